@@ -1,0 +1,153 @@
+/*
+ * pbrt_amd_rt.h -- C ABI of the MI355X (gfx950) ray-space accelerators.
+ *
+ * This is the drop-in boundary below pbrt's batched Aggregate interface.  It replaces the
+ * device side of the reference's three OpenCL accelerators:
+ *   Accelerator "naive"        src/accelerators/naive.cpp:204-437        (NaiveAccel)
+ *   Accelerator "rayhierarchy" src/accelerators/rayhierarchy.cpp:602-1326 (RayHieararchy)
+ *   Accelerator "ray-bvh"      src/accelerators/rayBVH.cpp:615-1323      (RayBVH)
+ * and their OpenCL plumbing (src/core/GPUparallel.cpp:479-712).  Plain pointers and sizes
+ * only; every entry point returns an rt_status (0 = success).  No C++ exception crosses it.
+ *
+ * Memory: scenes (triangles, UVs, the ray-bvh BVH) are copied once into device memory and
+ * stay resident until rt_scene_destroy.  Ray and hit arrays passed to the trace calls are
+ * DEVICE pointers owned by the caller; scratch (the ray hierarchy, reduction keys) is owned
+ * by the scene and grows on demand.  Calls on one scene must not overlap in time.
+ */
+#ifndef PBRT_AMD_RT_H
+#define PBRT_AMD_RT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rt_status {
+  RT_OK = 0,
+  RT_ERR_INVALID = 1,     /* bad argument (null pointer, size, shape, parameter) */
+  RT_ERR_HIP = 2,         /* a HIP runtime call failed */
+  RT_ERR_OOM = 3,         /* device allocation failed */
+  RT_ERR_UNSUPPORTED = 4, /* parameter combination this build does not implement */
+  RT_ERR_NO_DEVICE = 5,   /* no usable gfx950 device */
+  RT_ERR_STATE = 6        /* call order violated (e.g. ray-bvh trace before rt_scene_set_bvh) */
+} rt_status;
+
+/* Accelerator names of MakeAccelerator (src/core/api.cpp:576-596). */
+enum { RT_ACCEL_NAIVE = 0, RT_ACCEL_RAYHIERARCHY = 1, RT_ACCEL_RAYBVH = 2 };
+/* Ray-hierarchy node types (rayhierarchy.cpp:70-108): "box_dir" (6-D boxes, default), "ia". */
+enum { RT_NODE_BOX_DIR = 0, RT_NODE_IA = 1 };
+/* Closest-hit modes: primary camera rays (Intersect without bounce, rayhierarchy.cpp:816) and
+ * secondary rays with validity (Intersect(..., bounce), rayhierarchy.cpp:602). */
+enum { RT_CLOSEST_PRIMARY = 0, RT_CLOSEST_SECONDARY = 1 };
+
+/* Accelerator parameters of CreateNaiveAccelerator (naive.cpp:440-445), CreateRayHieararchy
+ * (rayhierarchy.cpp:1329-1348) and CreateRayBVH (rayBVH.cpp:1326-1344). */
+typedef struct rt_accel_desc {
+  int32_t kind;         /* RT_ACCEL_* */
+  int32_t node;         /* RT_NODE_*            ("node", default box_dir) */
+  int32_t chunk_x;      /* leaf tile width       ("chunkXSize", default 4) */
+  int32_t chunk_y;      /* leaf tile height      ("chunkYSize", default 4) */
+  int32_t height;       /* hierarchy height      ("height", default 3) */
+  int32_t repair_run;   /* repair passes         ("repairRun", default 1; rayhierarchy only) */
+  int32_t bvh_height;   /* BVH levels kept below the frontier ("BVHheight", default 3) */
+  int32_t max_bvh_prim; /* BVH leaf size         ("maxBVHPrim", default 1) */
+} rt_accel_desc;
+
+/* BVH node of the ray-bvh handoff, the reference's GPUNode (src/accelerators/bvh.h:75-79):
+ * interior nodes hold the index of child A in prim_offset (child B = A+1), leaves hold the
+ * first primitive position (into prim_order) and n_primitives > 0. */
+typedef struct rt_gpu_node {
+  float ax, ay, az, bx, by, bz;
+  uint32_t prim_offset;
+  uint32_t n_primitives;
+} rt_gpu_node;
+
+/* A ray batch in DEVICE memory as structure of arrays, laid out as a row-major ray image of
+ * width x height rays (the sampler order the reference's hierarchy assumes,
+ * rayhierarchy.cpp:258-265,377).  Use height = 1 for unstructured sets.
+ * valid: one byte per ray, nonzero = valid; NULL = all valid.  Ignored for
+ * RT_CLOSEST_PRIMARY, where every ray is traced (rayhierarchy.cpp:833-853). */
+typedef struct rt_rays {
+  const float *ox, *oy, *oz;
+  const float *dx, *dy, *dz;
+  const float *tmin, *tmax;
+  const uint8_t *valid;
+  int32_t width, height;
+} rt_rays;
+
+/* Closest-hit outputs in DEVICE memory, one entry per ray.  id is the ORIGINAL triangle index
+ * (ray-bvh ids are mapped back from BVH order) or -1 for a miss; t = INFINITY and u = v = 0 on
+ * a miss.  (u, v) are the interpolated surface coordinates of computeDpTuTv.cl:87-89.
+ * dpdu/dpdv (3 floats per ray each) are optional: NULL skips them. */
+typedef struct rt_hits {
+  int32_t *id;
+  float *t, *u, *v;
+  float *dpdu, *dpdv;
+} rt_hits;
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_scene rt_scene;
+
+/* Fills desc with the reference defaults for the accelerator kind. */
+int rt_accel_desc_default(int32_t kind, rt_accel_desc *desc);
+
+/* Opens the HIP device (replaces OpenCL::OpenCL, GPUparallel.cpp:39-124). */
+int rt_ctx_create(int32_t device, rt_ctx **out);
+int rt_ctx_destroy(rt_ctx *ctx);
+
+/* Uploads the flattened triangles once (replaces the per-call vertex uploads of
+ * naive.cpp:266-278, rayhierarchy.cpp:933-948, rayBVH.cpp:705-706).
+ * v9: HOST array of 9 floats per triangle (p1, p2, p3); uv6: HOST array of 6 floats per
+ * triangle or NULL for the default (0,0),(1,0),(1,1) (naive.cpp:91-97). */
+int rt_scene_create(rt_ctx *ctx, const float *v9, const float *uv6, uint32_t ntri,
+                    const rt_accel_desc *desc, rt_scene **out);
+/* ray-bvh only: the CPU-built BVH (BVHAccel::GPUbufferWide, bvh.cpp:154-206, one or more
+ * 60k-triangle parts concatenated).  roots are the frontier nodes (topLevelNodes of each part),
+ * prim_order maps BVH position -> original triangle index.  All HOST arrays. */
+int rt_scene_set_bvh(rt_scene *scene, const rt_gpu_node *nodes, uint32_t nnodes,
+                     const uint32_t *roots, uint32_t nroots, const uint32_t *prim_order);
+int rt_scene_destroy(rt_scene *scene);
+
+/* Host-side CPU BVH of ray-bvh, as RayBVH builds it before its first trace
+ * (rayBVH.cpp:144-146,204-276; BVHAccel bvh.cpp:209-434; GPUbufferWide bvh.cpp:154-206):
+ * global BVH order, parts of at most 60,000 triangles, per-part SAH BVH (12 buckets,
+ * max_prims <= 4 per leaf) flattened breadth first keeping the lowest bvh_height levels.
+ * Returns malloc'ed HOST arrays for rt_scene_set_bvh; release them with
+ * rt_raybvh_free_handoff. */
+int rt_raybvh_build_handoff(const float *v9, uint32_t ntri, uint32_t max_prims,
+                            uint32_t bvh_height, rt_gpu_node **nodes, uint32_t *nnodes,
+                            uint32_t **roots, uint32_t *nroots, uint32_t **prim_order);
+int rt_raybvh_free_handoff(rt_gpu_node *nodes, uint32_t *roots, uint32_t *prim_order);
+
+/* Batched closest hit: RayHieararchy::Intersect (rayhierarchy.cpp:602-813,816-1140),
+ * RayBVH::Intersect (rayBVH.cpp:615-836,839-1138), NaiveAccel::Intersect (naive.cpp:204-338).
+ * mode: RT_CLOSEST_PRIMARY or RT_CLOSEST_SECONDARY.  stream: hipStream_t or NULL. */
+int rt_trace_closest(rt_scene *scene, const rt_rays *rays, int32_t mode, const rt_hits *hits,
+                     void *stream);
+/* Batched occlusion: IntersectP (rayhierarchy.cpp:1170-1326, rayBVH.cpp:1170-1323,
+ * naive.cpp:357-437).  occluded[i] = 1 when ray i is valid and blocked in (tmin, tmax). */
+int rt_trace_occluded(rt_scene *scene, const rt_rays *rays, uint8_t *occluded, void *stream);
+
+/* Shadow rays toward a point light from closest hits, as the directlighting integrator builds
+ * them (VisibilityTester::SetSegment, src/core/light.h:79-84, PointLight::Sample_L,
+ * src/lights/point.cpp:42-49): o = p = o_r + d_r*t, d = (L-p)/|L-p|, tmin = 1e-3*t,
+ * tmax = |L-p|, valid = (id >= 0).  All DEVICE arrays of n rays. */
+int rt_make_shadow_rays(const rt_rays *primary, const int32_t *id, const float *t,
+                        const float light[3], float *ox, float *oy, float *oz, float *dx,
+                        float *dy, float *dz, float *tmin, float *tmax, uint8_t *valid,
+                        void *stream);
+
+/* HIP-event timing of the traversal kernels.  rt_scene_set_timing(scene, 1) starts recording
+ * one event pair around every traversal launch (and clears earlier records);
+ * rt_scene_kernel_time sums the recorded launches of one mode (RT_CLOSEST_PRIMARY,
+ * RT_CLOSEST_SECONDARY, or 2 for occlusion), waiting for them to finish.  This is the
+ * counterpart of the reference's GPU_TIMES accounting (GPUparallel.h:18-27,528-539). */
+int rt_scene_set_timing(rt_scene *scene, int32_t enable);
+int rt_scene_kernel_time(rt_scene *scene, int32_t mode, float *total_ms, int32_t *launches);
+
+const char *rt_error_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

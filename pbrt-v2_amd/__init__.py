@@ -1,0 +1,329 @@
+"""pbrt-v2_amd -- MI355X (gfx950) ray-space accelerators: naive, rayhierarchy, ray-bvh.
+
+Python view of the C ABI in include/pbrt_amd_rt.h (librt_amd.so, built by csrc/Makefile).
+torch is used only for device memory and streams; every computation runs in the HIP
+library.  Importing this package never falls back to a CPU path: a missing library raises.
+
+The directory name is not a Python identifier; load it with ``load_package()`` from the
+repository root helpers (tests/conftest.py, bench.py, __graft_entry__.py) or importlib.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "librt_amd.so"
+
+RT_OK = 0
+RT_ACCEL_NAIVE, RT_ACCEL_RAYHIERARCHY, RT_ACCEL_RAYBVH = 0, 1, 2
+RT_NODE_BOX_DIR, RT_NODE_IA = 0, 1
+RT_CLOSEST_PRIMARY, RT_CLOSEST_SECONDARY = 0, 1
+
+ACCEL_NAMES = {"naive": RT_ACCEL_NAIVE, "rayhierarchy": RT_ACCEL_RAYHIERARCHY,
+               "ray-bvh": RT_ACCEL_RAYBVH}
+
+# exported C symbols (every function declared in include/pbrt_amd_rt.h)
+EXPORTS = (
+    "rt_accel_desc_default", "rt_ctx_create", "rt_ctx_destroy", "rt_scene_create",
+    "rt_scene_set_bvh", "rt_scene_destroy", "rt_trace_closest", "rt_trace_occluded",
+    "rt_make_shadow_rays", "rt_scene_set_timing", "rt_scene_kernel_time",
+    "rt_error_string", "rt_raybvh_build_handoff", "rt_raybvh_free_handoff",
+)
+
+
+class RtError(RuntimeError):
+    def __init__(self, status: int, what: str, msg: str):
+        super().__init__(f"{what} failed: status {status} ({msg})")
+        self.status = status
+
+
+class AccelDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("node", ctypes.c_int32),
+                ("chunk_x", ctypes.c_int32), ("chunk_y", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("repair_run", ctypes.c_int32),
+                ("bvh_height", ctypes.c_int32), ("max_bvh_prim", ctypes.c_int32)]
+
+
+class GpuNode(ctypes.Structure):
+    _fields_ = [("ax", ctypes.c_float), ("ay", ctypes.c_float), ("az", ctypes.c_float),
+                ("bx", ctypes.c_float), ("by", ctypes.c_float), ("bz", ctypes.c_float),
+                ("prim_offset", ctypes.c_uint32), ("n_primitives", ctypes.c_uint32)]
+
+
+GPU_NODE_DTYPE = np.dtype([("ax", "<f4"), ("ay", "<f4"), ("az", "<f4"), ("bx", "<f4"),
+                           ("by", "<f4"), ("bz", "<f4"), ("prim_offset", "<u4"),
+                           ("n_primitives", "<u4")])
+
+_fp = ctypes.c_void_p
+
+
+class RtRays(ctypes.Structure):
+    _fields_ = [("ox", _fp), ("oy", _fp), ("oz", _fp), ("dx", _fp), ("dy", _fp), ("dz", _fp),
+                ("tmin", _fp), ("tmax", _fp), ("valid", _fp),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32)]
+
+
+class RtHits(ctypes.Structure):
+    _fields_ = [("id", _fp), ("t", _fp), ("u", _fp), ("v", _fp), ("dpdu", _fp), ("dpdv", _fp)]
+
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads librt_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR / 'csrc'}` "
+                           "(or __graft_entry__.build())")
+    L = ctypes.CDLL(str(LIB_PATH))
+    P = ctypes.POINTER
+    L.rt_accel_desc_default.argtypes = [ctypes.c_int32, P(AccelDesc)]
+    L.rt_ctx_create.argtypes = [ctypes.c_int32, P(ctypes.c_void_p)]
+    L.rt_ctx_destroy.argtypes = [ctypes.c_void_p]
+    L.rt_scene_create.argtypes = [ctypes.c_void_p, _fp, _fp, ctypes.c_uint32, P(AccelDesc),
+                                  P(ctypes.c_void_p)]
+    L.rt_scene_set_bvh.argtypes = [ctypes.c_void_p, _fp, ctypes.c_uint32, _fp, ctypes.c_uint32, _fp]
+    L.rt_scene_destroy.argtypes = [ctypes.c_void_p]
+    L.rt_trace_closest.argtypes = [ctypes.c_void_p, P(RtRays), ctypes.c_int32, P(RtHits), _fp]
+    L.rt_trace_occluded.argtypes = [ctypes.c_void_p, P(RtRays), _fp, _fp]
+    L.rt_make_shadow_rays.argtypes = [P(RtRays), _fp, _fp, P(ctypes.c_float)] + [_fp] * 9 + [_fp]
+    L.rt_scene_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.rt_scene_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_float),
+                                       P(ctypes.c_int32)]
+    L.rt_error_string.argtypes = [ctypes.c_int]
+    L.rt_error_string.restype = ctypes.c_char_p
+    L.rt_raybvh_build_handoff.argtypes = [_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                          P(ctypes.c_void_p), P(ctypes.c_uint32), P(ctypes.c_void_p),
+                                          P(ctypes.c_uint32), P(ctypes.c_void_p)]
+    L.rt_raybvh_free_handoff.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    _LIB = L
+    return L
+
+
+def check(status: int, what: str) -> None:
+    if status != RT_OK:
+        msg = lib().rt_error_string(status).decode()
+        raise RtError(status, what, msg)
+
+
+def default_desc(kind, **overrides) -> AccelDesc:
+    """Reference defaults of the named accelerator (naive / rayhierarchy / ray-bvh)."""
+    if isinstance(kind, str):
+        kind = ACCEL_NAMES[kind]
+    d = AccelDesc()
+    check(lib().rt_accel_desc_default(kind, ctypes.byref(d)), "rt_accel_desc_default")
+    for k, v in overrides.items():
+        if k == "node" and isinstance(v, str):
+            v = {"box_dir": RT_NODE_BOX_DIR, "ia": RT_NODE_IA}[v]
+        setattr(d, k, int(v))
+    return d
+
+
+def raybvh_handoff(v9: np.ndarray, max_prims: int = 1, bvh_height: int = 3):
+    """CPU-built ray-bvh handoff (global BVH order, 60k-triangle parts, per-part SAH BVH and
+    breadth-first GPUNode layout).  Returns (nodes, roots, prim_order) numpy arrays."""
+    L = lib()
+    v9 = np.ascontiguousarray(v9, dtype=np.float32).reshape(-1)
+    ntri = v9.size // 9
+    nodes_p, roots_p, order_p = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    nn, nr = ctypes.c_uint32(), ctypes.c_uint32()
+    check(L.rt_raybvh_build_handoff(v9.ctypes.data, ntri, max_prims, bvh_height,
+                                    ctypes.byref(nodes_p), ctypes.byref(nn), ctypes.byref(roots_p),
+                                    ctypes.byref(nr), ctypes.byref(order_p)),
+          "rt_raybvh_build_handoff")
+    try:
+        nodes = np.ctypeslib.as_array(ctypes.cast(nodes_p, ctypes.POINTER(ctypes.c_uint8)),
+                                      shape=(nn.value * 32,)).view(GPU_NODE_DTYPE).copy()
+        roots = np.ctypeslib.as_array(ctypes.cast(roots_p, ctypes.POINTER(ctypes.c_uint32)),
+                                      shape=(max(nr.value, 1),))[:nr.value].copy()
+        order = np.ctypeslib.as_array(ctypes.cast(order_p, ctypes.POINTER(ctypes.c_uint32)),
+                                      shape=(max(ntri, 1),))[:ntri].copy()
+    finally:
+        L.rt_raybvh_free_handoff(nodes_p, roots_p, order_p)
+    return nodes, roots, order
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclass
+class DeviceRays:
+    """A ray image in device memory (torch tensors on the HIP device)."""
+    ox: object
+    oy: object
+    oz: object
+    dx: object
+    dy: object
+    dz: object
+    tmin: object
+    tmax: object
+    valid: object  # uint8 tensor or None
+    width: int
+    height: int
+
+    @property
+    def count(self) -> int:
+        return self.width * self.height
+
+    @staticmethod
+    def from_numpy(r: dict, device="cuda"):
+        torch = _torch()
+
+        def f(a, dt=torch.float32):
+            return torch.as_tensor(np.ascontiguousarray(a)).to(device=device, dtype=dt)
+
+        valid = r.get("valid")
+        return DeviceRays(f(r["ox"]), f(r["oy"]), f(r["oz"]), f(r["dx"]), f(r["dy"]), f(r["dz"]),
+                          f(r["tmin"]), f(r["tmax"]),
+                          None if valid is None else f(valid, torch.uint8),
+                          int(r["width"]), int(r["height"]))
+
+    def c_struct(self) -> RtRays:
+        p = lambda t: None if t is None else t.data_ptr()
+        return RtRays(p(self.ox), p(self.oy), p(self.oz), p(self.dx), p(self.dy), p(self.dz),
+                      p(self.tmin), p(self.tmax), p(self.valid), self.width, self.height)
+
+
+@dataclass
+class DeviceHits:
+    id: object
+    t: object
+    u: object
+    v: object
+    dpdu: object = None
+    dpdv: object = None
+
+    @staticmethod
+    def empty(n: int, device="cuda", shading: bool = False):
+        torch = _torch()
+        h = DeviceHits(torch.empty(n, dtype=torch.int32, device=device),
+                       torch.empty(n, dtype=torch.float32, device=device),
+                       torch.empty(n, dtype=torch.float32, device=device),
+                       torch.empty(n, dtype=torch.float32, device=device))
+        if shading:
+            h.dpdu = torch.empty(3 * n, dtype=torch.float32, device=device)
+            h.dpdv = torch.empty(3 * n, dtype=torch.float32, device=device)
+        return h
+
+    def c_struct(self) -> RtHits:
+        p = lambda t: None if t is None else t.data_ptr()
+        return RtHits(p(self.id), p(self.t), p(self.u), p(self.v), p(self.dpdu), p(self.dpdv))
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return _torch().cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.handle = ctypes.c_void_p()
+        check(lib().rt_ctx_create(device, ctypes.byref(self.handle)), "rt_ctx_create")
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().rt_ctx_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
+class Scene:
+    """A resident scene behind one accelerator (the plain per-scene form)."""
+
+    def __init__(self, ctx: Context, v9: np.ndarray, uv6: np.ndarray | None = None,
+                 desc: AccelDesc | str = "rayhierarchy", build_bvh: bool = True):
+        if isinstance(desc, str):
+            desc = default_desc(desc)
+        self.desc = desc
+        self.ctx = ctx
+        self.v9 = np.ascontiguousarray(v9, dtype=np.float32).reshape(-1)
+        self.ntri = self.v9.size // 9
+        self.uv6 = None if uv6 is None else np.ascontiguousarray(uv6, dtype=np.float32).reshape(-1)
+        self.handle = ctypes.c_void_p()
+        check(lib().rt_scene_create(ctx.handle, self.v9.ctypes.data,
+                                    None if self.uv6 is None else self.uv6.ctypes.data,
+                                    self.ntri, ctypes.byref(desc), ctypes.byref(self.handle)),
+              "rt_scene_create")
+        self.handoff = None
+        if desc.kind == RT_ACCEL_RAYBVH and build_bvh and self.ntri:
+            self.set_bvh(*raybvh_handoff(self.v9, desc.max_bvh_prim, desc.bvh_height))
+
+    def set_bvh(self, nodes: np.ndarray, roots: np.ndarray, prim_order: np.ndarray):
+        nodes = np.ascontiguousarray(nodes, dtype=GPU_NODE_DTYPE)
+        roots = np.ascontiguousarray(roots, dtype=np.uint32)
+        prim_order = np.ascontiguousarray(prim_order, dtype=np.uint32)
+        check(lib().rt_scene_set_bvh(self.handle, nodes.ctypes.data, nodes.size,
+                                     roots.ctypes.data, roots.size, prim_order.ctypes.data),
+              "rt_scene_set_bvh")
+        self.handoff = (nodes, roots, prim_order)
+
+    def trace_closest(self, rays: DeviceRays, mode: int = RT_CLOSEST_PRIMARY, hits=None,
+                      stream=None, shading: bool = False) -> DeviceHits:
+        if hits is None:
+            hits = DeviceHits.empty(rays.count, device=rays.ox.device, shading=shading)
+        rs, hs = rays.c_struct(), hits.c_struct()
+        check(lib().rt_trace_closest(self.handle, ctypes.byref(rs), mode, ctypes.byref(hs),
+                                     _stream_ptr(stream)), "rt_trace_closest")
+        return hits
+
+    def trace_occluded(self, rays: DeviceRays, out=None, stream=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty(rays.count, dtype=torch.uint8, device=rays.ox.device)
+        rs = rays.c_struct()
+        check(lib().rt_trace_occluded(self.handle, ctypes.byref(rs), out.data_ptr(),
+                                      _stream_ptr(stream)), "rt_trace_occluded")
+        return out
+
+    def set_timing(self, on: bool):
+        check(lib().rt_scene_set_timing(self.handle, 1 if on else 0), "rt_scene_set_timing")
+
+    def kernel_time(self, mode: int):
+        """(total ms, launches) of the traversal kernels of one mode recorded since
+        set_timing(True); mode 0/1 closest primary/secondary, 2 occlusion."""
+        ms, n = ctypes.c_float(), ctypes.c_int32()
+        check(lib().rt_scene_kernel_time(self.handle, mode, ctypes.byref(ms), ctypes.byref(n)),
+              "rt_scene_kernel_time")
+        return ms.value, n.value
+
+    def close(self):
+        if self.handle:
+            lib().rt_scene_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
+def make_shadow_rays(primary: DeviceRays, hits: DeviceHits, light, stream=None,
+                     out: DeviceRays | None = None) -> DeviceRays:
+    """Point-light shadow rays from closest hits, built on the device (into `out` if given)."""
+    torch = _torch()
+    n = primary.count
+    dev = primary.ox.device
+    if out is None:
+        f = lambda: torch.empty(n, dtype=torch.float32, device=dev)
+        out = DeviceRays(f(), f(), f(), f(), f(), f(), f(), f(),
+                         torch.empty(n, dtype=torch.uint8, device=dev), primary.width,
+                         primary.height)
+    ps = primary.c_struct()
+    lt = (ctypes.c_float * 3)(*[float(x) for x in light])
+    check(lib().rt_make_shadow_rays(ctypes.byref(ps), hits.id.data_ptr(), hits.t.data_ptr(), lt,
+                                    out.ox.data_ptr(), out.oy.data_ptr(), out.oz.data_ptr(),
+                                    out.dx.data_ptr(), out.dy.data_ptr(), out.dz.data_ptr(),
+                                    out.tmin.data_ptr(), out.tmax.data_ptr(),
+                                    out.valid.data_ptr(), _stream_ptr(stream)),
+          "rt_make_shadow_rays")
+    return out
+
+
+__all__ = [n for n in dir() if not n.startswith("_")]
+del os

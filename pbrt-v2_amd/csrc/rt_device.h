@@ -1,0 +1,226 @@
+// rt_device.h -- FP32 device arithmetic of the ray-space accelerators (gfx950).
+//
+// Every function fixes its operation order so results are reproducible bit for bit:
+// the library is compiled with -ffp-contract=off (no FMA contraction) and correctly rounded
+// FP32 division and square root; min/max are written as selects.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rtd {
+
+// EPS of intersectAllLeaves.cl:8 and naive.cl:2 in FP32 (the kernels do not enable
+// cl_khr_fp64, so their double literals are single precision).
+constexpr float kEps = 0.002f;
+constexpr float kNaiveLo = -1e-3f - kEps;            // naive.cl:38,43
+constexpr float kNaiveHi = (1.0f + 1e-3f) + kEps;    // naive.cl:38,43
+constexpr float kNaivePLo = -1e-3f;                  // naive.cl:219,224
+constexpr float kNaivePHi = 1.0f + 1e-3f;            // naive.cl:219,224
+constexpr float kRepHi1 = 1.0f + kEps;               // intersectAllLeaves.cl:217
+constexpr float kRepHi2 = 1.0f - kEps;               // intersectAllLeaves.cl:222
+
+struct V3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ float fmn(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float fmx(float a, float b) { return b > a ? b : a; }
+__device__ __forceinline__ V3 vmin(V3 a, V3 b) { return {fmn(a.x, b.x), fmn(a.y, b.y), fmn(a.z, b.z)}; }
+__device__ __forceinline__ V3 vmax(V3 a, V3 b) { return {fmx(a.x, b.x), fmx(a.y, b.y), fmx(a.z, b.z)}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+// normalize(v) = v * (1/sqrt(v.v)); a zero vector is returned unchanged.
+__device__ __forceinline__ V3 normalize(V3 v) {
+  float l2 = dot(v, v);
+  if (l2 == 0.f) return v;
+  float inv = 1.0f / sqrtf(l2);
+  return {v.x * inv, v.y * inv, v.z * inv};
+}
+
+// Triangle as the kernels use it: vertices plus the two edges from v1.
+struct Tri {
+  V3 v1, v2, v3, e1, e2;
+};
+__device__ __forceinline__ Tri make_tri(V3 a, V3 b, V3 c) {
+  Tri t;
+  t.v1 = a;
+  t.v2 = b;
+  t.v3 = c;
+  t.e1 = sub(b, a);
+  t.e2 = sub(c, a);
+  return t;
+}
+
+// Strict Moller-Trumbore of intersectAllLeaves.cl:40-55: det != 0, b1 in (0,1), b2 > 0,
+// b1 + b2 < 1.  Returns false when the barycentric test fails; t/b1/b2 are then undefined.
+__device__ __forceinline__ bool mt_strict(V3 o, V3 d, const Tri& T, float& t, float& b1, float& b2) {
+  V3 s = cross(d, T.e2);
+  float det = dot(s, T.e1);
+  if (det == 0.0f) return false;
+  float inv = 1.0f / det;
+  V3 ov = sub(o, T.v1);
+  b1 = dot(ov, s) * inv;
+  if (!(b1 > 0.f && b1 < 1.f)) return false;
+  V3 s2 = cross(ov, T.e1);
+  b2 = dot(d, s2) * inv;
+  if (!(b2 > 0.f && (b1 + b2) < 1.f)) return false;
+  t = dot(T.e2, s2) * inv;
+  return true;
+}
+
+// Shrunk barycentric region of the repair pass (intersectAllLeaves.cl:216-222).  Only called
+// on values that already passed mt_strict, whose region contains this one.
+__device__ __forceinline__ bool repair_region(float b1, float b2) {
+  return !(b1 < kEps || b1 > kRepHi1) && !(b2 < kEps || (b1 + b2) > kRepHi2);
+}
+
+// naive.cl:30-46 (TriRay closest).
+__device__ __forceinline__ bool mt_naive(V3 o, V3 d, const Tri& T, float& t, float& b1, float& b2) {
+  V3 s1 = cross(d, T.e2);
+  float div = dot(s1, T.e1);
+  if (div < kEps && div > -kEps) return false;
+  float inv = 1.0f / div;
+  V3 dd = sub(o, T.v1);
+  b1 = dot(dd, s1) * inv;
+  if (b1 < kNaiveLo || b1 > kNaiveHi) return false;
+  V3 s2 = cross(dd, T.e1);
+  b2 = dot(d, s2) * inv;
+  if (b2 < kNaiveLo || (b1 + b2) > kNaiveHi) return false;
+  t = dot(T.e2, s2) * inv;
+  return true;
+}
+
+// naive.cl:211-227 (IntersectionP).
+__device__ __forceinline__ bool mt_naive_p(V3 o, V3 d, const Tri& T, float& t) {
+  V3 s1 = cross(d, T.e2);
+  float div = dot(s1, T.e1);
+  if (div == 0.0f) return false;
+  float inv = 1.0f / div;
+  V3 dd = sub(o, T.v1);
+  float b1 = dot(dd, s1) * inv;
+  if (b1 < kNaivePLo || b1 > kNaivePHi) return false;
+  V3 s2 = cross(dd, T.e1);
+  float b2 = dot(d, s2) * inv;
+  if (b2 < kNaivePLo || (b1 + b2) > kNaivePHi) return false;
+  t = dot(T.e2, s2) * inv;
+  return true;
+}
+
+// Barycentrics of a known winner (computeDpTuTv.cl:36-46, no range tests).
+__device__ __forceinline__ void bary(V3 o, V3 d, const Tri& T, float& b1, float& b2) {
+  V3 s1 = cross(d, T.e2);
+  float div = dot(s1, T.e1);
+  float inv = 1.0f / div;
+  V3 dd = sub(o, T.v1);
+  b1 = dot(dd, s1) * inv;
+  V3 s2 = cross(dd, T.e1);
+  b2 = dot(d, s2) * inv;
+}
+
+// Ray-hierarchy node: the component-wise box of directions and origins of its rays.
+struct Node {
+  V3 dmin, dmax, omin, omax;
+};
+
+// intersectsNode6DB.cl:53-63 / 88-98 with the uninitialised `ret` taken as false.
+__device__ __forceinline__ bool box_logic(V3 tmin, V3 tmax, const Node& n) {
+  V3 f0 = vmax(tmin, n.dmin);
+  V3 f1 = vmin(tmax, n.dmax);
+  bool ret = false;
+  if (tmin.x < 0.f && tmax.x > 0.f) ret = (f0.x < f1.x + kEps);
+  if (tmin.y < 0.f && tmax.y > 0.f) ret |= (f0.y < f1.y + kEps);
+  if (tmin.z < 0.f && tmax.z > 0.f) ret |= (f0.z < f1.z + kEps);
+  return ret || (f0.x < f1.x + kEps && f0.y < f1.y + kEps && f0.z < f1.z + kEps);
+}
+
+// Origin-box centre used by intersectsTri (intersectsNode6DB.cl:73).
+__device__ __forceinline__ V3 tri_center(const Node& n) {
+  return {(n.omin.x + n.omax.x) / 2, (n.omin.y + n.omax.y) / 2, (n.omin.z + n.omax.z) / 2};
+}
+// Direction box of a triangle seen from oc (intersectsNode6DB.cl:76-86).
+__device__ __forceinline__ void tri_dirbox(V3 oc, const Tri& T, V3& tmin, V3& tmax) {
+  V3 a = normalize(sub(T.v1, oc));
+  tmin = a;
+  tmax = a;
+  V3 b = normalize(sub(T.v2, oc));
+  tmin = vmin(tmin, b);
+  tmax = vmax(tmax, b);
+  V3 c = normalize(sub(T.v3, oc));
+  tmin = vmin(tmin, c);
+  tmax = vmax(tmax, c);
+}
+// intersectsTri, intersectsNode6DB.cl:66-99.
+__device__ __forceinline__ bool cull_tri(const Node& n, const Tri& T) {
+  V3 tmin, tmax;
+  tri_dirbox(tri_center(n), T, tmin, tmax);
+  return box_logic(tmin, tmax, n);
+}
+
+// intersectsNode, intersectsNode6DB.cl:6-64 (box Minkowski-grown by the origin half-extent).
+__device__ __forceinline__ bool cull_node6(const Node& n, V3 bmin, V3 bmax) {
+  V3 h = {(n.omax.x - n.omin.x) / 2, (n.omax.y - n.omin.y) / 2, (n.omax.z - n.omin.z) / 2};
+  bmin = sub(bmin, h);
+  bmax = add(bmax, h);
+  V3 oc = add(h, n.omin);
+  V3 r = normalize(sub(V3{bmin.x, bmin.y, bmin.z}, oc));
+  V3 tmin = r, tmax = r;
+#define RT_CORNER(X, Y, Z)              \
+  r = normalize(sub(V3{X, Y, Z}, oc)); \
+  tmin = vmin(tmin, r);                 \
+  tmax = vmax(tmax, r);
+  RT_CORNER(bmin.x, bmin.y, bmax.z)
+  RT_CORNER(bmin.x, bmax.y, bmin.z)
+  RT_CORNER(bmin.x, bmax.y, bmax.z)
+  RT_CORNER(bmax.x, bmin.y, bmin.z)
+  RT_CORNER(bmax.x, bmax.y, bmin.z)
+  RT_CORNER(bmax.x, bmin.y, bmax.z)
+  RT_CORNER(bmax.x, bmax.y, bmax.z)
+#undef RT_CORNER
+  return box_logic(tmin, tmax, n);
+}
+
+// intersectsNodeIA.cl:6-65 (interval slab test).
+__device__ __forceinline__ void ia_axis(float lo, float hi, float dlo, float dhi, float& smin,
+                                        float& smax) {
+  float a = lo * dlo, b = hi * dlo, c = hi * dhi, d = lo * dhi;
+  smin = fmn(fmn(fmn(a, b), c), d);
+  smax = fmx(fmx(fmx(a, b), c), d);
+}
+__device__ __forceinline__ bool cull_ia(const Node& n, V3 bmin, V3 bmax) {
+  V3 lo = sub(bmin, n.omax);
+  V3 hi = sub(bmax, n.omin);
+  if ((n.dmin.x <= 0.f && n.dmax.x >= 0.f) || (n.dmin.y <= 0.f && n.dmax.y >= 0.f) ||
+      (n.dmin.z <= 0.f && n.dmax.z >= 0.f))
+    return true;
+  float ix0 = 1 / n.dmax.x, ix1 = 1 / n.dmin.x;
+  float iy0 = 1 / n.dmax.y, iy1 = 1 / n.dmin.y;
+  float iz0 = 1 / n.dmax.z, iz1 = 1 / n.dmin.z;
+  float sx, sy, tx, ty, ux, uy;
+  ia_axis(lo.x, hi.x, ix0, ix1, sx, sy);
+  ia_axis(lo.y, hi.y, iy0, iy1, tx, ty);
+  ia_axis(lo.z, hi.z, iz0, iz1, ux, uy);
+  sx = fmx(fmx(sx, tx), ux);
+  sy = fmn(fmn(sy, ty), uy);
+  return sx < sy + kEps;
+}
+
+// Order-preserving 32-bit key of a float (+0 and -0 share the key of +0), and the 64-bit
+// nearest-hit key (t, id): the smallest key is the nearest hit, ties -> smallest id.
+__device__ __forceinline__ uint32_t fkey(float t) {
+  uint32_t b = __float_as_uint(t == 0.0f ? 0.0f : t);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_inv(uint32_t k) {
+  uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(b);
+}
+constexpr uint64_t kMissKey = ~0ull;
+__device__ __forceinline__ uint64_t hit_key(float t, int32_t id) {
+  return ((uint64_t)fkey(t) << 32) | (uint32_t)id;
+}
+
+}  // namespace rtd

@@ -1,0 +1,1048 @@
+// rt_kernels.hip -- gfx950 kernels of the ray-space accelerators.
+//
+// Data layout in HBM (all FP32 unless noted):
+//   triangles  v9[9*ntri] (p1, p2, p3), resident per scene; uv6[6*ntri] optional
+//   rays       structure of arrays ox..dz, tmin, tmax (+ valid bytes), row-major ray image
+//   hierarchy  4 float4 per node {dmin|valid, dmax, omin, omax}, level-major, row-major
+//   keys       one uint64 per ray: (order-preserving t bits << 32) | triangle id
+//   bvh        2 float4 per node {ax ay az bx}{by bz primOffset nPrimitives}
+//
+// Work decomposition (one 256-thread work-group per top-level hierarchy node, "block"):
+// the block's rays live in registers, triangles stream through LDS in tiles of 256, each
+// thread evaluates the reference's cull chain for one triangle against the block's subtree,
+// surviving (triangle, leaf-mask) pairs are compacted with a wavefront ballot into an LDS
+// list, and every ray then runs the leaf test against the list.  The nearest hit is reduced
+// in registers as a (t, id) key, so the result is deterministic (ties -> smallest id).
+#include "rt_device.h"
+#include "rt_internal.h"
+
+using namespace rtd;
+
+namespace rti {
+
+namespace {
+
+constexpr uint32_t kNoKey32 = 0xffffffffu;
+constexpr int kBvhCap = 2048;  // LDS list capacity of the ray-bvh kernel (entries per tile)
+constexpr int kStack = 32;     // per-thread stack of the ray-bvh product traversal
+
+__device__ __forceinline__ int pow4_off(int s) { return ((1 << (2 * s)) - 1) / 3; }
+
+__device__ __forceinline__ V3 ld3(const float* __restrict__ p) { return {p[0], p[1], p[2]}; }
+
+__device__ __forceinline__ Tri load_tri(const float* __restrict__ v9, uint32_t i) {
+  const float* p = v9 + 9 * (size_t)i;
+  return make_tri(ld3(p), ld3(p + 3), ld3(p + 6));
+}
+
+__device__ __forceinline__ void node_store(float4* __restrict__ nodes, size_t idx, const float* b,
+                                           bool valid) {
+  float4* p = nodes + 4 * idx;
+  p[0] = make_float4(b[0], b[1], b[2], valid ? 1.f : 0.f);
+  p[1] = make_float4(b[3], b[4], b[5], 0.f);
+  p[2] = make_float4(b[6], b[7], b[8], 0.f);
+  p[3] = make_float4(b[9], b[10], b[11], 0.f);
+}
+
+// Leaf-major mapping of a block-local ray slot: the rays of one leaf are consecutive slots,
+// so a wavefront covers whole leaves.
+__device__ __forceinline__ bool slot_xy(const RHGeom& g, int K, int J, int rl, int& x, int& y,
+                                        int& leaf) {
+  if (rl >= g.rb) return false;
+  leaf = rl / g.g;
+  int q = rl - leaf * g.g;
+  int lk = leaf % g.nlx, lj = leaf / g.nlx;
+  x = K * g.bw + lk * g.cx + q % g.cx;
+  y = J * g.bh + lj * g.cy + q / g.cx;
+  return x < g.W && y < g.H;
+}
+
+__device__ __forceinline__ void box_combine(float* a, bool& ha, const float* b, bool hb) {
+  if (!hb) return;
+  if (!ha) {
+    for (int k = 0; k < 12; ++k) a[k] = b[k];
+    ha = true;
+    return;
+  }
+  for (int k = 0; k < 3; ++k) a[k] = fmn(a[k], b[k]);
+  for (int k = 3; k < 6; ++k) a[k] = fmx(a[k], b[k]);
+  for (int k = 6; k < 9; ++k) a[k] = fmn(a[k], b[k]);
+  for (int k = 9; k < 12; ++k) a[k] = fmx(a[k], b[k]);
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused hierarchy build: one work-group builds the leaves and every level of one block
+// (rayhconstruct6DB.cl / rayhconstructP6DB.cl + levelConstruct6DB.cl / levelConstructP6DB.cl
+// folded into one launch instead of 1 + (height-1) launches with host waits,
+// rayhierarchy.cpp:375-549).
+template <bool USE_VALID>
+__global__ __launch_bounds__(256) void rh_build_kernel(RHGeom g, RaysDev r,
+                                                       float4* __restrict__ nodes) {
+  __shared__ float sBox[kMaxSub][12];
+  __shared__ int sVal[kMaxSub];
+  const int tid = threadIdx.x;
+  const int top = g.L - 1;
+  const int K = blockIdx.x % g.gw[top], J = blockIdx.x / g.gw[top];
+  const int nleaf = g.nlx * g.nlx;
+  const int leaf_off = g.nsub - nleaf;
+
+  const bool pow2 = g.g <= 64 && (g.g & (g.g - 1)) == 0;
+  if (pow2) {
+    for (int base = 0; base < g.rb; base += kThreads) {
+      int rl = base + tid;
+      int x = 0, y = 0, leaf = 0;
+      float v[12];
+      bool have = false;
+      bool in_block = rl < g.rb;
+      if (slot_xy(g, K, J, rl, x, y, leaf)) {
+        size_t i = (size_t)y * g.W + x;
+        have = !USE_VALID || r.valid == nullptr || r.valid[i] != 0;
+        if (have) {
+          v[0] = v[3] = r.dx[i];
+          v[1] = v[4] = r.dy[i];
+          v[2] = v[5] = r.dz[i];
+          v[6] = v[9] = r.ox[i];
+          v[7] = v[10] = r.oy[i];
+          v[8] = v[11] = r.oz[i];
+        }
+      }
+      if (!have)
+        for (int k = 0; k < 12; ++k) v[k] = 0.f;
+      for (int m = g.g >> 1; m >= 1; m >>= 1) {
+        float o[12];
+        for (int k = 0; k < 12; ++k) o[k] = __shfl_xor(v[k], m);
+        bool ho = __shfl_xor((int)have, m) != 0;
+        box_combine(v, have, o, ho);
+      }
+      if (in_block && (rl % g.g) == 0) {
+        int li = rl / g.g;
+        for (int k = 0; k < 12; ++k) sBox[leaf_off + li][k] = v[k];
+        sVal[leaf_off + li] = have ? 1 : 0;
+      }
+    }
+  } else {
+    for (int li = tid; li < nleaf; li += kThreads) {
+      float v[12];
+      bool have = false;
+      int lk = li % g.nlx, lj = li / g.nlx;
+      int x0 = K * g.bw + lk * g.cx, y0 = J * g.bh + lj * g.cy;
+      for (int y = y0; y < min(g.H, y0 + g.cy); ++y)
+        for (int x = x0; x < min(g.W, x0 + g.cx); ++x) {
+          size_t i = (size_t)y * g.W + x;
+          if (USE_VALID && r.valid != nullptr && r.valid[i] == 0) continue;
+          float b[12] = {r.dx[i], r.dy[i], r.dz[i], r.dx[i], r.dy[i], r.dz[i],
+                         r.ox[i], r.oy[i], r.oz[i], r.ox[i], r.oy[i], r.oz[i]};
+          box_combine(v, have, b, true);
+        }
+      if (!have)
+        for (int k = 0; k < 12; ++k) v[k] = 0.f;
+      for (int k = 0; k < 12; ++k) sBox[leaf_off + li][k] = v[k];
+      sVal[leaf_off + li] = have ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  for (int li = tid; li < nleaf; li += kThreads) {
+    int gk = K * g.nlx + li % g.nlx, gj = J * g.nlx + li / g.nlx;
+    if (gk < g.gw[0] && gj < g.gh[0])
+      node_store(nodes, (size_t)g.off[0] + (size_t)gj * g.gw[0] + gk, sBox[leaf_off + li],
+                 sVal[leaf_off + li] != 0);
+  }
+  for (int l = 1; l < g.L; ++l) {
+    const int s = top - l, nl = 1 << s, cnl = nl * 2;
+    const int soff = pow4_off(s), coff = pow4_off(s + 1);
+    for (int i = tid; i < nl * nl; i += kThreads) {
+      int dk = i % nl, dj = i / nl;
+      float v[12];
+      bool have = false;
+      for (int a = 0; a < 4; ++a) {
+        int c = (2 * dj + (a >> 1)) * cnl + 2 * dk + (a & 1);
+        box_combine(v, have, sBox[coff + c], sVal[coff + c] != 0);
+      }
+      if (!have)
+        for (int k = 0; k < 12; ++k) v[k] = 0.f;
+      for (int k = 0; k < 12; ++k) sBox[soff + i][k] = v[k];
+      sVal[soff + i] = have ? 1 : 0;
+      int gk = K * nl + dk, gj = J * nl + dj;
+      if (gk < g.gw[l] && gj < g.gh[l])
+        node_store(nodes, (size_t)g.off[l] + (size_t)gj * g.gw[l] + gk, v, have);
+    }
+    __syncthreads();
+  }
+}
+
+// Loads the subtree of block (K, J) into LDS: sub-level s = L-1-l, row-major within it.
+__device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J,
+                                             const float4* __restrict__ nodes, Node* sN, int* sV) {
+  const int top = g.L - 1;
+  for (int i = threadIdx.x; i < g.nsub; i += kThreads) {
+    int s = 0;
+    while (pow4_off(s + 1) <= i) ++s;
+    int local = i - pow4_off(s), nl = 1 << s, l = top - s;
+    int gk = K * nl + local % nl, gj = J * nl + local / nl;
+    int valid = 0;
+    Node n{};
+    if (gk < g.gw[l] && gj < g.gh[l]) {
+      const float4* p = nodes + 4 * ((size_t)g.off[l] + (size_t)gj * g.gw[l] + gk);
+      float4 a = p[0], b = p[1], c = p[2], d = p[3];
+      n.dmin = {a.x, a.y, a.z};
+      n.dmax = {b.x, b.y, b.z};
+      n.omin = {c.x, c.y, c.z};
+      n.omax = {d.x, d.y, d.z};
+      valid = a.w != 0.f;
+    }
+    sN[i] = n;
+    sV[i] = valid;
+  }
+}
+
+// Cull of one hierarchy node against one triangle, dispatched on the reference's kernel
+// variant: IA for node "ia"; intersectsTri for box_dir primary rays; intersectsNode
+// (triangle AABB) for box_dir secondary and shadow rays.  The direction box of the
+// triangle seen from the node's origin centre is reused while the centre is unchanged.
+template <int MODE, int NODE>
+struct Culler {
+  const Tri& T;
+  V3 bmin, bmax;
+  V3 coc, cmin, cmax;
+  bool have = false;
+  __device__ Culler(const Tri& t) : T(t) {
+    bmin = vmin(vmin(t.v1, t.v2), t.v3);
+    bmax = vmax(vmax(t.v1, t.v2), t.v3);
+  }
+  __device__ __forceinline__ bool operator()(const Node& n) {
+    if (NODE == 1) return cull_ia(n, bmin, bmax);
+    if (MODE == 0) {
+      V3 oc = tri_center(n);
+      if (!have || __float_as_uint(oc.x) != __float_as_uint(coc.x) ||
+          __float_as_uint(oc.y) != __float_as_uint(coc.y) ||
+          __float_as_uint(oc.z) != __float_as_uint(coc.z)) {
+        tri_dirbox(oc, T, cmin, cmax);
+        coc = oc;
+        have = true;
+      }
+      return box_logic(cmin, cmax, n);
+    }
+    return cull_node6(n, bmin, bmax);
+  }
+};
+
+// Leaf mask of the block subtree for one triangle: bit (lj*nlx + lk) is set when every node
+// on the path from the block root to leaf (lk, lj) passes the cull (the candidate set of the
+// reference's per-triangle DFS, intersection6DB.cl:51-119).
+template <int MODE, int NODE>
+__device__ __forceinline__ uint64_t leaf_mask(const Tri& T, const Node* sN, const int* sV, int L) {
+  Culler<MODE, NODE> cull(T);
+  if (!sV[0] || !cull(sN[0])) return 0;
+  uint64_t cur = 1;
+  int soff = 0;
+  for (int s = 1; s < L; ++s) {
+    const int pnl = 1 << (s - 1), nl = 1 << s, coff = soff + pnl * pnl;
+    uint64_t next = 0;
+    while (cur) {
+      int b = __ffsll((unsigned long long)cur) - 1;
+      cur &= cur - 1;
+      int dk = b & (pnl - 1), dj = b >> (s - 1);
+      for (int a = 0; a < 4; ++a) {
+        int c = (2 * dj + (a >> 1)) * nl + 2 * dk + (a & 1);
+        if (sV[coff + c] && cull(sN[coff + c])) next |= 1ull << c;
+      }
+    }
+    cur = next;
+    soff = coff;
+    if (!cur) break;
+  }
+  return cur;
+}
+
+struct RayReg {
+  V3 o, d;
+  float tmin, tmax;
+  int leaf;
+  size_t idx;
+  bool exists, valid;
+};
+
+template <int RPT>
+__device__ __forceinline__ void load_block_rays(const RHGeom& g, const RaysDev& r, int K, int J,
+                                                bool use_valid, RayReg* ray) {
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    int x, y, leaf = 0;
+    int rl = threadIdx.x + i * kThreads;
+    ray[i].exists = slot_xy(g, K, J, rl, x, y, leaf);
+    ray[i].leaf = leaf;
+    ray[i].valid = false;
+    if (ray[i].exists) {
+      size_t idx = (size_t)y * g.W + x;
+      ray[i].idx = idx;
+      ray[i].valid = !use_valid || r.valid == nullptr || r.valid[idx] != 0;
+      ray[i].o = {r.ox[idx], r.oy[idx], r.oz[idx]};
+      ray[i].d = {r.dx[idx], r.dy[idx], r.dz[idx]};
+      ray[i].tmin = r.tmin[idx];
+      ray[i].tmax = r.tmax[idx];
+    }
+  }
+}
+
+__device__ __forceinline__ void stage_tri(float4 (*sT)[3], int slot, const Tri& T) {
+  sT[slot][0] = make_float4(T.v1.x, T.v1.y, T.v1.z, T.e1.x);
+  sT[slot][1] = make_float4(T.e1.y, T.e1.z, T.e2.x, T.e2.y);
+  sT[slot][2] = make_float4(T.e2.z, 0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ Tri staged_tri(const float4 (*sT)[3], int slot) {
+  float4 a = sT[slot][0], b = sT[slot][1], c = sT[slot][2];
+  Tri T;
+  T.v1 = {a.x, a.y, a.z};
+  T.e1 = {a.w, b.x, b.y};
+  T.e2 = {b.z, b.w, c.x};
+  T.v2 = T.v3 = T.v1;  // unused by the leaf test
+  return T;
+}
+
+// Leaf test of one ray against one candidate triangle (intersectAllLeaves.cl:12-186).
+template <int MODE>
+__device__ __forceinline__ void ray_test(const RayReg& ry, const Tri& T, uint32_t tid,
+                                         uint64_t& best, uint32_t& rkey, bool& occ) {
+  if (MODE != 0 && !ry.valid) return;
+  if (MODE == 2 && occ) return;
+  float t, b1, b2;
+  if (!mt_strict(ry.o, ry.d, T, t, b1, b2)) return;
+  if (MODE == 2) {
+    if (t > ry.tmin && t < ry.tmax) occ = true;
+    return;
+  }
+  if (t > ry.tmin && t < __int_as_float(0x7f800000)) {
+    uint64_t k = hit_key(t, (int32_t)tid);
+    if (k < best) best = k;
+  }
+  if (MODE == 0 && repair_region(b1, b2) && t >= ry.tmin) {
+    uint32_t k = fkey(t);
+    if (k < rkey) rkey = k;
+  }
+}
+
+template <int MODE, int RPT>
+__device__ __forceinline__ void store_results(const RayReg* ray, const uint64_t* best,
+                                              const uint32_t* rkey, const bool* occ,
+                                              uint64_t* __restrict__ keys,
+                                              uint32_t* __restrict__ repkeys,
+                                              uint8_t* __restrict__ occ_out, int tchunks) {
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    if (!ray[i].exists) continue;
+    size_t idx = ray[i].idx;
+    if (MODE == 2) {
+      if (tchunks == 1) occ_out[idx] = occ[i] ? 1 : 0;
+      else if (occ[i]) occ_out[idx] = 1;
+    } else {
+      if (tchunks == 1) keys[idx] = best[i];
+      else if (best[i] != kMissKey) atomicMin((unsigned long long*)&keys[idx], (unsigned long long)best[i]);
+      if (MODE == 0 && repkeys) {
+        if (tchunks == 1) repkeys[idx] = rkey[i];
+        else if (rkey[i] != kNoKey32) atomicMin(&repkeys[idx], rkey[i]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Hierarchy x triangle traversal (intersection6DB.cl, intersection6DB2.cl, intersectionP6DB.cl,
+// intersectionIA.cl).  grid.x = blocks, grid.y = triangle chunks.
+template <int MODE, int NODE, int RPT>
+__global__ __launch_bounds__(256) void rh_trace_kernel(RHGeom g, RaysDev r,
+                                                       const float* __restrict__ v9, uint32_t ntri,
+                                                       uint32_t chunk,
+                                                       const float4* __restrict__ nodes,
+                                                       uint64_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ repkeys,
+                                                       uint8_t* __restrict__ occ_out, int tchunks) {
+  __shared__ Node sN[kMaxSub];
+  __shared__ int sV[kMaxSub];
+  __shared__ float4 sT[kThreads][3];
+  __shared__ uint64_t sM[kThreads];
+  __shared__ uint32_t sI[kThreads];
+  __shared__ uint32_t sCnt[2];
+
+  const int tid = threadIdx.x;
+  const int top = g.L - 1;
+  const int K = blockIdx.x % g.gw[top], J = blockIdx.x / g.gw[top];
+  load_subtree(g, K, J, nodes, sN, sV);
+  if (tid < 2) sCnt[tid] = 0;
+  __syncthreads();
+
+  RayReg ray[RPT];
+  load_block_rays<RPT>(g, r, K, J, MODE != 0, ray);
+  uint64_t best[RPT];
+  uint32_t rkey[RPT];
+  bool occ[RPT];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    best[i] = kMissKey;
+    rkey[i] = kNoKey32;
+    occ[i] = false;
+  }
+
+  if (sV[0]) {
+    const uint32_t t0 = blockIdx.y * chunk;
+    const uint32_t t1 = min(ntri, t0 + chunk);
+    const int lane = __lane_id();
+    const int wave = tid >> 6;
+    (void)wave;
+    int it = 0;
+    for (uint32_t base = t0; base < t1; base += kThreads, ++it) {
+      const uint32_t ti = base + tid;
+      uint64_t mask = 0;
+      if (ti < t1) {
+        Tri T = load_tri(v9, ti);
+        stage_tri(sT, tid, T);
+        mask = leaf_mask<MODE, NODE>(T, sN, sV, g.L);
+      }
+      const uint64_t bal = __ballot(mask != 0);
+      const int wcount = __popcll(bal);
+      const int prefix = __popcll(bal & ((1ull << lane) - 1ull));
+      uint32_t wbase = 0;
+      if (lane == 0 && wcount) wbase = atomicAdd(&sCnt[it & 1], (uint32_t)wcount);
+      wbase = __shfl(wbase, 0);
+      if (mask) {
+        sM[wbase + prefix] = mask;
+        sI[wbase + prefix] = (uint32_t)tid;
+      }
+      __syncthreads();
+      const int n = (int)sCnt[it & 1];
+      if (tid == 0) sCnt[(it + 1) & 1] = 0;
+      for (int e = 0; e < n; ++e) {
+        const uint64_t m = sM[e];
+        const uint32_t slot = sI[e];
+        bool need = false;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) need |= ray[i].exists && ((m >> ray[i].leaf) & 1ull);
+        if (!need) continue;
+        const Tri T = staged_tri(sT, (int)slot);
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+          if (ray[i].exists && ((m >> ray[i].leaf) & 1ull))
+            ray_test<MODE>(ray[i], T, base + slot, best[i], rkey[i], occ[i]);
+      }
+      if (MODE == 2) {
+        bool done = true;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) done &= !ray[i].exists || !ray[i].valid || occ[i];
+        if (__syncthreads_and(done)) break;
+      } else {
+        __syncthreads();
+      }
+    }
+  }
+  store_results<MODE, RPT>(ray, best, rkey, occ, keys, repkeys, occ_out, tchunks);
+}
+
+// ---------------------------------------------------------------------------------------
+// Hierarchy x BVH product traversal of ray-bvh (intersection6DB_BVH.cl, intersection6DB_BVH2.cl,
+// intersectionP6DB_BVH.cl).  Each thread walks one frontier root's BVH subtree in lockstep
+// with the block's hierarchy (both descend while interior) and emits (leaf mask, triangle)
+// entries for every BVH leaf it reaches.
+__device__ __forceinline__ void bvh_node(const float4* __restrict__ bvh, uint32_t n, V3& bmin,
+                                         V3& bmax, uint32_t& off, uint32_t& np) {
+  float4 a = bvh[2 * (size_t)n], b = bvh[2 * (size_t)n + 1];
+  bmin = {a.x, a.y, a.z};
+  bmax = {a.w, b.x, b.y};
+  off = __float_as_uint(b.z);
+  np = __float_as_uint(b.w);
+}
+
+__device__ __forceinline__ uint64_t children_mask(uint64_t A, int s) {
+  // nodes of sub-level s (side 1<<s) -> their children at sub-level s+1 (side 2<<s)
+  const int pnl = 1 << s, nl = pnl * 2;
+  uint64_t C = 0;
+  while (A) {
+    int b = __ffsll((unsigned long long)A) - 1;
+    A &= A - 1;
+    int dk = b & (pnl - 1), dj = b >> s;
+    for (int a = 0; a < 4; ++a) C |= 1ull << ((2 * dj + (a >> 1)) * nl + 2 * dk + (a & 1));
+  }
+  return C;
+}
+
+template <int MODE, int NODE, int RPT>
+__global__ __launch_bounds__(256) void bvh_trace_kernel(
+    RHGeom g, RaysDev r, const float* __restrict__ v9, const float4* __restrict__ bvh,
+    const uint32_t* __restrict__ roots, uint32_t nroots, const uint32_t* __restrict__ prim_order,
+    uint32_t chunk, int tile_roots, const float4* __restrict__ nodes,
+    uint64_t* __restrict__ keys, uint8_t* __restrict__ occ_out, int tchunks) {
+  __shared__ Node sN[kMaxSub];
+  __shared__ int sV[kMaxSub];
+  __shared__ uint64_t sM[kBvhCap];
+  __shared__ uint32_t sI[kBvhCap];
+  __shared__ uint32_t sCnt[2];
+
+  const int tid = threadIdx.x;
+  const int top = g.L - 1;
+  const int K = blockIdx.x % g.gw[top], J = blockIdx.x / g.gw[top];
+  load_subtree(g, K, J, nodes, sN, sV);
+  if (tid < 2) sCnt[tid] = 0;
+  __syncthreads();
+
+  RayReg ray[RPT];
+  load_block_rays<RPT>(g, r, K, J, MODE != 0, ray);
+  uint64_t best[RPT];
+  uint32_t rkey[RPT];
+  bool occ[RPT];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    best[i] = kMissKey;
+    rkey[i] = kNoKey32;
+    occ[i] = false;
+  }
+
+  if (sV[0]) {
+    const uint32_t r0 = blockIdx.y * chunk;
+    const uint32_t r1 = min(nroots, r0 + chunk);
+    int it = 0;
+    for (uint32_t base = r0; base < r1; base += (uint32_t)tile_roots, ++it) {
+      const uint32_t ri = base + tid;
+      if (tid < tile_roots && ri < r1) {
+        uint32_t stN[kStack];
+        int stS[kStack];
+        uint64_t stM[kStack];
+        int sp = 0;
+        stN[0] = roots[ri];
+        stS[0] = 0;
+        stM[0] = 1;
+        sp = 1;
+        while (sp > 0) {
+          --sp;
+          const uint32_t n = stN[sp];
+          const int s = stS[sp];
+          const uint64_t P = stM[sp];
+          V3 bmin, bmax;
+          uint32_t off, np;
+          bvh_node(bvh, n, bmin, bmax, off, np);
+          const int soff = pow4_off(s);
+          uint64_t A = 0, Pw = P;
+          while (Pw) {
+            int b = __ffsll((unsigned long long)Pw) - 1;
+            Pw &= Pw - 1;
+            const Node& nd = sN[soff + b];
+            if (!sV[soff + b]) continue;
+            bool pass = NODE == 1 ? cull_ia(nd, bmin, bmax) : cull_node6(nd, bmin, bmax);
+            if (pass) A |= 1ull << b;
+          }
+          if (!A) continue;
+          const bool rleaf = s == top, bleaf = np > 0;
+          if (rleaf && bleaf) {
+            const uint32_t cnt = MODE == 1 ? 1u : np;
+            for (uint32_t f = 0; f < cnt; ++f) {
+              uint32_t slot = atomicAdd(&sCnt[it & 1], 1u);
+              if (slot < (uint32_t)kBvhCap) {
+                sM[slot] = A;
+                sI[slot] = prim_order[off + f];
+              }
+            }
+          } else if (rleaf) {
+            if (sp + 2 <= kStack) {
+              stN[sp] = off + 1; stS[sp] = s; stM[sp] = A; ++sp;
+              stN[sp] = off; stS[sp] = s; stM[sp] = A; ++sp;
+            }
+          } else {
+            const uint64_t C = children_mask(A, s);
+            if (bleaf) {
+              if (sp + 1 <= kStack) {
+                stN[sp] = n; stS[sp] = s + 1; stM[sp] = C; ++sp;
+              }
+            } else if (sp + 2 <= kStack) {
+              stN[sp] = off + 1; stS[sp] = s + 1; stM[sp] = C; ++sp;
+              stN[sp] = off; stS[sp] = s + 1; stM[sp] = C; ++sp;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      const int n = min((int)sCnt[it & 1], kBvhCap);
+      if (tid == 0) sCnt[(it + 1) & 1] = 0;
+      for (int e = 0; e < n; ++e) {
+        const uint64_t m = sM[e];
+        bool need = false;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) need |= ray[i].exists && ((m >> ray[i].leaf) & 1ull);
+        if (!need) continue;
+        const uint32_t tri = (uint32_t)__builtin_amdgcn_readfirstlane((int)sI[e]);
+        const Tri T = load_tri(v9, tri);
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+          if (ray[i].exists && ((m >> ray[i].leaf) & 1ull))
+            ray_test<MODE == 0 ? 1 : MODE>(ray[i], T, tri, best[i], rkey[i], occ[i]);
+      }
+      if (MODE == 2) {
+        bool done = true;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) done &= !ray[i].exists || !ray[i].valid || occ[i];
+        if (__syncthreads_and(done)) break;
+      } else {
+        __syncthreads();
+      }
+    }
+  }
+  store_results<MODE == 0 ? 1 : MODE, RPT>(ray, best, rkey, occ, keys, nullptr, occ_out, tchunks);
+}
+
+// ---------------------------------------------------------------------------------------
+// Naive brute force (naive.cl "TriRay" semantics, ray-parallel): every ray of the block tests
+// every triangle of its chunk, triangles staged through LDS.
+template <int MODE>
+__global__ __launch_bounds__(256) void naive_kernel(RaysDev r, const float* __restrict__ v9,
+                                                    uint32_t ntri, uint32_t chunk,
+                                                    uint64_t* __restrict__ keys,
+                                                    uint8_t* __restrict__ occ_out, int tchunks) {
+  __shared__ float4 sT[kThreads][3];
+  const int tid = threadIdx.x;
+  const size_t n = (size_t)r.W * r.H;
+  const size_t i = (size_t)blockIdx.x * kThreads + tid;
+  const bool exists = i < n;
+  const bool valid = exists && (MODE == 0 || r.valid == nullptr || r.valid[i] != 0);
+  V3 o{}, d{};
+  float tmin = 0.f, tmax = 0.f;
+  if (valid) {
+    o = {r.ox[i], r.oy[i], r.oz[i]};
+    d = {r.dx[i], r.dy[i], r.dz[i]};
+    tmin = r.tmin[i];
+    tmax = r.tmax[i];
+  }
+  uint64_t best = kMissKey;
+  bool occ = false;
+  const uint32_t t0 = blockIdx.y * chunk;
+  const uint32_t t1 = min(ntri, t0 + chunk);
+  for (uint32_t base = t0; base < t1; base += kThreads) {
+    const uint32_t ti = base + tid;
+    if (ti < t1) stage_tri(sT, tid, load_tri(v9, ti));
+    __syncthreads();
+    const int cnt = (int)min((uint32_t)kThreads, t1 - base);
+    if (valid && !(MODE == 2 && occ)) {
+      for (int k = 0; k < cnt; ++k) {
+        const Tri T = staged_tri(sT, k);
+        float t, b1, b2;
+        if (MODE == 2) {
+          if (!mt_naive_p(o, d, T, t)) continue;
+          if (t < tmin || t > tmax) continue;
+          occ = true;
+          break;
+        } else {
+          if (!mt_naive(o, d, T, t, b1, b2)) continue;
+          if (t < tmin) continue;
+          uint64_t kk = hit_key(t, (int32_t)(base + k));
+          if (kk < best) best = kk;
+        }
+      }
+    }
+    if (MODE == 2) {
+      if (__syncthreads_and(!valid || occ)) break;
+    } else {
+      __syncthreads();
+    }
+  }
+  if (!exists) return;
+  if (MODE == 2) {
+    if (tchunks == 1) occ_out[i] = occ ? 1 : 0;
+    else if (occ) occ_out[i] = 1;
+  } else {
+    if (tchunks == 1) keys[i] = best;
+    else if (best != kMissKey) atomicMin((unsigned long long*)&keys[i], (unsigned long long)best);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Hit record: id, t, (u, v) and optionally dpdu/dpdv of computeDpTuTv.cl:8-91.
+__device__ void write_hit(const RaysDev& r, const float* __restrict__ v9,
+                          const float* __restrict__ uv6, size_t i, int32_t id, float t,
+                          HitOut out) {
+  if (id < 0) {
+    out.id[i] = -1;
+    out.t[i] = __int_as_float(0x7f800000);
+    out.u[i] = 0.f;
+    out.v[i] = 0.f;
+    if (out.dpdu)
+      for (int c = 0; c < 3; ++c) out.dpdu[3 * i + c] = 0.f;
+    if (out.dpdv)
+      for (int c = 0; c < 3; ++c) out.dpdv[3 * i + c] = 0.f;
+    return;
+  }
+  const Tri T = load_tri(v9, (uint32_t)id);
+  const V3 o = {r.ox[i], r.oy[i], r.oz[i]};
+  const V3 d = {r.dx[i], r.dy[i], r.dz[i]};
+  float b1, b2;
+  bary(o, d, T, b1, b2);
+  float uv[6] = {0.f, 0.f, 1.f, 0.f, 1.f, 1.f};
+  if (uv6)
+    for (int k = 0; k < 6; ++k) uv[k] = uv6[6 * (size_t)id + k];
+  const float b0 = 1 - b1 - b2;
+  out.id[i] = id;
+  out.t[i] = t;
+  out.u[i] = (b0 * uv[0] + b1 * uv[2]) + b2 * uv[4];
+  out.v[i] = (b0 * uv[1] + b1 * uv[3]) + b2 * uv[5];
+  if (out.dpdu || out.dpdv) {
+    const float du1 = uv[0] - uv[4], du2 = uv[2] - uv[4];
+    const float dv1 = uv[1] - uv[5], dv2 = uv[3] - uv[5];
+    const V3 dp1 = sub(T.v1, T.v3), dp2 = sub(T.v2, T.v3);
+    const float det = du1 * dv2 - dv1 * du2;
+    V3 pu, pv;
+    if (det == 0.f) {
+      const V3 tn = normalize(cross(T.e2, T.e1));
+      if (fabsf(tn.x) > fabsf(tn.y)) {
+        const float il = 1.0f / sqrtf(tn.x * tn.x + tn.z * tn.z);
+        pu = {-tn.z * il, 0.f, tn.x * il};
+      } else {
+        const float il = 1.0f / sqrtf(tn.y * tn.y + tn.z * tn.z);
+        pu = {0.f, tn.z * il, -tn.y * il};
+      }
+      pv = cross(tn, pu);
+    } else {
+      const float id2 = 1.f / det;
+      pu = {(dv2 * dp1.x - dv1 * dp2.x) * id2, (dv2 * dp1.y - dv1 * dp2.y) * id2,
+            (dv2 * dp1.z - dv1 * dp2.z) * id2};
+      pv = {(-du2 * dp1.x + du1 * dp2.x) * id2, (-du2 * dp1.y + du1 * dp2.y) * id2,
+            (-du2 * dp1.z + du1 * dp2.z) * id2};
+    }
+    if (out.dpdu) {
+      out.dpdu[3 * i] = pu.x; out.dpdu[3 * i + 1] = pu.y; out.dpdu[3 * i + 2] = pu.z;
+    }
+    if (out.dpdv) {
+      out.dpdv[3 * i] = pv.x; out.dpdv[3 * i + 1] = pv.y; out.dpdv[3 * i + 2] = pv.z;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void finalize_kernel(RaysDev r, const float* __restrict__ v9,
+                                                       const float* __restrict__ uv6,
+                                                       const uint64_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ repkeys,
+                                                       uint32_t* __restrict__ replist,
+                                                       HitOut out) {
+  const size_t n = (size_t)r.W * r.H;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t key = keys[i];
+  float tstar = __int_as_float(0x7f800000);
+  if (key == kMissKey) {
+    write_hit(r, v9, uv6, i, -1, 0.f, out);
+  } else {
+    tstar = fkey_inv((uint32_t)(key >> 32));
+    write_hit(r, v9, uv6, i, (int32_t)(uint32_t)key, tstar, out);
+  }
+  if (repkeys) {
+    const uint32_t rk = repkeys[i];
+    if (rk != kNoKey32) {
+      const float tr = fkey_inv(rk);
+      if (!(tr > tstar - kEps)) {
+        const uint32_t slot = atomicAdd(&replist[0], 1u);
+        replist[1 + slot] = (uint32_t)i;
+      }
+    }
+  }
+}
+
+// Exact sequential repair pass (yetAnotherIntersection6DB.cl with repairRun passes, in
+// increasing triangle id order) for the rays whose outcome it can change: the nearest
+// repair-region candidate lies at or below tHit - EPS (hits at exactly tmin, or ties at t
+// large enough that t - EPS rounds to t).  One work-group per flagged ray.
+__global__ __launch_bounds__(256) void repair_kernel(RHGeom g, RaysDev r, int node_kind,
+                                                     const float* __restrict__ v9,
+                                                     const float* __restrict__ uv6, uint32_t ntri,
+                                                     int repair_run,
+                                                     const float4* __restrict__ nodes,
+                                                     const uint32_t* __restrict__ replist,
+                                                     HitOut out) {
+  __shared__ Node sChain[kMaxLevels];
+  __shared__ int sWc[kThreads / 64];
+  __shared__ float sEt[kThreads];
+  __shared__ uint32_t sEi[kThreads];
+  __shared__ float sTH;
+  __shared__ int sIdx;
+  const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const uint32_t count = replist[0];
+  for (uint32_t w = blockIdx.x; w < count; w += gridDim.x) {
+    const size_t i = replist[1 + w];
+    const int x = (int)(i % (size_t)g.W), y = (int)(i / (size_t)g.W);
+    if (tid < g.L) {
+      const int l = tid;
+      const int gk = x / (g.cx << l), gj = y / (g.cy << l);
+      const float4* p = nodes + 4 * ((size_t)g.off[l] + (size_t)gj * g.gw[l] + gk);
+      float4 a = p[0], b = p[1], c = p[2], d = p[3];
+      sChain[l].dmin = {a.x, a.y, a.z};
+      sChain[l].dmax = {b.x, b.y, b.z};
+      sChain[l].omin = {c.x, c.y, c.z};
+      sChain[l].omax = {d.x, d.y, d.z};
+    }
+    if (tid == 0) {
+      sTH = out.t[i];
+      sIdx = out.id[i];
+    }
+    __syncthreads();
+    const V3 o = {r.ox[i], r.oy[i], r.oz[i]};
+    const V3 d = {r.dx[i], r.dy[i], r.dz[i]};
+    const float tmin = r.tmin[i];
+    for (int run = 0; run < repair_run; ++run) {
+      for (uint32_t base = 0; base < ntri; base += kThreads) {
+        const uint32_t ti = base + tid;
+        bool have = false;
+        float t = 0.f;
+        if (ti < ntri) {
+          const Tri T = load_tri(v9, ti);
+          bool pass = true;
+          if (node_kind == 1) {
+            const V3 bmin = vmin(vmin(T.v1, T.v2), T.v3), bmax = vmax(vmax(T.v1, T.v2), T.v3);
+            for (int l = g.L - 1; l >= 0 && pass; --l) pass = cull_ia(sChain[l], bmin, bmax);
+          } else {
+            for (int l = g.L - 1; l >= 0 && pass; --l) pass = cull_tri(sChain[l], T);
+          }
+          float b1, b2;
+          if (pass && mt_strict(o, d, T, t, b1, b2) && repair_region(b1, b2) && t >= tmin)
+            have = true;
+        }
+        const uint64_t bal = __ballot(have);
+        const int prefix = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) sWc[wave] = __popcll(bal);
+        __syncthreads();
+        int woff = 0, total = 0;
+        for (int k = 0; k < kThreads / 64; ++k) {
+          if (k < wave) woff += sWc[k];
+          total += sWc[k];
+        }
+        if (have) {
+          sEt[woff + prefix] = t;
+          sEi[woff + prefix] = ti;
+        }
+        __syncthreads();
+        if (tid == 0) {
+          for (int e = 0; e < total; ++e) {
+            if (sEt[e] > sTH - kEps || (int)sEi[e] == sIdx) continue;
+            sTH = sEt[e];
+            sIdx = (int)sEi[e];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (tid == 0) write_hit(r, v9, uv6, i, sIdx, sTH, out);
+    __syncthreads();
+  }
+}
+
+// Shadow rays toward a point light (light.h:79-84, point.cpp:42-49).
+__global__ __launch_bounds__(256) void shadow_kernel(RaysDev pr, const int32_t* __restrict__ id,
+                                                     const float* __restrict__ tt, float lx,
+                                                     float ly, float lz, float* ox, float* oy,
+                                                     float* oz, float* dx, float* dy, float* dz,
+                                                     float* tmin, float* tmax, uint8_t* valid) {
+  const size_t n = (size_t)pr.W * pr.H;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (id[i] < 0) {
+    ox[i] = oy[i] = oz[i] = 0.f;
+    dx[i] = dy[i] = dz[i] = 0.f;
+    tmin[i] = tmax[i] = 0.f;
+    valid[i] = 0;
+    return;
+  }
+  const float t = tt[i];
+  const float px = pr.ox[i] + pr.dx[i] * t;
+  const float py = pr.oy[i] + pr.dy[i] * t;
+  const float pz = pr.oz[i] + pr.dz[i] * t;
+  const float vx = lx - px, vy = ly - py, vz = lz - pz;
+  const float dist = sqrtf((vx * vx + vy * vy) + vz * vz);
+  const float inv = 1.f / dist;
+  ox[i] = px;
+  oy[i] = py;
+  oz[i] = pz;
+  dx[i] = vx * inv;
+  dy[i] = vy * inv;
+  dz[i] = vz * inv;
+  tmin[i] = 1e-3f * t;
+  tmax[i] = dist * (1.f - 0.f);
+  valid[i] = 1;
+}
+
+template <class T>
+__global__ void fill_kernel(T* p, T v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+inline dim3 grid1(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
+
+template <int MODE, int NODE>
+hipError_t rh_trace_dispatch(const RHGeom& g, const RaysDev& r, const float* v9, uint32_t ntri,
+                             int tchunks, const float4* nodes, uint64_t* keys, uint32_t* repkeys,
+                             uint8_t* occ, hipStream_t s) {
+  const uint32_t chunk = (uint32_t)((ntri + tchunks - 1) / tchunks);
+  dim3 grid(g.ntop, tchunks);
+  const int rpt = (g.rb + kThreads - 1) / kThreads;
+  if (rpt <= 1)
+    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 1>), grid, dim3(kThreads), 0, s, g, r, v9,
+                       ntri, chunk, nodes, keys, repkeys, occ, tchunks);
+  else if (rpt <= 4)
+    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 4>), grid, dim3(kThreads), 0, s, g, r, v9,
+                       ntri, chunk, nodes, keys, repkeys, occ, tchunks);
+  else
+    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 16>), grid, dim3(kThreads), 0, s, g, r, v9,
+                       ntri, chunk, nodes, keys, repkeys, occ, tchunks);
+  return hipGetLastError();
+}
+
+template <int MODE, int NODE>
+hipError_t bvh_trace_dispatch(const RHGeom& g, const RaysDev& r, const float* v9,
+                              const float4* bvh, const uint32_t* roots, uint32_t nroots,
+                              const uint32_t* prim_order, int tile_roots, int tchunks,
+                              const float4* nodes, uint64_t* keys, uint8_t* occ, hipStream_t s) {
+  const uint32_t chunk = (uint32_t)((nroots + tchunks - 1) / tchunks);
+  dim3 grid(g.ntop, tchunks);
+  const int rpt = (g.rb + kThreads - 1) / kThreads;
+  if (rpt <= 1)
+    hipLaunchKernelGGL((bvh_trace_kernel<MODE, NODE, 1>), grid, dim3(kThreads), 0, s, g, r, v9,
+                       bvh, roots, nroots, prim_order, chunk, tile_roots, nodes, keys, occ, tchunks);
+  else if (rpt <= 4)
+    hipLaunchKernelGGL((bvh_trace_kernel<MODE, NODE, 4>), grid, dim3(kThreads), 0, s, g, r, v9,
+                       bvh, roots, nroots, prim_order, chunk, tile_roots, nodes, keys, occ, tchunks);
+  else
+    hipLaunchKernelGGL((bvh_trace_kernel<MODE, NODE, 16>), grid, dim3(kThreads), 0, s, g, r, v9,
+                       bvh, roots, nroots, prim_order, chunk, tile_roots, nodes, keys, occ, tchunks);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int rh_geometry(int W, int H, int cx, int cy, int height, RHGeom* g) {
+  if (W <= 0 || H <= 0 || cx <= 0 || cy <= 0 || height <= 0) return 1;
+  int gw = (W + cx - 1) / cx, gh = (H + cy - 1) / cy;
+  g->W = W;
+  g->H = H;
+  g->cx = cx;
+  g->cy = cy;
+  g->gw[0] = gw;
+  g->gh[0] = gh;
+  int L = 1;
+  // Stop after a level whose width or height is odd (rayhierarchy.cpp:438-442).
+  for (int i = 1; i < height; ++i) {
+    if (L >= kMaxLevels) return 2;
+    gw = (gw + 1) / 2;
+    gh = (gh + 1) / 2;
+    g->gw[L] = gw;
+    g->gh[L] = gh;
+    ++L;
+    if ((gw & 1) || (gh & 1)) break;
+  }
+  g->L = L;
+  int off = 0;
+  for (int l = 0; l < L; ++l) {
+    g->off[l] = off;
+    off += g->gw[l] * g->gh[l];
+  }
+  g->nnodes = off;
+  g->nlx = 1 << (L - 1);
+  g->bw = cx * g->nlx;
+  g->bh = cy * g->nlx;
+  g->rb = g->bw * g->bh;
+  g->g = cx * cy;
+  g->nsub = ((1 << (2 * L)) - 1) / 3;
+  g->ntop = g->gw[L - 1] * g->gh[L - 1];
+  if (g->rb > kThreads * kMaxRpt) return 3;
+  return 0;
+}
+
+hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel<uint64_t>, grid1(n), dim3(kThreads), 0, s, p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, grid1(n), dim3(kThreads), 0, s, p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel<uint8_t>, grid1(n), dim3(kThreads), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_rh_build(const RHGeom& g, const RaysDev& r, bool use_valid, float4* nodes,
+                           hipStream_t s) {
+  if (use_valid)
+    hipLaunchKernelGGL(rh_build_kernel<true>, dim3(g.ntop), dim3(kThreads), 0, s, g, r, nodes);
+  else
+    hipLaunchKernelGGL(rh_build_kernel<false>, dim3(g.ntop), dim3(kThreads), 0, s, g, r, nodes);
+  return hipGetLastError();
+}
+
+hipError_t launch_rh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
+                           const float* v9, uint32_t ntri, int tchunks, const float4* nodes,
+                           uint64_t* keys, uint32_t* repkeys, uint8_t* occ, hipStream_t s) {
+  if (node_kind == 1) {
+    if (mode == 0) return rh_trace_dispatch<0, 1>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
+    if (mode == 1) return rh_trace_dispatch<1, 1>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
+    return rh_trace_dispatch<2, 1>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
+  }
+  if (mode == 0) return rh_trace_dispatch<0, 0>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
+  if (mode == 1) return rh_trace_dispatch<1, 0>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
+  return rh_trace_dispatch<2, 0>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
+}
+
+hipError_t launch_bvh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
+                            const float* v9, const float4* bvh, const uint32_t* roots,
+                            uint32_t nroots, const uint32_t* prim_order, int tile_roots,
+                            int tchunks, const float4* nodes, uint64_t* keys, uint8_t* occ,
+                            hipStream_t s) {
+#define RT_BVH(M, N)                                                                             \
+  return bvh_trace_dispatch<M, N>(g, r, v9, bvh, roots, nroots, prim_order, tile_roots, tchunks, \
+                                  nodes, keys, occ, s)
+  if (node_kind == 1) {
+    if (mode == 0) RT_BVH(0, 1);
+    if (mode == 1) RT_BVH(1, 1);
+    RT_BVH(2, 1);
+  }
+  if (mode == 0) RT_BVH(0, 0);
+  if (mode == 1) RT_BVH(1, 0);
+  RT_BVH(2, 0);
+#undef RT_BVH
+}
+
+hipError_t launch_naive_trace(const RaysDev& r, int mode, const float* v9, uint32_t ntri,
+                              int tchunks, uint64_t* keys, uint8_t* occ, hipStream_t s) {
+  const size_t n = (size_t)r.W * r.H;
+  const uint32_t chunk = (uint32_t)((ntri + tchunks - 1) / tchunks);
+  dim3 grid((unsigned)((n + kThreads - 1) / kThreads), tchunks);
+  if (mode == 0)
+    hipLaunchKernelGGL(naive_kernel<0>, grid, dim3(kThreads), 0, s, r, v9, ntri, chunk, keys, occ, tchunks);
+  else if (mode == 1)
+    hipLaunchKernelGGL(naive_kernel<1>, grid, dim3(kThreads), 0, s, r, v9, ntri, chunk, keys, occ, tchunks);
+  else
+    hipLaunchKernelGGL(naive_kernel<2>, grid, dim3(kThreads), 0, s, r, v9, ntri, chunk, keys, occ, tchunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const RaysDev& r, const float* v9, const float* uv6,
+                           const uint64_t* keys, const uint32_t* repkeys, uint32_t* replist,
+                           HitOut out, hipStream_t s) {
+  const size_t n = (size_t)r.W * r.H;
+  hipLaunchKernelGGL(finalize_kernel, grid1(n), dim3(kThreads), 0, s, r, v9, uv6, keys, repkeys,
+                     replist, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_repair(const RHGeom& g, const RaysDev& r, int node_kind, const float* v9,
+                         const float* uv6, uint32_t ntri, int repair_run, const float4* nodes,
+                         const uint32_t* replist, HitOut out, hipStream_t s) {
+  hipLaunchKernelGGL(repair_kernel, dim3(64), dim3(kThreads), 0, s, g, r, node_kind, v9, uv6,
+                     ntri, repair_run, nodes, replist, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_rays(const RaysDev& pr, const int32_t* id, const float* t, float lx,
+                              float ly, float lz, float* ox, float* oy, float* oz, float* dx,
+                              float* dy, float* dz, float* tmin, float* tmax, uint8_t* valid,
+                              hipStream_t s) {
+  const size_t n = (size_t)pr.W * pr.H;
+  hipLaunchKernelGGL(shadow_kernel, grid1(n), dim3(kThreads), 0, s, pr, id, t, lx, ly, lz, ox,
+                     oy, oz, dx, dy, dz, tmin, tmax, valid);
+  return hipGetLastError();
+}
+
+}  // namespace rti
