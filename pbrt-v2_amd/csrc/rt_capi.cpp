@@ -331,8 +331,13 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
     if (st) return st;
     RT_TRY(launch_rh_build(g, r, mode != RT_CLOSEST_PRIMARY, s->d_nodes, hs));
     if (kind == RT_ACCEL_RAYHIERARCHY) {
+      // LDS-staged blocks hold at most 1024 rays; queue entries pack the triangle id in 26 bits.
+      if (g.rb > 1024 || ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
       const bool repair = mode == RT_CLOSEST_PRIMARY && s->desc.repair_run > 0;
-      const int tchunks = pick_tchunks(g.ntop, ntri, 256);
+      // Split every block's triangles into chunks of at most 2048 so that blocks whose culls
+      // pass most triangles (wide origin boxes at silhouettes) spread over many work-groups.
+      const int tchunks = std::min(64, std::max(pick_tchunks(g.ntop, ntri, 256),
+                                                (int)((ntri + 2047) / 2048)));
       if (repair) {
         st = grow(s->d_repkeys, s->rep_cap, n);
         if (st) return st;

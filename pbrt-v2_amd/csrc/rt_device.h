@@ -208,6 +208,49 @@ __device__ __forceinline__ bool cull_ia(const Node& n, V3 bmin, V3 bmax) {
   return sx < sy + kEps;
 }
 
+// Exact conservative prefilters.  The normalised direction of every corner (or vertex) has
+// its squared length between the sums of the per-axis minimum and maximum squares, so with
+// correctly rounded sqrt and division (monotone) the normalised components lie between the
+// bounds computed here from the same FP32 values.  box_logic is monotone under growing the
+// direction box, so a reject here implies a reject of the exact test.  A zero-length corner
+// (normalize returns it unchanged) is left to the exact test.
+__device__ __forceinline__ void outer_axis(float a, float b, float inv_lo, float inv_hi, float& lo,
+                                           float& hi) {
+  const float la = a < 0.f ? a * inv_hi : a * inv_lo, ha = a < 0.f ? a * inv_lo : a * inv_hi;
+  const float lb = b < 0.f ? b * inv_hi : b * inv_lo, hb = b < 0.f ? b * inv_lo : b * inv_hi;
+  lo = fmn(la, lb);
+  hi = fmx(ha, hb);
+}
+__device__ __forceinline__ bool outer_box_logic(V3 lo, V3 hi, V3 mn, V3 mx, const Node& n) {
+  const float l2min = (mn.x + mn.y) + mn.z;
+  if (l2min == 0.f) return true;
+  const float l2max = (mx.x + mx.y) + mx.z;
+  const float inv_hi = 1.0f / sqrtf(l2min), inv_lo = 1.0f / sqrtf(l2max);
+  V3 tmin, tmax;
+  outer_axis(lo.x, hi.x, inv_lo, inv_hi, tmin.x, tmax.x);
+  outer_axis(lo.y, hi.y, inv_lo, inv_hi, tmin.y, tmax.y);
+  outer_axis(lo.z, hi.z, inv_lo, inv_hi, tmin.z, tmax.z);
+  return box_logic(tmin, tmax, n);
+}
+// May return true when cull_node6 is false; never false when it is true.
+__device__ __forceinline__ bool prefilter_node6(const Node& n, V3 bmin, V3 bmax) {
+  V3 h = {(n.omax.x - n.omin.x) / 2, (n.omax.y - n.omin.y) / 2, (n.omax.z - n.omin.z) / 2};
+  const V3 b0 = sub(bmin, h), b1 = add(bmax, h);
+  const V3 oc = add(h, n.omin);
+  const V3 lo = sub(b0, oc), hi = sub(b1, oc);
+  const V3 ql = {lo.x * lo.x, lo.y * lo.y, lo.z * lo.z}, qh = {hi.x * hi.x, hi.y * hi.y, hi.z * hi.z};
+  return outer_box_logic(lo, hi, vmin(ql, qh), vmax(ql, qh), n);
+}
+// May return true when cull_tri is false; never false when it is true.
+__device__ __forceinline__ bool prefilter_tri(const Node& n, const Tri& T) {
+  const V3 oc = tri_center(n);
+  const V3 a = sub(T.v1, oc), b = sub(T.v2, oc), c = sub(T.v3, oc);
+  const V3 qa = {a.x * a.x, a.y * a.y, a.z * a.z}, qb = {b.x * b.x, b.y * b.y, b.z * b.z},
+           qc = {c.x * c.x, c.y * c.y, c.z * c.z};
+  const V3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
+  return outer_box_logic(lo, hi, vmin(vmin(qa, qb), qc), vmax(vmax(qa, qb), qc), n);
+}
+
 // Order-preserving 32-bit key of a float (+0 and -0 share the key of +0), and the 64-bit
 // nearest-hit key (t, id): the smallest key is the nearest hit, ties -> smallest id.
 __device__ __forceinline__ uint32_t fkey(float t) {

@@ -195,65 +195,6 @@ __device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J,
   }
 }
 
-// Cull of one hierarchy node against one triangle, dispatched on the reference's kernel
-// variant: IA for node "ia"; intersectsTri for box_dir primary rays; intersectsNode
-// (triangle AABB) for box_dir secondary and shadow rays.  The direction box of the
-// triangle seen from the node's origin centre is reused while the centre is unchanged.
-template <int MODE, int NODE>
-struct Culler {
-  const Tri& T;
-  V3 bmin, bmax;
-  V3 coc, cmin, cmax;
-  bool have = false;
-  __device__ Culler(const Tri& t) : T(t) {
-    bmin = vmin(vmin(t.v1, t.v2), t.v3);
-    bmax = vmax(vmax(t.v1, t.v2), t.v3);
-  }
-  __device__ __forceinline__ bool operator()(const Node& n) {
-    if (NODE == 1) return cull_ia(n, bmin, bmax);
-    if (MODE == 0) {
-      V3 oc = tri_center(n);
-      if (!have || __float_as_uint(oc.x) != __float_as_uint(coc.x) ||
-          __float_as_uint(oc.y) != __float_as_uint(coc.y) ||
-          __float_as_uint(oc.z) != __float_as_uint(coc.z)) {
-        tri_dirbox(oc, T, cmin, cmax);
-        coc = oc;
-        have = true;
-      }
-      return box_logic(cmin, cmax, n);
-    }
-    return cull_node6(n, bmin, bmax);
-  }
-};
-
-// Leaf mask of the block subtree for one triangle: bit (lj*nlx + lk) is set when every node
-// on the path from the block root to leaf (lk, lj) passes the cull (the candidate set of the
-// reference's per-triangle DFS, intersection6DB.cl:51-119).
-template <int MODE, int NODE>
-__device__ __forceinline__ uint64_t leaf_mask(const Tri& T, const Node* sN, const int* sV, int L) {
-  Culler<MODE, NODE> cull(T);
-  if (!sV[0] || !cull(sN[0])) return 0;
-  uint64_t cur = 1;
-  int soff = 0;
-  for (int s = 1; s < L; ++s) {
-    const int pnl = 1 << (s - 1), nl = 1 << s, coff = soff + pnl * pnl;
-    uint64_t next = 0;
-    while (cur) {
-      int b = __ffsll((unsigned long long)cur) - 1;
-      cur &= cur - 1;
-      int dk = b & (pnl - 1), dj = b >> (s - 1);
-      for (int a = 0; a < 4; ++a) {
-        int c = (2 * dj + (a >> 1)) * nl + 2 * dk + (a & 1);
-        if (sV[coff + c] && cull(sN[coff + c])) next |= 1ull << c;
-      }
-    }
-    cur = next;
-    soff = coff;
-    if (!cur) break;
-  }
-  return cur;
-}
-
 struct RayReg {
   V3 o, d;
   float tmin, tmax;
@@ -347,8 +288,101 @@ __device__ __forceinline__ void store_results(const RayReg* ray, const uint64_t*
 
 // ---------------------------------------------------------------------------------------
 // Hierarchy x triangle traversal (intersection6DB.cl, intersection6DB2.cl, intersectionP6DB.cl,
-// intersectionIA.cl).  grid.x = blocks, grid.y = triangle chunks.
-template <int MODE, int NODE, int RPT>
+// intersectionIA.cl), breadth first with compaction at every level.  grid.x = blocks,
+// grid.y = triangle chunks.
+//   phase A  every thread culls one triangle of a 256-triangle tile against the block root;
+//            survivors are appended to an LDS queue with a wavefront ballot (up to 8 tiles);
+//   phase B  survivors are expanded 64 at a time into 256 (triangle, child) items, culled in
+//            parallel and compacted again, level by level down to the leaves;
+//   phase C  (triangle, leaf) pairs are expanded into (pair, ray) items over the leaf's rays,
+//            staged in LDS; the nearest hit is an LDS atomic min of the (t, id) key.
+// The candidate set is the reference's: a (leaf, triangle) pair is tested exactly when every
+// node from the block root to the leaf passes the cull.
+constexpr int kQ0Cap = 2048;     // root survivors held before expansion
+constexpr int kQCap = 256;       // survivors of one expansion round
+constexpr int kExpand = 64;      // parents expanded per round (4 children each)
+constexpr int kFlushTiles = 8;   // tiles accumulated before expanding
+static_assert(kFlushTiles * kThreads <= kQ0Cap, "root queue must hold the flushed tiles");
+
+template <int MODE, int NODE>
+__device__ __forceinline__ bool cull_one(const Node& n, const Tri& T, V3 bmin, V3 bmax) {
+  if (NODE == 1) return cull_ia(n, bmin, bmax);
+  if (MODE == 0) return prefilter_tri(n, T) && cull_tri(n, T);
+  return prefilter_node6(n, bmin, bmax) && cull_node6(n, bmin, bmax);
+}
+
+// Appends `item` for the lanes with `pass` set; one LDS atomic per wavefront.
+__device__ __forceinline__ void ballot_append(uint32_t* q, int* cnt, bool pass, uint32_t item) {
+  const uint64_t bal = __ballot(pass);
+  const int lane = __lane_id();
+  const int prefix = __popcll(bal & ((1ull << lane) - 1ull));
+  int base = 0;
+  if (lane == 0 && bal) base = atomicAdd(cnt, __popcll(bal));
+  base = __shfl(base, 0);
+  if (pass) q[base + prefix] = item;
+}
+
+// Chunk cursors of the expansion levels (kept in registers, identical in every thread).
+struct Cursors {
+  int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+  __device__ int get(int s) const { return s == 0 ? c0 : s == 1 ? c1 : s == 2 ? c2 : c3; }
+  __device__ void set(int s, int v) {
+    if (s == 0) c0 = v;
+    else if (s == 1) c1 = v;
+    else if (s == 2) c2 = v;
+    else c3 = v;
+  }
+};
+
+template <int MODE, int NODE, int RBMAX>
+struct RhShared {
+  Node sN[kMaxSub];
+  int sV[kMaxSub];
+  float4 sRo[RBMAX];  // origin, tmin
+  float4 sRd[RBMAX];  // direction, tmax
+  unsigned long long sKey[MODE == 2 ? 1 : RBMAX];
+  uint32_t sAux[RBMAX];  // repair key (primary) or occlusion flag (shadow)
+  uint8_t sRv[RBMAX];    // ray exists and is traced
+  uint32_t sQ0[kQ0Cap];
+  uint32_t sQ[kMaxLevels][kQCap];
+  int sCnt[kMaxLevels];
+  int sTraced, sOccluded;
+};
+
+// Phase C over `n` (triangle, leaf) pairs.
+template <int MODE>
+__device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restrict__ v9,
+                                          const uint32_t* q, int n, const float4* sRo,
+                                          const float4* sRd, unsigned long long* sKey,
+                                          uint32_t* sAux, const uint8_t* sRv, int* sOccluded) {
+  const int G = g.g;
+  const int total = n * G;
+  for (int item = threadIdx.x; item < total; item += kThreads) {
+    const int e = item / G;
+    const int qi = item - e * G;
+    const uint32_t P = q[e];
+    const uint32_t tri = P >> 6;
+    const int slot = (int)(P & 63u) * G + qi;
+    if (!sRv[slot]) continue;
+    if (MODE == 2 && sAux[slot]) continue;
+    const float4 o4 = sRo[slot], d4 = sRd[slot];
+    const Tri T = load_tri(v9, tri);
+    float t, b1, b2;
+    if (!mt_strict(V3{o4.x, o4.y, o4.z}, V3{d4.x, d4.y, d4.z}, T, t, b1, b2)) continue;
+    if (MODE == 2) {
+      // intersectAllLeaves.cl:177-179
+      if (t > o4.w && t < d4.w && atomicOr(&sAux[slot], 1u) == 0u) atomicAdd(sOccluded, 1);
+    } else {
+      // intersectAllLeaves.cl:57-61
+      if (t > o4.w && t < __int_as_float(0x7f800000))
+        atomicMin(&sKey[slot], (unsigned long long)hit_key(t, (int32_t)tri));
+      if (MODE == 0 && repair_region(b1, b2) && t >= o4.w) atomicMin(&sAux[slot], fkey(t));
+    }
+  }
+  __syncthreads();
+}
+
+template <int MODE, int NODE, int RBMAX>
 __global__ __launch_bounds__(256) void rh_trace_kernel(RHGeom g, RaysDev r,
                                                        const float* __restrict__ v9, uint32_t ntri,
                                                        uint32_t chunk,
@@ -356,84 +390,132 @@ __global__ __launch_bounds__(256) void rh_trace_kernel(RHGeom g, RaysDev r,
                                                        uint64_t* __restrict__ keys,
                                                        uint32_t* __restrict__ repkeys,
                                                        uint8_t* __restrict__ occ_out, int tchunks) {
-  __shared__ Node sN[kMaxSub];
-  __shared__ int sV[kMaxSub];
-  __shared__ float4 sT[kThreads][3];
-  __shared__ uint64_t sM[kThreads];
-  __shared__ uint32_t sI[kThreads];
-  __shared__ uint32_t sCnt[2];
-
+  __shared__ RhShared<MODE, 0, RBMAX> sh;
   const int tid = threadIdx.x;
   const int top = g.L - 1;
   const int K = blockIdx.x % g.gw[top], J = blockIdx.x / g.gw[top];
-  load_subtree(g, K, J, nodes, sN, sV);
-  if (tid < 2) sCnt[tid] = 0;
+  if (tid < kMaxLevels) sh.sCnt[tid] = 0;
+  if (tid == 0) {
+    sh.sTraced = 0;
+    sh.sOccluded = 0;
+  }
+  load_subtree(g, K, J, nodes, sh.sN, sh.sV);
   __syncthreads();
 
-  RayReg ray[RPT];
-  load_block_rays<RPT>(g, r, K, J, MODE != 0, ray);
-  uint64_t best[RPT];
-  uint32_t rkey[RPT];
-  bool occ[RPT];
-#pragma unroll
-  for (int i = 0; i < RPT; ++i) {
-    best[i] = kMissKey;
-    rkey[i] = kNoKey32;
-    occ[i] = false;
+  // Stage the block's rays (leaf-major slots) and count the traced ones.
+  for (int slot = tid; slot < RBMAX; slot += kThreads) {
+    int x = 0, y = 0, leaf = 0;
+    bool traced = false;
+    if (slot_xy(g, K, J, slot, x, y, leaf)) {
+      const size_t idx = (size_t)y * g.W + x;
+      traced = MODE == 0 || r.valid == nullptr || r.valid[idx] != 0;
+      sh.sRo[slot] = make_float4(r.ox[idx], r.oy[idx], r.oz[idx], r.tmin[idx]);
+      sh.sRd[slot] = make_float4(r.dx[idx], r.dy[idx], r.dz[idx], r.tmax[idx]);
+    }
+    sh.sRv[slot] = traced ? 1 : 0;
+    if (MODE != 2) sh.sKey[slot] = kMissKey;
+    sh.sAux[slot] = MODE == 0 ? kNoKey32 : 0u;
+    const uint64_t bal = __ballot(traced);
+    if (__lane_id() == 0 && bal) atomicAdd(&sh.sTraced, __popcll(bal));
   }
+  __syncthreads();
 
-  if (sV[0]) {
+  if (sh.sV[0] && sh.sTraced > 0) {
     const uint32_t t0 = blockIdx.y * chunk;
     const uint32_t t1 = min(ntri, t0 + chunk);
-    const int lane = __lane_id();
-    const int wave = tid >> 6;
-    (void)wave;
-    int it = 0;
-    for (uint32_t base = t0; base < t1; base += kThreads, ++it) {
+    int tiles = 0;
+    for (uint32_t base = t0; base < t1; base += kThreads) {
+      // phase A: root cull
       const uint32_t ti = base + tid;
-      uint64_t mask = 0;
+      bool pass = false;
       if (ti < t1) {
-        Tri T = load_tri(v9, ti);
-        stage_tri(sT, tid, T);
-        mask = leaf_mask<MODE, NODE>(T, sN, sV, g.L);
+        const Tri T = load_tri(v9, ti);
+        pass = cull_one<MODE, NODE>(sh.sN[0], T, vmin(vmin(T.v1, T.v2), T.v3),
+                                    vmax(vmax(T.v1, T.v2), T.v3));
       }
-      const uint64_t bal = __ballot(mask != 0);
-      const int wcount = __popcll(bal);
-      const int prefix = __popcll(bal & ((1ull << lane) - 1ull));
-      uint32_t wbase = 0;
-      if (lane == 0 && wcount) wbase = atomicAdd(&sCnt[it & 1], (uint32_t)wcount);
-      wbase = __shfl(wbase, 0);
-      if (mask) {
-        sM[wbase + prefix] = mask;
-        sI[wbase + prefix] = (uint32_t)tid;
-      }
+      ballot_append(sh.sQ0, &sh.sCnt[0], pass, ti << 6);
+      ++tiles;
       __syncthreads();
-      const int n = (int)sCnt[it & 1];
-      if (tid == 0) sCnt[(it + 1) & 1] = 0;
-      for (int e = 0; e < n; ++e) {
-        const uint64_t m = sM[e];
-        const uint32_t slot = sI[e];
-        bool need = false;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) need |= ray[i].exists && ((m >> ray[i].leaf) & 1ull);
-        if (!need) continue;
-        const Tri T = staged_tri(sT, (int)slot);
-#pragma unroll
-        for (int i = 0; i < RPT; ++i)
-          if (ray[i].exists && ((m >> ray[i].leaf) & 1ull))
-            ray_test<MODE>(ray[i], T, base + slot, best[i], rkey[i], occ[i]);
-      }
-      if (MODE == 2) {
-        bool done = true;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) done &= !ray[i].exists || !ray[i].valid || occ[i];
-        if (__syncthreads_and(done)) break;
+      // The decision uses only values every thread holds (no LDS read racing the next
+      // tile's appends); kFlushTiles tiles of 256 fit the root queue.
+      const bool last = base + kThreads >= t1;
+      if (!(last || tiles >= kFlushTiles)) continue;
+      tiles = 0;
+      // phase B: expand level by level, depth first over chunks of 64 parents
+      if (g.L == 1) {
+        leaf_rays<MODE>(g, v9, sh.sQ0, sh.sCnt[0], sh.sRo, sh.sRd, sh.sKey, sh.sAux, sh.sRv,
+                        &sh.sOccluded);
       } else {
-        __syncthreads();
+        Cursors cur;
+        int s = 0;
+        while (true) {
+          const int cnt = sh.sCnt[s];
+          if (s == top) {
+            leaf_rays<MODE>(g, v9, sh.sQ[s], cnt, sh.sRo, sh.sRd, sh.sKey, sh.sAux, sh.sRv,
+                            &sh.sOccluded);
+            --s;
+            continue;
+          }
+          const int c = cur.get(s);
+          if (c >= cnt) {
+            cur.set(s, 0);
+            if (s == 0) break;
+            --s;
+            continue;
+          }
+          cur.set(s, c + kExpand);
+          const int np = min(kExpand, cnt - c);
+          const uint32_t* src = s == 0 ? sh.sQ0 : sh.sQ[s];
+          __syncthreads();
+          if (tid == 0) sh.sCnt[s + 1] = 0;
+          __syncthreads();
+          bool ok = false;
+          uint32_t item = 0;
+          if (tid < np * 4) {
+            const uint32_t P = src[c + (tid >> 2)];
+            const uint32_t tri = P >> 6;
+            const int pl = (int)(P & 63u), a = tid & 3;
+            const int pnl = 1 << s;
+            const int dk = pl & (pnl - 1), dj = pl >> s;
+            const int ch = (2 * dj + (a >> 1)) * (2 * pnl) + 2 * dk + (a & 1);
+            const int ni = pow4_off(s + 1) + ch;
+            item = (tri << 6) | (uint32_t)ch;
+            if (sh.sV[ni]) {
+              const Tri T = load_tri(v9, tri);
+              ok = cull_one<MODE, NODE>(sh.sN[ni], T, vmin(vmin(T.v1, T.v2), T.v3),
+                                        vmax(vmax(T.v1, T.v2), T.v3));
+            }
+          }
+          ballot_append(sh.sQ[s + 1], &sh.sCnt[s + 1], ok, item);
+          __syncthreads();
+          ++s;
+        }
+      }
+      if (tid == 0) sh.sCnt[0] = 0;
+      __syncthreads();
+      if (MODE == 2 && sh.sOccluded >= sh.sTraced) break;
+    }
+  }
+
+  for (int slot = tid; slot < g.rb; slot += kThreads) {
+    int x, y, leaf;
+    if (!slot_xy(g, K, J, slot, x, y, leaf)) continue;
+    const size_t idx = (size_t)y * g.W + x;
+    if (MODE == 2) {
+      const bool o = sh.sAux[slot] != 0;
+      if (tchunks == 1) occ_out[idx] = o ? 1 : 0;
+      else if (o) occ_out[idx] = 1;
+    } else {
+      const unsigned long long k = sh.sKey[slot];
+      if (tchunks == 1) keys[idx] = k;
+      else if (k != kMissKey) atomicMin((unsigned long long*)&keys[idx], k);
+      if (MODE == 0 && repkeys) {
+        const uint32_t rk = sh.sAux[slot];
+        if (tchunks == 1) repkeys[idx] = rk;
+        else if (rk != kNoKey32) atomicMin(&repkeys[idx], rk);
       }
     }
   }
-  store_results<MODE, RPT>(ray, best, rkey, occ, keys, repkeys, occ_out, tchunks);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -874,16 +956,14 @@ hipError_t rh_trace_dispatch(const RHGeom& g, const RaysDev& r, const float* v9,
                              uint8_t* occ, hipStream_t s) {
   const uint32_t chunk = (uint32_t)((ntri + tchunks - 1) / tchunks);
   dim3 grid(g.ntop, tchunks);
-  const int rpt = (g.rb + kThreads - 1) / kThreads;
-  if (rpt <= 1)
-    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 1>), grid, dim3(kThreads), 0, s, g, r, v9,
+  if (g.rb <= 256)
+    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 256>), grid, dim3(kThreads), 0, s, g, r, v9,
                        ntri, chunk, nodes, keys, repkeys, occ, tchunks);
-  else if (rpt <= 4)
-    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 4>), grid, dim3(kThreads), 0, s, g, r, v9,
+  else if (g.rb <= 1024)
+    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 1024>), grid, dim3(kThreads), 0, s, g, r, v9,
                        ntri, chunk, nodes, keys, repkeys, occ, tchunks);
   else
-    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 16>), grid, dim3(kThreads), 0, s, g, r, v9,
-                       ntri, chunk, nodes, keys, repkeys, occ, tchunks);
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
