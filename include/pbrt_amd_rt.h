@@ -137,6 +137,13 @@ int rt_make_shadow_rays(const rt_rays *primary, const int32_t *id, const float *
                         float *dy, float *dz, float *tmin, float *tmax, uint8_t *valid,
                         void *stream);
 
+/* Launch-shape parameters (set by the autotuner; 0 restores the built-in choice).
+ *   RT_PARAM_CLUSTERS_CLOSEST / RT_PARAM_CLUSTERS_OCCLUDED: passing triangle clusters (64
+ *   triangles each) per rayhierarchy work unit for closest-hit / occlusion traces; smaller
+ *   values split a block's work over more units of the persistent traversal kernel. */
+enum { RT_PARAM_CLUSTERS_CLOSEST = 1, RT_PARAM_CLUSTERS_OCCLUDED = 2 };
+int rt_scene_set_param(rt_scene *scene, int32_t param, int32_t value);
+
 /* HIP-event timing of the traversal kernels.  rt_scene_set_timing(scene, 1) starts recording
  * one event pair around every traversal launch (and clears earlier records);
  * rt_scene_kernel_time sums the recorded launches of one mode (RT_CLOSEST_PRIMARY,

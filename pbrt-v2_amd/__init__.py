@@ -32,8 +32,9 @@ EXPORTS = (
     "rt_accel_desc_default", "rt_ctx_create", "rt_ctx_destroy", "rt_scene_create",
     "rt_scene_set_bvh", "rt_scene_destroy", "rt_trace_closest", "rt_trace_occluded",
     "rt_make_shadow_rays", "rt_scene_set_timing", "rt_scene_kernel_time",
-    "rt_error_string", "rt_raybvh_build_handoff", "rt_raybvh_free_handoff",
+    "rt_error_string", "rt_raybvh_build_handoff", "rt_raybvh_free_handoff", "rt_scene_set_param",
 )
+RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED = 1, 2
 
 
 class RtError(RuntimeError):
@@ -96,6 +97,7 @@ def lib() -> ctypes.CDLL:
     L.rt_trace_occluded.argtypes = [ctypes.c_void_p, P(RtRays), _fp, _fp]
     L.rt_make_shadow_rays.argtypes = [P(RtRays), _fp, _fp, P(ctypes.c_float)] + [_fp] * 9 + [_fp]
     L.rt_scene_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.rt_scene_set_param.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
     L.rt_scene_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_float),
                                        P(ctypes.c_int32)]
     L.rt_error_string.argtypes = [ctypes.c_int]
@@ -285,6 +287,9 @@ class Scene:
         check(lib().rt_trace_occluded(self.handle, ctypes.byref(rs), out.data_ptr(),
                                       _stream_ptr(stream)), "rt_trace_occluded")
         return out
+
+    def set_param(self, param: int, value: int):
+        check(lib().rt_scene_set_param(self.handle, param, int(value)), "rt_scene_set_param")
 
     def set_timing(self, on: bool):
         check(lib().rt_scene_set_timing(self.handle, 1 if on else 0), "rt_scene_set_timing")

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -26,6 +27,12 @@ struct rt_scene {
   uint32_t ntri = 0;
   float* d_v9 = nullptr;
   float* d_uv6 = nullptr;
+  // traversal copy for rayhierarchy: Morton-sorted triangles in clusters of 64
+  float* d_v9s = nullptr;
+  uint32_t* d_perm = nullptr;
+  float4* d_cbox = nullptr;
+  uint32_t nclusters = 0;
+  int clusters_closest = 0, clusters_occluded = 0;  // 0 = built-in choice
   // ray-bvh handoff
   float4* d_bvh = nullptr;
   uint32_t* d_roots = nullptr;
@@ -42,6 +49,12 @@ struct rt_scene {
   size_t rep_cap = 0;
   uint32_t* d_replist = nullptr;
   size_t replist_cap = 0;
+  uint32_t* d_clist = nullptr;  // passing clusters per block
+  size_t clist_cap = 0;
+  uint4* d_units = nullptr;  // work units {block, first, count}
+  size_t units_cap = 0;
+  uint32_t* d_ucount = nullptr;
+  size_t ucount_cap = 0;
   // HIP-event timing of the traversal kernels (one event pair per launch while enabled)
   struct Timed {
     int mode;
@@ -118,6 +131,70 @@ int desc_ok(const rt_accel_desc* d) {
     if (d->max_bvh_prim <= 0 || d->max_bvh_prim > 4) return RT_ERR_UNSUPPORTED;
   }
   return RT_OK;
+}
+
+// Morton order of triangle centroids (10 bits per axis), ties by triangle id, cut into clusters
+// of kClusterSize with their vertex AABBs.  Only the traversal order changes: hits carry the
+// original ids through perm.
+void sort_clusters(const float* v9, uint32_t ntri, std::vector<float>& v9s,
+                   std::vector<uint32_t>& perm, std::vector<float4>& cbox) {
+  std::vector<float> c(3 * (size_t)ntri);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = 0; i < ntri; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const float* p = v9 + 9 * (size_t)i;
+      const float v = (p[a] + p[3 + a] + p[6 + a]) * (1.0f / 3.0f);
+      c[3 * (size_t)i + a] = v;
+      lo[a] = std::min(lo[a], v);
+      hi[a] = std::max(hi[a], v);
+    }
+  auto spread = [](uint32_t x) {
+    x &= 1023u;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+  };
+  std::vector<uint32_t> code(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) {
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) {
+      const float ext = hi[a] - lo[a];
+      float f = ext > 0.f ? (c[3 * (size_t)i + a] - lo[a]) / ext : 0.f;
+      f = std::min(std::max(f, 0.f), 1.f);
+      q[a] = (uint32_t)(f * 1023.f);
+    }
+    code[i] = (spread(q[0]) << 2) | (spread(q[1]) << 1) | spread(q[2]);
+  }
+  perm.resize(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+  v9s.resize(9 * (size_t)ntri);
+  for (uint32_t i = 0; i < ntri; ++i)
+    std::memcpy(&v9s[9 * (size_t)i], v9 + 9 * (size_t)perm[i], 9 * sizeof(float));
+  const uint32_t nc = (ntri + kClusterSize - 1) / kClusterSize;
+  cbox.assign(2 * (size_t)nc, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (uint32_t k = 0; k < nc; ++k) {
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = k * kClusterSize; i < std::min(ntri, (k + 1) * kClusterSize); ++i)
+      for (int v = 0; v < 3; ++v)
+        for (int a = 0; a < 3; ++a) {
+          const float x = v9s[9 * (size_t)i + 3 * v + a];
+          mn[a] = std::min(mn[a], x);
+          mx[a] = std::max(mx[a], x);
+        }
+    cbox[2 * (size_t)k] = make_float4(mn[0], mn[1], mn[2], 0.f);
+    cbox[2 * (size_t)k + 1] = make_float4(mx[0], mx[1], mx[2], 0.f);
+  }
+}
+
+template <class T>
+hipError_t upload(T*& d, const T* h, size_t n) {
+  hipError_t e = hipMalloc((void**)&d, sizeof(T) * std::max<size_t>(n, 1));
+  if (e == hipSuccess && n) e = hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice);
+  return e;
 }
 
 // Records the start event of a traversal launch; returns the slot or -1.
@@ -199,6 +276,16 @@ int rt_scene_create(rt_ctx* ctx, const float* v9, const float* uv6, uint32_t ntr
         e = hipMemcpy(s->d_uv6, uv6, sizeof(float) * 6 * (size_t)ntri, hipMemcpyHostToDevice);
     }
   }
+  if (e == hipSuccess && desc->kind == RT_ACCEL_RAYHIERARCHY && ntri) {
+    std::vector<float> v9s;
+    std::vector<uint32_t> perm;
+    std::vector<float4> cbox;
+    sort_clusters(v9, ntri, v9s, perm, cbox);
+    e = upload(s->d_v9s, v9s.data(), v9s.size());
+    if (e == hipSuccess) e = upload(s->d_perm, perm.data(), perm.size());
+    if (e == hipSuccess) e = upload(s->d_cbox, cbox.data(), cbox.size());
+    s->nclusters = (uint32_t)(cbox.size() / 2);
+  }
   if (e != hipSuccess) {
     rt_scene_destroy(s);
     return hip_status(e);
@@ -277,6 +364,9 @@ int rt_scene_destroy(rt_scene* s) {
   (void)hipSetDevice(s->ctx->device);
   (void)hipFree(s->d_v9);
   (void)hipFree(s->d_uv6);
+  (void)hipFree(s->d_v9s);
+  (void)hipFree(s->d_perm);
+  (void)hipFree(s->d_cbox);
   (void)hipFree(s->d_bvh);
   (void)hipFree(s->d_roots);
   (void)hipFree(s->d_prim_order);
@@ -284,6 +374,9 @@ int rt_scene_destroy(rt_scene* s) {
   (void)hipFree(s->d_keys);
   (void)hipFree(s->d_repkeys);
   (void)hipFree(s->d_replist);
+  (void)hipFree(s->d_clist);
+  (void)hipFree(s->d_units);
+  (void)hipFree(s->d_ucount);
   for (auto& t : s->timed) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -334,25 +427,35 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
       // LDS-staged blocks hold at most 1024 rays; queue entries pack the triangle id in 26 bits.
       if (g.rb > 1024 || ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
       const bool repair = mode == RT_CLOSEST_PRIMARY && s->desc.repair_run > 0;
-      // Split every block's triangles into chunks of at most 2048 so that blocks whose culls
-      // pass most triangles (wide origin boxes at silhouettes) spread over many work-groups.
-      const int tchunks = std::min(64, std::max(pick_tchunks(g.ntop, ntri, 256),
-                                                (int)((ntri + 2047) / 2048)));
+      // Work units of at most `per` passing clusters (16 for closest hits, 4 for occlusion by
+      // default): blocks whose culls pass many clusters (wide origin boxes at silhouettes)
+      // become many units that the persistent kernel spreads over the whole GPU.
+      const int per_set = closest ? s->clusters_closest : s->clusters_occluded;
+      const int per = per_set > 0 ? per_set : (closest ? 16 : 4);
+      const size_t max_units = (size_t)g.ntop * ((s->nclusters + per - 1) / per);
+      st = grow(s->d_clist, s->clist_cap, (size_t)g.ntop * s->nclusters);
+      if (st) return st;
+      st = grow(s->d_units, s->units_cap, std::max<size_t>(max_units, 1));
+      if (st) return st;
+      st = grow(s->d_ucount, s->ucount_cap, 1);
+      if (st) return st;
+      RT_TRY(launch_fill_u32(s->d_ucount, 0u, 1, hs));
       if (repair) {
         st = grow(s->d_repkeys, s->rep_cap, n);
         if (st) return st;
         st = grow(s->d_replist, s->replist_cap, n + 1);
         if (st) return st;
         RT_TRY(launch_fill_u32(s->d_replist, 0u, 1, hs));
+        RT_TRY(launch_fill_u32(s->d_repkeys, 0xffffffffu, n, hs));
       }
-      if (tchunks > 1) {
-        if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
-        else RT_TRY(launch_fill_u8(occ, 0, n, hs));
-        if (repair) RT_TRY(launch_fill_u32(s->d_repkeys, 0xffffffffu, n, hs));
-      }
+      if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
+      else RT_TRY(launch_fill_u8(occ, 0, n, hs));
       const int slot = timing_begin(s, mode, hs);
-      RT_TRY(launch_rh_trace(g, r, mode, s->desc.node, s->d_v9, ntri, tchunks, s->d_nodes,
-                             s->d_keys, repair ? s->d_repkeys : nullptr, occ, hs));
+      const SortedTris st{s->d_v9s, s->d_perm, s->d_cbox, ntri, s->nclusters};
+      const RhWork w{s->d_clist, s->d_units, s->d_ucount, per,
+                     (int)std::min<size_t>(std::max<size_t>(max_units, 1), 2048)};
+      RT_TRY(launch_rh_trace(g, r, mode, s->desc.node, st, w, s->d_nodes, s->d_keys,
+                             repair ? s->d_repkeys : nullptr, occ, hs));
       timing_end(s, slot, hs);
       if (closest) {
         HitOut out{hits->id, hits->t, hits->u, hits->v, hits->dpdu, hits->dpdv};
@@ -407,6 +510,15 @@ int rt_make_shadow_rays(const rt_rays* primary, const int32_t* id, const float* 
     return RT_ERR_INVALID;
   RT_TRY(launch_shadow_rays(to_dev(primary), id, t, light[0], light[1], light[2], ox, oy, oz, dx,
                             dy, dz, tmin, tmax, valid, (hipStream_t)stream));
+  return RT_OK;
+}
+
+int rt_scene_set_param(rt_scene* s, int32_t param, int32_t value) {
+  if (!s || value < 0) return RT_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (param == RT_PARAM_CLUSTERS_CLOSEST) s->clusters_closest = value;
+  else if (param == RT_PARAM_CLUSTERS_OCCLUDED) s->clusters_occluded = value;
+  else return RT_ERR_INVALID;
   return RT_OK;
 }
 

@@ -251,6 +251,35 @@ __device__ __forceinline__ bool prefilter_tri(const Node& n, const Tri& T) {
   return outer_box_logic(lo, hi, vmin(vmin(qa, qb), qc), vmax(vmax(qa, qb), qc), n);
 }
 
+// Cluster tests: conservative for every triangle whose vertices lie in [cmin, cmax].  A
+// vertex or grown-box corner component lies between the endpoint differences (FP32
+// subtraction is monotone), its square at least that of the endpoint nearest to zero (0 when
+// the interval straddles zero).
+__device__ __forceinline__ void interval_squares(V3 lo, V3 hi, V3& mn, V3& mx) {
+  const V3 ql = {lo.x * lo.x, lo.y * lo.y, lo.z * lo.z}, qh = {hi.x * hi.x, hi.y * hi.y, hi.z * hi.z};
+  mn = vmin(ql, qh);
+  mx = vmax(ql, qh);
+  if (lo.x <= 0.f && hi.x >= 0.f) mn.x = 0.f;
+  if (lo.y <= 0.f && hi.y >= 0.f) mn.y = 0.f;
+  if (lo.z <= 0.f && hi.z >= 0.f) mn.z = 0.f;
+}
+__device__ __forceinline__ bool cluster_tri(const Node& n, V3 cmin, V3 cmax) {
+  const V3 oc = tri_center(n);
+  const V3 lo = sub(cmin, oc), hi = sub(cmax, oc);
+  V3 mn, mx;
+  interval_squares(lo, hi, mn, mx);
+  return outer_box_logic(lo, hi, mn, mx, n);
+}
+__device__ __forceinline__ bool cluster_node6(const Node& n, V3 cmin, V3 cmax) {
+  const V3 h = {(n.omax.x - n.omin.x) / 2, (n.omax.y - n.omin.y) / 2, (n.omax.z - n.omin.z) / 2};
+  const V3 b0 = sub(cmin, h), b1 = add(cmax, h);
+  const V3 oc = add(h, n.omin);
+  const V3 lo = sub(b0, oc), hi = sub(b1, oc);
+  V3 mn, mx;
+  interval_squares(lo, hi, mn, mx);
+  return outer_box_logic(lo, hi, mn, mx, n);
+}
+
 // Order-preserving 32-bit key of a float (+0 and -0 share the key of +0), and the 64-bit
 // nearest-hit key (t, id): the smallest key is the nearest hit, ties -> smallest id.
 __device__ __forceinline__ uint32_t fkey(float t) {

@@ -34,6 +34,27 @@ struct RHGeom {
 // Returns 0 on success, nonzero when the shape is not supported.
 int rh_geometry(int W, int H, int cx, int cy, int height, RHGeom* g);
 
+// Triangles Morton-sorted into clusters of 64 (traversal copy of a scene): v9s[9*slot],
+// perm[slot] = original triangle id, cbox[2*c] / cbox[2*c+1] = cluster AABB min / max.
+struct SortedTris {
+  const float* v9s;
+  const uint32_t* perm;
+  const float4* cbox;
+  uint32_t ntri, nclusters;
+};
+constexpr int kClusterSize = 64;
+
+// Work queue of the rayhierarchy traversal: passing clusters per block (clist, ntop x
+// nclusters), units {block, first, count, 0} of at most per_unit passing clusters, the unit
+// count (zeroed before the cluster pass) and the grid of the persistent unit kernel.
+struct RhWork {
+  uint32_t* clist;
+  uint4* units;
+  uint32_t* unit_count;
+  int per_unit;
+  int grid;
+};
+
 struct HitOut {
   int32_t* id;
   float *t, *u, *v, *dpdu, *dpdv;
@@ -47,7 +68,7 @@ hipError_t launch_rh_build(const RHGeom& g, const RaysDev& r, bool use_valid, fl
                            hipStream_t s);
 // mode 0 closest primary, 1 closest secondary, 2 occluded
 hipError_t launch_rh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
-                           const float* v9, uint32_t ntri, int tchunks, const float4* nodes,
+                           const SortedTris& st, const RhWork& w, const float4* nodes,
                            uint64_t* keys, uint32_t* repkeys, uint8_t* occ, hipStream_t s);
 hipError_t launch_bvh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
                             const float* v9, const float4* bvh, const uint32_t* roots,

@@ -289,26 +289,47 @@ __device__ __forceinline__ void store_results(const RayReg* ray, const uint64_t*
 // ---------------------------------------------------------------------------------------
 // Hierarchy x triangle traversal (intersection6DB.cl, intersection6DB2.cl, intersectionP6DB.cl,
 // intersectionIA.cl), breadth first with compaction at every level.  grid.x = blocks,
-// grid.y = triangle chunks.
-//   phase A  every thread culls one triangle of a 256-triangle tile against the block root;
-//            survivors are appended to an LDS queue with a wavefront ballot (up to 8 tiles);
-//   phase B  survivors are expanded 64 at a time into 256 (triangle, child) items, culled in
-//            parallel and compacted again, level by level down to the leaves;
-//   phase C  (triangle, leaf) pairs are expanded into (pair, ray) items over the leaf's rays,
-//            staged in LDS; the nearest hit is an LDS atomic min of the (t, id) key.
-// The candidate set is the reference's: a (leaf, triangle) pair is tested exactly when every
-// node from the block root to the leaf passes the cull.
-constexpr int kQ0Cap = 2048;     // root survivors held before expansion
+// grid.y = cluster chunks.  Triangles are stored Morton-sorted in clusters of 64 with AABBs.
+//
+// The result of a ray depends only on triangles that pass the whole cull chain (the
+// reference's candidate set) AND pass the leaf test.  The traversal therefore descends with
+// cheap exact-conservative bounds of the culls (never false where the cull is true) and
+// evaluates the exact cull chain only for (triangle, leaf) pairs in which some ray passes the
+// leaf test:
+//   phase A  each thread tests one cluster against the block root (cluster bound); the
+//            triangles of passing clusters get the per-triangle bound; survivors queue in LDS
+//            (ballot compaction);
+//   phase B  queued triangles are expanded 64 at a time into 256 (triangle, child) items,
+//            bounded in parallel and compacted again, level by level down to the leaves;
+//   phase C  (triangle, leaf) pairs are expanded into (pair, ray) leaf tests over the rays
+//            staged in LDS; pairs with a passing test get the exact cull chain; the tests of
+//            verified pairs update the ray (LDS atomic min of the (t, id) key / occlusion flag).
+constexpr int kCluster = 64;     // triangles per cluster
+constexpr int kQ0Cap = 1024;     // root survivors held before expansion
 constexpr int kQCap = 256;       // survivors of one expansion round
 constexpr int kExpand = 64;      // parents expanded per round (4 children each)
-constexpr int kFlushTiles = 8;   // tiles accumulated before expanding
-static_assert(kFlushTiles * kThreads <= kQ0Cap, "root queue must hold the flushed tiles");
 
 template <int MODE, int NODE>
-__device__ __forceinline__ bool cull_one(const Node& n, const Tri& T, V3 bmin, V3 bmax) {
-  if (NODE == 1) return cull_ia(n, bmin, bmax);
-  if (MODE == 0) return prefilter_tri(n, T) && cull_tri(n, T);
-  return prefilter_node6(n, bmin, bmax) && cull_node6(n, bmin, bmax);
+__device__ __forceinline__ bool cluster_bound(const Node& n, V3 cmin, V3 cmax) {
+  if (NODE == 1) return cull_ia(n, cmin, cmax);
+  if (MODE == 0) return cluster_tri(n, cmin, cmax);
+  return cluster_node6(n, cmin, cmax);
+}
+__device__ __forceinline__ V3 tri_bmin(const Tri& T) { return vmin(vmin(T.v1, T.v2), T.v3); }
+__device__ __forceinline__ V3 tri_bmax(const Tri& T) { return vmax(vmax(T.v1, T.v2), T.v3); }
+// Exact-conservative bound of the cull (IA is cheap enough to be its own bound).
+template <int MODE, int NODE>
+__device__ __forceinline__ bool cull_bound(const Node& n, const Tri& T) {
+  if (NODE == 1) return cull_ia(n, tri_bmin(T), tri_bmax(T));
+  if (MODE == 0) return prefilter_tri(n, T);
+  return prefilter_node6(n, tri_bmin(T), tri_bmax(T));
+}
+// The reference's cull.
+template <int MODE, int NODE>
+__device__ __forceinline__ bool cull_exact(const Node& n, const Tri& T) {
+  if (NODE == 1) return cull_ia(n, tri_bmin(T), tri_bmax(T));
+  if (MODE == 0) return cull_tri(n, T);
+  return cull_node6(n, tri_bmin(T), tri_bmax(T));
 }
 
 // Appends `item` for the lanes with `pass` set; one LDS atomic per wavefront.
@@ -334,187 +355,285 @@ struct Cursors {
   }
 };
 
-template <int MODE, int NODE, int RBMAX>
+template <int LMAX>
+struct SubSize {
+  static constexpr int value = ((1 << (2 * LMAX)) - 1) / 3;
+};
+
+template <int MODE, int RBMAX, int LMAX>
 struct RhShared {
-  Node sN[kMaxSub];
-  int sV[kMaxSub];
+  Node sN[SubSize<LMAX>::value];
+  int sV[SubSize<LMAX>::value];
   float4 sRo[RBMAX];  // origin, tmin
   float4 sRd[RBMAX];  // direction, tmax
   unsigned long long sKey[MODE == 2 ? 1 : RBMAX];
   uint32_t sAux[RBMAX];  // repair key (primary) or occlusion flag (shadow)
   uint8_t sRv[RBMAX];    // ray exists and is traced
+  uint8_t sPair[kQ0Cap]; // phase C pair state: 1 = some ray passes, 2 = exact chain verified
   uint32_t sQ0[kQ0Cap];
-  uint32_t sQ[kMaxLevels][kQCap];
+  uint32_t sQ[LMAX > 1 ? LMAX : 2][kQCap];  // level s survivors live in sQ[s], s >= 1
   int sCnt[kMaxLevels];
   int sTraced, sOccluded;
 };
 
-// Phase C over `n` (triangle, leaf) pairs.
-template <int MODE>
-__device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restrict__ v9,
-                                          const uint32_t* q, int n, const float4* sRo,
-                                          const float4* sRd, unsigned long long* sKey,
-                                          uint32_t* sAux, const uint8_t* sRv, int* sOccluded) {
+// The exact cull chain from the block root to leaf `li` (reference candidate test).
+template <int MODE, int NODE, int RBMAX, int LMAX>
+__device__ __forceinline__ bool exact_chain(const RHGeom& g, const RhShared<MODE, RBMAX, LMAX>& sh,
+                                            const Tri& T, int li) {
+  const int top = g.L - 1;
+  const int lx = li & (g.nlx - 1), ly = li >> top;
+  for (int s = 0; s <= top; ++s) {
+    const int sh_ = top - s;
+    const int idx = pow4_off(s) + (ly >> sh_) * (1 << s) + (lx >> sh_);
+    if (!sh.sV[idx] || !cull_exact<MODE, NODE>(sh.sN[idx], T)) return false;
+  }
+  return true;
+}
+
+// Leaf test of ray slot `slot` against T; returns whether it would update the ray.
+template <int MODE, int RBMAX, int LMAX>
+__device__ __forceinline__ bool leaf_hit(const RhShared<MODE, RBMAX, LMAX>& sh, int slot, const Tri& T,
+                                         float& t, float& b1, float& b2) {
+  const float4 o4 = sh.sRo[slot], d4 = sh.sRd[slot];
+  if (!mt_strict(V3{o4.x, o4.y, o4.z}, V3{d4.x, d4.y, d4.z}, T, t, b1, b2)) return false;
+  if (MODE == 2) return t > o4.w && t < d4.w;  // intersectAllLeaves.cl:177-179
+  const bool main = t > o4.w && t < __int_as_float(0x7f800000);  // intersectAllLeaves.cl:57-61
+  return main || (MODE == 0 && repair_region(b1, b2) && t >= o4.w);
+}
+
+// Phase C over `n` (triangle slot, leaf) pairs.
+template <int MODE, int NODE, int RBMAX, int LMAX>
+__device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restrict__ v9s,
+                                          const uint32_t* __restrict__ perm, const uint32_t* q,
+                                          int n, RhShared<MODE, RBMAX, LMAX>& sh) {
   const int G = g.g;
   const int total = n * G;
+  for (int e = threadIdx.x; e < n; e += kThreads) sh.sPair[e] = 0;
+  __syncthreads();
+  // C1: leaf tests; flag pairs with a passing test
   for (int item = threadIdx.x; item < total; item += kThreads) {
     const int e = item / G;
-    const int qi = item - e * G;
     const uint32_t P = q[e];
-    const uint32_t tri = P >> 6;
-    const int slot = (int)(P & 63u) * G + qi;
-    if (!sRv[slot]) continue;
-    if (MODE == 2 && sAux[slot]) continue;
-    const float4 o4 = sRo[slot], d4 = sRd[slot];
-    const Tri T = load_tri(v9, tri);
+    const int slot = (int)(P & 63u) * G + (item - e * G);
+    if (!sh.sRv[slot]) continue;
+    if (MODE == 2 && sh.sAux[slot]) continue;
     float t, b1, b2;
-    if (!mt_strict(V3{o4.x, o4.y, o4.z}, V3{d4.x, d4.y, d4.z}, T, t, b1, b2)) continue;
+    if (leaf_hit<MODE, RBMAX, LMAX>(sh, slot, load_tri(v9s, P >> 6), t, b1, b2)) sh.sPair[e] = 1;
+  }
+  __syncthreads();
+  // C2: exact cull chain of the flagged pairs
+  int verified = 0;
+  for (int e = threadIdx.x; e < n; e += kThreads) {
+    if (sh.sPair[e] != 1) continue;
+    const uint32_t P = q[e];
+    const bool ok = exact_chain<MODE, NODE, RBMAX, LMAX>(g, sh, load_tri(v9s, P >> 6), (int)(P & 63u));
+    sh.sPair[e] = ok ? 2 : 0;
+    verified |= ok;
+  }
+  if (!__syncthreads_or(verified)) return;
+  // C3: apply the leaf tests of verified pairs
+  for (int item = threadIdx.x; item < total; item += kThreads) {
+    const int e = item / G;
+    if (sh.sPair[e] != 2) continue;
+    const uint32_t P = q[e];
+    const uint32_t slot_t = P >> 6;
+    const int slot = (int)(P & 63u) * G + (item - e * G);
+    if (!sh.sRv[slot]) continue;
+    float t, b1, b2;
+    if (!leaf_hit<MODE, RBMAX, LMAX>(sh, slot, load_tri(v9s, slot_t), t, b1, b2)) continue;
+    const float tmin = sh.sRo[slot].w;
     if (MODE == 2) {
-      // intersectAllLeaves.cl:177-179
-      if (t > o4.w && t < d4.w && atomicOr(&sAux[slot], 1u) == 0u) atomicAdd(sOccluded, 1);
+      if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
     } else {
-      // intersectAllLeaves.cl:57-61
-      if (t > o4.w && t < __int_as_float(0x7f800000))
-        atomicMin(&sKey[slot], (unsigned long long)hit_key(t, (int32_t)tri));
-      if (MODE == 0 && repair_region(b1, b2) && t >= o4.w) atomicMin(&sAux[slot], fkey(t));
+      if (t > tmin && t < __int_as_float(0x7f800000))
+        atomicMin(&sh.sKey[slot], (unsigned long long)hit_key(t, (int32_t)perm[slot_t]));
+      if (MODE == 0 && repair_region(b1, b2) && t >= tmin) atomicMin(&sh.sAux[slot], fkey(t));
     }
   }
   __syncthreads();
 }
 
-template <int MODE, int NODE, int RBMAX>
-__global__ __launch_bounds__(256) void rh_trace_kernel(RHGeom g, RaysDev r,
-                                                       const float* __restrict__ v9, uint32_t ntri,
-                                                       uint32_t chunk,
-                                                       const float4* __restrict__ nodes,
-                                                       uint64_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ repkeys,
-                                                       uint8_t* __restrict__ occ_out, int tchunks) {
-  __shared__ RhShared<MODE, 0, RBMAX> sh;
+// Phases B and C for the triangles queued in sQ0; empties the queue.
+template <int MODE, int NODE, int RBMAX, int LMAX>
+__device__ void expand_queue(const RHGeom& g, const float* __restrict__ v9s,
+                             const uint32_t* __restrict__ perm, RhShared<MODE, RBMAX, LMAX>& sh) {
   const int tid = threadIdx.x;
   const int top = g.L - 1;
-  const int K = blockIdx.x % g.gw[top], J = blockIdx.x / g.gw[top];
-  if (tid < kMaxLevels) sh.sCnt[tid] = 0;
-  if (tid == 0) {
-    sh.sTraced = 0;
-    sh.sOccluded = 0;
-  }
-  load_subtree(g, K, J, nodes, sh.sN, sh.sV);
-  __syncthreads();
-
-  // Stage the block's rays (leaf-major slots) and count the traced ones.
-  for (int slot = tid; slot < RBMAX; slot += kThreads) {
-    int x = 0, y = 0, leaf = 0;
-    bool traced = false;
-    if (slot_xy(g, K, J, slot, x, y, leaf)) {
-      const size_t idx = (size_t)y * g.W + x;
-      traced = MODE == 0 || r.valid == nullptr || r.valid[idx] != 0;
-      sh.sRo[slot] = make_float4(r.ox[idx], r.oy[idx], r.oz[idx], r.tmin[idx]);
-      sh.sRd[slot] = make_float4(r.dx[idx], r.dy[idx], r.dz[idx], r.tmax[idx]);
-    }
-    sh.sRv[slot] = traced ? 1 : 0;
-    if (MODE != 2) sh.sKey[slot] = kMissKey;
-    sh.sAux[slot] = MODE == 0 ? kNoKey32 : 0u;
-    const uint64_t bal = __ballot(traced);
-    if (__lane_id() == 0 && bal) atomicAdd(&sh.sTraced, __popcll(bal));
-  }
-  __syncthreads();
-
-  if (sh.sV[0] && sh.sTraced > 0) {
-    const uint32_t t0 = blockIdx.y * chunk;
-    const uint32_t t1 = min(ntri, t0 + chunk);
-    int tiles = 0;
-    for (uint32_t base = t0; base < t1; base += kThreads) {
-      // phase A: root cull
-      const uint32_t ti = base + tid;
-      bool pass = false;
-      if (ti < t1) {
-        const Tri T = load_tri(v9, ti);
-        pass = cull_one<MODE, NODE>(sh.sN[0], T, vmin(vmin(T.v1, T.v2), T.v3),
-                                    vmax(vmax(T.v1, T.v2), T.v3));
+  if (top == 0) {
+    leaf_rays<MODE, NODE, RBMAX, LMAX>(g, v9s, perm, sh.sQ0, sh.sCnt[0], sh);
+  } else {
+    Cursors cur;
+    int s = 0;
+    while (true) {
+      const int cnt = sh.sCnt[s];
+      if (s == top) {
+        leaf_rays<MODE, NODE, RBMAX, LMAX>(g, v9s, perm, sh.sQ[s], cnt, sh);
+        --s;
+        continue;
       }
-      ballot_append(sh.sQ0, &sh.sCnt[0], pass, ti << 6);
-      ++tiles;
+      const int c = cur.get(s);
+      if (c >= cnt) {
+        cur.set(s, 0);
+        if (s == 0) break;
+        --s;
+        continue;
+      }
+      cur.set(s, c + kExpand);
+      const int np = min(kExpand, cnt - c);
+      const uint32_t* src = s == 0 ? sh.sQ0 : sh.sQ[s];
       __syncthreads();
-      // The decision uses only values every thread holds (no LDS read racing the next
-      // tile's appends); kFlushTiles tiles of 256 fit the root queue.
-      const bool last = base + kThreads >= t1;
-      if (!(last || tiles >= kFlushTiles)) continue;
-      tiles = 0;
-      // phase B: expand level by level, depth first over chunks of 64 parents
-      if (g.L == 1) {
-        leaf_rays<MODE>(g, v9, sh.sQ0, sh.sCnt[0], sh.sRo, sh.sRd, sh.sKey, sh.sAux, sh.sRv,
-                        &sh.sOccluded);
-      } else {
-        Cursors cur;
-        int s = 0;
-        while (true) {
-          const int cnt = sh.sCnt[s];
-          if (s == top) {
-            leaf_rays<MODE>(g, v9, sh.sQ[s], cnt, sh.sRo, sh.sRd, sh.sKey, sh.sAux, sh.sRv,
-                            &sh.sOccluded);
-            --s;
-            continue;
-          }
-          const int c = cur.get(s);
-          if (c >= cnt) {
-            cur.set(s, 0);
-            if (s == 0) break;
-            --s;
-            continue;
-          }
-          cur.set(s, c + kExpand);
-          const int np = min(kExpand, cnt - c);
-          const uint32_t* src = s == 0 ? sh.sQ0 : sh.sQ[s];
-          __syncthreads();
-          if (tid == 0) sh.sCnt[s + 1] = 0;
-          __syncthreads();
-          bool ok = false;
-          uint32_t item = 0;
-          if (tid < np * 4) {
-            const uint32_t P = src[c + (tid >> 2)];
-            const uint32_t tri = P >> 6;
-            const int pl = (int)(P & 63u), a = tid & 3;
-            const int pnl = 1 << s;
-            const int dk = pl & (pnl - 1), dj = pl >> s;
-            const int ch = (2 * dj + (a >> 1)) * (2 * pnl) + 2 * dk + (a & 1);
-            const int ni = pow4_off(s + 1) + ch;
-            item = (tri << 6) | (uint32_t)ch;
-            if (sh.sV[ni]) {
-              const Tri T = load_tri(v9, tri);
-              ok = cull_one<MODE, NODE>(sh.sN[ni], T, vmin(vmin(T.v1, T.v2), T.v3),
-                                        vmax(vmax(T.v1, T.v2), T.v3));
-            }
-          }
-          ballot_append(sh.sQ[s + 1], &sh.sCnt[s + 1], ok, item);
-          __syncthreads();
-          ++s;
+      if (tid == 0) sh.sCnt[s + 1] = 0;
+      __syncthreads();
+      bool ok = false;
+      uint32_t item = 0;
+      if (tid < np * 4) {
+        const uint32_t P = src[c + (tid >> 2)];
+        const uint32_t slot_t = P >> 6;
+        const int pl = (int)(P & 63u), a = tid & 3;
+        const int pnl = 1 << s;
+        const int dk = pl & (pnl - 1), dj = pl >> s;
+        const int ch = (2 * dj + (a >> 1)) * (2 * pnl) + 2 * dk + (a & 1);
+        const int ni = pow4_off(s + 1) + ch;
+        item = (slot_t << 6) | (uint32_t)ch;
+        if (sh.sV[ni]) ok = cull_bound<MODE, NODE>(sh.sN[ni], load_tri(v9s, slot_t));
+      }
+      ballot_append(sh.sQ[s + 1], &sh.sCnt[s + 1], ok, item);
+      __syncthreads();
+      ++s;
+    }
+  }
+  if (tid == 0) sh.sCnt[0] = 0;
+  __syncthreads();
+}
+
+// Cluster pass: one work-group per block tests every cluster against the block root (cluster
+// bound), stores the passing cluster ids in clist[block * nclusters + ...] and appends work
+// units of at most `per_unit` passing clusters to `units` ({block, first, count}).
+template <int MODE, int NODE>
+__global__ __launch_bounds__(256) void rh_cluster_kernel(RHGeom g, const float4* __restrict__ cbox,
+                                                         uint32_t nclusters,
+                                                         const float4* __restrict__ nodes,
+                                                         uint32_t* __restrict__ clist,
+                                                         uint4* __restrict__ units,
+                                                         uint32_t* __restrict__ unit_count,
+                                                         int per_unit) {
+  __shared__ int sCnt;
+  __shared__ uint32_t sBase;
+  const int tid = threadIdx.x;
+  const int top = g.L - 1;
+  const int b = blockIdx.x;
+  const int K = b % g.gw[top], J = b / g.gw[top];
+  const float4* p = nodes + 4 * ((size_t)g.off[top] + (size_t)J * g.gw[top] + K);
+  const float4 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3];
+  if (a0.w == 0.f) return;  // no traced ray below this root (uniform)
+  Node root;
+  root.dmin = {a0.x, a0.y, a0.z};
+  root.dmax = {a1.x, a1.y, a1.z};
+  root.omin = {a2.x, a2.y, a2.z};
+  root.omax = {a3.x, a3.y, a3.z};
+  if (tid == 0) sCnt = 0;
+  __syncthreads();
+  uint32_t* out = clist + (size_t)b * nclusters;
+  for (uint32_t cb = 0; cb < nclusters; cb += kThreads) {
+    const uint32_t ci = cb + tid;
+    bool ok = false;
+    if (ci < nclusters) {
+      const float4 lo = cbox[2 * (size_t)ci], hi = cbox[2 * (size_t)ci + 1];
+      ok = cluster_bound<MODE, NODE>(root, V3{lo.x, lo.y, lo.z}, V3{hi.x, hi.y, hi.z});
+    }
+    ballot_append(out, &sCnt, ok, ci);
+  }
+  __syncthreads();
+  const int cnt = sCnt;
+  const int nu = (cnt + per_unit - 1) / per_unit;
+  if (tid == 0 && nu > 0) sBase = atomicAdd(unit_count, (uint32_t)nu);
+  __syncthreads();
+  for (int u = tid; u < nu; u += kThreads)
+    units[sBase + u] = make_uint4((uint32_t)b, (uint32_t)(u * per_unit),
+                                  (uint32_t)min(per_unit, cnt - u * per_unit), 0u);
+}
+
+// Persistent unit kernel: a fixed grid walks the unit queue.  A unit stages its block's rays
+// and subtree, takes the triangles of its clusters through the per-triangle bound (phase A),
+// expands the survivors (phase B), runs the verified leaf tests (phase C) and merges the block
+// results into the global per-ray keys / occlusion flags with atomics.
+template <int MODE, int NODE, int RBMAX, int LMAX>
+__global__ __launch_bounds__(256) void rh_unit_kernel(
+    RHGeom g, RaysDev r, const float* __restrict__ v9s, const uint32_t* __restrict__ perm,
+    uint32_t ntri, uint32_t nclusters, const float4* __restrict__ nodes,
+    const uint32_t* __restrict__ clist, const uint4* __restrict__ units,
+    const uint32_t* __restrict__ unit_count, uint64_t* __restrict__ keys,
+    uint32_t* __restrict__ repkeys, uint8_t* __restrict__ occ_out) {
+  __shared__ RhShared<MODE, RBMAX, LMAX> sh;
+  const int tid = threadIdx.x;
+  const int top = g.L - 1;
+  const uint32_t nunits = *unit_count;
+  for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const uint4 unit = units[u];
+    const int b = (int)unit.x;
+    const int K = b % g.gw[top], J = b / g.gw[top];
+    if (tid < kMaxLevels) sh.sCnt[tid] = 0;
+    if (tid == 0) {
+      sh.sTraced = 0;
+      sh.sOccluded = 0;
+    }
+    load_subtree(g, K, J, nodes, sh.sN, sh.sV);
+    __syncthreads();
+    for (int slot = tid; slot < RBMAX; slot += kThreads) {
+      int x = 0, y = 0, leaf = 0;
+      bool traced = false;
+      if (slot_xy(g, K, J, slot, x, y, leaf)) {
+        const size_t idx = (size_t)y * g.W + x;
+        traced = MODE == 0 || r.valid == nullptr || r.valid[idx] != 0;
+        // a ray another unit already found occluded needs no more tests
+        if (MODE == 2 && traced && occ_out[idx]) traced = false;
+        sh.sRo[slot] = make_float4(r.ox[idx], r.oy[idx], r.oz[idx], r.tmin[idx]);
+        sh.sRd[slot] = make_float4(r.dx[idx], r.dy[idx], r.dz[idx], r.tmax[idx]);
+      }
+      sh.sRv[slot] = traced ? 1 : 0;
+      if (MODE != 2) sh.sKey[slot] = kMissKey;
+      sh.sAux[slot] = MODE == 0 ? kNoKey32 : 0u;
+      const uint64_t bal = __ballot(traced);
+      if (__lane_id() == 0 && bal) atomicAdd(&sh.sTraced, __popcll(bal));
+    }
+    __syncthreads();
+    if (sh.sV[0] && sh.sTraced > 0) {
+      const Node& root = sh.sN[0];
+      const uint32_t* cl = clist + (size_t)b * nclusters + unit.y;
+      const int ncl = (int)unit.z;
+      bool done = false;
+      for (int k0 = 0; k0 < ncl && !done; k0 += kThreads / kCluster) {
+        const int k = k0 + tid / kCluster;
+        const uint32_t slot = k < ncl ? cl[k] * kCluster + (uint32_t)(tid % kCluster) : ntri;
+        bool pass = false;
+        if (slot < ntri) pass = cull_bound<MODE, NODE>(root, load_tri(v9s, slot));
+        ballot_append(sh.sQ0, &sh.sCnt[0], pass, slot << 6);
+        __syncthreads();
+        const int cnt0 = sh.sCnt[0];
+        __syncthreads();  // every thread has read the count before the next round appends
+        if (cnt0 > kQ0Cap - kThreads) {
+          expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, perm, sh);
+          if (MODE == 2 && sh.sOccluded >= sh.sTraced) done = true;
         }
       }
-      if (tid == 0) sh.sCnt[0] = 0;
-      __syncthreads();
-      if (MODE == 2 && sh.sOccluded >= sh.sTraced) break;
+      if (!done && sh.sCnt[0] > 0) expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, perm, sh);
     }
-  }
-
-  for (int slot = tid; slot < g.rb; slot += kThreads) {
-    int x, y, leaf;
-    if (!slot_xy(g, K, J, slot, x, y, leaf)) continue;
-    const size_t idx = (size_t)y * g.W + x;
-    if (MODE == 2) {
-      const bool o = sh.sAux[slot] != 0;
-      if (tchunks == 1) occ_out[idx] = o ? 1 : 0;
-      else if (o) occ_out[idx] = 1;
-    } else {
-      const unsigned long long k = sh.sKey[slot];
-      if (tchunks == 1) keys[idx] = k;
-      else if (k != kMissKey) atomicMin((unsigned long long*)&keys[idx], k);
-      if (MODE == 0 && repkeys) {
-        const uint32_t rk = sh.sAux[slot];
-        if (tchunks == 1) repkeys[idx] = rk;
-        else if (rk != kNoKey32) atomicMin(&repkeys[idx], rk);
+    for (int slot = tid; slot < g.rb; slot += kThreads) {
+      int x, y, leaf;
+      if (!slot_xy(g, K, J, slot, x, y, leaf)) continue;
+      const size_t idx = (size_t)y * g.W + x;
+      if (MODE == 2) {
+        if (sh.sAux[slot]) occ_out[idx] = 1;
+      } else {
+        const unsigned long long k = sh.sKey[slot];
+        if (k != kMissKey) atomicMin((unsigned long long*)&keys[idx], k);
+        if (MODE == 0 && repkeys) {
+          const uint32_t rk = sh.sAux[slot];
+          if (rk != kNoKey32) atomicMin(&repkeys[idx], rk);
+        }
       }
     }
+    __syncthreads();  // LDS is restaged by the next unit
   }
 }
 
@@ -951,19 +1070,23 @@ __global__ void fill_kernel(T* p, T v, size_t n) {
 inline dim3 grid1(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
 template <int MODE, int NODE>
-hipError_t rh_trace_dispatch(const RHGeom& g, const RaysDev& r, const float* v9, uint32_t ntri,
-                             int tchunks, const float4* nodes, uint64_t* keys, uint32_t* repkeys,
-                             uint8_t* occ, hipStream_t s) {
-  const uint32_t chunk = (uint32_t)((ntri + tchunks - 1) / tchunks);
-  dim3 grid(g.ntop, tchunks);
-  if (g.rb <= 256)
-    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 256>), grid, dim3(kThreads), 0, s, g, r, v9,
-                       ntri, chunk, nodes, keys, repkeys, occ, tchunks);
-  else if (g.rb <= 1024)
-    hipLaunchKernelGGL((rh_trace_kernel<MODE, NODE, 1024>), grid, dim3(kThreads), 0, s, g, r, v9,
-                       ntri, chunk, nodes, keys, repkeys, occ, tchunks);
+hipError_t rh_trace_dispatch(const RHGeom& g, const RaysDev& r, const SortedTris& st,
+                             const RhWork& w, const float4* nodes, uint64_t* keys,
+                             uint32_t* repkeys, uint8_t* occ, hipStream_t s) {
+  if (g.rb > 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE>), dim3(g.ntop), dim3(kThreads), 0, s, g,
+                     st.cbox, st.nclusters, nodes, w.clist, w.units, w.unit_count, w.per_unit);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const dim3 grid(w.grid);
+  if (g.rb <= 256 && g.L <= 3)
+    hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 256, 3>), grid, dim3(kThreads), 0, s, g, r,
+                       st.v9s, st.perm, st.ntri, st.nclusters, nodes, w.clist, w.units,
+                       w.unit_count, keys, repkeys, occ);
   else
-    return hipErrorInvalidValue;
+    hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 1024, 4>), grid, dim3(kThreads), 0, s, g, r,
+                       st.v9s, st.perm, st.ntri, st.nclusters, nodes, w.clist, w.units,
+                       w.unit_count, keys, repkeys, occ);
   return hipGetLastError();
 }
 
@@ -1053,16 +1176,16 @@ hipError_t launch_rh_build(const RHGeom& g, const RaysDev& r, bool use_valid, fl
 }
 
 hipError_t launch_rh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
-                           const float* v9, uint32_t ntri, int tchunks, const float4* nodes,
+                           const SortedTris& st, const RhWork& w, const float4* nodes,
                            uint64_t* keys, uint32_t* repkeys, uint8_t* occ, hipStream_t s) {
   if (node_kind == 1) {
-    if (mode == 0) return rh_trace_dispatch<0, 1>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
-    if (mode == 1) return rh_trace_dispatch<1, 1>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
-    return rh_trace_dispatch<2, 1>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
+    if (mode == 0) return rh_trace_dispatch<0, 1>(g, r, st, w, nodes, keys, repkeys, occ, s);
+    if (mode == 1) return rh_trace_dispatch<1, 1>(g, r, st, w, nodes, keys, repkeys, occ, s);
+    return rh_trace_dispatch<2, 1>(g, r, st, w, nodes, keys, repkeys, occ, s);
   }
-  if (mode == 0) return rh_trace_dispatch<0, 0>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
-  if (mode == 1) return rh_trace_dispatch<1, 0>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
-  return rh_trace_dispatch<2, 0>(g, r, v9, ntri, tchunks, nodes, keys, repkeys, occ, s);
+  if (mode == 0) return rh_trace_dispatch<0, 0>(g, r, st, w, nodes, keys, repkeys, occ, s);
+  if (mode == 1) return rh_trace_dispatch<1, 0>(g, r, st, w, nodes, keys, repkeys, occ, s);
+  return rh_trace_dispatch<2, 0>(g, r, st, w, nodes, keys, repkeys, occ, s);
 }
 
 hipError_t launch_bvh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
