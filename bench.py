@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="torus_rayhierarchy_512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the timed steps, run one plain device copy of this many MB "
+                         "(calibrates profiler byte counters against a kernel of known traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -99,6 +102,14 @@ def main():
     value = rays_per_step * args.steps / (t1 - t0) / 1e6
     gpu_ids = hits.id.cpu().numpy()
     gpu_occ = occ.cpu().numpy()
+    if args.copy_mb > 0:
+        src = torch.empty(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     result = {
         "metric": "Mrays/s batched vs plain per-scene launch, same hits; p50/p99 request latency",

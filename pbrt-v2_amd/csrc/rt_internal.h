@@ -67,7 +67,11 @@ hipError_t launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 hipError_t launch_rh_build(const RHGeom& g, const RaysDev& r, bool use_valid, float4* nodes,
                            hipStream_t s);
 // mode 0 closest primary, 1 closest secondary, 2 occluded
-hipError_t launch_rh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
+// rayhierarchy traversal: the cluster pass fills the work queue, the persistent unit kernel
+// walks it (keys / repkeys / occlusion flags must be initialised before).
+hipError_t launch_rh_clusters(const RHGeom& g, int mode, int node_kind, const SortedTris& st,
+                              const RhWork& w, const float4* nodes, hipStream_t s);
+hipError_t launch_rh_units(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
                            const SortedTris& st, const RhWork& w, const float4* nodes,
                            uint64_t* keys, uint32_t* repkeys, uint8_t* occ, hipStream_t s);
 hipError_t launch_bvh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,

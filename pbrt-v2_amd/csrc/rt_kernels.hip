@@ -1070,14 +1070,18 @@ __global__ void fill_kernel(T* p, T v, size_t n) {
 inline dim3 grid1(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
 template <int MODE, int NODE>
-hipError_t rh_trace_dispatch(const RHGeom& g, const RaysDev& r, const SortedTris& st,
-                             const RhWork& w, const float4* nodes, uint64_t* keys,
-                             uint32_t* repkeys, uint8_t* occ, hipStream_t s) {
-  if (g.rb > 1024) return hipErrorInvalidValue;
+hipError_t rh_cluster_dispatch(const RHGeom& g, const SortedTris& st, const RhWork& w,
+                               const float4* nodes, hipStream_t s) {
   hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE>), dim3(g.ntop), dim3(kThreads), 0, s, g,
                      st.cbox, st.nclusters, nodes, w.clist, w.units, w.unit_count, w.per_unit);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+template <int MODE, int NODE>
+hipError_t rh_unit_dispatch(const RHGeom& g, const RaysDev& r, const SortedTris& st,
+                            const RhWork& w, const float4* nodes, uint64_t* keys,
+                            uint32_t* repkeys, uint8_t* occ, hipStream_t s) {
+  if (g.rb > 1024) return hipErrorInvalidValue;
   const dim3 grid(w.grid);
   if (g.rb <= 256 && g.L <= 3)
     hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 256, 3>), grid, dim3(kThreads), 0, s, g, r,
@@ -1175,17 +1179,33 @@ hipError_t launch_rh_build(const RHGeom& g, const RaysDev& r, bool use_valid, fl
   return hipGetLastError();
 }
 
-hipError_t launch_rh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
+hipError_t launch_rh_clusters(const RHGeom& g, int mode, int node_kind, const SortedTris& st,
+                              const RhWork& w, const float4* nodes, hipStream_t s) {
+#define RT_CALL(M, N) rh_cluster_dispatch<M, N>(g, st, w, nodes, s)
+  if (node_kind == 1) {
+    if (mode == 0) return RT_CALL(0, 1);
+    if (mode == 1) return RT_CALL(1, 1);
+    return RT_CALL(2, 1);
+  }
+  if (mode == 0) return RT_CALL(0, 0);
+  if (mode == 1) return RT_CALL(1, 0);
+  return RT_CALL(2, 0);
+#undef RT_CALL
+}
+
+hipError_t launch_rh_units(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
                            const SortedTris& st, const RhWork& w, const float4* nodes,
                            uint64_t* keys, uint32_t* repkeys, uint8_t* occ, hipStream_t s) {
+#define RT_CALL(M, N) rh_unit_dispatch<M, N>(g, r, st, w, nodes, keys, repkeys, occ, s)
   if (node_kind == 1) {
-    if (mode == 0) return rh_trace_dispatch<0, 1>(g, r, st, w, nodes, keys, repkeys, occ, s);
-    if (mode == 1) return rh_trace_dispatch<1, 1>(g, r, st, w, nodes, keys, repkeys, occ, s);
-    return rh_trace_dispatch<2, 1>(g, r, st, w, nodes, keys, repkeys, occ, s);
+    if (mode == 0) return RT_CALL(0, 1);
+    if (mode == 1) return RT_CALL(1, 1);
+    return RT_CALL(2, 1);
   }
-  if (mode == 0) return rh_trace_dispatch<0, 0>(g, r, st, w, nodes, keys, repkeys, occ, s);
-  if (mode == 1) return rh_trace_dispatch<1, 0>(g, r, st, w, nodes, keys, repkeys, occ, s);
-  return rh_trace_dispatch<2, 0>(g, r, st, w, nodes, keys, repkeys, occ, s);
+  if (mode == 0) return RT_CALL(0, 0);
+  if (mode == 1) return RT_CALL(1, 0);
+  return RT_CALL(2, 0);
+#undef RT_CALL
 }
 
 hipError_t launch_bvh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
