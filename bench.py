@@ -38,6 +38,20 @@ def _load(name, path, search=None):
     return mod
 
 
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters (FETCH_SIZE,
+    WRITE_SIZE, calibrated against a plain copy) and recorded in profiles/*.md as a line
+    `traffic-per-launch <workload> <kernel> <bytes> ...`; None when not profiled."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            parts = line.split()
+            if len(parts) >= 4 and parts[0] == "traffic-per-launch" and parts[1] == workload:
+                pairs = dict(zip(parts[2::2], parts[3::2]))
+                if kernel in pairs:
+                    return int(float(pairs[kernel]))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,7 +172,7 @@ def main():
         achieved = b / (t * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None, "bytes_per_launch": int(b),
+                    "traffic": profiled_traffic(args.workload, name), "bytes_per_launch": int(b),
                     "counts": {"cull_tests": int((os_ if name[0] == "o" else cs).cull_tests),
                                "leaf_pairs": int((os_ if name[0] == "o" else cs).leaf_pairs)}}
         # CPU baseline: pbrt's BVHAccel on the same primary + shadow rays, all host threads

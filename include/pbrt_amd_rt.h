@@ -78,7 +78,8 @@ typedef struct rt_rays {
 /* Closest-hit outputs in DEVICE memory, one entry per ray.  id is the ORIGINAL triangle index
  * (ray-bvh ids are mapped back from BVH order) or -1 for a miss; t = INFINITY and u = v = 0 on
  * a miss.  (u, v) are the interpolated surface coordinates of computeDpTuTv.cl:87-89.
- * dpdu/dpdv (3 floats per ray each) are optional: NULL skips them. */
+ * dpdu/dpdv are optional (NULL skips them): 3 floats per ray each, x y z of ray i at
+ * [3i, 3i+3). */
 typedef struct rt_hits {
   int32_t *id;
   float *t, *u, *v;
