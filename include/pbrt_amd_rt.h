@@ -138,11 +138,22 @@ int rt_make_shadow_rays(const rt_rays *primary, const int32_t *id, const float *
                         float *dy, float *dz, float *tmin, float *tmax, uint8_t *valid,
                         void *stream);
 
-/* Launch-shape parameters (set by the autotuner; 0 restores the built-in choice).
+/* Launch-shape parameters (set by the autotuner; 0 restores the built-in choice).  None of
+ * them changes any result.
  *   RT_PARAM_CLUSTERS_CLOSEST / RT_PARAM_CLUSTERS_OCCLUDED: passing triangle clusters (64
  *   triangles each) per rayhierarchy work unit for closest-hit / occlusion traces; smaller
- *   values split a block's work over more units of the persistent traversal kernel. */
-enum { RT_PARAM_CLUSTERS_CLOSEST = 1, RT_PARAM_CLUSTERS_OCCLUDED = 2 };
+ *   values split a block's work over more units of the persistent traversal kernel.
+ *   RT_PARAM_GRID_CLOSEST / RT_PARAM_GRID_OCCLUDED: work-groups of the persistent traversal
+ *   kernel (batches only).
+ *   RT_PARAM_GRAPH: 1 = replay the batch's launch sequence as a HIP graph while its jobs are
+ *   unchanged (batches only; needs a non-null stream). */
+enum {
+  RT_PARAM_CLUSTERS_CLOSEST = 1,
+  RT_PARAM_CLUSTERS_OCCLUDED = 2,
+  RT_PARAM_GRID_CLOSEST = 3,
+  RT_PARAM_GRID_OCCLUDED = 4,
+  RT_PARAM_GRAPH = 5
+};
 int rt_scene_set_param(rt_scene *scene, int32_t param, int32_t value);
 
 /* HIP-event timing of the traversal kernels.  rt_scene_set_timing(scene, 1) starts recording
@@ -152,6 +163,35 @@ int rt_scene_set_param(rt_scene *scene, int32_t param, int32_t value);
  * counterpart of the reference's GPU_TIMES accounting (GPUparallel.h:18-27,528-539). */
 int rt_scene_set_timing(rt_scene *scene, int32_t enable);
 int rt_scene_kernel_time(rt_scene *scene, int32_t mode, float *total_ms, int32_t *launches);
+
+/* Batched form: many rayhierarchy jobs -- scenes and ray images of any size, closest-hit and
+ * occlusion mixed -- run as one launch sequence per (mode, node type) group.  Every group
+ * builds all its hierarchies in one launch and walks one shared work queue with one
+ * persistent, work-stealing traversal kernel.  Results are identical to running each job
+ * alone with rt_trace_closest / rt_trace_occluded.  No counterpart in the reference, which
+ * serialises every accelerator call (rayhierarchy.cpp:208,636,855,1212); it serves the
+ * breadth-first renderer's per-task calls of many scenes and the request loop of
+ * rt_server_*.
+ * The jobs' ray and output arrays are DEVICE arrays that must stay valid until the stream
+ * has run the batch.  Calls on one batch must not overlap in time. */
+typedef struct rt_batch rt_batch;
+int rt_batch_create(rt_ctx *ctx, rt_batch **out);
+int rt_batch_destroy(rt_batch *batch);
+/* Removes every job (keeps the device scratch and tuned parameters). */
+int rt_batch_clear(rt_batch *batch);
+int rt_batch_add_closest(rt_batch *batch, rt_scene *scene, const rt_rays *rays, int32_t mode,
+                         const rt_hits *hits);
+int rt_batch_add_occluded(rt_batch *batch, rt_scene *scene, const rt_rays *rays,
+                          uint8_t *occluded);
+int rt_batch_run(rt_batch *batch, void *stream);
+int rt_batch_set_param(rt_batch *batch, int32_t param, int32_t value);
+int rt_batch_get_param(rt_batch *batch, int32_t param, int32_t *value);
+/* Autotuner: times the batch's current jobs over the launch-shape candidates (clusters per
+ * work unit x persistent grid, per mode), `reps` runs each, and keeps the fastest. */
+int rt_batch_autotune(rt_batch *batch, void *stream, int32_t reps);
+/* Event timing of the batch's traversal kernels, as rt_scene_set_timing / kernel_time. */
+int rt_batch_set_timing(rt_batch *batch, int32_t enable);
+int rt_batch_kernel_time(rt_batch *batch, int32_t mode, float *total_ms, int32_t *launches);
 
 const char *rt_error_string(int status);
 

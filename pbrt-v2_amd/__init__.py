@@ -33,8 +33,12 @@ EXPORTS = (
     "rt_scene_set_bvh", "rt_scene_destroy", "rt_trace_closest", "rt_trace_occluded",
     "rt_make_shadow_rays", "rt_scene_set_timing", "rt_scene_kernel_time",
     "rt_error_string", "rt_raybvh_build_handoff", "rt_raybvh_free_handoff", "rt_scene_set_param",
+    "rt_batch_create", "rt_batch_destroy", "rt_batch_clear", "rt_batch_add_closest",
+    "rt_batch_add_occluded", "rt_batch_run", "rt_batch_set_param", "rt_batch_get_param",
+    "rt_batch_autotune", "rt_batch_set_timing", "rt_batch_kernel_time",
 )
-RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED = 1, 2
+(RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED, RT_PARAM_GRID_CLOSEST,
+ RT_PARAM_GRID_OCCLUDED, RT_PARAM_GRAPH) = 1, 2, 3, 4, 5
 
 
 class RtError(RuntimeError):
@@ -106,6 +110,19 @@ def lib() -> ctypes.CDLL:
                                           P(ctypes.c_void_p), P(ctypes.c_uint32), P(ctypes.c_void_p),
                                           P(ctypes.c_uint32), P(ctypes.c_void_p)]
     L.rt_raybvh_free_handoff.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.rt_batch_create.argtypes = [ctypes.c_void_p, P(ctypes.c_void_p)]
+    L.rt_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.rt_batch_clear.argtypes = [ctypes.c_void_p]
+    L.rt_batch_add_closest.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(RtRays), ctypes.c_int32,
+                                       P(RtHits)]
+    L.rt_batch_add_occluded.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(RtRays), _fp]
+    L.rt_batch_run.argtypes = [ctypes.c_void_p, _fp]
+    L.rt_batch_set_param.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+    L.rt_batch_get_param.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_int32)]
+    L.rt_batch_autotune.argtypes = [ctypes.c_void_p, _fp, ctypes.c_int32]
+    L.rt_batch_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.rt_batch_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_float),
+                                       P(ctypes.c_int32)]
     _LIB = L
     return L
 
@@ -306,6 +323,71 @@ class Scene:
         if self.handle:
             lib().rt_scene_destroy(self.handle)
             self.handle = ctypes.c_void_p()
+
+
+class Batch:
+    """Batched form: many rayhierarchy jobs (scenes x ray images, closest and occlusion) run as
+    one launch sequence per (mode, node type) group with a shared persistent traversal kernel.
+    The jobs' tensors are kept referenced until the batch is cleared."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.handle = ctypes.c_void_p()
+        check(lib().rt_batch_create(ctx.handle, ctypes.byref(self.handle)), "rt_batch_create")
+        self._keep = []
+
+    def add_closest(self, scene: Scene, rays: DeviceRays, mode: int = RT_CLOSEST_PRIMARY,
+                    hits: DeviceHits | None = None, shading: bool = False) -> DeviceHits:
+        if hits is None:
+            hits = DeviceHits.empty(rays.count, device=rays.ox.device, shading=shading)
+        rs, hs = rays.c_struct(), hits.c_struct()
+        check(lib().rt_batch_add_closest(self.handle, scene.handle, ctypes.byref(rs), mode,
+                                         ctypes.byref(hs)), "rt_batch_add_closest")
+        self._keep.append((scene, rays, hits))
+        return hits
+
+    def add_occluded(self, scene: Scene, rays: DeviceRays, out=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty(rays.count, dtype=torch.uint8, device=rays.ox.device)
+        rs = rays.c_struct()
+        check(lib().rt_batch_add_occluded(self.handle, scene.handle, ctypes.byref(rs),
+                                          out.data_ptr()), "rt_batch_add_occluded")
+        self._keep.append((scene, rays, out))
+        return out
+
+    def run(self, stream=None):
+        check(lib().rt_batch_run(self.handle, _stream_ptr(stream)), "rt_batch_run")
+
+    def clear(self):
+        check(lib().rt_batch_clear(self.handle), "rt_batch_clear")
+        self._keep = []
+
+    def set_param(self, param: int, value: int):
+        check(lib().rt_batch_set_param(self.handle, param, int(value)), "rt_batch_set_param")
+
+    def get_param(self, param: int) -> int:
+        v = ctypes.c_int32()
+        check(lib().rt_batch_get_param(self.handle, param, ctypes.byref(v)), "rt_batch_get_param")
+        return v.value
+
+    def autotune(self, reps: int = 3, stream=None):
+        check(lib().rt_batch_autotune(self.handle, _stream_ptr(stream), reps), "rt_batch_autotune")
+
+    def set_timing(self, on: bool):
+        check(lib().rt_batch_set_timing(self.handle, 1 if on else 0), "rt_batch_set_timing")
+
+    def kernel_time(self, mode: int):
+        ms, n = ctypes.c_float(), ctypes.c_int32()
+        check(lib().rt_batch_kernel_time(self.handle, mode, ctypes.byref(ms), ctypes.byref(n)),
+              "rt_batch_kernel_time")
+        return ms.value, n.value
+
+    def close(self):
+        if self.handle:
+            lib().rt_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+        self._keep = []
 
 
 def make_shadow_rays(primary: DeviceRays, hits: DeviceHits, light, stream=None,

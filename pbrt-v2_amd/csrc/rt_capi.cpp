@@ -21,6 +21,91 @@ struct rt_ctx {
   int device;
 };
 
+// HIP-event timing of the traversal kernels: one event pair per launch while enabled.
+struct Timer {
+  struct Slot {
+    int mode;
+    hipEvent_t a, b;
+  };
+  bool on = false;
+  std::vector<Slot> slots;
+  size_t used = 0;
+  ~Timer() {
+    for (auto& t : slots) {
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+  }
+  // The event pair for the next launch of `mode`, or null when timing is off.
+  Slot* next(int mode) {
+    if (!on) return nullptr;
+    if (used == slots.size()) {
+      Slot t{mode, nullptr, nullptr};
+      if (hipEventCreate(&t.a) != hipSuccess) return nullptr;
+      if (hipEventCreate(&t.b) != hipSuccess) {
+        (void)hipEventDestroy(t.a);
+        return nullptr;
+      }
+      slots.push_back(t);
+    }
+    Slot& t = slots[used++];
+    t.mode = mode;
+    return &t;
+  }
+  int total(int mode, float* ms_out, int32_t* n_out) {
+    float sum = 0.f;
+    int32_t n = 0;
+    for (size_t i = 0; i < used; ++i) {
+      const Slot& t = slots[i];
+      if (t.mode != mode) continue;
+      if (hipEventSynchronize(t.b) != hipSuccess) return RT_ERR_HIP;
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) return RT_ERR_HIP;
+      sum += ms;
+      ++n;
+    }
+    *ms_out = sum;
+    *n_out = n;
+    return RT_OK;
+  }
+};
+
+// Scratch of rayhierarchy launch groups: one device arena (job tables, per-job hierarchy,
+// cluster lists and keys, per-group work queue) and a pinned host copy of the job tables.
+// The tables are uploaded only when they change, so repeated calls on the same buffers cost
+// no host-to-device copy.
+struct RhWorkspace {
+  char* d_arena = nullptr;
+  size_t arena_cap = 0;
+  RhItem* h_table = nullptr;  // pinned
+  size_t table_cap = 0;
+  std::vector<RhItem> last;  // tables as last uploaded
+  const char* last_arena = nullptr;
+  hipEvent_t uploaded = nullptr;
+  ~RhWorkspace() {
+    if (uploaded) (void)hipEventSynchronize(uploaded);
+    (void)hipFree(d_arena);
+    (void)hipHostFree(h_table);
+    if (uploaded) (void)hipEventDestroy(uploaded);
+  }
+};
+
+// One rayhierarchy job of a plain call or a batch.
+struct RhJob {
+  rt_scene* s;
+  RaysDev r;
+  int mode;  // 0 primary, 1 secondary, 2 occluded
+  HitOut out;
+  uint8_t* occ;
+};
+
+// Launch-shape parameters: passing clusters per work unit and persistent grid, per mode
+// (index 0 closest, 1 occluded); 0 = built-in choice.
+struct RhParams {
+  int per[2] = {0, 0};
+  int grid[2] = {0, 0};
+};
+
 struct rt_scene {
   rt_ctx* ctx = nullptr;
   rt_accel_desc desc{};
@@ -40,30 +125,38 @@ struct rt_scene {
   uint32_t nroots = 0;
   int tile_roots = 256;
   bool has_bvh = false;
-  // scratch, grown on demand
+  // scratch of the naive and ray-bvh paths, grown on demand
   float4* d_nodes = nullptr;
   size_t nodes_cap = 0;
   uint64_t* d_keys = nullptr;
   size_t keys_cap = 0;
-  uint32_t* d_repkeys = nullptr;
-  size_t rep_cap = 0;
-  uint32_t* d_replist = nullptr;
-  size_t replist_cap = 0;
-  uint32_t* d_clist = nullptr;  // passing clusters per block
-  size_t clist_cap = 0;
-  uint4* d_units = nullptr;  // work units {block, first, count}
-  size_t units_cap = 0;
-  uint32_t* d_ucount = nullptr;
-  size_t ucount_cap = 0;
-  // HIP-event timing of the traversal kernels (one event pair per launch while enabled)
-  struct Timed {
-    int mode;
-    hipEvent_t a, b;
-  };
-  bool timing = false;
-  std::vector<Timed> timed;
-  size_t timed_used = 0;
+  // scratch of the rayhierarchy path (a launch group of one job)
+  RhWorkspace ws;
+  Timer timer;
   std::mutex mu;
+};
+
+struct rt_batch {
+  rt_ctx* ctx = nullptr;
+  std::vector<RhJob> jobs;
+  RhWorkspace ws;
+  RhParams prm;
+  bool use_graph = false;
+  // captured launch sequence, valid while the jobs, parameters and arena are unchanged
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  bool graph_valid = false;
+  Timer timer;
+  std::mutex mu;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_valid = false;
+  }
+  ~rt_batch() { drop_graph(); }
 };
 
 namespace {
@@ -197,25 +290,185 @@ hipError_t upload(T*& d, const T* h, size_t n) {
   return e;
 }
 
-// Records the start event of a traversal launch; returns the slot or -1.
-int timing_begin(rt_scene* s, int mode, hipStream_t st) {
-  if (!s->timing) return -1;
-  if (s->timed_used == s->timed.size()) {
-    rt_scene::Timed t{mode, nullptr, nullptr};
-    if (hipEventCreate(&t.a) != hipSuccess) return -1;
-    if (hipEventCreate(&t.b) != hipSuccess) {
-      (void)hipEventDestroy(t.a);
-      return -1;
-    }
-    s->timed.push_back(t);
+// Launches `fn` bracketed by the timer's next event pair for `mode` (if timing is on).
+template <class F>
+hipError_t timed_launch(Timer& tm, int mode, hipStream_t st, F fn) {
+  Timer::Slot* t = tm.next(mode);
+  if (t) {
+    hipError_t e = hipEventRecord(t->a, st);
+    if (e != hipSuccess) return e;
   }
-  rt_scene::Timed& t = s->timed[s->timed_used];
-  t.mode = mode;
-  if (hipEventRecord(t.a, st) != hipSuccess) return -1;
-  return (int)s->timed_used++;
+  hipError_t e = fn();
+  if (e == hipSuccess && t) e = hipEventRecord(t->b, st);
+  return e;
 }
-void timing_end(rt_scene* s, int slot, hipStream_t st) {
-  if (slot >= 0 && hipEventRecord(s->timed[slot].b, st) != hipSuccess) s->timed[slot].mode = -1;
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+constexpr int kDefaultPer[2] = {16, 4};  // passing clusters per unit: closest, occluded
+constexpr int kDefaultGrid = 2048;       // persistent work-groups (8 per CU)
+
+// Lays out the jobs as launch groups (one per mode and node type, in job order) in the
+// workspace arena, fills the job tables and uploads them when they changed.  `changed` tells
+// whether the device tables or arena were rewritten.
+int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& prm,
+               hipStream_t st, std::vector<RhGroup>& groups, bool* changed) {
+  groups.clear();
+  *changed = false;
+  // group membership in job order
+  std::vector<std::pair<int, int>> keys;  // (mode, node)
+  std::vector<std::vector<int>> members;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const std::pair<int, int> k{jobs[j].mode, jobs[j].s->desc.node};
+    size_t gi = 0;
+    while (gi < keys.size() && keys[gi] != k) ++gi;
+    if (gi == keys.size()) {
+      keys.push_back(k);
+      members.emplace_back();
+    }
+    members[gi].push_back((int)j);
+  }
+  // geometry and sizes
+  std::vector<RhItem> items(jobs.size());
+  std::vector<size_t> off_nodes(jobs.size()), off_clist(jobs.size()), off_keys(jobs.size()),
+      off_rep(jobs.size());
+  size_t total = 0;
+  const size_t tables = align256(sizeof(RhItem) * std::max<size_t>(jobs.size(), 1));
+  total += tables;
+  struct GroupLayout {
+    size_t units, ctr, replist, max_units;
+    uint32_t nblocks, nrays;
+    bool big, repair;
+    int per, grid;
+  };
+  std::vector<GroupLayout> gl(keys.size());
+  for (size_t gi = 0; gi < keys.size(); ++gi) {
+    const int mode = keys[gi].first;
+    const int mi = mode == 2 ? 1 : 0;
+    GroupLayout& L = gl[gi];
+    L.per = prm.per[mi] > 0 ? prm.per[mi] : kDefaultPer[mi];
+    L.max_units = 0;
+    L.nblocks = 0;
+    L.nrays = 0;
+    L.big = false;
+    L.repair = false;
+    uint64_t blocks = 0, rays = 0;
+    for (int j : members[gi]) {
+      const RhJob& jb = jobs[j];
+      RhItem& it = items[j];
+      std::memset(&it, 0, sizeof(it));
+      if (rh_geometry(jb.r.W, jb.r.H, jb.s->desc.chunk_x, jb.s->desc.chunk_y, jb.s->desc.height,
+                      &it.g))
+        return RT_ERR_UNSUPPORTED;
+      // LDS-staged blocks hold at most 1024 rays; queue entries pack the triangle slot in 26 bits
+      if (it.g.rb > 1024 || jb.s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
+      const size_t n = (size_t)jb.r.W * jb.r.H;
+      it.r = jb.r;
+      it.v9 = jb.s->d_v9;
+      it.uv6 = jb.s->d_uv6;
+      it.v9s = jb.s->d_v9s;
+      it.perm = jb.s->d_perm;
+      it.cbox = jb.s->d_cbox;
+      it.ntri = jb.s->ntri;
+      it.nclusters = jb.s->nclusters;
+      it.out = jb.out;
+      it.occ = jb.occ;
+      it.repair_run = mode == 0 ? jb.s->desc.repair_run : 0;
+      it.block_base = (uint32_t)blocks;
+      it.ray_base = (uint32_t)rays;
+      blocks += (uint64_t)it.g.ntop;
+      rays += n;
+      if (it.g.rb > 256 || it.g.L > 3) L.big = true;
+      if (it.repair_run > 0) L.repair = true;
+      L.max_units += (size_t)it.g.ntop * ((it.nclusters + L.per - 1) / L.per);
+      off_nodes[j] = total;
+      total += align256(sizeof(float4) * 4 * (size_t)it.g.nnodes);
+      off_clist[j] = total;
+      total += align256(sizeof(uint32_t) * (size_t)it.g.ntop * std::max<uint32_t>(it.nclusters, 1));
+      off_keys[j] = total;
+      if (mode != 2) total += align256(sizeof(uint64_t) * n);
+      off_rep[j] = total;
+      if (it.repair_run > 0) total += align256(sizeof(uint32_t) * n);
+    }
+    if (blocks >= (1ull << 31) || rays >= (1ull << 31)) return RT_ERR_UNSUPPORTED;
+    L.nblocks = (uint32_t)blocks;
+    L.nrays = (uint32_t)rays;
+    const int gset = prm.grid[mi] > 0 ? prm.grid[mi] : kDefaultGrid;
+    L.grid = (int)std::max<size_t>(1, std::min<size_t>(L.max_units, (size_t)gset));
+    L.units = total;
+    total += align256(sizeof(uint4) * std::max<size_t>(L.max_units, 1));
+    L.ctr = total;
+    total += 256;
+    L.replist = total;
+    if (L.repair) total += align256(sizeof(uint32_t) * L.nrays);
+  }
+  // arena
+  if (total > ws.arena_cap || !ws.d_arena) {
+    if (ws.d_arena) {
+      hipError_t e = hipFree(ws.d_arena);  // waits for work still using it
+      ws.d_arena = nullptr;
+      ws.arena_cap = 0;
+      if (e != hipSuccess) return hip_status(e);
+    }
+    ws.last_arena = nullptr;  // the new arena holds no table yet
+    const size_t cap = total + total / 4;
+    hipError_t e = hipMalloc((void**)&ws.d_arena, cap);
+    if (e != hipSuccess) {
+      ws.d_arena = nullptr;
+      return hip_status(e);
+    }
+    ws.arena_cap = cap;
+  }
+  char* A = ws.d_arena;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    RhItem& it = items[j];
+    it.nodes = (float4*)(A + off_nodes[j]);
+    it.clist = (uint32_t*)(A + off_clist[j]);
+    it.keys = jobs[j].mode != 2 ? (uint64_t*)(A + off_keys[j]) : nullptr;
+    it.repkeys = it.repair_run > 0 ? (uint32_t*)(A + off_rep[j]) : nullptr;
+  }
+  // tables in group order
+  std::vector<RhItem> table;
+  table.reserve(jobs.size());
+  for (size_t gi = 0; gi < keys.size(); ++gi) {
+    RhGroup G{};
+    G.items = (const RhItem*)A + table.size();
+    for (int j : members[gi]) table.push_back(items[j]);
+    G.nitems = (int)members[gi].size();
+    G.mode = keys[gi].first;
+    G.node = keys[gi].second;
+    G.big = gl[gi].big ? 1 : 0;
+    G.nblocks = gl[gi].nblocks;
+    G.nrays = gl[gi].nrays;
+    G.units = (uint4*)(A + gl[gi].units);
+    G.ctr = (uint32_t*)(A + gl[gi].ctr);
+    G.replist = gl[gi].repair ? (uint32_t*)(A + gl[gi].replist) : nullptr;
+    G.per_unit = gl[gi].per;
+    G.grid = gl[gi].grid;
+    groups.push_back(G);
+  }
+  const bool same = ws.last_arena == A && ws.last.size() == table.size() &&
+                    (table.empty() ||
+                     std::memcmp(ws.last.data(), table.data(), sizeof(RhItem) * table.size()) == 0);
+  if (!same && !table.empty()) {
+    if (!ws.uploaded) RT_TRY(hipEventCreateWithFlags(&ws.uploaded, hipEventDisableTiming));
+    else RT_TRY(hipEventSynchronize(ws.uploaded));  // the pinned copy is free again
+    if (table.size() > ws.table_cap) {
+      (void)hipHostFree(ws.h_table);
+      ws.h_table = nullptr;
+      ws.table_cap = 0;
+      RT_TRY(hipHostMalloc((void**)&ws.h_table, sizeof(RhItem) * table.size(), 0));
+      ws.table_cap = table.size();
+    }
+    std::memcpy(ws.h_table, table.data(), sizeof(RhItem) * table.size());
+    RT_TRY(hipMemcpyAsync(A, ws.h_table, sizeof(RhItem) * table.size(), hipMemcpyHostToDevice,
+                          st));
+    RT_TRY(hipEventRecord(ws.uploaded, st));
+    ws.last = table;
+    ws.last_arena = A;
+    *changed = true;
+  }
+  return RT_OK;
 }
 
 }  // namespace
@@ -372,15 +625,6 @@ int rt_scene_destroy(rt_scene* s) {
   (void)hipFree(s->d_prim_order);
   (void)hipFree(s->d_nodes);
   (void)hipFree(s->d_keys);
-  (void)hipFree(s->d_repkeys);
-  (void)hipFree(s->d_replist);
-  (void)hipFree(s->d_clist);
-  (void)hipFree(s->d_units);
-  (void)hipFree(s->d_ucount);
-  for (auto& t : s->timed) {
-    (void)hipEventDestroy(t.a);
-    (void)hipEventDestroy(t.b);
-  }
   delete s;
   return RT_OK;
 }
@@ -395,12 +639,28 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
   const size_t n = (size_t)rays->width * rays->height;
   const RaysDev r = to_dev(rays);
   const bool closest = mode != 2;
+  const HitOut out = closest ? HitOut{hits->id, hits->t, hits->u, hits->v, hits->dpdu, hits->dpdv}
+                             : HitOut{};
+  const uint32_t ntri = s->ntri;
+  const int kind = s->desc.kind;
+  if (kind == RT_ACCEL_RAYHIERARCHY) {
+    // a launch group of one job (an empty scene yields no work units: every ray misses)
+    const std::vector<RhJob> jobs{RhJob{s, r, mode, out, occ}};
+    std::vector<RhGroup> groups;
+    bool changed = false;
+    RhParams prm;
+    prm.per[0] = s->clusters_closest;
+    prm.per[1] = s->clusters_occluded;
+    st = rh_prepare(s->ws, jobs, prm, hs, groups, &changed);
+    if (st) return st;
+    Timer::Slot* t = s->timer.next(mode);
+    RT_TRY(launch_rh_group(groups[0], hs, t ? t->a : nullptr, t ? t->b : nullptr));
+    return RT_OK;
+  }
   if (closest) {
     st = grow(s->d_keys, s->keys_cap, n);
     if (st) return st;
   }
-  const uint32_t ntri = s->ntri;
-  const int kind = s->desc.kind;
   if (ntri == 0 || (kind == RT_ACCEL_RAYBVH && s->nroots == 0)) {
     // empty scene: everything misses
     if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
@@ -412,80 +672,31 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
       if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
       else RT_TRY(launch_fill_u8(occ, 0, n, hs));
     }
-    const int slot = timing_begin(s, mode, hs);
-    RT_TRY(launch_naive_trace(r, mode, s->d_v9, ntri, tchunks, s->d_keys, occ, hs));
-    timing_end(s, slot, hs);
+    RT_TRY(timed_launch(s->timer, mode, hs, [&] {
+      return launch_naive_trace(r, mode, s->d_v9, ntri, tchunks, s->d_keys, occ, hs);
+    }));
   } else {
+    // ray-bvh
+    if (!s->has_bvh) return RT_ERR_STATE;
     RHGeom g;
-    int ge = rh_geometry(rays->width, rays->height, s->desc.chunk_x, s->desc.chunk_y,
-                         s->desc.height, &g);
-    if (ge) return RT_ERR_UNSUPPORTED;
+    if (rh_geometry(rays->width, rays->height, s->desc.chunk_x, s->desc.chunk_y, s->desc.height,
+                    &g))
+      return RT_ERR_UNSUPPORTED;
     st = grow(s->d_nodes, s->nodes_cap, 4 * (size_t)g.nnodes);
     if (st) return st;
     RT_TRY(launch_rh_build(g, r, mode != RT_CLOSEST_PRIMARY, s->d_nodes, hs));
-    if (kind == RT_ACCEL_RAYHIERARCHY) {
-      // LDS-staged blocks hold at most 1024 rays; queue entries pack the triangle id in 26 bits.
-      if (g.rb > 1024 || ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
-      const bool repair = mode == RT_CLOSEST_PRIMARY && s->desc.repair_run > 0;
-      // Work units of at most `per` passing clusters (16 for closest hits, 4 for occlusion by
-      // default): blocks whose culls pass many clusters (wide origin boxes at silhouettes)
-      // become many units that the persistent kernel spreads over the whole GPU.
-      const int per_set = closest ? s->clusters_closest : s->clusters_occluded;
-      const int per = per_set > 0 ? per_set : (closest ? 16 : 4);
-      const size_t max_units = (size_t)g.ntop * ((s->nclusters + per - 1) / per);
-      st = grow(s->d_clist, s->clist_cap, (size_t)g.ntop * s->nclusters);
-      if (st) return st;
-      st = grow(s->d_units, s->units_cap, std::max<size_t>(max_units, 1));
-      if (st) return st;
-      st = grow(s->d_ucount, s->ucount_cap, 1);
-      if (st) return st;
-      RT_TRY(launch_fill_u32(s->d_ucount, 0u, 1, hs));
-      if (repair) {
-        st = grow(s->d_repkeys, s->rep_cap, n);
-        if (st) return st;
-        st = grow(s->d_replist, s->replist_cap, n + 1);
-        if (st) return st;
-        RT_TRY(launch_fill_u32(s->d_replist, 0u, 1, hs));
-        RT_TRY(launch_fill_u32(s->d_repkeys, 0xffffffffu, n, hs));
-      }
-      if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
-      else RT_TRY(launch_fill_u8(occ, 0, n, hs));
-      const SortedTris st{s->d_v9s, s->d_perm, s->d_cbox, ntri, s->nclusters};
-      const RhWork w{s->d_clist, s->d_units, s->d_ucount, per,
-                     (int)std::min<size_t>(std::max<size_t>(max_units, 1), 2048)};
-      RT_TRY(launch_rh_clusters(g, mode, s->desc.node, st, w, s->d_nodes, hs));
-      const int slot = timing_begin(s, mode, hs);  // the dominant kernel
-      RT_TRY(launch_rh_units(g, r, mode, s->desc.node, st, w, s->d_nodes, s->d_keys,
-                             repair ? s->d_repkeys : nullptr, occ, hs));
-      timing_end(s, slot, hs);
-      if (closest) {
-        HitOut out{hits->id, hits->t, hits->u, hits->v, hits->dpdu, hits->dpdv};
-        RT_TRY(launch_finalize(r, s->d_v9, s->d_uv6, s->d_keys, repair ? s->d_repkeys : nullptr,
-                               s->d_replist, out, hs));
-        if (repair)
-          RT_TRY(launch_repair(g, r, s->desc.node, s->d_v9, s->d_uv6, ntri, s->desc.repair_run,
-                               s->d_nodes, s->d_replist, out, hs));
-        return RT_OK;
-      }
-      return RT_OK;
-    }
-    // ray-bvh
-    if (!s->has_bvh) return RT_ERR_STATE;
     const int tchunks = pick_tchunks(g.ntop, s->nroots, (uint32_t)s->tile_roots);
     if (tchunks > 1) {
       if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
       else RT_TRY(launch_fill_u8(occ, 0, n, hs));
     }
-    const int slot = timing_begin(s, mode, hs);
-    RT_TRY(launch_bvh_trace(g, r, mode, s->desc.node, s->d_v9, s->d_bvh, s->d_roots, s->nroots,
-                            s->d_prim_order, s->tile_roots, tchunks, s->d_nodes, s->d_keys, occ,
-                            hs));
-    timing_end(s, slot, hs);
+    RT_TRY(timed_launch(s->timer, mode, hs, [&] {
+      return launch_bvh_trace(g, r, mode, s->desc.node, s->d_v9, s->d_bvh, s->d_roots,
+                              s->nroots, s->d_prim_order, s->tile_roots, tchunks, s->d_nodes,
+                              s->d_keys, occ, hs);
+    }));
   }
-  if (closest) {
-    HitOut out{hits->id, hits->t, hits->u, hits->v, hits->dpdu, hits->dpdv};
-    RT_TRY(launch_finalize(r, s->d_v9, s->d_uv6, s->d_keys, nullptr, nullptr, out, hs));
-  }
+  if (closest) RT_TRY(launch_finalize(r, s->d_v9, s->d_uv6, s->d_keys, out, hs));
   return RT_OK;
 }
 
@@ -526,28 +737,225 @@ int rt_scene_set_param(rt_scene* s, int32_t param, int32_t value) {
 int rt_scene_set_timing(rt_scene* s, int32_t enable) {
   if (!s) return RT_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);
-  s->timing = enable != 0;
-  s->timed_used = 0;
+  s->timer.on = enable != 0;
+  s->timer.used = 0;
   return RT_OK;
 }
 
 int rt_scene_kernel_time(rt_scene* s, int32_t mode, float* total_ms, int32_t* launches) {
   if (!s || !total_ms || !launches) return RT_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);
-  float sum = 0.f;
-  int32_t n = 0;
-  for (size_t i = 0; i < s->timed_used; ++i) {
-    const rt_scene::Timed& t = s->timed[i];
-    if (t.mode != mode) continue;
-    RT_TRY(hipEventSynchronize(t.b));
-    float ms = 0.f;
-    RT_TRY(hipEventElapsedTime(&ms, t.a, t.b));
-    sum += ms;
-    ++n;
-  }
-  *total_ms = sum;
-  *launches = n;
+  return s->timer.total(mode, total_ms, launches);
+}
+
+int rt_batch_create(rt_ctx* ctx, rt_batch** out) {
+  if (!ctx || !out) return RT_ERR_INVALID;
+  *out = nullptr;
+  rt_batch* b = new (std::nothrow) rt_batch();
+  if (!b) return RT_ERR_OOM;
+  b->ctx = ctx;
+  *out = b;
   return RT_OK;
+}
+
+int rt_batch_destroy(rt_batch* b) {
+  if (!b) return RT_OK;
+  (void)hipSetDevice(b->ctx->device);
+  delete b;
+  return RT_OK;
+}
+
+int rt_batch_clear(rt_batch* b) {
+  if (!b) return RT_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->jobs.clear();
+  return RT_OK;
+}
+
+static int batch_add(rt_batch* b, rt_scene* s, const rt_rays* rays, int mode, const rt_hits* hits,
+                     uint8_t* occ) {
+  if (!b || !s) return RT_ERR_INVALID;
+  if (s->ctx->device != b->ctx->device) return RT_ERR_INVALID;
+  if (s->desc.kind != RT_ACCEL_RAYHIERARCHY) return RT_ERR_UNSUPPORTED;
+  int st = check_rays(rays, true);
+  if (st) return st;
+  RHGeom g;
+  if (rh_geometry(rays->width, rays->height, s->desc.chunk_x, s->desc.chunk_y, s->desc.height, &g))
+    return RT_ERR_UNSUPPORTED;
+  if (g.rb > 1024 || s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(b->mu);
+  const HitOut out = hits ? HitOut{hits->id, hits->t, hits->u, hits->v, hits->dpdu, hits->dpdv}
+                          : HitOut{};
+  b->jobs.push_back(RhJob{s, to_dev(rays), mode, out, occ});
+  return RT_OK;
+}
+
+int rt_batch_add_closest(rt_batch* b, rt_scene* s, const rt_rays* rays, int32_t mode,
+                         const rt_hits* hits) {
+  if (!hits || !hits->id || !hits->t || !hits->u || !hits->v) return RT_ERR_INVALID;
+  if (mode != RT_CLOSEST_PRIMARY && mode != RT_CLOSEST_SECONDARY) return RT_ERR_INVALID;
+  return batch_add(b, s, rays, mode, hits, nullptr);
+}
+
+int rt_batch_add_occluded(rt_batch* b, rt_scene* s, const rt_rays* rays, uint8_t* occluded) {
+  if (!occluded) return RT_ERR_INVALID;
+  return batch_add(b, s, rays, 2, nullptr, occluded);
+}
+
+// Launches every group of the batch on `hs` (eagerly).
+static int batch_launch(rt_batch* b, const std::vector<RhGroup>& groups, hipStream_t hs,
+                        bool timed) {
+  for (const RhGroup& G : groups) {
+    Timer::Slot* t = timed ? b->timer.next(G.mode) : nullptr;
+    RT_TRY(launch_rh_group(G, hs, t ? t->a : nullptr, t ? t->b : nullptr));
+  }
+  return RT_OK;
+}
+
+static int batch_run_locked(rt_batch* b, hipStream_t hs) {
+  RT_TRY(hipSetDevice(b->ctx->device));
+  if (b->jobs.empty()) return RT_OK;
+  std::vector<RhGroup> groups;
+  bool changed = false;
+  int st = rh_prepare(b->ws, b->jobs, b->prm, hs, groups, &changed);
+  if (st) return st;
+  // Graph replay needs a capturable stream and no per-launch timing events.
+  if (!b->use_graph || !hs || b->timer.on) return batch_launch(b, groups, hs, true);
+  if (changed || !b->graph_valid || b->graph_stream != hs) {
+    b->drop_graph();
+    RT_TRY(hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal));
+    st = batch_launch(b, groups, hs, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(hs, &g);
+    if (st) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    RT_TRY(e);
+    b->graph = g;
+    RT_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->graph_stream = hs;
+    b->graph_valid = true;
+  }
+  RT_TRY(hipGraphLaunch(b->exec, hs));
+  return RT_OK;
+}
+
+int rt_batch_run(rt_batch* b, void* stream) {
+  if (!b) return RT_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->mu);
+  return batch_run_locked(b, (hipStream_t)stream);
+}
+
+static int* batch_param(rt_batch* b, int32_t param) {
+  switch (param) {
+    case RT_PARAM_CLUSTERS_CLOSEST: return &b->prm.per[0];
+    case RT_PARAM_CLUSTERS_OCCLUDED: return &b->prm.per[1];
+    case RT_PARAM_GRID_CLOSEST: return &b->prm.grid[0];
+    case RT_PARAM_GRID_OCCLUDED: return &b->prm.grid[1];
+    default: return nullptr;
+  }
+}
+
+int rt_batch_set_param(rt_batch* b, int32_t param, int32_t value) {
+  if (!b || value < 0) return RT_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (param == RT_PARAM_GRAPH) {
+    b->use_graph = value != 0;
+    if (!b->use_graph) b->drop_graph();
+    return RT_OK;
+  }
+  int* p = batch_param(b, param);
+  if (!p) return RT_ERR_INVALID;
+  if (param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED) {
+    if (value > (1 << 20)) return RT_ERR_INVALID;
+  } else if (value > (1 << 16)) {
+    return RT_ERR_INVALID;
+  }
+  *p = value;
+  b->graph_valid = false;
+  return RT_OK;
+}
+
+int rt_batch_get_param(rt_batch* b, int32_t param, int32_t* value) {
+  if (!b || !value) return RT_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (param == RT_PARAM_GRAPH) {
+    *value = b->use_graph ? 1 : 0;
+    return RT_OK;
+  }
+  int* p = batch_param(b, param);
+  if (!p) return RT_ERR_INVALID;
+  const int mi = (param == RT_PARAM_CLUSTERS_OCCLUDED || param == RT_PARAM_GRID_OCCLUDED) ? 1 : 0;
+  const bool grid = param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED;
+  *value = *p > 0 ? *p : (grid ? kDefaultGrid : kDefaultPer[mi]);
+  return RT_OK;
+}
+
+int rt_batch_autotune(rt_batch* b, void* stream, int32_t reps) {
+  if (!b || reps <= 0) return RT_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->mu);
+  RT_TRY(hipSetDevice(b->ctx->device));
+  hipStream_t hs = (hipStream_t)stream;
+  if (b->jobs.empty()) return RT_OK;
+  static const int kPer[] = {2, 4, 8, 16, 32};
+  static const int kGrid[] = {1024, 2048, 4096};
+  const bool was_on = b->timer.on;
+  const bool was_graph = b->use_graph;
+  b->use_graph = false;
+  b->timer.on = true;
+  float best[2] = {INFINITY, INFINITY};
+  RhParams best_prm = b->prm;
+  int st = RT_OK;
+  for (int gi = 0; gi < 3 && !st; ++gi)
+    for (int pi = 0; pi < 5 && !st; ++pi) {
+      b->prm.per[0] = b->prm.per[1] = kPer[pi];
+      b->prm.grid[0] = b->prm.grid[1] = kGrid[gi];
+      st = batch_run_locked(b, hs);  // warm-up (allocates)
+      b->timer.used = 0;
+      for (int r = 0; r < reps && !st; ++r) st = batch_run_locked(b, hs);
+      if (st) break;
+      for (int m = 0; m < 2; ++m) {
+        float ms = 0.f;
+        int32_t n = 0, n2 = 0;
+        float ms2 = 0.f;
+        if (m == 0) {
+          st = b->timer.total(0, &ms, &n);
+          if (!st) st = b->timer.total(1, &ms2, &n2);
+          ms += ms2;
+          n += n2;
+        } else {
+          st = b->timer.total(2, &ms, &n);
+        }
+        if (st) break;
+        if (n > 0 && ms < best[m]) {
+          best[m] = ms;
+          best_prm.per[m] = kPer[pi];
+          best_prm.grid[m] = kGrid[gi];
+        }
+      }
+      b->timer.used = 0;
+    }
+  b->prm = best_prm;
+  b->timer.on = was_on;
+  b->timer.used = 0;
+  b->use_graph = was_graph;
+  b->graph_valid = false;
+  return st;
+}
+
+int rt_batch_set_timing(rt_batch* b, int32_t enable) {
+  if (!b) return RT_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->timer.on = enable != 0;
+  b->timer.used = 0;
+  return RT_OK;
+}
+
+int rt_batch_kernel_time(rt_batch* b, int32_t mode, float* total_ms, int32_t* launches) {
+  if (!b || !total_ms || !launches) return RT_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->mu);
+  return b->timer.total(mode, total_ms, launches);
 }
 
 const char* rt_error_string(int status) {

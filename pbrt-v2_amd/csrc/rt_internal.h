@@ -9,6 +9,7 @@ constexpr int kMaxLevels = 4;   // leaves per block <= 64 (uint64 masks)
 constexpr int kMaxSub = 85;     // 1 + 4 + 16 + 64 nodes in a block subtree
 constexpr int kThreads = 256;   // work-group size of every traversal kernel
 constexpr int kMaxRpt = 16;     // rays per thread (block of <= 4096 rays)
+constexpr int kClusterSize = 64;  // triangles per Morton cluster
 
 // Ray batch in device memory (structure of arrays, row-major ray image).
 struct RaysDev {
@@ -34,46 +35,58 @@ struct RHGeom {
 // Returns 0 on success, nonzero when the shape is not supported.
 int rh_geometry(int W, int H, int cx, int cy, int height, RHGeom* g);
 
-// Triangles Morton-sorted into clusters of 64 (traversal copy of a scene): v9s[9*slot],
-// perm[slot] = original triangle id, cbox[2*c] / cbox[2*c+1] = cluster AABB min / max.
-struct SortedTris {
-  const float* v9s;
-  const uint32_t* perm;
-  const float4* cbox;
-  uint32_t ntri, nclusters;
-};
-constexpr int kClusterSize = 64;
-
-// Work queue of the rayhierarchy traversal: passing clusters per block (clist, ntop x
-// nclusters), units {block, first, count, 0} of at most per_unit passing clusters, the unit
-// count (zeroed before the cluster pass) and the grid of the persistent unit kernel.
-struct RhWork {
-  uint32_t* clist;
-  uint4* units;
-  uint32_t* unit_count;
-  int per_unit;
-  int grid;
-};
-
 struct HitOut {
   int32_t* id;
   float *t, *u, *v, *dpdu, *dpdv;
 };
 
+// One rayhierarchy traversal job: a resident scene, a ray image and its outputs, with the
+// job's scratch (hierarchy nodes, passing-cluster lists, reduction keys) in device memory.
+// A launch group holds any number of jobs of one mode and node type; the kernels find their
+// job from a global block or ray index (block_base / ray_base are prefix sums).
+struct RhItem {
+  RHGeom g;
+  RaysDev r;
+  const float* v9;       // original triangle order (finalize, repair)
+  const float* uv6;      // or null
+  const float* v9s;      // Morton-sorted traversal copy, clusters of 64
+  const uint32_t* perm;  // sorted slot -> original triangle id
+  const float4* cbox;    // cluster AABBs (min, max)
+  uint32_t ntri, nclusters;
+  float4* nodes;         // 4 float4 per hierarchy node
+  uint32_t* clist;       // ntop x nclusters passing clusters
+  uint64_t* keys;        // closest: one (t, id) key per ray
+  uint32_t* repkeys;     // primary with repair: nearest repair-region t key per ray, else null
+  uint8_t* occ;          // occlusion flags
+  HitOut out;            // closest-hit outputs
+  uint32_t block_base, ray_base;
+  int32_t repair_run, pad;
+};
+
+// A launch group: `items` is the device copy of the job table; units, counters and the
+// repair list are shared by the group's jobs.
+struct RhGroup {
+  const RhItem* items;
+  int nitems;
+  int mode;  // 0 closest primary, 1 closest secondary, 2 occluded
+  int node;  // 0 box_dir, 1 ia
+  int big;   // some job has more than 256 rays per block or 4 levels
+  uint32_t nblocks, nrays;
+  uint4* units;       // {item, block, first cluster, count}
+  uint32_t* ctr;      // [0] units, [1] next unit to take, [2] rays flagged for repair
+  uint32_t* replist;  // global ray indices flagged for repair (primary with repair only)
+  int per_unit;       // passing clusters per work unit
+  int grid;           // work-groups of the persistent unit kernel
+};
+
 // Launchers (all asynchronous on `s`).
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s);
-hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 hipError_t launch_rh_build(const RHGeom& g, const RaysDev& r, bool use_valid, float4* nodes,
                            hipStream_t s);
-// mode 0 closest primary, 1 closest secondary, 2 occluded
-// rayhierarchy traversal: the cluster pass fills the work queue, the persistent unit kernel
-// walks it (keys / repkeys / occlusion flags must be initialised before).
-hipError_t launch_rh_clusters(const RHGeom& g, int mode, int node_kind, const SortedTris& st,
-                              const RhWork& w, const float4* nodes, hipStream_t s);
-hipError_t launch_rh_units(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
-                           const SortedTris& st, const RhWork& w, const float4* nodes,
-                           uint64_t* keys, uint32_t* repkeys, uint8_t* occ, hipStream_t s);
+// rayhierarchy: init, fused build, cluster pass, persistent unit kernel (bracketed by the
+// events t0 / t1 when they are given) and, for closest hits, finalize and repair.
+hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t t1);
 hipError_t launch_bvh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
                             const float* v9, const float4* bvh, const uint32_t* roots,
                             uint32_t nroots, const uint32_t* prim_order, int tile_roots,
@@ -81,12 +94,9 @@ hipError_t launch_bvh_trace(const RHGeom& g, const RaysDev& r, int mode, int nod
                             hipStream_t s);
 hipError_t launch_naive_trace(const RaysDev& r, int mode, const float* v9, uint32_t ntri,
                               int tchunks, uint64_t* keys, uint8_t* occ, hipStream_t s);
+// Hit records of the naive and ray-bvh paths from the reduction keys.
 hipError_t launch_finalize(const RaysDev& r, const float* v9, const float* uv6,
-                           const uint64_t* keys, const uint32_t* repkeys, uint32_t* replist,
-                           HitOut out, hipStream_t s);
-hipError_t launch_repair(const RHGeom& g, const RaysDev& r, int node_kind, const float* v9,
-                         const float* uv6, uint32_t ntri, int repair_run, const float4* nodes,
-                         const uint32_t* replist, HitOut out, hipStream_t s);
+                           const uint64_t* keys, HitOut out, hipStream_t s);
 hipError_t launch_shadow_rays(const RaysDev& pr, const int32_t* id, const float* t, float lx,
                               float ly, float lz, float* ox, float* oy, float* oz, float* dx,
                               float* dy, float* dz, float* tmin, float* tmax, uint8_t* valid,

@@ -13,6 +13,8 @@
 // surviving (triangle, leaf-mask) pairs are compacted with a wavefront ballot into an LDS
 // list, and every ray then runs the leaf test against the list.  The nearest hit is reduced
 // in registers as a (t, id) key, so the result is deterministic (ties -> smallest id).
+#include <algorithm>
+
 #include "rt_device.h"
 #include "rt_internal.h"
 
@@ -76,13 +78,11 @@ __device__ __forceinline__ void box_combine(float* a, bool& ha, const float* b, 
 // folded into one launch instead of 1 + (height-1) launches with host waits,
 // rayhierarchy.cpp:375-549).
 template <bool USE_VALID>
-__global__ __launch_bounds__(256) void rh_build_kernel(RHGeom g, RaysDev r,
-                                                       float4* __restrict__ nodes) {
-  __shared__ float sBox[kMaxSub][12];
-  __shared__ int sVal[kMaxSub];
+__device__ void build_block(const RHGeom& g, const RaysDev& r, float4* __restrict__ nodes, int b,
+                            float (*sBox)[12], int* sVal) {
   const int tid = threadIdx.x;
   const int top = g.L - 1;
-  const int K = blockIdx.x % g.gw[top], J = blockIdx.x / g.gw[top];
+  const int K = b % g.gw[top], J = b / g.gw[top];
   const int nleaf = g.nlx * g.nlx;
   const int leaf_off = g.nsub - nleaf;
 
@@ -167,6 +167,63 @@ __global__ __launch_bounds__(256) void rh_build_kernel(RHGeom g, RaysDev r,
         node_store(nodes, (size_t)g.off[l] + (size_t)gj * g.gw[l] + gk, v, have);
     }
     __syncthreads();
+  }
+}
+
+template <bool USE_VALID>
+__global__ __launch_bounds__(256) void rh_build_kernel(RHGeom g, RaysDev r,
+                                                       float4* __restrict__ nodes) {
+  __shared__ float sBox[kMaxSub][12];
+  __shared__ int sVal[kMaxSub];
+  build_block<USE_VALID>(g, r, nodes, (int)blockIdx.x, sBox, sVal);
+}
+
+// Job of a global block / ray index in a launch group (uniform when the index is).
+__device__ __forceinline__ int find_block_item(const RhItem* __restrict__ items, int n,
+                                               uint32_t gb) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block_base <= gb) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+__device__ __forceinline__ int find_ray_item(const RhItem* __restrict__ items, int n,
+                                             uint32_t gr) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].ray_base <= gr) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool USE_VALID>
+__global__ __launch_bounds__(256) void rh_group_build_kernel(const RhItem* __restrict__ items,
+                                                             int nitems) {
+  __shared__ float sBox[kMaxSub][12];
+  __shared__ int sVal[kMaxSub];
+  const int k = find_block_item(items, nitems, blockIdx.x);
+  const RhItem& it = items[k];
+  build_block<USE_VALID>(it.g, it.r, it.nodes, (int)(blockIdx.x - it.block_base), sBox, sVal);
+}
+
+// Clears the reduction state of every job: keys, repair keys, occlusion flags, counters.
+template <int MODE>
+__global__ __launch_bounds__(256) void rh_init_kernel(const RhItem* __restrict__ items, int nitems,
+                                                      uint32_t nrays, uint32_t* __restrict__ ctr) {
+  const uint32_t gi = blockIdx.x * kThreads + threadIdx.x;
+  if (gi < 4) ctr[gi] = 0u;
+  if (gi >= nrays) return;
+  const RhItem& it = items[find_ray_item(items, nitems, gi)];
+  const uint32_t i = gi - it.ray_base;
+  if (MODE == 2) {
+    it.occ[i] = 0;
+  } else {
+    it.keys[i] = kMissKey;
+    if (MODE == 0 && it.repkeys) it.repkeys[i] = 0xffffffffu;
   }
 }
 
@@ -455,7 +512,7 @@ __device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restri
 
 // Phases B and C for the triangles queued in sQ0; empties the queue.
 template <int MODE, int NODE, int RBMAX, int LMAX>
-__device__ void expand_queue(const RHGeom& g, const float* __restrict__ v9s,
+__device__ __forceinline__ void expand_queue(const RHGeom& g, const float* __restrict__ v9s,
                              const uint32_t* __restrict__ perm, RhShared<MODE, RBMAX, LMAX>& sh) {
   const int tid = threadIdx.x;
   const int top = g.L - 1;
@@ -506,24 +563,25 @@ __device__ void expand_queue(const RHGeom& g, const float* __restrict__ v9s,
   __syncthreads();
 }
 
-// Cluster pass: one work-group per block tests every cluster against the block root (cluster
-// bound), stores the passing cluster ids in clist[block * nclusters + ...] and appends work
-// units of at most `per_unit` passing clusters to `units` ({block, first, count}).
+// Cluster pass: one work-group per block (of any job of the group) tests every cluster against
+// the block root (cluster bound), stores the passing cluster ids in the job's
+// clist[block * nclusters + ...] and appends work units {job, block, first, count} of at most
+// `per_unit` passing clusters.
 template <int MODE, int NODE>
-__global__ __launch_bounds__(256) void rh_cluster_kernel(RHGeom g, const float4* __restrict__ cbox,
-                                                         uint32_t nclusters,
-                                                         const float4* __restrict__ nodes,
-                                                         uint32_t* __restrict__ clist,
-                                                         uint4* __restrict__ units,
-                                                         uint32_t* __restrict__ unit_count,
+__global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
+                                                         int nitems, uint4* __restrict__ units,
+                                                         uint32_t* __restrict__ ctr,
                                                          int per_unit) {
   __shared__ int sCnt;
   __shared__ uint32_t sBase;
   const int tid = threadIdx.x;
+  const int k = find_block_item(items, nitems, blockIdx.x);
+  const RhItem& it = items[k];
+  const RHGeom& g = it.g;
   const int top = g.L - 1;
-  const int b = blockIdx.x;
+  const int b = (int)(blockIdx.x - it.block_base);
   const int K = b % g.gw[top], J = b / g.gw[top];
-  const float4* p = nodes + 4 * ((size_t)g.off[top] + (size_t)J * g.gw[top] + K);
+  const float4* p = it.nodes + 4 * ((size_t)g.off[top] + (size_t)J * g.gw[top] + K);
   const float4 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3];
   if (a0.w == 0.f) return;  // no traced ray below this root (uniform)
   Node root;
@@ -533,7 +591,9 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(RHGeom g, const float4*
   root.omax = {a3.x, a3.y, a3.z};
   if (tid == 0) sCnt = 0;
   __syncthreads();
-  uint32_t* out = clist + (size_t)b * nclusters;
+  const uint32_t nclusters = it.nclusters;
+  const float4* __restrict__ cbox = it.cbox;
+  uint32_t* out = it.clist + (size_t)b * nclusters;
   for (uint32_t cb = 0; cb < nclusters; cb += kThreads) {
     const uint32_t ci = cb + tid;
     bool ok = false;
@@ -546,38 +606,47 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(RHGeom g, const float4*
   __syncthreads();
   const int cnt = sCnt;
   const int nu = (cnt + per_unit - 1) / per_unit;
-  if (tid == 0 && nu > 0) sBase = atomicAdd(unit_count, (uint32_t)nu);
+  if (tid == 0 && nu > 0) sBase = atomicAdd(&ctr[0], (uint32_t)nu);
   __syncthreads();
   for (int u = tid; u < nu; u += kThreads)
-    units[sBase + u] = make_uint4((uint32_t)b, (uint32_t)(u * per_unit),
-                                  (uint32_t)min(per_unit, cnt - u * per_unit), 0u);
+    units[sBase + u] = make_uint4((uint32_t)k, (uint32_t)b, (uint32_t)(u * per_unit),
+                                  (uint32_t)min(per_unit, cnt - u * per_unit));
 }
 
-// Persistent unit kernel: a fixed grid walks the unit queue.  A unit stages its block's rays
-// and subtree, takes the triangles of its clusters through the per-triangle bound (phase A),
+// Persistent unit kernel: a fixed grid takes units from the group's queue with an atomic
+// counter (work stealing across every job of the group).  A unit stages its block's rays and
+// subtree, takes the triangles of its clusters through the per-triangle bound (phase A),
 // expands the survivors (phase B), runs the verified leaf tests (phase C) and merges the block
-// results into the global per-ray keys / occlusion flags with atomics.
+// results into the job's per-ray keys / occlusion flags with atomics.
 template <int MODE, int NODE, int RBMAX, int LMAX>
-__global__ __launch_bounds__(256) void rh_unit_kernel(
-    RHGeom g, RaysDev r, const float* __restrict__ v9s, const uint32_t* __restrict__ perm,
-    uint32_t ntri, uint32_t nclusters, const float4* __restrict__ nodes,
-    const uint32_t* __restrict__ clist, const uint4* __restrict__ units,
-    const uint32_t* __restrict__ unit_count, uint64_t* __restrict__ keys,
-    uint32_t* __restrict__ repkeys, uint8_t* __restrict__ occ_out) {
+__global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__ items,
+                                                      const uint4* __restrict__ units,
+                                                      uint32_t* __restrict__ ctr) {
   __shared__ RhShared<MODE, RBMAX, LMAX> sh;
+  __shared__ uint32_t sUnit;
   const int tid = threadIdx.x;
-  const int top = g.L - 1;
-  const uint32_t nunits = *unit_count;
-  for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+  const uint32_t nunits = __builtin_amdgcn_readfirstlane(ctr[0]);
+  for (;;) {
+    if (tid == 0) sUnit = atomicAdd(&ctr[1], 1u);
+    __syncthreads();
+    const uint32_t u = __builtin_amdgcn_readfirstlane(sUnit);
+    if (u >= nunits) break;
     const uint4 unit = units[u];
-    const int b = (int)unit.x;
+    const RhItem& it = items[__builtin_amdgcn_readfirstlane(unit.x)];
+    const RHGeom& g = it.g;
+    const RaysDev& r = it.r;
+    const float* __restrict__ v9s = it.v9s;
+    const uint32_t* __restrict__ perm = it.perm;
+    const uint32_t ntri = it.ntri;
+    const int top = g.L - 1;
+    const int b = (int)__builtin_amdgcn_readfirstlane(unit.y);
     const int K = b % g.gw[top], J = b / g.gw[top];
     if (tid < kMaxLevels) sh.sCnt[tid] = 0;
     if (tid == 0) {
       sh.sTraced = 0;
       sh.sOccluded = 0;
     }
-    load_subtree(g, K, J, nodes, sh.sN, sh.sV);
+    load_subtree(g, K, J, it.nodes, sh.sN, sh.sV);
     __syncthreads();
     for (int slot = tid; slot < RBMAX; slot += kThreads) {
       int x = 0, y = 0, leaf = 0;
@@ -586,7 +655,7 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(
         const size_t idx = (size_t)y * g.W + x;
         traced = MODE == 0 || r.valid == nullptr || r.valid[idx] != 0;
         // a ray another unit already found occluded needs no more tests
-        if (MODE == 2 && traced && occ_out[idx]) traced = false;
+        if (MODE == 2 && traced && it.occ[idx]) traced = false;
         sh.sRo[slot] = make_float4(r.ox[idx], r.oy[idx], r.oz[idx], r.tmin[idx]);
         sh.sRd[slot] = make_float4(r.dx[idx], r.dy[idx], r.dz[idx], r.tmax[idx]);
       }
@@ -599,8 +668,8 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(
     __syncthreads();
     if (sh.sV[0] && sh.sTraced > 0) {
       const Node& root = sh.sN[0];
-      const uint32_t* cl = clist + (size_t)b * nclusters + unit.y;
-      const int ncl = (int)unit.z;
+      const uint32_t* cl = it.clist + (size_t)b * it.nclusters + unit.z;
+      const int ncl = (int)unit.w;
       bool done = false;
       for (int k0 = 0; k0 < ncl && !done; k0 += kThreads / kCluster) {
         const int k = k0 + tid / kCluster;
@@ -623,17 +692,17 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(
       if (!slot_xy(g, K, J, slot, x, y, leaf)) continue;
       const size_t idx = (size_t)y * g.W + x;
       if (MODE == 2) {
-        if (sh.sAux[slot]) occ_out[idx] = 1;
+        if (sh.sAux[slot]) it.occ[idx] = 1;
       } else {
-        const unsigned long long k = sh.sKey[slot];
-        if (k != kMissKey) atomicMin((unsigned long long*)&keys[idx], k);
-        if (MODE == 0 && repkeys) {
+        const unsigned long long key = sh.sKey[slot];
+        if (key != kMissKey) atomicMin((unsigned long long*)&it.keys[idx], key);
+        if (MODE == 0 && it.repkeys) {
           const uint32_t rk = sh.sAux[slot];
-          if (rk != kNoKey32) atomicMin(&repkeys[idx], rk);
+          if (rk != kNoKey32) atomicMin(&it.repkeys[idx], rk);
         }
       }
     }
-    __syncthreads();  // LDS is restaged by the next unit
+    __syncthreads();  // LDS (and sUnit) are rewritten by the next unit
   }
 }
 
@@ -911,32 +980,44 @@ __device__ void write_hit(const RaysDev& r, const float* __restrict__ v9,
   }
 }
 
+// Hit record of one ray from its reduction key; returns t (INFINITY on a miss).
+__device__ __forceinline__ float finalize_ray(const RaysDev& r, const float* __restrict__ v9,
+                                              const float* __restrict__ uv6, uint64_t key,
+                                              size_t i, const HitOut& out) {
+  if (key == kMissKey) {
+    write_hit(r, v9, uv6, i, -1, 0.f, out);
+    return __int_as_float(0x7f800000);
+  }
+  const float t = fkey_inv((uint32_t)(key >> 32));
+  write_hit(r, v9, uv6, i, (int32_t)(uint32_t)key, t, out);
+  return t;
+}
+
 __global__ __launch_bounds__(256) void finalize_kernel(RaysDev r, const float* __restrict__ v9,
                                                        const float* __restrict__ uv6,
                                                        const uint64_t* __restrict__ keys,
-                                                       const uint32_t* __restrict__ repkeys,
-                                                       uint32_t* __restrict__ replist,
                                                        HitOut out) {
   const size_t n = (size_t)r.W * r.H;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint64_t key = keys[i];
-  float tstar = __int_as_float(0x7f800000);
-  if (key == kMissKey) {
-    write_hit(r, v9, uv6, i, -1, 0.f, out);
-  } else {
-    tstar = fkey_inv((uint32_t)(key >> 32));
-    write_hit(r, v9, uv6, i, (int32_t)(uint32_t)key, tstar, out);
-  }
-  if (repkeys) {
-    const uint32_t rk = repkeys[i];
-    if (rk != kNoKey32) {
-      const float tr = fkey_inv(rk);
-      if (!(tr > tstar - kEps)) {
-        const uint32_t slot = atomicAdd(&replist[0], 1u);
-        replist[1 + slot] = (uint32_t)i;
-      }
-    }
+  finalize_ray(r, v9, uv6, keys[i], i, out);
+}
+
+// Hit records of every job of a closest-hit group.  Rays whose nearest repair-region
+// candidate lies at or below tHit - EPS are listed for the repair pass (only those can
+// change, see rh_repair_kernel).
+__global__ __launch_bounds__(256) void rh_finalize_kernel(const RhItem* __restrict__ items,
+                                                          int nitems, uint32_t nrays,
+                                                          uint32_t* __restrict__ ctr,
+                                                          uint32_t* __restrict__ replist) {
+  const uint32_t gi = blockIdx.x * kThreads + threadIdx.x;
+  if (gi >= nrays) return;
+  const RhItem& it = items[find_ray_item(items, nitems, gi)];
+  const uint32_t i = gi - it.ray_base;
+  const float tstar = finalize_ray(it.r, it.v9, it.uv6, it.keys[i], i, it.out);
+  if (it.repkeys) {
+    const uint32_t rk = it.repkeys[i];
+    if (rk != kNoKey32 && !(fkey_inv(rk) > tstar - kEps)) replist[atomicAdd(&ctr[2], 1u)] = gi;
   }
 }
 
@@ -944,13 +1025,11 @@ __global__ __launch_bounds__(256) void finalize_kernel(RaysDev r, const float* _
 // increasing triangle id order) for the rays whose outcome it can change: the nearest
 // repair-region candidate lies at or below tHit - EPS (hits at exactly tmin, or ties at t
 // large enough that t - EPS rounds to t).  One work-group per flagged ray.
-__global__ __launch_bounds__(256) void repair_kernel(RHGeom g, RaysDev r, int node_kind,
-                                                     const float* __restrict__ v9,
-                                                     const float* __restrict__ uv6, uint32_t ntri,
-                                                     int repair_run,
-                                                     const float4* __restrict__ nodes,
-                                                     const uint32_t* __restrict__ replist,
-                                                     HitOut out) {
+template <int NODE>
+__global__ __launch_bounds__(256) void rh_repair_kernel(const RhItem* __restrict__ items,
+                                                        int nitems,
+                                                        const uint32_t* __restrict__ ctr,
+                                                        const uint32_t* __restrict__ replist) {
   __shared__ Node sChain[kMaxLevels];
   __shared__ int sWc[kThreads / 64];
   __shared__ float sEt[kThreads];
@@ -958,14 +1037,20 @@ __global__ __launch_bounds__(256) void repair_kernel(RHGeom g, RaysDev r, int no
   __shared__ float sTH;
   __shared__ int sIdx;
   const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
-  const uint32_t count = replist[0];
+  const uint32_t count = ctr[2];
   for (uint32_t w = blockIdx.x; w < count; w += gridDim.x) {
-    const size_t i = replist[1 + w];
+    const uint32_t gi = __builtin_amdgcn_readfirstlane(replist[w]);
+    const RhItem& it = items[find_ray_item(items, nitems, gi)];
+    const RHGeom& g = it.g;
+    const RaysDev& r = it.r;
+    const float* __restrict__ v9 = it.v9;
+    const uint32_t ntri = it.ntri;
+    const size_t i = gi - it.ray_base;
     const int x = (int)(i % (size_t)g.W), y = (int)(i / (size_t)g.W);
     if (tid < g.L) {
       const int l = tid;
       const int gk = x / (g.cx << l), gj = y / (g.cy << l);
-      const float4* p = nodes + 4 * ((size_t)g.off[l] + (size_t)gj * g.gw[l] + gk);
+      const float4* p = it.nodes + 4 * ((size_t)g.off[l] + (size_t)gj * g.gw[l] + gk);
       float4 a = p[0], b = p[1], c = p[2], d = p[3];
       sChain[l].dmin = {a.x, a.y, a.z};
       sChain[l].dmax = {b.x, b.y, b.z};
@@ -973,14 +1058,14 @@ __global__ __launch_bounds__(256) void repair_kernel(RHGeom g, RaysDev r, int no
       sChain[l].omax = {d.x, d.y, d.z};
     }
     if (tid == 0) {
-      sTH = out.t[i];
-      sIdx = out.id[i];
+      sTH = it.out.t[i];
+      sIdx = it.out.id[i];
     }
     __syncthreads();
     const V3 o = {r.ox[i], r.oy[i], r.oz[i]};
     const V3 d = {r.dx[i], r.dy[i], r.dz[i]};
     const float tmin = r.tmin[i];
-    for (int run = 0; run < repair_run; ++run) {
+    for (int run = 0; run < it.repair_run; ++run) {
       for (uint32_t base = 0; base < ntri; base += kThreads) {
         const uint32_t ti = base + tid;
         bool have = false;
@@ -988,7 +1073,7 @@ __global__ __launch_bounds__(256) void repair_kernel(RHGeom g, RaysDev r, int no
         if (ti < ntri) {
           const Tri T = load_tri(v9, ti);
           bool pass = true;
-          if (node_kind == 1) {
+          if (NODE == 1) {
             const V3 bmin = vmin(vmin(T.v1, T.v2), T.v3), bmax = vmax(vmax(T.v1, T.v2), T.v3);
             for (int l = g.L - 1; l >= 0 && pass; --l) pass = cull_ia(sChain[l], bmin, bmax);
           } else {
@@ -1022,7 +1107,7 @@ __global__ __launch_bounds__(256) void repair_kernel(RHGeom g, RaysDev r, int no
         __syncthreads();
       }
     }
-    if (tid == 0) write_hit(r, v9, uv6, i, sIdx, sTH, out);
+    if (tid == 0) write_hit(r, v9, it.uv6, i, sIdx, sTH, it.out);
     __syncthreads();
   }
 }
@@ -1069,28 +1154,41 @@ __global__ void fill_kernel(T* p, T v, size_t n) {
 
 inline dim3 grid1(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
+// Launch sequence of one rayhierarchy group with the mode and node type fixed.
 template <int MODE, int NODE>
-hipError_t rh_cluster_dispatch(const RHGeom& g, const SortedTris& st, const RhWork& w,
-                               const float4* nodes, hipStream_t s) {
-  hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE>), dim3(g.ntop), dim3(kThreads), 0, s, g,
-                     st.cbox, st.nclusters, nodes, w.clist, w.units, w.unit_count, w.per_unit);
-  return hipGetLastError();
-}
-
-template <int MODE, int NODE>
-hipError_t rh_unit_dispatch(const RHGeom& g, const RaysDev& r, const SortedTris& st,
-                            const RhWork& w, const float4* nodes, uint64_t* keys,
-                            uint32_t* repkeys, uint8_t* occ, hipStream_t s) {
-  if (g.rb > 1024) return hipErrorInvalidValue;
-  const dim3 grid(w.grid);
-  if (g.rb <= 256 && g.L <= 3)
-    hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 256, 3>), grid, dim3(kThreads), 0, s, g, r,
-                       st.v9s, st.perm, st.ntri, st.nclusters, nodes, w.clist, w.units,
-                       w.unit_count, keys, repkeys, occ);
+hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+  hipLaunchKernelGGL((rh_init_kernel<MODE>), grid1(std::max<uint32_t>(G.nrays, 4)),
+                     dim3(kThreads), 0, s, G.items, G.nitems, G.nrays, G.ctr);
+  hipLaunchKernelGGL((rh_group_build_kernel<MODE != 0>), dim3(G.nblocks), dim3(kThreads), 0, s,
+                     G.items, G.nitems);
+  hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE>), dim3(G.nblocks), dim3(kThreads), 0, s,
+                     G.items, G.nitems, G.units, G.ctr, G.per_unit);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (t0) {
+    e = hipEventRecord(t0, s);
+    if (e != hipSuccess) return e;
+  }
+  const dim3 grid(std::max(G.grid, 1));
+  if (G.big)
+    hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 1024, 4>), grid, dim3(kThreads), 0, s,
+                       G.items, G.units, G.ctr);
   else
-    hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 1024, 4>), grid, dim3(kThreads), 0, s, g, r,
-                       st.v9s, st.perm, st.ntri, st.nclusters, nodes, w.clist, w.units,
-                       w.unit_count, keys, repkeys, occ);
+    hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 256, 3>), grid, dim3(kThreads), 0, s,
+                       G.items, G.units, G.ctr);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (t1) {
+    e = hipEventRecord(t1, s);
+    if (e != hipSuccess) return e;
+  }
+  if (MODE != 2) {
+    hipLaunchKernelGGL(rh_finalize_kernel, grid1(G.nrays), dim3(kThreads), 0, s, G.items,
+                       G.nitems, G.nrays, G.ctr, G.replist);
+    if (MODE == 0 && G.replist)
+      hipLaunchKernelGGL((rh_repair_kernel<NODE>), dim3(64), dim3(kThreads), 0, s, G.items,
+                         G.nitems, G.ctr, G.replist);
+  }
   return hipGetLastError();
 }
 
@@ -1159,11 +1257,6 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(fill_kernel<uint64_t>, grid1(n), dim3(kThreads), 0, s, p, v, n);
   return hipGetLastError();
 }
-hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
-  if (!n) return hipSuccess;
-  hipLaunchKernelGGL(fill_kernel<uint32_t>, grid1(n), dim3(kThreads), 0, s, p, v, n);
-  return hipGetLastError();
-}
 hipError_t launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(fill_kernel<uint8_t>, grid1(n), dim3(kThreads), 0, s, p, v, n);
@@ -1179,31 +1272,16 @@ hipError_t launch_rh_build(const RHGeom& g, const RaysDev& r, bool use_valid, fl
   return hipGetLastError();
 }
 
-hipError_t launch_rh_clusters(const RHGeom& g, int mode, int node_kind, const SortedTris& st,
-                              const RhWork& w, const float4* nodes, hipStream_t s) {
-#define RT_CALL(M, N) rh_cluster_dispatch<M, N>(g, st, w, nodes, s)
-  if (node_kind == 1) {
-    if (mode == 0) return RT_CALL(0, 1);
-    if (mode == 1) return RT_CALL(1, 1);
+hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+  if (G.nitems <= 0 || G.nrays == 0) return hipSuccess;
+#define RT_CALL(M, N) rh_group_dispatch<M, N>(G, s, t0, t1)
+  if (G.node == 1) {
+    if (G.mode == 0) return RT_CALL(0, 1);
+    if (G.mode == 1) return RT_CALL(1, 1);
     return RT_CALL(2, 1);
   }
-  if (mode == 0) return RT_CALL(0, 0);
-  if (mode == 1) return RT_CALL(1, 0);
-  return RT_CALL(2, 0);
-#undef RT_CALL
-}
-
-hipError_t launch_rh_units(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
-                           const SortedTris& st, const RhWork& w, const float4* nodes,
-                           uint64_t* keys, uint32_t* repkeys, uint8_t* occ, hipStream_t s) {
-#define RT_CALL(M, N) rh_unit_dispatch<M, N>(g, r, st, w, nodes, keys, repkeys, occ, s)
-  if (node_kind == 1) {
-    if (mode == 0) return RT_CALL(0, 1);
-    if (mode == 1) return RT_CALL(1, 1);
-    return RT_CALL(2, 1);
-  }
-  if (mode == 0) return RT_CALL(0, 0);
-  if (mode == 1) return RT_CALL(1, 0);
+  if (G.mode == 0) return RT_CALL(0, 0);
+  if (G.mode == 1) return RT_CALL(1, 0);
   return RT_CALL(2, 0);
 #undef RT_CALL
 }
@@ -1242,19 +1320,9 @@ hipError_t launch_naive_trace(const RaysDev& r, int mode, const float* v9, uint3
 }
 
 hipError_t launch_finalize(const RaysDev& r, const float* v9, const float* uv6,
-                           const uint64_t* keys, const uint32_t* repkeys, uint32_t* replist,
-                           HitOut out, hipStream_t s) {
+                           const uint64_t* keys, HitOut out, hipStream_t s) {
   const size_t n = (size_t)r.W * r.H;
-  hipLaunchKernelGGL(finalize_kernel, grid1(n), dim3(kThreads), 0, s, r, v9, uv6, keys, repkeys,
-                     replist, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_repair(const RHGeom& g, const RaysDev& r, int node_kind, const float* v9,
-                         const float* uv6, uint32_t ntri, int repair_run, const float4* nodes,
-                         const uint32_t* replist, HitOut out, hipStream_t s) {
-  hipLaunchKernelGGL(repair_kernel, dim3(64), dim3(kThreads), 0, s, g, r, node_kind, v9, uv6,
-                     ntri, repair_run, nodes, replist, out);
+  hipLaunchKernelGGL(finalize_kernel, grid1(n), dim3(kThreads), 0, s, r, v9, uv6, keys, out);
   return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@ device and to a CPU checker.  Workloads:
                200 small random triangles; 512x512 camera
   room         closed 10x6x20 box room (12 tris) + 400 Gaussian clusters of 200 triangles
   clusters     Gaussian triangle clusters in [-5,5]^3 (the batched/serving scene generator)
+  uneven_batch 64 cluster scenes of log-uniform size with camera + random shadow rays
 """
 from __future__ import annotations
 
@@ -178,6 +179,28 @@ def subimage(r: dict, x0: int, y0: int, w: int, h: int):
         a = r.get(k)
         out[k] = None if a is None else a.reshape(-1, W)[y0:y0 + h, x0:x0 + w].reshape(-1).copy()
     out["width"], out["height"] = w, h
+    return out
+
+
+def uneven_batch(nscenes: int = 64, seed0: int = 100):
+    """The uneven batch: `nscenes` independent scenes.  Scene i (seed seed0 + i): triangle
+    count log-uniform in [1e3, 3e5], Gaussian clusters of 200 triangles (sigma 0.15) with
+    centres in [-5,5]^3; ray count log-uniform in [1.6e4, 1e6], half camera primaries (a
+    square image, camera at (0,0,-12) looking +z, 50 deg FOV) and half random shadow rays
+    (origins in [-5,5]^3, uniform directions, maxt 10) laid out 256 rays wide.  Image sides
+    are rounded to multiples of 16 so that the default 4x4-leaf, height-3 hierarchy is exactly
+    the reference's shape."""
+    out = []
+    for i in range(nscenes):
+        g = rng(seed0 + i)
+        ntri = int(round(np.exp(g.uniform(np.log(1e3), np.log(3e5)))))
+        nray = int(round(np.exp(g.uniform(np.log(1.6e4), np.log(1e6)))))
+        side = max(16, int(round(np.sqrt(nray / 2) / 16)) * 16)
+        nsh = max(4096, int(round((nray - side * side) / 4096)) * 4096)
+        out.append(dict(v9=cluster_scene(ntri, seed=seed0 + i),
+                        primary=pinhole_rays(side, side, origin=(0.0, 0.0, -12.0), fov_deg=50.0),
+                        shadow=random_rays(nsh, seed=10000 + seed0 + i, lo=-5.0, hi=5.0,
+                                           maxt=10.0, width=256)))
     return out
 
 

@@ -1,0 +1,150 @@
+"""GPU: the batched form (rt_batch_*) against the plain per-scene calls and the oracle.
+
+The batched form must give, bit for bit, the results of running every job alone: the same
+hit ids, t, u, v and occlusion flags.  Covered: scenes of different sizes and node types in
+one batch, closest primary / secondary and occlusion jobs mixed, jobs of the 256-ray and the
+1024-ray block kernels in one group, HIP-graph replay (several replays with eager work in
+between, and re-capture after the jobs change), and the autotuner (which must not change any
+result).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(h):
+    return dict(id=h.id.cpu().numpy(), t=h.t.cpu().numpy(), u=h.u.cpu().numpy(),
+                v=h.v.cpu().numpy())
+
+
+def _same(a, b, label):
+    for k in ("id", "t", "u", "v"):
+        x, y = a[k], b[k]
+        if x.dtype == np.float32:
+            x, y = x.view(np.uint32), y.view(np.uint32)
+        mism = int((x != y).sum())
+        assert mism == 0, f"{label}: {mism} {k} mismatches"
+
+
+def _jobs(rt, scenes, gpu_ctx):
+    """(scene, rays dict, mode) jobs: torus window box_dir, random1k box_dir and ia, a
+    cluster scene with an unstructured ray set, a height-4 hierarchy with 2x4 leaves (512-ray
+    blocks, the large-block kernel)."""
+    torus = scenes.torus_scene()
+    rnd = scenes.random_triangles(1024, seed=1)
+    clus = scenes.cluster_scene(3000, seed=7)
+    sc = {
+        "torus": rt.Scene(gpu_ctx, torus, desc="rayhierarchy"),
+        "rnd": rt.Scene(gpu_ctx, rnd, desc="rayhierarchy"),
+        "rnd_ia": rt.Scene(gpu_ctx, rnd, desc=rt.default_desc("rayhierarchy", node="ia")),
+        "clus": rt.Scene(gpu_ctx, clus, desc="rayhierarchy"),
+        "h4": rt.Scene(gpu_ctx, rnd, desc=rt.default_desc("rayhierarchy", chunk_x=2, chunk_y=4,
+                                                          height=4)),
+    }
+    full = scenes.pinhole_rays(512, 512)
+    win = scenes.subimage(full, 192, 256, 128, 128)
+    win["width"], win["height"] = 128, 128
+    cam = scenes.pinhole_rays(96, 64)
+    rr = scenes.random_rays(64 * 64, seed=5, lo=-1.5, hi=1.5, maxt=np.inf, width=64)
+    rr["valid"] = (np.random.default_rng(9).random(64 * 64) < 0.75).astype(np.uint8)
+    jobs = [
+        ("torus", win, 0), ("rnd", cam, 0), ("rnd_ia", cam, 0), ("clus", rr, 1),
+        ("h4", scenes.pinhole_rays(128, 64), 0), ("rnd", rr, 1),
+    ]
+    return sc, jobs
+
+
+def _shadow_of(scenes, rays, ref, light=(2.0, 2.0, -2.0)):
+    sh = scenes.shadow_rays_np(rays, ref["id"], ref["t"], light)
+    sh["width"], sh["height"] = rays["width"], rays["height"]
+    return sh
+
+
+def test_batch_equals_plain_and_oracle(rt, scenes, orc, gpu_ctx):
+    sc, jobs = _jobs(rt, scenes, gpu_ctx)
+    plain, dev = [], []
+    for name, rays, mode in jobs:
+        d = rt.DeviceRays.from_numpy(rays)
+        dev.append(d)
+        plain.append(_np(sc[name].trace_closest(d, mode)))
+    # shadow jobs from the plain hits
+    shadows = [_shadow_of(scenes, rays, plain[i]) for i, (_, rays, _) in enumerate(jobs)]
+    dsh = [rt.DeviceRays.from_numpy(s) for s in shadows]
+    plain_occ = [sc[name].trace_occluded(dsh[i]).cpu().numpy()
+                 for i, (name, _, _) in enumerate(jobs)]
+
+    b = rt.Batch(gpu_ctx)
+    outs, occs = [], []
+    for i, (name, rays, mode) in enumerate(jobs):
+        outs.append(b.add_closest(sc[name], dev[i], mode))
+        occs.append(b.add_occluded(sc[name], dsh[i]))
+    print(f"[batch] {len(jobs)} closest + {len(jobs)} occlusion jobs", flush=True)
+    b.run()
+    for i, (name, rays, mode) in enumerate(jobs):
+        _same(_np(outs[i]), plain[i], f"job {i} {name} mode {mode}")
+        assert np.array_equal(occs[i].cpu().numpy(), plain_occ[i]), f"job {i} occlusion"
+    # oracle on the small jobs (the torus window and the cluster scene are covered by the
+    # plain-path parity tests)
+    for i in (1, 2, 4, 5):
+        name, rays, mode = jobs[i]
+        p = orc.params("rayhierarchy", node="ia" if name == "rnd_ia" else "box_dir",
+                       cx=2 if name == "h4" else 4, height=4 if name == "h4" else 3)
+        ref = orc.trace(sc[name].v9.reshape(-1, 9), rays, p, mode)
+        assert np.array_equal(plain[i]["id"], ref["id"]), f"job {i} vs oracle"
+        oref = orc.trace(sc[name].v9.reshape(-1, 9), shadows[i], p, orc.OCCLUDED)["occ"]
+        assert np.array_equal(plain_occ[i], oref), f"job {i} occlusion vs oracle"
+    # run again: identical
+    b.run()
+    for i in range(len(jobs)):
+        _same(_np(outs[i]), plain[i], f"rerun job {i}")
+    b.close()
+    for s in sc.values():
+        s.close()
+
+
+def test_batch_graph_replay_and_autotune(rt, scenes, gpu_ctx):
+    import torch
+
+    sc, jobs = _jobs(rt, scenes, gpu_ctx)
+    dev = [rt.DeviceRays.from_numpy(r) for _, r, _ in jobs]
+    ref = [_np(sc[n].trace_closest(dev[i], m)) for i, (n, _, m) in enumerate(jobs)]
+    stream = torch.cuda.Stream()
+    b = rt.Batch(gpu_ctx)
+    b.set_param(rt.RT_PARAM_GRAPH, 1)
+    outs = [b.add_closest(sc[n], dev[i], m) for i, (n, _, m) in enumerate(jobs)]
+    for rep in range(4):
+        for o in outs:  # poison the outputs between replays
+            o.id.fill_(-7)
+            o.t.fill_(-1.0)
+        torch.cuda.synchronize()
+        b.run(stream=stream)
+        stream.synchronize()
+        for i in range(len(jobs)):
+            _same(_np(outs[i]), ref[i], f"graph replay {rep} job {i}")
+        # eager work between replays
+        sc["rnd"].trace_closest(dev[1], 0)
+        torch.cuda.synchronize()
+    print("[batch] graph replays ok", flush=True)
+    # change the job list: re-capture
+    b.clear()
+    outs = [b.add_closest(sc[n], dev[i], m) for i, (n, _, m) in enumerate(jobs) if i % 2 == 0]
+    b.run(stream=stream)
+    stream.synchronize()
+    for k, i in enumerate(range(0, len(jobs), 2)):
+        _same(_np(outs[k]), ref[i], f"re-captured job {i}")
+    # autotune, then results unchanged
+    b.set_param(rt.RT_PARAM_GRAPH, 0)
+    b.autotune(reps=1, stream=stream)
+    stream.synchronize()
+    tuned = [b.get_param(p) for p in (rt.RT_PARAM_CLUSTERS_CLOSEST, rt.RT_PARAM_GRID_CLOSEST)]
+    print(f"[batch] tuned clusters/unit, grid = {tuned}", flush=True)
+    for o in outs:
+        o.id.fill_(-7)
+    b.run(stream=stream)
+    stream.synchronize()
+    for k, i in enumerate(range(0, len(jobs), 2)):
+        _same(_np(outs[k]), ref[i], f"tuned job {i}")
+    b.close()
+    for s in sc.values():
+        s.close()
