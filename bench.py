@@ -52,6 +52,142 @@ def profiled_traffic(workload, kernel):
     return None
 
 
+def bench_uneven(args, rt, scenes, torch):
+    """The uneven batch (64 scenes of log-uniform size, camera + random shadow rays): the
+    plain per-scene form (every scene's closest and occlusion call in turn on one stream)
+    against the batched form (one batch of 128 jobs, graph-replayed), same hits required."""
+    print("[bench] uneven batch: generating 64 scenes", file=sys.stderr, flush=True)
+    jobs = scenes.uneven_batch(64, seed0=100)
+    ctx = rt.Context(0)
+    stream = torch.cuda.Stream()
+    scs, prims, shads = [], [], []
+    for j in jobs:
+        scs.append(rt.Scene(ctx, j["v9"], desc="rayhierarchy"))
+        prims.append(rt.DeviceRays.from_numpy(j["primary"]))
+        shads.append(rt.DeviceRays.from_numpy(j["shadow"]))
+    n_prim = sum(p.count for p in prims)
+    n_shad = sum(int(j["shadow"]["valid"].sum()) for j in jobs)
+    rays_per_step = n_prim + n_shad
+    hits_p = [rt.DeviceHits.empty(p.count) for p in prims]
+    occ_p = [torch.empty(s.count, dtype=torch.uint8, device="cuda") for s in shads]
+
+    def plain_step():
+        for k in range(len(jobs)):
+            scs[k].trace_closest(prims[k], rt.RT_CLOSEST_PRIMARY, hits=hits_p[k], stream=stream)
+            scs[k].trace_occluded(shads[k], out=occ_p[k], stream=stream)
+
+    b = rt.Batch(ctx)
+    hits_b = [b.add_closest(scs[k], prims[k]) for k in range(len(jobs))]
+    occ_b = [b.add_occluded(scs[k], shads[k]) for k in range(len(jobs))]
+    if args.autotune:
+        print("[bench] autotuning the batch", file=sys.stderr, flush=True)
+        b.autotune(reps=1, stream=stream)
+    b.set_param(rt.RT_PARAM_GRAPH, 1)
+
+    def batch_step():
+        b.run(stream=stream)
+
+    def timed(fn, label):
+        for _ in range(args.warmup):
+            fn()
+        stream.synchronize()
+        print(f"[bench] timing {args.steps} {label} steps", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        stream.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    t_plain = timed(plain_step, "plain")
+    t_batch = timed(batch_step, "batched")
+    mism_id = sum(int((hits_b[k].id != hits_p[k].id).sum().item()) for k in range(len(jobs)))
+    mism_t = sum(int((hits_b[k].t.view(torch.int32) != hits_p[k].t.view(torch.int32)).sum().item())
+                 for k in range(len(jobs)))
+    mism_occ = sum(int((occ_b[k] != occ_p[k]).sum().item()) for k in range(len(jobs)))
+    v_plain = rays_per_step / t_plain / 1e6
+    v_batch = rays_per_step / t_batch / 1e6
+    tuned = {name: b.get_param(p) for name, p in (
+        ("clusters_closest", rt.RT_PARAM_CLUSTERS_CLOSEST),
+        ("clusters_occluded", rt.RT_PARAM_CLUSTERS_OCCLUDED),
+        ("grid_closest", rt.RT_PARAM_GRID_CLOSEST), ("grid_occluded", rt.RT_PARAM_GRID_OCCLUDED))}
+    result = {
+        "metric": "Mrays/s batched vs plain per-scene launch, same hits; p50/p99 request latency",
+        "value": round(v_batch, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "uneven_batch_64", "form": "batched (128 jobs, HIP graph)",
+                   "scenes": len(jobs), "triangles": int(sum(j["v9"].shape[0] for j in jobs)),
+                   "primary_rays": n_prim, "shadow_rays": n_shad, "accelerator": "rayhierarchy",
+                   "leaf": "4x4", "height": 3, "node": "box_dir", "launch_shape": tuned},
+        "plain": {"value": round(v_plain, 3), "ms_per_step": round(t_plain * 1e3, 3)},
+        "batched_over_plain": round(v_batch / v_plain, 3),
+        "parity": {"id_mismatches": mism_id, "t_mismatches": mism_t,
+                   "occlusion_mismatches": mism_occ},
+        "roofline": None, "cpu_baseline": None,
+    }
+    b.close()
+    for sc in scs:
+        sc.close()
+    ctx.close()
+    print(json.dumps(result))
+
+
+def bench_serving(rt, scenes, torch, rates, nreq, seed0=200):
+    """Serving form: 16 resident cluster scenes (triangle counts log-uniform in [1e3, 1e6]),
+    Poisson arrivals at each rate, resolutions uniform in {128, 256, 512, 1024} squared,
+    1-4 spp, primary + shadow rays.  Returns p50 / p99 latency and aggregate Mrays/s per
+    rate."""
+    ctx = rt.Context(0)
+    print("[bench] serving: building 16 resident scenes", file=sys.stderr, flush=True)
+    scs = []
+    for i in range(16):
+        g = np.random.Generator(np.random.PCG64(seed0 + i))
+        ntri = int(round(np.exp(g.uniform(np.log(1e3), np.log(1e6)))))
+        scs.append(rt.Scene(ctx, scenes.cluster_scene(ntri, seed=seed0 + i), desc="rayhierarchy"))
+    srv = rt.Server(ctx, scs, max_active=256)
+    points = []
+    try:
+        for rate in rates:
+            g = np.random.Generator(np.random.PCG64(seed0 + int(rate)))
+            gaps = g.exponential(1.0 / rate, size=nreq)
+            reqs = [dict(scene=int(g.integers(0, 16)), width=int(s), height=int(s),
+                         spp=int(g.integers(1, 5)), origin=(0.0, 0.0, -12.0), fov_deg=50.0,
+                         light=(0.0, 8.0, -8.0), seed=k)
+                    for k, s in enumerate(g.choice([128, 256, 512, 1024], size=nreq))]
+            print(f"[bench] serving: {nreq} requests at {rate:g} req/s", file=sys.stderr,
+                  flush=True)
+            t_start = time.perf_counter()
+            due = t_start
+            tickets = []
+            for q, gap in zip(reqs, gaps):
+                due += gap
+                while True:
+                    left = due - time.perf_counter()
+                    if left <= 0:
+                        break
+                    time.sleep(min(left, 0.002))
+                tickets.append(srv.submit(**q, outputs=True))
+            res = [srv.wait(t, timeout_s=300.0) for t in tickets]
+            t_end = time.perf_counter()
+            lat = np.array([r["latency_s"] for r in res]) * 1e3
+            rays = sum(int(r["primary_rays"]) + int((r["id"] >= 0).sum()) for r in res)
+            first = min(r["submit_s"] for r in res)
+            last = max(r["done_s"] for r in res)
+            points.append({"rate_req_s": rate, "requests": nreq,
+                           "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                           "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                           "mrays_s": round(rays / (last - first) / 1e6, 1),
+                           "wall_s": round(t_end - t_start, 2)})
+        ticks, most = srv.counters()
+    finally:
+        srv.close()
+        for sc in scs:
+            sc.close()
+        ctx.close()
+    return {"workload": "serving_16_scenes", "points": points, "ticks": ticks,
+            "largest_tick_jobs": most}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +195,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="torus_rayhierarchy_512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--serve-rates", default="20,100,400",
+                    help="request rates (req/s) of the serving leg; empty to skip it")
+    ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--autotune", action="store_true",
+                    help="uneven_batch_64: autotune the batch's launch shape before timing")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed steps, run one plain device copy of this many MB "
                          "(calibrates profiler byte counters against a kernel of known traffic)")
@@ -72,8 +213,11 @@ def main():
 
     rt = _load("pbrt_v2_amd", ROOT / "pbrt-v2_amd" / "__init__.py", ROOT / "pbrt-v2_amd")
     scenes = _load("pbrt_v2_amd.scenes", ROOT / "pbrt-v2_amd" / "scenes.py")
-    wl = scenes.WORKLOADS[args.workload]
     torch.cuda.set_device(0)
+    if args.workload == "uneven_batch_64":
+        bench_uneven(args, rt, scenes, torch)
+        return
+    wl = scenes.WORKLOADS[args.workload]
     print(f"[bench] workload {args.workload}: building scene", file=sys.stderr, flush=True)
     v9 = scenes.make_scene(wl["scene"])
     rays_np = scenes.pinhole_rays(wl["W"], wl["H"])
@@ -192,6 +336,9 @@ def main():
                          f"pbrt BVHAccel SAH maxnodeprims 4 restated in oracle/"}
     result["roofline"] = roofline
     result["cpu_baseline"] = cpu
+    if args.serve_rates:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        result["serving"] = bench_serving(rt, scenes, torch, rates, args.serve_requests)
     sc.close()
     ctx.close()
     print(json.dumps(result))

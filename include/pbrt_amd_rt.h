@@ -193,6 +193,47 @@ int rt_batch_autotune(rt_batch *batch, void *stream, int32_t reps);
 int rt_batch_set_timing(rt_batch *batch, int32_t enable);
 int rt_batch_kernel_time(rt_batch *batch, int32_t mode, float *total_ms, int32_t *launches);
 
+/* Serving form: a request loop over resident scenes.  Each request renders one pinhole
+ * camera image (width*spp x height rays, the spp samples of a pixel adjacent in x; pixel
+ * centres for spp = 1, hashed jitter from `seed` otherwise) with one shadow ray per hit toward
+ * a point light.  Requests are admitted in arrival order while their device memory fits the
+ * budget and fewer than max_active are in flight; every tick of the loop runs ONE batch that
+ * holds the primary rays of newly admitted requests and the shadow rays of requests whose
+ * primaries finished in the previous tick (continuous batching), so work of requests that
+ * arrive at different times overlaps on the device.  Each request's result is the same
+ * alone and in any mix.  No counterpart in the reference (the breadth-first renderer traces
+ * one image task at a time, breadth-first.cpp:134-153); it is for render services. */
+typedef struct rt_server rt_server;
+typedef struct rt_request {
+  int32_t scene;           /* index into the server's scene list */
+  int32_t width, height, spp;
+  float origin[3];         /* camera position; the camera looks along +z */
+  float fov_deg;           /* vertical field of view */
+  float light[3];          /* point light of the shadow rays */
+  uint32_t seed;           /* jitter seed (spp > 1) */
+  /* optional HOST outputs, width*spp*height entries each (NULL = not copied back) */
+  int32_t *host_id;
+  float *host_t;
+  uint8_t *host_occluded;
+} rt_request;
+typedef struct rt_request_stats {
+  double submit_s, admit_s, done_s; /* steady-clock seconds */
+  uint64_t primary_rays;             /* camera rays; one shadow ray slot per camera ray */
+  int32_t status;                   /* rt_status of the request */
+} rt_request_stats;
+/* scenes: rayhierarchy scenes that stay resident for the server's lifetime.
+ * mem_budget: device bytes the in-flight requests may hold (0 = half the free memory). */
+int rt_server_create(rt_ctx *ctx, rt_scene *const *scenes, int32_t nscenes, uint64_t mem_budget,
+                     int32_t max_active, rt_server **out);
+/* Queues a request (thread-safe, returns at once). */
+int rt_server_submit(rt_server *server, const rt_request *req, uint64_t *ticket);
+/* Waits for a request (timeout_s < 0: no limit); RT_ERR_STATE on time-out. */
+int rt_server_wait(rt_server *server, uint64_t ticket, double timeout_s, rt_request_stats *stats);
+/* Ticks run so far and the largest number of requests one tick carried. */
+int rt_server_counters(rt_server *server, uint64_t *ticks, int32_t *max_tick_requests);
+/* Finishes every queued request, then stops the loop. */
+int rt_server_destroy(rt_server *server);
+
 const char *rt_error_string(int status);
 
 #ifdef __cplusplus

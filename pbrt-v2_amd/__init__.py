@@ -36,6 +36,8 @@ EXPORTS = (
     "rt_batch_create", "rt_batch_destroy", "rt_batch_clear", "rt_batch_add_closest",
     "rt_batch_add_occluded", "rt_batch_run", "rt_batch_set_param", "rt_batch_get_param",
     "rt_batch_autotune", "rt_batch_set_timing", "rt_batch_kernel_time",
+    "rt_server_create", "rt_server_submit", "rt_server_wait", "rt_server_counters",
+    "rt_server_destroy",
 )
 (RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED, RT_PARAM_GRID_CLOSEST,
  RT_PARAM_GRID_OCCLUDED, RT_PARAM_GRAPH) = 1, 2, 3, 4, 5
@@ -65,6 +67,20 @@ GPU_NODE_DTYPE = np.dtype([("ax", "<f4"), ("ay", "<f4"), ("az", "<f4"), ("bx", "
                            ("n_primitives", "<u4")])
 
 _fp = ctypes.c_void_p
+
+
+class RtRequest(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("spp", ctypes.c_int32), ("origin", ctypes.c_float * 3),
+                ("fov_deg", ctypes.c_float), ("light", ctypes.c_float * 3),
+                ("seed", ctypes.c_uint32), ("host_id", ctypes.c_void_p),
+                ("host_t", ctypes.c_void_p), ("host_occluded", ctypes.c_void_p)]
+
+
+class RtRequestStats(ctypes.Structure):
+    _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("primary_rays", ctypes.c_uint64),
+                ("status", ctypes.c_int32)]
 
 
 class RtRays(ctypes.Structure):
@@ -123,6 +139,13 @@ def lib() -> ctypes.CDLL:
     L.rt_batch_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.rt_batch_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_float),
                                        P(ctypes.c_int32)]
+    L.rt_server_create.argtypes = [ctypes.c_void_p, P(ctypes.c_void_p), ctypes.c_int32,
+                                   ctypes.c_uint64, ctypes.c_int32, P(ctypes.c_void_p)]
+    L.rt_server_submit.argtypes = [ctypes.c_void_p, P(RtRequest), P(ctypes.c_uint64)]
+    L.rt_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double,
+                                 P(RtRequestStats)]
+    L.rt_server_counters.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_int32)]
+    L.rt_server_destroy.argtypes = [ctypes.c_void_p]
     _LIB = L
     return L
 
@@ -388,6 +411,59 @@ class Batch:
             lib().rt_batch_destroy(self.handle)
             self.handle = ctypes.c_void_p()
         self._keep = []
+
+
+class Server:
+    """Serving form: a request loop over resident rayhierarchy scenes with continuous
+    batching.  submit() returns a ticket at once; wait() returns the request's stats and, if
+    outputs were asked for, numpy arrays of hit ids, t and occlusion flags."""
+
+    def __init__(self, ctx: Context, scenes: list, mem_budget: int = 0, max_active: int = 64):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        arr = (ctypes.c_void_p * len(self.scenes))(*[s.handle.value for s in self.scenes])
+        self.handle = ctypes.c_void_p()
+        check(lib().rt_server_create(ctx.handle, arr, len(self.scenes), mem_budget, max_active,
+                                     ctypes.byref(self.handle)), "rt_server_create")
+        self._out = {}
+
+    def submit(self, scene: int, width: int, height: int, spp: int = 1,
+               origin=(0.0, 0.0, -3.0), fov_deg: float = 60.0, light=(2.0, 2.0, -2.0),
+               seed: int = 0, outputs: bool = False) -> int:
+        n = width * spp * height
+        out = None
+        q = RtRequest(scene, width, height, spp, (ctypes.c_float * 3)(*origin), fov_deg,
+                      (ctypes.c_float * 3)(*light), seed, None, None, None)
+        if outputs:
+            out = (np.empty(n, np.int32), np.empty(n, np.float32), np.empty(n, np.uint8))
+            q.host_id, q.host_t, q.host_occluded = (a.ctypes.data for a in out)
+        t = ctypes.c_uint64()
+        check(lib().rt_server_submit(self.handle, ctypes.byref(q), ctypes.byref(t)),
+              "rt_server_submit")
+        self._out[t.value] = out
+        return t.value
+
+    def wait(self, ticket: int, timeout_s: float = -1.0):
+        st = RtRequestStats()
+        check(lib().rt_server_wait(self.handle, ticket, timeout_s, ctypes.byref(st)),
+              "rt_server_wait")
+        out = self._out.pop(ticket, None)
+        stats = dict(submit_s=st.submit_s, admit_s=st.admit_s, done_s=st.done_s,
+                     latency_s=st.done_s - st.submit_s, primary_rays=st.primary_rays)
+        if out is not None:
+            stats.update(id=out[0], t=out[1], occ=out[2])
+        return stats
+
+    def counters(self):
+        t, m = ctypes.c_uint64(), ctypes.c_int32()
+        check(lib().rt_server_counters(self.handle, ctypes.byref(t), ctypes.byref(m)),
+              "rt_server_counters")
+        return t.value, m.value
+
+    def close(self):
+        if self.handle:
+            lib().rt_server_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
 
 
 def make_shadow_rays(primary: DeviceRays, hits: DeviceHits, light, stream=None,

@@ -17,10 +17,6 @@
 
 using namespace rti;
 
-struct rt_ctx {
-  int device;
-};
-
 // HIP-event timing of the traversal kernels: one event pair per launch while enabled.
 struct Timer {
   struct Slot {
@@ -472,6 +468,10 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
 }
 
 }  // namespace
+
+namespace rti {
+uint32_t scene_nclusters(const rt_scene* s) { return s->nclusters; }
+}  // namespace rti
 
 extern "C" {
 

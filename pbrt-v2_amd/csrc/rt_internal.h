@@ -3,7 +3,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct rt_ctx {
+  int device;
+};
+struct rt_scene;
+
 namespace rti {
+
+// Morton clusters of a rayhierarchy scene (0 for other accelerators).
+uint32_t scene_nclusters(const rt_scene* s);
 
 constexpr int kMaxLevels = 4;   // leaves per block <= 64 (uint64 masks)
 constexpr int kMaxSub = 85;     // 1 + 4 + 16 + 64 nodes in a block subtree
@@ -97,6 +105,10 @@ hipError_t launch_naive_trace(const RaysDev& r, int mode, const float* v9, uint3
 // Hit records of the naive and ray-bvh paths from the reduction keys.
 hipError_t launch_finalize(const RaysDev& r, const float* v9, const float* uv6,
                            const uint64_t* keys, HitOut out, hipStream_t s);
+hipError_t launch_camera_rays(int W, int H, int spp, const float origin[3], float tan_half,
+                              float aspect, uint32_t seed, float* ox, float* oy, float* oz,
+                              float* dx, float* dy, float* dz, float* tmin, float* tmax,
+                              hipStream_t s);
 hipError_t launch_shadow_rays(const RaysDev& pr, const int32_t* id, const float* t, float lx,
                               float ly, float lz, float* ox, float* oy, float* oz, float* dx,
                               float* dy, float* dz, float* tmin, float* tmax, uint8_t* valid,

@@ -1146,6 +1146,51 @@ __global__ __launch_bounds__(256) void shadow_kernel(RaysDev pr, const int32_t* 
   valid[i] = 1;
 }
 
+// Pinhole camera rays of a render request: image (W*spp) x H, the spp samples of a pixel
+// adjacent in x (the reference sampler's order); pixel centres for spp == 1, otherwise a
+// counter-based hash of (seed, x, y, sample) gives the jitter.  The FP32 operation order is
+// that of the workloads' numpy camera, so spp == 1 images are identical to it.
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+__global__ __launch_bounds__(256) void camera_kernel(int W, int H, int spp, float cx, float cy,
+                                                     float cz, float tan_half, float aspect,
+                                                     uint32_t seed, float* ox, float* oy,
+                                                     float* oz, float* dx, float* dy, float* dz,
+                                                     float* tmin, float* tmax) {
+  const size_t n = (size_t)W * spp * H;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int row = W * spp;
+  const int y = (int)(i / (size_t)row), c = (int)(i % (size_t)row);
+  const int x = c / spp, k = c % spp;
+  float jx = 0.5f, jy = 0.5f;
+  if (spp > 1) {
+    const uint32_t h = mix32(seed ^ mix32((uint32_t)x * 0x9e3779b9u ^ mix32((uint32_t)y ^
+                                                                          ((uint32_t)k << 20))));
+    jx = (float)(h >> 8) * (1.0f / 16777216.0f);
+    jy = (float)(mix32(h) >> 8) * (1.0f / 16777216.0f);
+  }
+  const float px = ((((float)x + jx) / (float)W) * 2.0f - 1.0f) * tan_half * aspect;
+  const float py = (1.0f - (((float)y + jy) / (float)H) * 2.0f) * tan_half;
+  const float pz = 1.0f;
+  const float l = sqrtf((px * px + py * py) + pz * pz);
+  const float inv = 1.0f / l;
+  ox[i] = cx;
+  oy[i] = cy;
+  oz[i] = cz;
+  dx[i] = px * inv;
+  dy[i] = py * inv;
+  dz[i] = pz * inv;
+  tmin[i] = 0.f;
+  tmax[i] = __int_as_float(0x7f800000);
+}
+
 template <class T>
 __global__ void fill_kernel(T* p, T v, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1323,6 +1368,17 @@ hipError_t launch_finalize(const RaysDev& r, const float* v9, const float* uv6,
                            const uint64_t* keys, HitOut out, hipStream_t s) {
   const size_t n = (size_t)r.W * r.H;
   hipLaunchKernelGGL(finalize_kernel, grid1(n), dim3(kThreads), 0, s, r, v9, uv6, keys, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_camera_rays(int W, int H, int spp, const float origin[3], float tan_half,
+                              float aspect, uint32_t seed, float* ox, float* oy, float* oz,
+                              float* dx, float* dy, float* dz, float* tmin, float* tmax,
+                              hipStream_t s) {
+  const size_t n = (size_t)W * spp * H;
+  hipLaunchKernelGGL(camera_kernel, grid1(n), dim3(kThreads), 0, s, W, H, spp, origin[0],
+                     origin[1], origin[2], tan_half, aspect, seed, ox, oy, oz, dx, dy, dz, tmin,
+                     tmax);
   return hipGetLastError();
 }
 

@@ -1,0 +1,85 @@
+"""GPU: the serving form (rt_server_*): device camera rays, continuous batching, and every
+result identical alone and in a mix of concurrent requests.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes(rt, scenes, ctx):
+    return [rt.Scene(ctx, scenes.torus_scene(), desc="rayhierarchy"),
+            rt.Scene(ctx, scenes.random_triangles(1024, seed=1), desc="rayhierarchy"),
+            rt.Scene(ctx, scenes.cluster_scene(20000, seed=11), desc="rayhierarchy")]
+
+
+def test_request_matches_plain_trace(rt, scenes, gpu_ctx):
+    """spp = 1: the device camera reproduces the numpy pinhole camera bit for bit, so a
+    served request equals the plain per-scene traces of the same rays."""
+    scs = _scenes(rt, scenes, gpu_ctx)
+    srv = rt.Server(gpu_ctx, scs, max_active=8)
+    try:
+        t = srv.submit(0, 128, 96, 1, origin=(0.0, 0.0, -3.0), fov_deg=60.0,
+                       light=(2.0, 2.0, -2.0), outputs=True)
+        res = srv.wait(t, timeout_s=60.0)
+    finally:
+        srv.close()
+    rays = scenes.pinhole_rays(128, 96)
+    d = rt.DeviceRays.from_numpy(rays)
+    h = scs[0].trace_closest(d, rt.RT_CLOSEST_PRIMARY)
+    ids, ts = h.id.cpu().numpy(), h.t.cpu().numpy()
+    print(f"[serve] hits {int((ids >= 0).sum())} / {ids.size}", flush=True)
+    assert np.array_equal(res["id"], ids)
+    assert np.array_equal(res["t"].view(np.uint32), ts.view(np.uint32))
+    sh = rt.make_shadow_rays(d, h, (2.0, 2.0, -2.0))
+    occ = scs[0].trace_occluded(sh).cpu().numpy()
+    assert np.array_equal(res["occ"], occ)
+    for s in scs:
+        s.close()
+
+
+def test_alone_equals_mix(rt, scenes, gpu_ctx):
+    scs = _scenes(rt, scenes, gpu_ctx)
+    g = np.random.default_rng(5)
+    reqs = []
+    for k in range(16):
+        reqs.append(dict(scene=int(g.integers(0, 3)), width=int(g.choice([32, 64, 128])),
+                         height=int(g.choice([32, 64, 96])), spp=int(g.integers(1, 4)),
+                         origin=(0.0, 0.0, -12.0 if k % 2 else -3.0),
+                         fov_deg=50.0 if k % 2 else 60.0, light=(2.0, 2.0, -2.0), seed=k))
+    srv = rt.Server(gpu_ctx, scs, max_active=64)
+    try:
+        alone = []
+        for q in reqs:  # one at a time
+            alone.append(srv.wait(srv.submit(**q, outputs=True), timeout_s=60.0))
+        t0, m0 = srv.counters()
+        tickets = [srv.submit(**q, outputs=True) for q in reqs]  # all at once
+        mixed = [srv.wait(t, timeout_s=120.0) for t in tickets]
+        t1, m1 = srv.counters()
+    finally:
+        srv.close()
+    print(f"[serve] alone ticks {t0}, mix ticks {t1 - t0}, largest tick {m1} jobs", flush=True)
+    assert m1 > 2, "the mix was not batched"
+    for k, (a, b) in enumerate(zip(alone, mixed)):
+        assert np.array_equal(a["id"], b["id"]), f"request {k}: ids differ alone vs mix"
+        assert np.array_equal(a["t"].view(np.uint32), b["t"].view(np.uint32)), f"request {k} t"
+        assert np.array_equal(a["occ"], b["occ"]), f"request {k}: occlusion differs"
+    for s in scs:
+        s.close()
+
+
+def test_budget_admission(rt, scenes, gpu_ctx):
+    """A budget smaller than two requests admits them one by one; all complete."""
+    scs = _scenes(rt, scenes, gpu_ctx)
+    srv = rt.Server(gpu_ctx, scs, mem_budget=1 << 20, max_active=64)
+    try:
+        tickets = [srv.submit(1, 64, 64, 1, outputs=False) for _ in range(6)]
+        stats = [srv.wait(t, timeout_s=60.0) for t in tickets]
+        ticks, most = srv.counters()
+    finally:
+        srv.close()
+    assert all(s["done_s"] >= s["admit_s"] >= s["submit_s"] for s in stats)
+    assert most <= 2, f"{most} jobs in one tick despite the budget"
+    print(f"[serve] budget run: {ticks} ticks", flush=True)
+    for s in scs:
+        s.close()
