@@ -220,7 +220,9 @@ def main():
     wl = scenes.WORKLOADS[args.workload]
     print(f"[bench] workload {args.workload}: building scene", file=sys.stderr, flush=True)
     v9 = scenes.make_scene(wl["scene"])
-    rays_np = scenes.pinhole_rays(wl["W"], wl["H"])
+    rays_np = scenes.pinhole_rays(wl["W"], wl["H"], origin=wl.get("origin", (0.0, 0.0, -3.0)),
+                                  fov_deg=wl.get("fov", 60.0), spp=wl.get("spp", 1),
+                                  seed=wl.get("seed"))
     light = wl["light"]
     accel = wl["accel"]
 
@@ -283,7 +285,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "form": "plain per-scene", "accelerator": accel,
-                   "triangles": int(len(v9)), "image": f"{wl['W']}x{wl['H']}",
+                   "triangles": int(len(v9)),
+                   "image": f"{wl['W']}x{wl['H']}x{wl.get('spp', 1)}spp",
                    "primary_rays": n, "shadow_rays": n_shadow, "leaf": "4x4", "height": 3,
                    "node": "box_dir"},
         "kernels_ms": {"closest_traversal": round(closest_ms / max(nc, 1), 4),
@@ -296,17 +299,24 @@ def main():
         orc = _load("rt_oracle_py", ROOT / "oracle" / "oracle.py")
         print("[bench] oracle counts + CPU baseline", file=sys.stderr, flush=True)
         p = orc.params(accel)
-        ref = orc.trace(v9, rays_np, p, orc.CLOSEST_PRIMARY)
+        if accel == "ray-bvh":
+            nodes, roots, order = sc.handoff
+            trace = lambda r, m: orc.trace_raybvh(v9, nodes, roots, order, r, p, m)
+        else:
+            trace = lambda r, m: orc.trace(v9, r, p, m)
+        ref = trace(rays_np, orc.CLOSEST_PRIMARY)
         sh_np = scenes.shadow_rays_np(rays_np, ref["id"], ref["t"], light)
-        oref = orc.trace(v9, sh_np, p, orc.OCCLUDED)
+        oref = trace(sh_np, orc.OCCLUDED)
         result["parity"] = {"id_mismatches": int((gpu_ids != ref["id"]).sum()),
                             "occlusion_mismatches": int((gpu_occ != oref["occ"]).sum())}
         # Algorithmic bytes per launch (reference candidate set at the reference defaults):
         # rays 32 B + hit 16 B (closest) or 1 B (occlusion), 36 B per (leaf, triangle) pair,
         # 24 B per (hierarchy node, triangle) cull evaluation.
         cs, os_ = ref["stats"], oref["stats"]
-        bytes_closest = n * 48 + cs.leaf_pairs * 36 + cs.cull_tests * 24
-        bytes_occl = n_shadow * 33 + os_.leaf_pairs * 36 + os_.cull_tests * 24
+        # a ray-bvh cull evaluation reads one hierarchy node (24 B) and one GPUNode (32 B)
+        per_cull = 56 if accel == "ray-bvh" else 24
+        bytes_closest = n * 48 + cs.leaf_pairs * 36 + cs.cull_tests * per_cull
+        bytes_occl = n_shadow * 33 + os_.leaf_pairs * 36 + os_.cull_tests * per_cull
         per_c = closest_ms / max(nc, 1)
         per_o = occl_ms / max(no, 1)
         if per_o >= per_c:

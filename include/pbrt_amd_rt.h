@@ -138,6 +138,17 @@ int rt_make_shadow_rays(const rt_rays *primary, const int32_t *id, const float *
                         float *dy, float *dz, float *tmin, float *tmax, uint8_t *valid,
                         void *stream);
 
+/* Bounce rays of the path integrator from closest hits (PathIntegrator::Li's BSDF sampling
+ * step, src/integrators/path.cpp:43-157, with a cosine-weighted Lambertian lobe): o = p,
+ * a direction about the hit triangle's geometric normal turned toward the incoming ray,
+ * tmin = 1e-3*t (rayEpsilon), tmax = INFINITY, valid = (id >= 0 and the input ray valid).
+ * Random numbers are hashed from (seed, ray index, depth), so the rays do not depend on the
+ * launch.  All DEVICE arrays of the input image's size. */
+int rt_make_bounce_rays(rt_scene *scene, const rt_rays *in, const int32_t *id, const float *t,
+                        uint32_t seed, int32_t depth, float *ox, float *oy, float *oz, float *dx,
+                        float *dy, float *dz, float *tmin, float *tmax, uint8_t *valid,
+                        void *stream);
+
 /* Launch-shape parameters (set by the autotuner; 0 restores the built-in choice).  None of
  * them changes any result.
  *   RT_PARAM_CLUSTERS_CLOSEST / RT_PARAM_CLUSTERS_OCCLUDED: passing triangle clusters (64

@@ -37,7 +37,7 @@ EXPORTS = (
     "rt_batch_add_occluded", "rt_batch_run", "rt_batch_set_param", "rt_batch_get_param",
     "rt_batch_autotune", "rt_batch_set_timing", "rt_batch_kernel_time",
     "rt_server_create", "rt_server_submit", "rt_server_wait", "rt_server_counters",
-    "rt_server_destroy",
+    "rt_server_destroy", "rt_make_bounce_rays",
 )
 (RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED, RT_PARAM_GRID_CLOSEST,
  RT_PARAM_GRID_OCCLUDED, RT_PARAM_GRAPH) = 1, 2, 3, 4, 5
@@ -146,6 +146,8 @@ def lib() -> ctypes.CDLL:
                                  P(RtRequestStats)]
     L.rt_server_counters.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_int32)]
     L.rt_server_destroy.argtypes = [ctypes.c_void_p]
+    L.rt_make_bounce_rays.argtypes = [ctypes.c_void_p, P(RtRays), _fp, _fp, ctypes.c_uint32,
+                                      ctypes.c_int32] + [_fp] * 9 + [_fp]
     _LIB = L
     return L
 
@@ -346,6 +348,26 @@ class Scene:
         if self.handle:
             lib().rt_scene_destroy(self.handle)
             self.handle = ctypes.c_void_p()
+
+
+def make_bounce_rays(scene: "Scene", rays: DeviceRays, hits: DeviceHits, seed: int = 0,
+                     depth: int = 0, stream=None, out: DeviceRays | None = None) -> DeviceRays:
+    """Cosine-weighted path bounce rays from closest hits, built on the device."""
+    torch = _torch()
+    n = rays.count
+    dev = rays.ox.device
+    if out is None:
+        f = lambda: torch.empty(n, dtype=torch.float32, device=dev)
+        out = DeviceRays(f(), f(), f(), f(), f(), f(), f(), f(),
+                         torch.empty(n, dtype=torch.uint8, device=dev), rays.width, rays.height)
+    rs = rays.c_struct()
+    check(lib().rt_make_bounce_rays(scene.handle, ctypes.byref(rs), hits.id.data_ptr(),
+                                    hits.t.data_ptr(), seed, depth, out.ox.data_ptr(),
+                                    out.oy.data_ptr(), out.oz.data_ptr(), out.dx.data_ptr(),
+                                    out.dy.data_ptr(), out.dz.data_ptr(), out.tmin.data_ptr(),
+                                    out.tmax.data_ptr(), out.valid.data_ptr(),
+                                    _stream_ptr(stream)), "rt_make_bounce_rays")
+    return out
 
 
 class Batch:

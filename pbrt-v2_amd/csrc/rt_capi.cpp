@@ -725,6 +725,24 @@ int rt_make_shadow_rays(const rt_rays* primary, const int32_t* id, const float* 
   return RT_OK;
 }
 
+int rt_make_bounce_rays(rt_scene* s, const rt_rays* in, const int32_t* id, const float* t,
+                        uint32_t seed, int32_t depth, float* ox, float* oy, float* oz, float* dx,
+                        float* dy, float* dz, float* tmin, float* tmax, uint8_t* valid,
+                        void* stream) {
+  if (!s) return RT_ERR_INVALID;
+  int st = check_rays(in, true);
+  if (st) return st;
+  if (!id || !t || !ox || !oy || !oz || !dx || !dy || !dz || !tmin || !tmax || !valid)
+    return RT_ERR_INVALID;
+  if (s->ntri == 0) {
+    RT_TRY(launch_fill_u8(valid, 0, (size_t)in->width * in->height, (hipStream_t)stream));
+    return RT_OK;
+  }
+  RT_TRY(launch_bounce_rays(to_dev(in), s->d_v9, id, t, seed, depth, ox, oy, oz, dx, dy, dz,
+                            tmin, tmax, valid, (hipStream_t)stream));
+  return RT_OK;
+}
+
 int rt_scene_set_param(rt_scene* s, int32_t param, int32_t value) {
   if (!s || value < 0) return RT_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);

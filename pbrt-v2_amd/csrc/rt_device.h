@@ -209,11 +209,12 @@ __device__ __forceinline__ bool cull_ia(const Node& n, V3 bmin, V3 bmax) {
 }
 
 // Exact conservative prefilters.  The normalised direction of every corner (or vertex) has
-// its squared length between the sums of the per-axis minimum and maximum squares, so with
-// correctly rounded sqrt and division (monotone) the normalised components lie between the
-// bounds computed here from the same FP32 values.  box_logic is monotone under growing the
-// direction box, so a reject here implies a reject of the exact test.  A zero-length corner
-// (normalize returns it unchanged) is left to the exact test.
+// its squared length between the sums of the per-axis minimum and maximum squares, and the
+// exact test's scale factor (correctly rounded sqrt and division, both monotone) lies between
+// the outward-widened hardware reciprocal square roots of those sums, so the normalised
+// components lie between the bounds computed here from the same FP32 values.  box_logic is
+// monotone under growing the direction box, so a reject here implies a reject of the exact
+// test.  A zero-length corner (normalize returns it unchanged) is left to the exact test.
 __device__ __forceinline__ void outer_axis(float a, float b, float inv_lo, float inv_hi, float& lo,
                                            float& hi) {
   const float la = a < 0.f ? a * inv_hi : a * inv_lo, ha = a < 0.f ? a * inv_lo : a * inv_hi;
@@ -221,11 +222,16 @@ __device__ __forceinline__ void outer_axis(float a, float b, float inv_lo, float
   lo = fmn(la, lb);
   hi = fmx(ha, hb);
 }
+// The exact test scales by fl(1/fl(sqrt(l2))), within 2^-23 (relative) of 1/sqrt(l2) and
+// monotone in l2.  The hardware reciprocal square root is within 1 ulp (2^-23) of
+// 1/sqrt(l2); widened by 2^-16 outward it brackets the exact factor over [l2min, l2max]
+// with a wide margin.  Tiny squared lengths (where the hardware may flush) pass.
 __device__ __forceinline__ bool outer_box_logic(V3 lo, V3 hi, V3 mn, V3 mx, const Node& n) {
   const float l2min = (mn.x + mn.y) + mn.z;
-  if (l2min == 0.f) return true;
+  if (!(l2min >= 0x1p-100f)) return true;
   const float l2max = (mx.x + mx.y) + mx.z;
-  const float inv_hi = 1.0f / sqrtf(l2min), inv_lo = 1.0f / sqrtf(l2max);
+  const float inv_hi = __builtin_amdgcn_rsqf(l2min) * (1.0f + 0x1p-16f);
+  const float inv_lo = __builtin_amdgcn_rsqf(l2max) * (1.0f - 0x1p-16f);
   V3 tmin, tmax;
   outer_axis(lo.x, hi.x, inv_lo, inv_hi, tmin.x, tmax.x);
   outer_axis(lo.y, hi.y, inv_lo, inv_hi, tmin.y, tmax.y);

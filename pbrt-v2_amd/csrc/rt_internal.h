@@ -109,6 +109,10 @@ hipError_t launch_camera_rays(int W, int H, int spp, const float origin[3], floa
                               float aspect, uint32_t seed, float* ox, float* oy, float* oz,
                               float* dx, float* dy, float* dz, float* tmin, float* tmax,
                               hipStream_t s);
+hipError_t launch_bounce_rays(const RaysDev& in, const float* v9, const int32_t* id,
+                              const float* t, uint32_t seed, int depth, float* ox, float* oy,
+                              float* oz, float* dx, float* dy, float* dz, float* tmin,
+                              float* tmax, uint8_t* valid, hipStream_t s);
 hipError_t launch_shadow_rays(const RaysDev& pr, const int32_t* id, const float* t, float lx,
                               float ly, float lz, float* ox, float* oy, float* oz, float* dx,
                               float* dy, float* dz, float* tmin, float* tmax, uint8_t* valid,

@@ -458,18 +458,22 @@ __device__ __forceinline__ bool leaf_hit(const RhShared<MODE, RBMAX, LMAX>& sh, 
   return main || (MODE == 0 && repair_region(b1, b2) && t >= o4.w);
 }
 
-// Phase C over `n` (triangle slot, leaf) pairs.
+// Phase C over `n` (triangle slot, leaf) pairs.  Item = (pair, ray of its leaf): consecutive
+// lanes take consecutive ray slots of one pair, so the LDS ray reads are conflict-free and
+// the triangle load is a broadcast.
 template <int MODE, int NODE, int RBMAX, int LMAX>
 __device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restrict__ v9s,
                                           const uint32_t* __restrict__ perm, const uint32_t* q,
                                           int n, RhShared<MODE, RBMAX, LMAX>& sh) {
   const int G = g.g;
   const int total = n * G;
+  const bool pow2 = (G & (G - 1)) == 0;
+  const int lg = __ffs(G) - 1;
   for (int e = threadIdx.x; e < n; e += kThreads) sh.sPair[e] = 0;
   __syncthreads();
   // C1: leaf tests; flag pairs with a passing test
   for (int item = threadIdx.x; item < total; item += kThreads) {
-    const int e = item / G;
+    const int e = pow2 ? item >> lg : item / G;
     const uint32_t P = q[e];
     const int slot = (int)(P & 63u) * G + (item - e * G);
     if (!sh.sRv[slot]) continue;
@@ -490,7 +494,7 @@ __device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restri
   if (!__syncthreads_or(verified)) return;
   // C3: apply the leaf tests of verified pairs
   for (int item = threadIdx.x; item < total; item += kThreads) {
-    const int e = item / G;
+    const int e = pow2 ? item >> lg : item / G;
     if (sh.sPair[e] != 2) continue;
     const uint32_t P = q[e];
     const uint32_t slot_t = P >> 6;
@@ -1191,6 +1195,57 @@ __global__ __launch_bounds__(256) void camera_kernel(int W, int H, int spp, floa
   tmax[i] = __int_as_float(0x7f800000);
 }
 
+// Path-tracing bounce rays from closest hits: origin p = o + d*t, a cosine-weighted direction
+// about the geometric normal of the hit triangle turned toward the incoming ray, tmin =
+// 1e-3*t (pbrt's rayEpsilon), tmax = INFINITY; random numbers hashed from (seed, ray, depth)
+// so the rays do not depend on the launch order.
+__global__ __launch_bounds__(256) void bounce_kernel(RaysDev in, const float* __restrict__ v9,
+                                                     const int32_t* __restrict__ id,
+                                                     const float* __restrict__ tt, uint32_t seed,
+                                                     int depth, float* ox, float* oy, float* oz,
+                                                     float* dx, float* dy, float* dz,
+                                                     float* tmin, float* tmax, uint8_t* valid) {
+  const size_t n = (size_t)in.W * in.H;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t tri = id[i];
+  if (tri < 0 || (in.valid != nullptr && in.valid[i] == 0)) {
+    ox[i] = oy[i] = oz[i] = 0.f;
+    dx[i] = dy[i] = dz[i] = 0.f;
+    tmin[i] = tmax[i] = 0.f;
+    valid[i] = 0;
+    return;
+  }
+  const float t = tt[i];
+  const V3 d = {in.dx[i], in.dy[i], in.dz[i]};
+  const V3 p = {in.ox[i] + d.x * t, in.oy[i] + d.y * t, in.oz[i] + d.z * t};
+  const Tri T = load_tri(v9, (uint32_t)tri);
+  V3 nn = normalize(cross(T.e1, T.e2));
+  if (dot(nn, d) > 0.f) nn = {-nn.x, -nn.y, -nn.z};
+  const uint32_t h = mix32(seed ^ mix32((uint32_t)i * 0x9e3779b9u ^ mix32((uint32_t)depth)));
+  const float u1 = (float)(h >> 8) * (1.0f / 16777216.0f);
+  const float u2 = (float)(mix32(h) >> 8) * (1.0f / 16777216.0f);
+  const float r = sqrtf(u1), phi = 6.28318530717958647692f * u2;
+  const float lx = r * cosf(phi), ly = r * sinf(phi), lz = sqrtf(fmaxf(0.f, 1.f - u1));
+  // orthonormal basis about nn (Duff et al. 2017)
+  const float sgn = nn.z >= 0.f ? 1.f : -1.f;
+  const float a = -1.f / (sgn + nn.z);
+  const float b = nn.x * nn.y * a;
+  const V3 t1 = {1.f + sgn * nn.x * nn.x * a, sgn * b, -sgn * nn.x};
+  const V3 t2 = {b, sgn + nn.y * nn.y * a, -nn.y};
+  const V3 w = {t1.x * lx + t2.x * ly + nn.x * lz, t1.y * lx + t2.y * ly + nn.y * lz,
+                t1.z * lx + t2.z * ly + nn.z * lz};
+  ox[i] = p.x;
+  oy[i] = p.y;
+  oz[i] = p.z;
+  dx[i] = w.x;
+  dy[i] = w.y;
+  dz[i] = w.z;
+  tmin[i] = 1e-3f * t;
+  tmax[i] = __int_as_float(0x7f800000);
+  valid[i] = 1;
+}
+
 template <class T>
 __global__ void fill_kernel(T* p, T v, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1379,6 +1434,16 @@ hipError_t launch_camera_rays(int W, int H, int spp, const float origin[3], floa
   hipLaunchKernelGGL(camera_kernel, grid1(n), dim3(kThreads), 0, s, W, H, spp, origin[0],
                      origin[1], origin[2], tan_half, aspect, seed, ox, oy, oz, dx, dy, dz, tmin,
                      tmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_bounce_rays(const RaysDev& in, const float* v9, const int32_t* id,
+                              const float* t, uint32_t seed, int depth, float* ox, float* oy,
+                              float* oz, float* dx, float* dy, float* dz, float* tmin,
+                              float* tmax, uint8_t* valid, hipStream_t s) {
+  const size_t n = (size_t)in.W * in.H;
+  hipLaunchKernelGGL(bounce_kernel, grid1(n), dim3(kThreads), 0, s, in, v9, id, t, seed, depth,
+                     ox, oy, oz, dx, dy, dz, tmin, tmax, valid);
   return hipGetLastError();
 }
 

@@ -208,6 +208,9 @@ WORKLOADS = {
     "random1k_naive_128": dict(scene="random1k", W=128, H=128, accel="naive", light=(2, 2, -2)),
     "torus_rayhierarchy_512": dict(scene="torus", W=512, H=512, accel="rayhierarchy",
                                    light=(2, 2, -2)),
+    # one task of the 1024x1024 x 4 spp render: 256x256 pixels x 4 spp (1024x256 ray image)
+    "room_raybvh_task": dict(scene="room", W=256, H=256, spp=4, seed=3, accel="ray-bvh",
+                             origin=(0.0, 0.0, -9.0), fov=50.0, light=(1.0, 2.5, -8.0)),
 }
 
 

@@ -230,3 +230,50 @@ def test_torus_512_full_properties(rt, scenes, orc, gpu_ctx):
     occ = sc.trace_occluded(sh).cpu().numpy()
     assert np.all(occ[h1["id"] < 0] == 0)
     sc.close()
+
+
+# ---------------------------------------------------------------- bounce rays (path integrator)
+@pytest.mark.parametrize("accel", ["ray-bvh", "rayhierarchy"])
+def test_bounce_rays_secondary_trace(rt, scenes, orc, gpu_ctx, accel):
+    """Incoherent secondary rays: cosine-weighted bounces from the room window's primary hits,
+    traced in secondary mode (validity from the bounce) and checked against the oracle on the
+    rays read back from the device."""
+    v9 = scenes.room_scene(nclusters=60)
+    full = scenes.pinhole_rays(256, 256, origin=(0.0, 0.0, -9.0), fov_deg=50)
+    rays = scenes.subimage(full, 64, 64, 96, 96)
+    sc = rt.Scene(gpu_ctx, v9, desc=accel)
+    d = _dev(rt, rays)
+    h = sc.trace_closest(d, rt.RT_CLOSEST_PRIMARY)
+    b = rt.make_bounce_rays(sc, d, h, seed=7, depth=0)
+    b2 = rt.make_bounce_rays(sc, d, h, seed=7, depth=0)
+    assert all(torch_equal(x, y) for x, y in zip(
+        (b.dx, b.dy, b.dz, b.valid), (b2.dx, b2.dy, b2.dz, b2.valid))), "not deterministic"
+    br = {k: getattr(b, k).cpu().numpy() for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin",
+                                                   "tmax", "valid")}
+    br["width"], br["height"] = b.width, b.height
+    ids = h.id.cpu().numpy()
+    assert np.array_equal(br["valid"].astype(bool), ids >= 0)
+    ok = ids >= 0
+    dl = np.sqrt(br["dx"] ** 2 + br["dy"] ** 2 + br["dz"] ** 2)[ok]
+    assert np.all(np.abs(dl - 1) < 1e-5)
+    # direction on the side of the incoming ray
+    tri = v9.reshape(-1, 9)[ids[ok]]
+    n = np.cross(tri[:, 3:6] - tri[:, 0:3], tri[:, 6:9] - tri[:, 0:3])
+    din = np.stack([rays["dx"], rays["dy"], rays["dz"]], 1)[ok]
+    flip = np.where(np.sum(n * din, 1) > 0, -1.0, 1.0)[:, None]
+    dout = np.stack([br["dx"], br["dy"], br["dz"]], 1)[ok]
+    assert np.all(np.sum(n * flip * dout, 1) >= -1e-4)
+    hb = _to_np(sc.trace_closest(b, rt.RT_CLOSEST_SECONDARY))
+    p = orc.params(accel)
+    if accel == "ray-bvh":
+        nodes, roots, order = sc.handoff
+        ref = orc.trace_raybvh(v9, nodes, roots, order, br, p, orc.CLOSEST_SECONDARY)
+    else:
+        ref = orc.trace(v9, br, p, orc.CLOSEST_SECONDARY)
+    _assert_closest(hb, ref, f"{accel} bounce")
+    sc.close()
+
+
+def torch_equal(a, b):
+    import torch
+    return bool(torch.equal(a, b))
