@@ -118,15 +118,15 @@ struct rt_scene {
   float4* d_bvh = nullptr;
   uint32_t* d_roots = nullptr;
   uint32_t* d_prim_order = nullptr;
+  uint32_t* d_bvh_parent = nullptr;  // parent of every node below a frontier root
+  uint8_t* d_bvh_depth = nullptr;    // its depth below that root
   uint32_t nroots = 0;
-  int tile_roots = 256;
+  int roots_per_unit = 0;  // 0 = built-in choice
   bool has_bvh = false;
-  // scratch of the naive and ray-bvh paths, grown on demand
-  float4* d_nodes = nullptr;
-  size_t nodes_cap = 0;
+  // scratch of the naive path, grown on demand
   uint64_t* d_keys = nullptr;
   size_t keys_cap = 0;
-  // scratch of the rayhierarchy path (a launch group of one job)
+  // scratch of the rayhierarchy and ray-bvh paths (a launch group of one job)
   RhWorkspace ws;
   Timer timer;
   std::mutex mu;
@@ -303,6 +303,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int kDefaultPer[2] = {16, 4};  // passing clusters per unit: closest, occluded
 constexpr int kDefaultGrid = 2048;       // persistent work-groups (8 per CU)
+constexpr int kDefaultRootsPerUnit = 32;  // passing frontier roots per ray-bvh unit
 
 // Lays out the jobs as launch groups (one per mode and node type, in job order) in the
 // workspace arena, fills the job tables and uploads them when they changed.  `changed` tells
@@ -312,10 +313,11 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
   groups.clear();
   *changed = false;
   // group membership in job order
-  std::vector<std::pair<int, int>> keys;  // (mode, node)
+  std::vector<std::pair<int, int>> keys;  // (mode, node + 2 * is_raybvh)
   std::vector<std::vector<int>> members;
   for (size_t j = 0; j < jobs.size(); ++j) {
-    const std::pair<int, int> k{jobs[j].mode, jobs[j].s->desc.node};
+    const int bvh = jobs[j].s->desc.kind == RT_ACCEL_RAYBVH ? 1 : 0;
+    const std::pair<int, int> k{jobs[j].mode, jobs[j].s->desc.node + 2 * bvh};
     size_t gi = 0;
     while (gi < keys.size() && keys[gi] != k) ++gi;
     if (gi == keys.size()) {
@@ -340,9 +342,13 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
   std::vector<GroupLayout> gl(keys.size());
   for (size_t gi = 0; gi < keys.size(); ++gi) {
     const int mode = keys[gi].first;
+    const bool bvh = keys[gi].second >= 2;
     const int mi = mode == 2 ? 1 : 0;
     GroupLayout& L = gl[gi];
-    L.per = prm.per[mi] > 0 ? prm.per[mi] : kDefaultPer[mi];
+    // work units: passing clusters (rayhierarchy) or passing frontier roots (ray-bvh, at most
+    // 256 so that one unit's first emissions fit the pair queue)
+    L.per = prm.per[mi] > 0 ? prm.per[mi] : (bvh ? kDefaultRootsPerUnit : kDefaultPer[mi]);
+    if (bvh) L.per = std::min(L.per, 256);
     L.max_units = 0;
     L.nblocks = 0;
     L.nrays = 0;
@@ -358,6 +364,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
         return RT_ERR_UNSUPPORTED;
       // LDS-staged blocks hold at most 1024 rays; queue entries pack the triangle slot in 26 bits
       if (it.g.rb > 1024 || jb.s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
+      if (bvh && jb.s->ntri > 0 && !jb.s->has_bvh) return RT_ERR_STATE;
       const size_t n = (size_t)jb.r.W * jb.r.H;
       it.r = jb.r;
       it.v9 = jb.s->d_v9;
@@ -366,10 +373,17 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
       it.perm = jb.s->d_perm;
       it.cbox = jb.s->d_cbox;
       it.ntri = jb.s->ntri;
-      it.nclusters = jb.s->nclusters;
+      it.nclusters = bvh ? (jb.s->ntri ? jb.s->nroots : 0) : jb.s->nclusters;
+      if (bvh) {
+        it.bvh = jb.s->d_bvh;
+        it.roots = jb.s->d_roots;
+        it.prim_order = jb.s->d_prim_order;
+        it.bvh_parent = jb.s->d_bvh_parent;
+        it.bvh_depth = jb.s->d_bvh_depth;
+      }
       it.out = jb.out;
       it.occ = jb.occ;
-      it.repair_run = mode == 0 ? jb.s->desc.repair_run : 0;
+      it.repair_run = (mode == 0 && !bvh) ? jb.s->desc.repair_run : 0;
       it.block_base = (uint32_t)blocks;
       it.ray_base = (uint32_t)rays;
       blocks += (uint64_t)it.g.ntop;
@@ -432,7 +446,8 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     for (int j : members[gi]) table.push_back(items[j]);
     G.nitems = (int)members[gi].size();
     G.mode = keys[gi].first;
-    G.node = keys[gi].second;
+    G.node = keys[gi].second & 1;
+    G.bvh = keys[gi].second >= 2 ? 1 : 0;
     G.big = gl[gi].big ? 1 : 0;
     G.nblocks = gl[gi].nblocks;
     G.nrays = gl[gi].nrays;
@@ -570,44 +585,48 @@ int rt_scene_set_bvh(rt_scene* s, const rt_gpu_node* nodes, uint32_t nnodes, con
   }
   for (uint32_t i = 0; i < s->ntri; ++i)
     if (prim_order[i] >= s->ntri) return RT_ERR_INVALID;
-  // Entries one frontier root can emit (primitives below it) bound the per-tile LDS list.
-  // The device walks each root's subtree with a 32-entry stack: subtrees deeper than 24
-  // levels (the BVHheight limit) are rejected here, as are cycles.
-  uint32_t max_emit = 1;
+  // Every node below a frontier root gets its parent and its depth below that root (the
+  // device rebuilds the exact lockstep cull chain of a hit from them).  Subtrees deeper than
+  // 12 levels, nodes shared by two roots and cycles are rejected: the device traversal's
+  // bounded stack holds 256 + 224 * depth items.
+  if (nnodes >= (1u << 25)) return RT_ERR_UNSUPPORTED;
+  std::vector<uint32_t> parent(nnodes, 0xffffffffu);
+  std::vector<uint8_t> depth(nnodes, 0), seen(nnodes, 0);
   for (uint32_t r = 0; r < nroots; ++r) {
-    if (roots[r] >= nnodes) return RT_ERR_INVALID;
-    std::vector<std::pair<uint32_t, uint32_t>> stack{{roots[r], 0u}};
-    uint32_t emit = 0;
+    if (roots[r] >= nnodes || seen[roots[r]]) return RT_ERR_INVALID;
+    std::vector<uint32_t> stack{roots[r]};
+    seen[roots[r]] = 1;
+    parent[roots[r]] = roots[r];
     while (!stack.empty()) {
-      auto [n, depth] = stack.back();
+      const uint32_t n = stack.back();
       stack.pop_back();
-      if (depth > 24) return RT_ERR_UNSUPPORTED;
-      if (nodes[n].n_primitives) emit += nodes[n].n_primitives;
-      else {
-        stack.push_back({nodes[n].prim_offset, depth + 1});
-        stack.push_back({nodes[n].prim_offset + 1, depth + 1});
+      if (nodes[n].n_primitives) continue;
+      if (depth[n] >= 12) return RT_ERR_UNSUPPORTED;
+      for (uint32_t c = nodes[n].prim_offset; c <= nodes[n].prim_offset + 1; ++c) {
+        if (seen[c]) return RT_ERR_INVALID;
+        seen[c] = 1;
+        parent[c] = n;
+        depth[c] = (uint8_t)(depth[n] + 1);
+        stack.push_back(c);
       }
     }
-    max_emit = std::max(max_emit, emit);
   }
-  if (max_emit > 2048) return RT_ERR_UNSUPPORTED;
   RT_TRY(hipSetDevice(s->ctx->device));
   if (s->d_bvh) (void)hipFree(s->d_bvh);
   if (s->d_roots) (void)hipFree(s->d_roots);
   if (s->d_prim_order) (void)hipFree(s->d_prim_order);
+  if (s->d_bvh_parent) (void)hipFree(s->d_bvh_parent);
+  if (s->d_bvh_depth) (void)hipFree(s->d_bvh_depth);
   s->d_bvh = nullptr;
-  s->d_roots = s->d_prim_order = nullptr;
+  s->d_roots = s->d_prim_order = s->d_bvh_parent = nullptr;
+  s->d_bvh_depth = nullptr;
   s->has_bvh = false;
-  RT_TRY(hipMalloc((void**)&s->d_bvh, sizeof(float4) * packed.size()));
-  RT_TRY(hipMemcpy(s->d_bvh, packed.data(), sizeof(float4) * packed.size(), hipMemcpyHostToDevice));
-  RT_TRY(hipMalloc((void**)&s->d_roots, sizeof(uint32_t) * std::max<uint32_t>(nroots, 1)));
-  if (nroots)
-    RT_TRY(hipMemcpy(s->d_roots, roots, sizeof(uint32_t) * nroots, hipMemcpyHostToDevice));
-  RT_TRY(hipMalloc((void**)&s->d_prim_order, sizeof(uint32_t) * std::max<uint32_t>(s->ntri, 1)));
-  if (s->ntri)
-    RT_TRY(hipMemcpy(s->d_prim_order, prim_order, sizeof(uint32_t) * s->ntri, hipMemcpyHostToDevice));
+  RT_TRY(upload(s->d_bvh, packed.data(), packed.size()));
+  RT_TRY(upload(s->d_roots, roots, nroots));
+  RT_TRY(upload(s->d_prim_order, prim_order, s->ntri));
+  RT_TRY(upload(s->d_bvh_parent, parent.data(), parent.size()));
+  RT_TRY(upload(s->d_bvh_depth, depth.data(), depth.size()));
   s->nroots = nroots;
-  s->tile_roots = (int)std::max<uint32_t>(1, std::min<uint32_t>(256, 2048 / max_emit));
   s->has_bvh = true;
   return RT_OK;
 }
@@ -623,7 +642,8 @@ int rt_scene_destroy(rt_scene* s) {
   (void)hipFree(s->d_bvh);
   (void)hipFree(s->d_roots);
   (void)hipFree(s->d_prim_order);
-  (void)hipFree(s->d_nodes);
+  (void)hipFree(s->d_bvh_parent);
+  (void)hipFree(s->d_bvh_depth);
   (void)hipFree(s->d_keys);
   delete s;
   return RT_OK;
@@ -643,14 +663,15 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
                              : HitOut{};
   const uint32_t ntri = s->ntri;
   const int kind = s->desc.kind;
-  if (kind == RT_ACCEL_RAYHIERARCHY) {
+  if (kind == RT_ACCEL_RAYHIERARCHY || kind == RT_ACCEL_RAYBVH) {
     // a launch group of one job (an empty scene yields no work units: every ray misses)
+    if (kind == RT_ACCEL_RAYBVH && !s->has_bvh && ntri > 0) return RT_ERR_STATE;
     const std::vector<RhJob> jobs{RhJob{s, r, mode, out, occ}};
     std::vector<RhGroup> groups;
     bool changed = false;
     RhParams prm;
-    prm.per[0] = s->clusters_closest;
-    prm.per[1] = s->clusters_occluded;
+    prm.per[0] = kind == RT_ACCEL_RAYBVH ? s->roots_per_unit : s->clusters_closest;
+    prm.per[1] = kind == RT_ACCEL_RAYBVH ? s->roots_per_unit : s->clusters_occluded;
     st = rh_prepare(s->ws, jobs, prm, hs, groups, &changed);
     if (st) return st;
     Timer::Slot* t = s->timer.next(mode);
@@ -661,7 +682,7 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
     st = grow(s->d_keys, s->keys_cap, n);
     if (st) return st;
   }
-  if (ntri == 0 || (kind == RT_ACCEL_RAYBVH && s->nroots == 0)) {
+  if (ntri == 0) {
     // empty scene: everything misses
     if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
     else RT_TRY(launch_fill_u8(occ, 0, n, hs));
@@ -674,26 +695,6 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
     }
     RT_TRY(timed_launch(s->timer, mode, hs, [&] {
       return launch_naive_trace(r, mode, s->d_v9, ntri, tchunks, s->d_keys, occ, hs);
-    }));
-  } else {
-    // ray-bvh
-    if (!s->has_bvh) return RT_ERR_STATE;
-    RHGeom g;
-    if (rh_geometry(rays->width, rays->height, s->desc.chunk_x, s->desc.chunk_y, s->desc.height,
-                    &g))
-      return RT_ERR_UNSUPPORTED;
-    st = grow(s->d_nodes, s->nodes_cap, 4 * (size_t)g.nnodes);
-    if (st) return st;
-    RT_TRY(launch_rh_build(g, r, mode != RT_CLOSEST_PRIMARY, s->d_nodes, hs));
-    const int tchunks = pick_tchunks(g.ntop, s->nroots, (uint32_t)s->tile_roots);
-    if (tchunks > 1) {
-      if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
-      else RT_TRY(launch_fill_u8(occ, 0, n, hs));
-    }
-    RT_TRY(timed_launch(s->timer, mode, hs, [&] {
-      return launch_bvh_trace(g, r, mode, s->desc.node, s->d_v9, s->d_bvh, s->d_roots,
-                              s->nroots, s->d_prim_order, s->tile_roots, tchunks, s->d_nodes,
-                              s->d_keys, occ, hs);
     }));
   }
   if (closest) RT_TRY(launch_finalize(r, s->d_v9, s->d_uv6, s->d_keys, out, hs));
@@ -746,7 +747,10 @@ int rt_make_bounce_rays(rt_scene* s, const rt_rays* in, const int32_t* id, const
 int rt_scene_set_param(rt_scene* s, int32_t param, int32_t value) {
   if (!s || value < 0) return RT_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);
-  if (param == RT_PARAM_CLUSTERS_CLOSEST) s->clusters_closest = value;
+  if (s->desc.kind == RT_ACCEL_RAYBVH &&
+      (param == RT_PARAM_CLUSTERS_CLOSEST || param == RT_PARAM_CLUSTERS_OCCLUDED))
+    s->roots_per_unit = value;
+  else if (param == RT_PARAM_CLUSTERS_CLOSEST) s->clusters_closest = value;
   else if (param == RT_PARAM_CLUSTERS_OCCLUDED) s->clusters_occluded = value;
   else return RT_ERR_INVALID;
   return RT_OK;
@@ -794,7 +798,9 @@ static int batch_add(rt_batch* b, rt_scene* s, const rt_rays* rays, int mode, co
                      uint8_t* occ) {
   if (!b || !s) return RT_ERR_INVALID;
   if (s->ctx->device != b->ctx->device) return RT_ERR_INVALID;
-  if (s->desc.kind != RT_ACCEL_RAYHIERARCHY) return RT_ERR_UNSUPPORTED;
+  if (s->desc.kind != RT_ACCEL_RAYHIERARCHY && s->desc.kind != RT_ACCEL_RAYBVH)
+    return RT_ERR_UNSUPPORTED;
+  if (s->desc.kind == RT_ACCEL_RAYBVH && s->ntri > 0 && !s->has_bvh) return RT_ERR_STATE;
   int st = check_rays(rays, true);
   if (st) return st;
   RHGeom g;

@@ -60,7 +60,15 @@ struct RhItem {
   const float* v9s;      // Morton-sorted traversal copy, clusters of 64
   const uint32_t* perm;  // sorted slot -> original triangle id
   const float4* cbox;    // cluster AABBs (min, max)
-  uint32_t ntri, nclusters;
+  uint32_t ntri;
+  uint32_t nclusters;    // first-pass candidates: clusters (rayhierarchy) or frontier roots
+  // ray-bvh: BVH nodes (2 float4 each), frontier roots, BVH position -> triangle id, and per
+  // node its parent and its depth below its frontier root
+  const float4* bvh;
+  const uint32_t* roots;
+  const uint32_t* prim_order;
+  const uint32_t* bvh_parent;
+  const uint8_t* bvh_depth;
   float4* nodes;         // 4 float4 per hierarchy node
   uint32_t* clist;       // ntop x nclusters passing clusters
   uint64_t* keys;        // closest: one (t, id) key per ray
@@ -79,6 +87,7 @@ struct RhGroup {
   int mode;  // 0 closest primary, 1 closest secondary, 2 occluded
   int node;  // 0 box_dir, 1 ia
   int big;   // some job has more than 256 rays per block or 4 levels
+  int bvh;   // ray-bvh jobs (product traversal with the scene's BVH)
   uint32_t nblocks, nrays;
   uint4* units;       // {item, block, first cluster, count}
   uint32_t* ctr;      // [0] units, [1] next unit to take, [2] rays flagged for repair
@@ -90,19 +99,13 @@ struct RhGroup {
 // Launchers (all asynchronous on `s`).
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
-hipError_t launch_rh_build(const RHGeom& g, const RaysDev& r, bool use_valid, float4* nodes,
-                           hipStream_t s);
-// rayhierarchy: init, fused build, cluster pass, persistent unit kernel (bracketed by the
-// events t0 / t1 when they are given) and, for closest hits, finalize and repair.
+// rayhierarchy / ray-bvh group: init, fused build, cluster (root) pass, persistent unit
+// kernel (bracketed by the events t0 / t1 when they are given) and, for closest hits,
+// finalize and (rayhierarchy) repair.
 hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t t1);
-hipError_t launch_bvh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
-                            const float* v9, const float4* bvh, const uint32_t* roots,
-                            uint32_t nroots, const uint32_t* prim_order, int tile_roots,
-                            int tchunks, const float4* nodes, uint64_t* keys, uint8_t* occ,
-                            hipStream_t s);
 hipError_t launch_naive_trace(const RaysDev& r, int mode, const float* v9, uint32_t ntri,
                               int tchunks, uint64_t* keys, uint8_t* occ, hipStream_t s);
-// Hit records of the naive and ray-bvh paths from the reduction keys.
+// Hit records of the naive path from the reduction keys.
 hipError_t launch_finalize(const RaysDev& r, const float* v9, const float* uv6,
                            const uint64_t* keys, HitOut out, hipStream_t s);
 hipError_t launch_camera_rays(int W, int H, int spp, const float origin[3], float tan_half,

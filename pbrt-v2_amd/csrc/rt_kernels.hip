@@ -25,8 +25,6 @@ namespace rti {
 namespace {
 
 constexpr uint32_t kNoKey32 = 0xffffffffu;
-constexpr int kBvhCap = 2048;  // LDS list capacity of the ray-bvh kernel (entries per tile)
-constexpr int kStack = 32;     // per-thread stack of the ray-bvh product traversal
 
 __device__ __forceinline__ int pow4_off(int s) { return ((1 << (2 * s)) - 1) / 3; }
 
@@ -170,14 +168,6 @@ __device__ void build_block(const RHGeom& g, const RaysDev& r, float4* __restric
   }
 }
 
-template <bool USE_VALID>
-__global__ __launch_bounds__(256) void rh_build_kernel(RHGeom g, RaysDev r,
-                                                       float4* __restrict__ nodes) {
-  __shared__ float sBox[kMaxSub][12];
-  __shared__ int sVal[kMaxSub];
-  build_block<USE_VALID>(g, r, nodes, (int)blockIdx.x, sBox, sVal);
-}
-
 // Job of a global block / ray index in a launch group (uniform when the index is).
 __device__ __forceinline__ int find_block_item(const RhItem* __restrict__ items, int n,
                                                uint32_t gb) {
@@ -249,97 +239,6 @@ __device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J,
     }
     sN[i] = n;
     sV[i] = valid;
-  }
-}
-
-struct RayReg {
-  V3 o, d;
-  float tmin, tmax;
-  int leaf;
-  size_t idx;
-  bool exists, valid;
-};
-
-template <int RPT>
-__device__ __forceinline__ void load_block_rays(const RHGeom& g, const RaysDev& r, int K, int J,
-                                                bool use_valid, RayReg* ray) {
-#pragma unroll
-  for (int i = 0; i < RPT; ++i) {
-    int x, y, leaf = 0;
-    int rl = threadIdx.x + i * kThreads;
-    ray[i].exists = slot_xy(g, K, J, rl, x, y, leaf);
-    ray[i].leaf = leaf;
-    ray[i].valid = false;
-    if (ray[i].exists) {
-      size_t idx = (size_t)y * g.W + x;
-      ray[i].idx = idx;
-      ray[i].valid = !use_valid || r.valid == nullptr || r.valid[idx] != 0;
-      ray[i].o = {r.ox[idx], r.oy[idx], r.oz[idx]};
-      ray[i].d = {r.dx[idx], r.dy[idx], r.dz[idx]};
-      ray[i].tmin = r.tmin[idx];
-      ray[i].tmax = r.tmax[idx];
-    }
-  }
-}
-
-__device__ __forceinline__ void stage_tri(float4 (*sT)[3], int slot, const Tri& T) {
-  sT[slot][0] = make_float4(T.v1.x, T.v1.y, T.v1.z, T.e1.x);
-  sT[slot][1] = make_float4(T.e1.y, T.e1.z, T.e2.x, T.e2.y);
-  sT[slot][2] = make_float4(T.e2.z, 0.f, 0.f, 0.f);
-}
-__device__ __forceinline__ Tri staged_tri(const float4 (*sT)[3], int slot) {
-  float4 a = sT[slot][0], b = sT[slot][1], c = sT[slot][2];
-  Tri T;
-  T.v1 = {a.x, a.y, a.z};
-  T.e1 = {a.w, b.x, b.y};
-  T.e2 = {b.z, b.w, c.x};
-  T.v2 = T.v3 = T.v1;  // unused by the leaf test
-  return T;
-}
-
-// Leaf test of one ray against one candidate triangle (intersectAllLeaves.cl:12-186).
-template <int MODE>
-__device__ __forceinline__ void ray_test(const RayReg& ry, const Tri& T, uint32_t tid,
-                                         uint64_t& best, uint32_t& rkey, bool& occ) {
-  if (MODE != 0 && !ry.valid) return;
-  if (MODE == 2 && occ) return;
-  float t, b1, b2;
-  if (!mt_strict(ry.o, ry.d, T, t, b1, b2)) return;
-  if (MODE == 2) {
-    if (t > ry.tmin && t < ry.tmax) occ = true;
-    return;
-  }
-  if (t > ry.tmin && t < __int_as_float(0x7f800000)) {
-    uint64_t k = hit_key(t, (int32_t)tid);
-    if (k < best) best = k;
-  }
-  if (MODE == 0 && repair_region(b1, b2) && t >= ry.tmin) {
-    uint32_t k = fkey(t);
-    if (k < rkey) rkey = k;
-  }
-}
-
-template <int MODE, int RPT>
-__device__ __forceinline__ void store_results(const RayReg* ray, const uint64_t* best,
-                                              const uint32_t* rkey, const bool* occ,
-                                              uint64_t* __restrict__ keys,
-                                              uint32_t* __restrict__ repkeys,
-                                              uint8_t* __restrict__ occ_out, int tchunks) {
-#pragma unroll
-  for (int i = 0; i < RPT; ++i) {
-    if (!ray[i].exists) continue;
-    size_t idx = ray[i].idx;
-    if (MODE == 2) {
-      if (tchunks == 1) occ_out[idx] = occ[i] ? 1 : 0;
-      else if (occ[i]) occ_out[idx] = 1;
-    } else {
-      if (tchunks == 1) keys[idx] = best[i];
-      else if (best[i] != kMissKey) atomicMin((unsigned long long*)&keys[idx], (unsigned long long)best[i]);
-      if (MODE == 0 && repkeys) {
-        if (tchunks == 1) repkeys[idx] = rkey[i];
-        else if (rkey[i] != kNoKey32) atomicMin(&repkeys[idx], rkey[i]);
-      }
-    }
   }
 }
 
@@ -448,9 +347,9 @@ __device__ __forceinline__ bool exact_chain(const RHGeom& g, const RhShared<MODE
 }
 
 // Leaf test of ray slot `slot` against T; returns whether it would update the ray.
-template <int MODE, int RBMAX, int LMAX>
-__device__ __forceinline__ bool leaf_hit(const RhShared<MODE, RBMAX, LMAX>& sh, int slot, const Tri& T,
-                                         float& t, float& b1, float& b2) {
+template <int MODE, class SH>
+__device__ __forceinline__ bool leaf_hit(const SH& sh, int slot, const Tri& T, float& t, float& b1,
+                                         float& b2) {
   const float4 o4 = sh.sRo[slot], d4 = sh.sRd[slot];
   if (!mt_strict(V3{o4.x, o4.y, o4.z}, V3{d4.x, d4.y, d4.z}, T, t, b1, b2)) return false;
   if (MODE == 2) return t > o4.w && t < d4.w;  // intersectAllLeaves.cl:177-179
@@ -479,7 +378,7 @@ __device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restri
     if (!sh.sRv[slot]) continue;
     if (MODE == 2 && sh.sAux[slot]) continue;
     float t, b1, b2;
-    if (leaf_hit<MODE, RBMAX, LMAX>(sh, slot, load_tri(v9s, P >> 6), t, b1, b2)) sh.sPair[e] = 1;
+    if (leaf_hit<MODE>(sh, slot, load_tri(v9s, P >> 6), t, b1, b2)) sh.sPair[e] = 1;
   }
   __syncthreads();
   // C2: exact cull chain of the flagged pairs
@@ -501,7 +400,7 @@ __device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restri
     const int slot = (int)(P & 63u) * G + (item - e * G);
     if (!sh.sRv[slot]) continue;
     float t, b1, b2;
-    if (!leaf_hit<MODE, RBMAX, LMAX>(sh, slot, load_tri(v9s, slot_t), t, b1, b2)) continue;
+    if (!leaf_hit<MODE>(sh, slot, load_tri(v9s, slot_t), t, b1, b2)) continue;
     const float tmin = sh.sRo[slot].w;
     if (MODE == 2) {
       if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
@@ -711,152 +610,365 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
 }
 
 // ---------------------------------------------------------------------------------------
-// Hierarchy x BVH product traversal of ray-bvh (intersection6DB_BVH.cl, intersection6DB_BVH2.cl,
-// intersectionP6DB_BVH.cl).  Each thread walks one frontier root's BVH subtree in lockstep
-// with the block's hierarchy (both descend while interior) and emits (leaf mask, triangle)
-// entries for every BVH leaf it reaches.
-__device__ __forceinline__ void bvh_node(const float4* __restrict__ bvh, uint32_t n, V3& bmin,
-                                         V3& bmax, uint32_t& off, uint32_t& np) {
-  float4 a = bvh[2 * (size_t)n], b = bvh[2 * (size_t)n + 1];
+// ray-bvh as a launch group (intersection6DB_BVH.cl, intersection6DB_BVH2.cl,
+// intersectionP6DB_BVH.cl).  The reference walks, for every block root and every frontier
+// root of the CPU-built BVH, the product of the block's hierarchy and the root's BVH subtree
+// in lockstep (both descend while interior; a leaf waits for the other side), culling each
+// (hierarchy node, BVH node) pair with intersectsNode / the IA test, and tests the triangles
+// of every (hierarchy leaf, BVH leaf) pair it reaches (only the first one for secondary rays).
+// A (leaf, BVH leaf) pair is reached iff every pair on its unique lockstep path passes, so the
+// same hit-first scheme as the rayhierarchy path applies: the traversal descends with the
+// exact-conservative bound of the cull, runs the leaf tests, and evaluates the exact cull
+// chain only for pairs in which some ray passes its leaf test.
+constexpr int kBvhQCap = 3072;   // traversal stack: 256 + 224 * depth <= 3072 for depth <= 12
+constexpr int kPairCap = 2048;   // (triangle, BVH leaf, hierarchy leaf) pairs awaiting tests
+constexpr int kBvhPop = 32;      // traversal items expanded per round (8 children each)
+
+template <int MODE, int RBMAX, int LMAX>
+struct BvhShared {
+  Node sN[SubSize<LMAX>::value];
+  int sV[SubSize<LMAX>::value];
+  float4 sRo[RBMAX];
+  float4 sRd[RBMAX];
+  unsigned long long sKey[MODE == 2 ? 1 : RBMAX];
+  uint32_t sAux[RBMAX];  // occlusion flag (shadow)
+  uint8_t sRv[RBMAX];
+  uint32_t sT[kBvhQCap];   // traversal items: BVH node << 7 | subtree node index
+  uint32_t sPt[kPairCap];  // pair: original triangle id
+  uint32_t sPn[kPairCap];  // pair: BVH leaf << 6 | hierarchy leaf
+  uint8_t sPair[kPairCap];
+  int sTq, sPq;
+  int sTraced, sOccluded;
+};
+
+__device__ __forceinline__ void bvh_box(const float4* __restrict__ bvh, uint32_t n, V3& bmin,
+                                        V3& bmax, uint32_t& off, uint32_t& np) {
+  const float4 a = bvh[2 * (size_t)n], b = bvh[2 * (size_t)n + 1];
   bmin = {a.x, a.y, a.z};
   bmax = {a.w, b.x, b.y};
   off = __float_as_uint(b.z);
   np = __float_as_uint(b.w);
 }
 
-__device__ __forceinline__ uint64_t children_mask(uint64_t A, int s) {
-  // nodes of sub-level s (side 1<<s) -> their children at sub-level s+1 (side 2<<s)
-  const int pnl = 1 << s, nl = pnl * 2;
-  uint64_t C = 0;
-  while (A) {
-    int b = __ffsll((unsigned long long)A) - 1;
-    A &= A - 1;
-    int dk = b & (pnl - 1), dj = b >> s;
-    for (int a = 0; a < 4; ++a) C |= 1ull << ((2 * dj + (a >> 1)) * nl + 2 * dk + (a & 1));
-  }
-  return C;
+template <int NODE>
+__device__ __forceinline__ bool bvh_bound(const Node& n, V3 bmin, V3 bmax) {
+  if (NODE == 1) return cull_ia(n, bmin, bmax);
+  return prefilter_node6(n, bmin, bmax);
+}
+template <int NODE>
+__device__ __forceinline__ bool bvh_cull(const Node& n, V3 bmin, V3 bmax) {
+  if (NODE == 1) return cull_ia(n, bmin, bmax);
+  return cull_node6(n, bmin, bmax);
 }
 
-template <int MODE, int NODE, int RPT>
-__global__ __launch_bounds__(256) void bvh_trace_kernel(
-    RHGeom g, RaysDev r, const float* __restrict__ v9, const float4* __restrict__ bvh,
-    const uint32_t* __restrict__ roots, uint32_t nroots, const uint32_t* __restrict__ prim_order,
-    uint32_t chunk, int tile_roots, const float4* __restrict__ nodes,
-    uint64_t* __restrict__ keys, uint8_t* __restrict__ occ_out, int tchunks) {
-  __shared__ Node sN[kMaxSub];
-  __shared__ int sV[kMaxSub];
-  __shared__ uint64_t sM[kBvhCap];
-  __shared__ uint32_t sI[kBvhCap];
-  __shared__ uint32_t sCnt[2];
+// Sub-level of subtree node index i (0 root, then 4, 16, 64 nodes).
+__device__ __forceinline__ int sub_level(int i) { return i == 0 ? 0 : i < 5 ? 1 : i < 21 ? 2 : 3; }
 
-  const int tid = threadIdx.x;
+// The exact lockstep chain of the pair (BVH leaf b, hierarchy leaf li): step s visits the
+// hierarchy node of level max(top - s, 0) above li and the BVH node of depth min(s, d) above b.
+template <int NODE, class SH>
+__device__ __forceinline__ bool bvh_chain(const RHGeom& g, const SH& sh, const RhItem& it,
+                                          uint32_t b, int li) {
   const int top = g.L - 1;
-  const int K = blockIdx.x % g.gw[top], J = blockIdx.x / g.gw[top];
-  load_subtree(g, K, J, nodes, sN, sV);
-  if (tid < 2) sCnt[tid] = 0;
-  __syncthreads();
-
-  RayReg ray[RPT];
-  load_block_rays<RPT>(g, r, K, J, MODE != 0, ray);
-  uint64_t best[RPT];
-  uint32_t rkey[RPT];
-  bool occ[RPT];
-#pragma unroll
-  for (int i = 0; i < RPT; ++i) {
-    best[i] = kMissKey;
-    rkey[i] = kNoKey32;
-    occ[i] = false;
+  const int lx = li & (g.nlx - 1), ly = li >> top;
+  const int d = (int)it.bvh_depth[b];
+  uint32_t node = b;
+  int dn = d;
+  for (int st = max(top, d); st >= 0; --st) {
+    const int bd = min(st, d);
+    while (dn > bd) {
+      node = it.bvh_parent[node];
+      --dn;
+    }
+    const int ss = min(st, top);  // subtree sub-level of the hierarchy node
+    const int sh_ = top - ss;
+    const int idx = pow4_off(ss) + (ly >> sh_) * (1 << ss) + (lx >> sh_);
+    V3 bmin, bmax;
+    uint32_t off, np;
+    bvh_box(it.bvh, node, bmin, bmax, off, np);
+    if (!sh.sV[idx] || !bvh_cull<NODE>(sh.sN[idx], bmin, bmax)) return false;
   }
+  return true;
+}
 
-  if (sV[0]) {
-    const uint32_t r0 = blockIdx.y * chunk;
-    const uint32_t r1 = min(nroots, r0 + chunk);
-    int it = 0;
-    for (uint32_t base = r0; base < r1; base += (uint32_t)tile_roots, ++it) {
-      const uint32_t ri = base + tid;
-      if (tid < tile_roots && ri < r1) {
-        uint32_t stN[kStack];
-        int stS[kStack];
-        uint64_t stM[kStack];
-        int sp = 0;
-        stN[0] = roots[ri];
-        stS[0] = 0;
-        stM[0] = 1;
-        sp = 1;
-        while (sp > 0) {
-          --sp;
-          const uint32_t n = stN[sp];
-          const int s = stS[sp];
-          const uint64_t P = stM[sp];
-          V3 bmin, bmax;
-          uint32_t off, np;
-          bvh_node(bvh, n, bmin, bmax, off, np);
-          const int soff = pow4_off(s);
-          uint64_t A = 0, Pw = P;
-          while (Pw) {
-            int b = __ffsll((unsigned long long)Pw) - 1;
-            Pw &= Pw - 1;
-            const Node& nd = sN[soff + b];
-            if (!sV[soff + b]) continue;
-            bool pass = NODE == 1 ? cull_ia(nd, bmin, bmax) : cull_node6(nd, bmin, bmax);
-            if (pass) A |= 1ull << b;
-          }
-          if (!A) continue;
-          const bool rleaf = s == top, bleaf = np > 0;
-          if (rleaf && bleaf) {
-            const uint32_t cnt = MODE == 1 ? 1u : np;
-            for (uint32_t f = 0; f < cnt; ++f) {
-              uint32_t slot = atomicAdd(&sCnt[it & 1], 1u);
-              if (slot < (uint32_t)kBvhCap) {
-                sM[slot] = A;
-                sI[slot] = prim_order[off + f];
-              }
-            }
-          } else if (rleaf) {
-            if (sp + 2 <= kStack) {
-              stN[sp] = off + 1; stS[sp] = s; stM[sp] = A; ++sp;
-              stN[sp] = off; stS[sp] = s; stM[sp] = A; ++sp;
-            }
-          } else {
-            const uint64_t C = children_mask(A, s);
-            if (bleaf) {
-              if (sp + 1 <= kStack) {
-                stN[sp] = n; stS[sp] = s + 1; stM[sp] = C; ++sp;
-              }
-            } else if (sp + 2 <= kStack) {
-              stN[sp] = off + 1; stS[sp] = s + 1; stM[sp] = C; ++sp;
-              stN[sp] = off; stS[sp] = s + 1; stM[sp] = C; ++sp;
-            }
-          }
-        }
-      }
-      __syncthreads();
-      const int n = min((int)sCnt[it & 1], kBvhCap);
-      if (tid == 0) sCnt[(it + 1) & 1] = 0;
-      for (int e = 0; e < n; ++e) {
-        const uint64_t m = sM[e];
-        bool need = false;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) need |= ray[i].exists && ((m >> ray[i].leaf) & 1ull);
-        if (!need) continue;
-        const uint32_t tri = (uint32_t)__builtin_amdgcn_readfirstlane((int)sI[e]);
-        const Tri T = load_tri(v9, tri);
-#pragma unroll
-        for (int i = 0; i < RPT; ++i)
-          if (ray[i].exists && ((m >> ray[i].leaf) & 1ull))
-            ray_test<MODE == 0 ? 1 : MODE>(ray[i], T, tri, best[i], rkey[i], occ[i]);
-      }
+// Leaf tests of the queued pairs (test, exact chain of pairs with a passing test, apply).
+template <int MODE, int NODE, int RBMAX, int LMAX>
+__device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
+                                              BvhShared<MODE, RBMAX, LMAX>& sh) {
+  constexpr int TM = MODE == 0 ? 1 : MODE;  // ray-bvh has no repair pass
+  const int n = sh.sPq;
+  const int G = g.g;
+  const int total = n * G;
+  const bool pow2 = (G & (G - 1)) == 0;
+  const int lg = __ffs(G) - 1;
+  const float* __restrict__ v9 = it.v9;
+  for (int e = threadIdx.x; e < n; e += kThreads) sh.sPair[e] = 0;
+  __syncthreads();
+  for (int item = threadIdx.x; item < total; item += kThreads) {
+    const int e = pow2 ? item >> lg : item / G;
+    const int slot = (int)(sh.sPn[e] & 63u) * G + (item - e * G);
+    if (!sh.sRv[slot]) continue;
+    if (MODE == 2 && sh.sAux[slot]) continue;
+    float t, b1, b2;
+    if (leaf_hit<TM>(sh, slot, load_tri(v9, sh.sPt[e]), t, b1, b2)) sh.sPair[e] = 1;
+  }
+  __syncthreads();
+  int verified = 0;
+  for (int e = threadIdx.x; e < n; e += kThreads) {
+    if (sh.sPair[e] != 1) continue;
+    const bool ok = bvh_chain<NODE>(g, sh, it, sh.sPn[e] >> 6, (int)(sh.sPn[e] & 63u));
+    sh.sPair[e] = ok ? 2 : 0;
+    verified |= ok;
+  }
+  if (__syncthreads_or(verified)) {
+    for (int item = threadIdx.x; item < total; item += kThreads) {
+      const int e = pow2 ? item >> lg : item / G;
+      if (sh.sPair[e] != 2) continue;
+      const int slot = (int)(sh.sPn[e] & 63u) * G + (item - e * G);
+      if (!sh.sRv[slot]) continue;
+      const uint32_t tri = sh.sPt[e];
+      float t, b1, b2;
+      if (!leaf_hit<TM>(sh, slot, load_tri(v9, tri), t, b1, b2)) continue;
       if (MODE == 2) {
-        bool done = true;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) done &= !ray[i].exists || !ray[i].valid || occ[i];
-        if (__syncthreads_and(done)) break;
+        if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
       } else {
-        __syncthreads();
+        atomicMin(&sh.sKey[slot], (unsigned long long)hit_key(t, (int32_t)tri));
       }
     }
   }
-  store_results<MODE == 0 ? 1 : MODE, RPT>(ray, best, rkey, occ, keys, nullptr, occ_out, tchunks);
+  __syncthreads();
+  if (threadIdx.x == 0) sh.sPq = 0;
+  __syncthreads();
+}
+
+// Root pass: one work-group per block (of any job of the group) tests every frontier root's
+// box against the block root (bound), lists the passing roots and appends work units.
+template <int MODE, int NODE>
+__global__ __launch_bounds__(256) void bvh_root_kernel(const RhItem* __restrict__ items,
+                                                       int nitems, uint4* __restrict__ units,
+                                                       uint32_t* __restrict__ ctr, int per_unit) {
+  __shared__ int sCnt;
+  __shared__ uint32_t sBase;
+  const int tid = threadIdx.x;
+  const int k = find_block_item(items, nitems, blockIdx.x);
+  const RhItem& it = items[k];
+  const RHGeom& g = it.g;
+  const int top = g.L - 1;
+  const int b = (int)(blockIdx.x - it.block_base);
+  const int K = b % g.gw[top], J = b / g.gw[top];
+  const float4* p = it.nodes + 4 * ((size_t)g.off[top] + (size_t)J * g.gw[top] + K);
+  const float4 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3];
+  if (a0.w == 0.f) return;
+  Node root;
+  root.dmin = {a0.x, a0.y, a0.z};
+  root.dmax = {a1.x, a1.y, a1.z};
+  root.omin = {a2.x, a2.y, a2.z};
+  root.omax = {a3.x, a3.y, a3.z};
+  if (tid == 0) sCnt = 0;
+  __syncthreads();
+  const uint32_t nr = it.nclusters;
+  uint32_t* out = it.clist + (size_t)b * nr;
+  for (uint32_t base = 0; base < nr; base += kThreads) {
+    const uint32_t ri = base + tid;
+    bool ok = false;
+    uint32_t node = 0;
+    if (ri < nr) {
+      node = it.roots[ri];
+      V3 bmin, bmax;
+      uint32_t off, np;
+      bvh_box(it.bvh, node, bmin, bmax, off, np);
+      ok = bvh_bound<NODE>(root, bmin, bmax);
+    }
+    ballot_append(out, &sCnt, ok, node);
+  }
+  __syncthreads();
+  const int cnt = sCnt;
+  const int nu = (cnt + per_unit - 1) / per_unit;
+  if (tid == 0 && nu > 0) sBase = atomicAdd(&ctr[0], (uint32_t)nu);
+  __syncthreads();
+  for (int u = tid; u < nu; u += kThreads)
+    units[sBase + u] = make_uint4((uint32_t)k, (uint32_t)b, (uint32_t)(u * per_unit),
+                                  (uint32_t)min(per_unit, cnt - u * per_unit));
+}
+
+// Appends the (triangle, BVH leaf, hierarchy leaf) pairs of a reached leaf x leaf pair.
+template <int MODE, class SH>
+__device__ __forceinline__ void bvh_emit(SH& sh, const RhItem& it, uint32_t b, uint32_t off,
+                                         uint32_t np, int li) {
+  const uint32_t cnt = MODE == 1 ? 1u : np;  // secondary rays: first primitive only
+  const int base = atomicAdd(&sh.sPq, (int)cnt);
+  for (uint32_t f = 0; f < cnt; ++f) {
+    if (base + (int)f < kPairCap) {
+      sh.sPt[base + f] = it.prim_order[off + f];
+      sh.sPn[base + f] = (b << 6) | (uint32_t)li;
+    }
+  }
+}
+
+template <int MODE, int NODE, int RBMAX, int LMAX>
+__global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict__ items,
+                                                       const uint4* __restrict__ units,
+                                                       uint32_t* __restrict__ ctr) {
+  __shared__ BvhShared<MODE, RBMAX, LMAX> sh;
+  __shared__ uint32_t sUnit;
+  const int tid = threadIdx.x;
+  const uint32_t nunits = __builtin_amdgcn_readfirstlane(ctr[0]);
+  for (;;) {
+    if (tid == 0) sUnit = atomicAdd(&ctr[1], 1u);
+    __syncthreads();
+    const uint32_t u = __builtin_amdgcn_readfirstlane(sUnit);
+    if (u >= nunits) break;
+    const uint4 unit = units[u];
+    const RhItem& it = items[__builtin_amdgcn_readfirstlane(unit.x)];
+    const RHGeom& g = it.g;
+    const RaysDev& r = it.r;
+    const int top = g.L - 1;
+    const int b = (int)__builtin_amdgcn_readfirstlane(unit.y);
+    const int K = b % g.gw[top], J = b / g.gw[top];
+    if (tid == 0) {
+      sh.sTq = 0;
+      sh.sPq = 0;
+      sh.sTraced = 0;
+      sh.sOccluded = 0;
+    }
+    load_subtree(g, K, J, it.nodes, sh.sN, sh.sV);
+    __syncthreads();
+    for (int slot = tid; slot < RBMAX; slot += kThreads) {
+      int x = 0, y = 0, leaf = 0;
+      bool traced = false;
+      if (slot_xy(g, K, J, slot, x, y, leaf)) {
+        const size_t idx = (size_t)y * g.W + x;
+        traced = MODE == 0 || r.valid == nullptr || r.valid[idx] != 0;
+        if (MODE == 2 && traced && it.occ[idx]) traced = false;
+        sh.sRo[slot] = make_float4(r.ox[idx], r.oy[idx], r.oz[idx], r.tmin[idx]);
+        sh.sRd[slot] = make_float4(r.dx[idx], r.dy[idx], r.dz[idx], r.tmax[idx]);
+      }
+      sh.sRv[slot] = traced ? 1 : 0;
+      if (MODE != 2) sh.sKey[slot] = kMissKey;
+      sh.sAux[slot] = 0u;
+      const uint64_t bal = __ballot(traced);
+      if (__lane_id() == 0 && bal) atomicAdd(&sh.sTraced, __popcll(bal));
+    }
+    __syncthreads();
+    if (sh.sV[0] && sh.sTraced > 0) {
+      // the unit's roots passed the bound against the block root: expand them
+      const uint32_t* rl = it.clist + (size_t)b * it.nclusters + unit.z;
+      const int nroot = (int)unit.w;
+      for (int i = tid; i < nroot; i += kThreads) {
+        const uint32_t node = rl[i];
+        V3 bmin, bmax;
+        uint32_t off, np;
+        bvh_box(it.bvh, node, bmin, bmax, off, np);
+        if (top == 0 && np > 0) {
+          bvh_emit<MODE>(sh, it, node, off, np, 0);
+        } else {
+          const int slot = atomicAdd(&sh.sTq, 1);
+          if (slot < kBvhQCap) sh.sT[slot] = node << 7;
+        }
+      }
+      __syncthreads();
+      for (;;) {
+        const int q = min(sh.sTq, kBvhQCap);
+        if (sh.sPq > kPairCap - 8 * kBvhPop * 4 || (q == 0 && sh.sPq > 0)) {
+          bvh_leaf_rays<MODE, NODE, RBMAX, LMAX>(g, it, sh);
+          if (MODE == 2 && sh.sOccluded >= sh.sTraced) break;
+        }
+        if (q == 0) break;
+        // pop the newest items (bounded stack), room for 8 children each
+        int npop = min(kBvhPop, q);
+        while (npop > 0 && q - npop + 8 * npop > kBvhQCap) --npop;
+        if (npop == 0) {  // cannot happen for subtrees of depth <= 12 (checked on the host)
+          if (tid == 0) atomicOr(&ctr[3], 1u);
+          break;
+        }
+        uint32_t item = 0;
+        const int pi = tid >> 3, c = tid & 7;
+        if (pi < npop) item = sh.sT[q - npop + pi];
+        __syncthreads();
+        if (tid == 0) sh.sTq = q - npop;
+        __syncthreads();
+        bool push = false;
+        uint32_t child = 0;
+        if (pi < npop) {
+          const uint32_t B = item >> 7;
+          const int R = (int)(item & 127u);
+          const int s = sub_level(R);
+          V3 bmin, bmax;
+          uint32_t off, np;
+          bvh_box(it.bvh, B, bmin, bmax, off, np);
+          const bool bleaf = np > 0, rleaf = s == top;
+          int rc = -1;
+          uint32_t bc = B;
+          if (!rleaf && !bleaf) {
+            rc = c >> 1;
+            bc = off + (uint32_t)(c & 1);
+          } else if (!rleaf) {
+            rc = c < 4 ? c : -2;
+          } else if (!bleaf) {
+            rc = c < 2 ? -1 : -2;
+            bc = off + (uint32_t)c;
+          } else {
+            rc = -2;  // leaf x leaf items are emitted, never queued
+          }
+          if (rc != -2) {
+            int Rn = R;
+            if (rc >= 0) {
+              const int pl = R - pow4_off(s), pnl = 1 << s;
+              const int dk = pl & (pnl - 1), dj = pl >> s;
+              Rn = pow4_off(s + 1) + (2 * dj + (rc >> 1)) * (2 * pnl) + 2 * dk + (rc & 1);
+            }
+            if (sh.sV[Rn]) {
+              V3 cmin, cmax;
+              uint32_t coff, cnp;
+              bvh_box(it.bvh, bc, cmin, cmax, coff, cnp);
+              if (bvh_bound<NODE>(sh.sN[Rn], cmin, cmax)) {
+                const int cs = sub_level(Rn);
+                if (cs == top && cnp > 0)
+                  bvh_emit<MODE>(sh, it, bc, coff, cnp, Rn - pow4_off(top));
+                else {
+                  push = true;
+                  child = (bc << 7) | (uint32_t)Rn;
+                }
+              }
+            }
+          }
+        }
+        ballot_append(sh.sT, &sh.sTq, push, child);
+        __syncthreads();
+      }
+      if (sh.sPq > 0 && !(MODE == 2 && sh.sOccluded >= sh.sTraced))
+        bvh_leaf_rays<MODE, NODE, RBMAX, LMAX>(g, it, sh);
+    }
+    __syncthreads();
+    for (int slot = tid; slot < g.rb; slot += kThreads) {
+      int x, y, leaf;
+      if (!slot_xy(g, K, J, slot, x, y, leaf)) continue;
+      const size_t idx = (size_t)y * g.W + x;
+      if (MODE == 2) {
+        if (sh.sAux[slot]) it.occ[idx] = 1;
+      } else {
+        const unsigned long long key = sh.sKey[slot];
+        if (key != kMissKey) atomicMin((unsigned long long*)&it.keys[idx], key);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Triangle staged in LDS as v1 and the two edges (the leaf test's operands).
+__device__ __forceinline__ void stage_tri(float4 (*sT)[3], int slot, const Tri& T) {
+  sT[slot][0] = make_float4(T.v1.x, T.v1.y, T.v1.z, T.e1.x);
+  sT[slot][1] = make_float4(T.e1.y, T.e1.z, T.e2.x, T.e2.y);
+  sT[slot][2] = make_float4(T.e2.z, 0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ Tri staged_tri(const float4 (*sT)[3], int slot) {
+  float4 a = sT[slot][0], b = sT[slot][1], c = sT[slot][2];
+  Tri T;
+  T.v1 = {a.x, a.y, a.z};
+  T.e1 = {a.w, b.x, b.y};
+  T.e2 = {b.z, b.w, c.x};
+  T.v2 = T.v3 = T.v1;  // unused by the leaf test
+  return T;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1261,8 +1373,12 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
                      dim3(kThreads), 0, s, G.items, G.nitems, G.nrays, G.ctr);
   hipLaunchKernelGGL((rh_group_build_kernel<MODE != 0>), dim3(G.nblocks), dim3(kThreads), 0, s,
                      G.items, G.nitems);
-  hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE>), dim3(G.nblocks), dim3(kThreads), 0, s,
-                     G.items, G.nitems, G.units, G.ctr, G.per_unit);
+  if (G.bvh)
+    hipLaunchKernelGGL((bvh_root_kernel<MODE, NODE>), dim3(G.nblocks), dim3(kThreads), 0, s,
+                       G.items, G.nitems, G.units, G.ctr, G.per_unit);
+  else
+    hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE>), dim3(G.nblocks), dim3(kThreads), 0, s,
+                       G.items, G.nitems, G.units, G.ctr, G.per_unit);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (t0) {
@@ -1270,7 +1386,13 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
     if (e != hipSuccess) return e;
   }
   const dim3 grid(std::max(G.grid, 1));
-  if (G.big)
+  if (G.bvh && G.big)
+    hipLaunchKernelGGL((bvh_unit_kernel<MODE, NODE, 1024, 4>), grid, dim3(kThreads), 0, s,
+                       G.items, G.units, G.ctr);
+  else if (G.bvh)
+    hipLaunchKernelGGL((bvh_unit_kernel<MODE, NODE, 256, 3>), grid, dim3(kThreads), 0, s,
+                       G.items, G.units, G.ctr);
+  else if (G.big)
     hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 1024, 4>), grid, dim3(kThreads), 0, s,
                        G.items, G.units, G.ctr);
   else
@@ -1285,30 +1407,10 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
   if (MODE != 2) {
     hipLaunchKernelGGL(rh_finalize_kernel, grid1(G.nrays), dim3(kThreads), 0, s, G.items,
                        G.nitems, G.nrays, G.ctr, G.replist);
-    if (MODE == 0 && G.replist)
+    if (MODE == 0 && G.replist && !G.bvh)
       hipLaunchKernelGGL((rh_repair_kernel<NODE>), dim3(64), dim3(kThreads), 0, s, G.items,
                          G.nitems, G.ctr, G.replist);
   }
-  return hipGetLastError();
-}
-
-template <int MODE, int NODE>
-hipError_t bvh_trace_dispatch(const RHGeom& g, const RaysDev& r, const float* v9,
-                              const float4* bvh, const uint32_t* roots, uint32_t nroots,
-                              const uint32_t* prim_order, int tile_roots, int tchunks,
-                              const float4* nodes, uint64_t* keys, uint8_t* occ, hipStream_t s) {
-  const uint32_t chunk = (uint32_t)((nroots + tchunks - 1) / tchunks);
-  dim3 grid(g.ntop, tchunks);
-  const int rpt = (g.rb + kThreads - 1) / kThreads;
-  if (rpt <= 1)
-    hipLaunchKernelGGL((bvh_trace_kernel<MODE, NODE, 1>), grid, dim3(kThreads), 0, s, g, r, v9,
-                       bvh, roots, nroots, prim_order, chunk, tile_roots, nodes, keys, occ, tchunks);
-  else if (rpt <= 4)
-    hipLaunchKernelGGL((bvh_trace_kernel<MODE, NODE, 4>), grid, dim3(kThreads), 0, s, g, r, v9,
-                       bvh, roots, nroots, prim_order, chunk, tile_roots, nodes, keys, occ, tchunks);
-  else
-    hipLaunchKernelGGL((bvh_trace_kernel<MODE, NODE, 16>), grid, dim3(kThreads), 0, s, g, r, v9,
-                       bvh, roots, nroots, prim_order, chunk, tile_roots, nodes, keys, occ, tchunks);
   return hipGetLastError();
 }
 
@@ -1363,15 +1465,6 @@ hipError_t launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_rh_build(const RHGeom& g, const RaysDev& r, bool use_valid, float4* nodes,
-                           hipStream_t s) {
-  if (use_valid)
-    hipLaunchKernelGGL(rh_build_kernel<true>, dim3(g.ntop), dim3(kThreads), 0, s, g, r, nodes);
-  else
-    hipLaunchKernelGGL(rh_build_kernel<false>, dim3(g.ntop), dim3(kThreads), 0, s, g, r, nodes);
-  return hipGetLastError();
-}
-
 hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
   if (G.nitems <= 0 || G.nrays == 0) return hipSuccess;
 #define RT_CALL(M, N) rh_group_dispatch<M, N>(G, s, t0, t1)
@@ -1384,25 +1477,6 @@ hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEv
   if (G.mode == 1) return RT_CALL(1, 0);
   return RT_CALL(2, 0);
 #undef RT_CALL
-}
-
-hipError_t launch_bvh_trace(const RHGeom& g, const RaysDev& r, int mode, int node_kind,
-                            const float* v9, const float4* bvh, const uint32_t* roots,
-                            uint32_t nroots, const uint32_t* prim_order, int tile_roots,
-                            int tchunks, const float4* nodes, uint64_t* keys, uint8_t* occ,
-                            hipStream_t s) {
-#define RT_BVH(M, N)                                                                             \
-  return bvh_trace_dispatch<M, N>(g, r, v9, bvh, roots, nroots, prim_order, tile_roots, tchunks, \
-                                  nodes, keys, occ, s)
-  if (node_kind == 1) {
-    if (mode == 0) RT_BVH(0, 1);
-    if (mode == 1) RT_BVH(1, 1);
-    RT_BVH(2, 1);
-  }
-  if (mode == 0) RT_BVH(0, 0);
-  if (mode == 1) RT_BVH(1, 0);
-  RT_BVH(2, 0);
-#undef RT_BVH
 }
 
 hipError_t launch_naive_trace(const RaysDev& r, int mode, const float* v9, uint32_t ntri,
