@@ -118,10 +118,12 @@ struct rt_scene {
   float4* d_bvh = nullptr;
   uint32_t* d_roots = nullptr;
   uint32_t* d_prim_order = nullptr;
+  float4* d_rcbox = nullptr;          // boxes of the clusters of 64 Morton-ordered roots
+  uint32_t nrclusters = 0;
   uint32_t* d_bvh_parent = nullptr;  // parent of every node below a frontier root
   uint8_t* d_bvh_depth = nullptr;    // its depth below that root
   uint32_t nroots = 0;
-  int roots_per_unit = 0;  // 0 = built-in choice
+  int roots_per_unit = 0;  // root clusters per ray-bvh unit; 0 = built-in choice
   bool has_bvh = false;
   // scratch of the naive path, grown on demand
   uint64_t* d_keys = nullptr;
@@ -222,6 +224,26 @@ int desc_ok(const rt_accel_desc* d) {
   return RT_OK;
 }
 
+// 30-bit Morton code of point p quantised to 10 bits per axis over [lo, hi].
+uint32_t morton3(const float* p, const float* lo, const float* hi) {
+  auto spread = [](uint32_t x) {
+    x &= 1023u;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+  };
+  uint32_t q[3];
+  for (int a = 0; a < 3; ++a) {
+    const float ext = hi[a] - lo[a];
+    float f = ext > 0.f ? (p[a] - lo[a]) / ext : 0.f;
+    f = std::min(std::max(f, 0.f), 1.f);
+    q[a] = (uint32_t)(f * 1023.f);
+  }
+  return (spread(q[0]) << 2) | (spread(q[1]) << 1) | spread(q[2]);
+}
+
 // Morton order of triangle centroids (10 bits per axis), ties by triangle id, cut into clusters
 // of kClusterSize with their vertex AABBs.  Only the traversal order changes: hits carry the
 // original ids through perm.
@@ -237,25 +259,8 @@ void sort_clusters(const float* v9, uint32_t ntri, std::vector<float>& v9s,
       lo[a] = std::min(lo[a], v);
       hi[a] = std::max(hi[a], v);
     }
-  auto spread = [](uint32_t x) {
-    x &= 1023u;
-    x = (x | (x << 16)) & 0x030000FFu;
-    x = (x | (x << 8)) & 0x0300F00Fu;
-    x = (x | (x << 4)) & 0x030C30C3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
-  };
   std::vector<uint32_t> code(ntri);
-  for (uint32_t i = 0; i < ntri; ++i) {
-    uint32_t q[3];
-    for (int a = 0; a < 3; ++a) {
-      const float ext = hi[a] - lo[a];
-      float f = ext > 0.f ? (c[3 * (size_t)i + a] - lo[a]) / ext : 0.f;
-      f = std::min(std::max(f, 0.f), 1.f);
-      q[a] = (uint32_t)(f * 1023.f);
-    }
-    code[i] = (spread(q[0]) << 2) | (spread(q[1]) << 1) | spread(q[2]);
-  }
+  for (uint32_t i = 0; i < ntri; ++i) code[i] = morton3(c.data() + 3 * (size_t)i, lo, hi);
   perm.resize(ntri);
   for (uint32_t i = 0; i < ntri; ++i) perm[i] = i;
   std::stable_sort(perm.begin(), perm.end(),
@@ -303,7 +308,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int kDefaultPer[2] = {16, 4};  // passing clusters per unit: closest, occluded
 constexpr int kDefaultGrid = 2048;       // persistent work-groups (8 per CU)
-constexpr int kDefaultRootsPerUnit = 32;  // passing frontier roots per ray-bvh unit
+constexpr int kDefaultRootClustersPerUnit = 8;  // passing root clusters per ray-bvh unit
 
 // Lays out the jobs as launch groups (one per mode and node type, in job order) in the
 // workspace arena, fills the job tables and uploads them when they changed.  `changed` tells
@@ -345,10 +350,8 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     const bool bvh = keys[gi].second >= 2;
     const int mi = mode == 2 ? 1 : 0;
     GroupLayout& L = gl[gi];
-    // work units: passing clusters (rayhierarchy) or passing frontier roots (ray-bvh, at most
-    // 256 so that one unit's first emissions fit the pair queue)
-    L.per = prm.per[mi] > 0 ? prm.per[mi] : (bvh ? kDefaultRootsPerUnit : kDefaultPer[mi]);
-    if (bvh) L.per = std::min(L.per, 256);
+    // work units: passing clusters of triangles (rayhierarchy) or of frontier roots (ray-bvh)
+    L.per = prm.per[mi] > 0 ? prm.per[mi] : (bvh ? kDefaultRootClustersPerUnit : kDefaultPer[mi]);
     L.max_units = 0;
     L.nblocks = 0;
     L.nrays = 0;
@@ -373,8 +376,10 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
       it.perm = jb.s->d_perm;
       it.cbox = jb.s->d_cbox;
       it.ntri = jb.s->ntri;
-      it.nclusters = bvh ? (jb.s->ntri ? jb.s->nroots : 0) : jb.s->nclusters;
+      it.nclusters = bvh ? (jb.s->ntri ? jb.s->nrclusters : 0) : jb.s->nclusters;
       if (bvh) {
+        it.cbox = jb.s->d_rcbox;
+        it.nroots = jb.s->nroots;
         it.bvh = jb.s->d_bvh;
         it.roots = jb.s->d_roots;
         it.prim_order = jb.s->d_prim_order;
@@ -611,7 +616,51 @@ int rt_scene_set_bvh(rt_scene* s, const rt_gpu_node* nodes, uint32_t nnodes, con
       }
     }
   }
+  // Frontier roots in Morton order of their box centres, cut into clusters of 64 with the
+  // union of their boxes (the traversal order does not change any result).
+  std::vector<uint32_t> rsorted(roots, roots + nroots);
+  std::vector<float4> rcbox;
+  {
+    std::vector<float> c(3 * (size_t)nroots);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t r = 0; r < nroots; ++r) {
+      const rt_gpu_node& n = nodes[roots[r]];
+      const float ctr[3] = {0.5f * (n.ax + n.bx), 0.5f * (n.ay + n.by), 0.5f * (n.az + n.bz)};
+      for (int a = 0; a < 3; ++a) {
+        c[3 * (size_t)r + a] = ctr[a];
+        lo[a] = std::min(lo[a], ctr[a]);
+        hi[a] = std::max(hi[a], ctr[a]);
+      }
+    }
+    std::vector<uint32_t> code(nroots), idx(nroots);
+    for (uint32_t r = 0; r < nroots; ++r) {
+      code[r] = morton3(c.data() + 3 * (size_t)r, lo, hi);
+      idx[r] = r;
+    }
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+    for (uint32_t r = 0; r < nroots; ++r) rsorted[r] = roots[idx[r]];
+    const uint32_t nc = (nroots + kClusterSize - 1) / kClusterSize;
+    rcbox.assign(2 * (size_t)std::max<uint32_t>(nc, 1), make_float4(0.f, 0.f, 0.f, 0.f));
+    for (uint32_t k = 0; k < nc; ++k) {
+      float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (uint32_t r = k * kClusterSize; r < std::min(nroots, (k + 1) * kClusterSize); ++r) {
+        const rt_gpu_node& n = nodes[rsorted[r]];
+        mn[0] = std::min(mn[0], n.ax);
+        mn[1] = std::min(mn[1], n.ay);
+        mn[2] = std::min(mn[2], n.az);
+        mx[0] = std::max(mx[0], n.bx);
+        mx[1] = std::max(mx[1], n.by);
+        mx[2] = std::max(mx[2], n.bz);
+      }
+      rcbox[2 * (size_t)k] = make_float4(mn[0], mn[1], mn[2], 0.f);
+      rcbox[2 * (size_t)k + 1] = make_float4(mx[0], mx[1], mx[2], 0.f);
+    }
+    s->nrclusters = nc;
+  }
   RT_TRY(hipSetDevice(s->ctx->device));
+  if (s->d_rcbox) (void)hipFree(s->d_rcbox);
+  s->d_rcbox = nullptr;
   if (s->d_bvh) (void)hipFree(s->d_bvh);
   if (s->d_roots) (void)hipFree(s->d_roots);
   if (s->d_prim_order) (void)hipFree(s->d_prim_order);
@@ -622,7 +671,8 @@ int rt_scene_set_bvh(rt_scene* s, const rt_gpu_node* nodes, uint32_t nnodes, con
   s->d_bvh_depth = nullptr;
   s->has_bvh = false;
   RT_TRY(upload(s->d_bvh, packed.data(), packed.size()));
-  RT_TRY(upload(s->d_roots, roots, nroots));
+  RT_TRY(upload(s->d_roots, rsorted.data(), nroots));
+  RT_TRY(upload(s->d_rcbox, rcbox.data(), rcbox.size()));
   RT_TRY(upload(s->d_prim_order, prim_order, s->ntri));
   RT_TRY(upload(s->d_bvh_parent, parent.data(), parent.size()));
   RT_TRY(upload(s->d_bvh_depth, depth.data(), depth.size()));
@@ -644,6 +694,7 @@ int rt_scene_destroy(rt_scene* s) {
   (void)hipFree(s->d_prim_order);
   (void)hipFree(s->d_bvh_parent);
   (void)hipFree(s->d_bvh_depth);
+  (void)hipFree(s->d_rcbox);
   (void)hipFree(s->d_keys);
   delete s;
   return RT_OK;

@@ -61,9 +61,11 @@ struct RhItem {
   const uint32_t* perm;  // sorted slot -> original triangle id
   const float4* cbox;    // cluster AABBs (min, max)
   uint32_t ntri;
-  uint32_t nclusters;    // first-pass candidates: clusters (rayhierarchy) or frontier roots
-  // ray-bvh: BVH nodes (2 float4 each), frontier roots, BVH position -> triangle id, and per
-  // node its parent and its depth below its frontier root
+  uint32_t nclusters;    // clusters of 64 triangles (rayhierarchy) or 64 frontier roots
+  // ray-bvh: BVH nodes (2 float4 each), frontier roots (Morton-ordered; cbox holds the boxes
+  // of their clusters), BVH position -> triangle id, and per node its parent and its depth
+  // below its frontier root
+  uint32_t nroots;
   const float4* bvh;
   const uint32_t* roots;
   const uint32_t* prim_order;

@@ -265,10 +265,12 @@ constexpr int kQ0Cap = 1024;     // root survivors held before expansion
 constexpr int kQCap = 256;       // survivors of one expansion round
 constexpr int kExpand = 64;      // parents expanded per round (4 children each)
 
-template <int MODE, int NODE>
+// Cluster bound: conservative for every triangle (rayhierarchy) or BVH node box (ray-bvh, whose
+// cull is intersectsNode in every mode) inside [cmin, cmax].
+template <int MODE, int NODE, bool BVH = false>
 __device__ __forceinline__ bool cluster_bound(const Node& n, V3 cmin, V3 cmax) {
   if (NODE == 1) return cull_ia(n, cmin, cmax);
-  if (MODE == 0) return cluster_tri(n, cmin, cmax);
+  if (MODE == 0 && !BVH) return cluster_tri(n, cmin, cmax);
   return cluster_node6(n, cmin, cmax);
 }
 __device__ __forceinline__ V3 tri_bmin(const Tri& T) { return vmin(vmin(T.v1, T.v2), T.v3); }
@@ -466,11 +468,11 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, const float* __res
   __syncthreads();
 }
 
-// Cluster pass: one work-group per block (of any job of the group) tests every cluster against
-// the block root (cluster bound), stores the passing cluster ids in the job's
-// clist[block * nclusters + ...] and appends work units {job, block, first, count} of at most
-// `per_unit` passing clusters.
-template <int MODE, int NODE>
+// Cluster pass: one work-group per block (of any job of the group) tests every cluster (of
+// triangles, or of ray-bvh frontier roots) against the block root (cluster bound), stores the
+// passing cluster ids in the job's clist[block * nclusters + ...] and appends work units
+// {job, block, first, count} of at most `per_unit` passing clusters.
+template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
                                                          int nitems, uint4* __restrict__ units,
                                                          uint32_t* __restrict__ ctr,
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
     bool ok = false;
     if (ci < nclusters) {
       const float4 lo = cbox[2 * (size_t)ci], hi = cbox[2 * (size_t)ci + 1];
-      ok = cluster_bound<MODE, NODE>(root, V3{lo.x, lo.y, lo.z}, V3{hi.x, hi.y, hi.z});
+      ok = cluster_bound<MODE, NODE, BVH>(root, V3{lo.x, lo.y, lo.z}, V3{hi.x, hi.y, hi.z});
     }
     ballot_append(out, &sCnt, ok, ci);
   }
@@ -741,56 +743,6 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
   __syncthreads();
 }
 
-// Root pass: one work-group per block (of any job of the group) tests every frontier root's
-// box against the block root (bound), lists the passing roots and appends work units.
-template <int MODE, int NODE>
-__global__ __launch_bounds__(256) void bvh_root_kernel(const RhItem* __restrict__ items,
-                                                       int nitems, uint4* __restrict__ units,
-                                                       uint32_t* __restrict__ ctr, int per_unit) {
-  __shared__ int sCnt;
-  __shared__ uint32_t sBase;
-  const int tid = threadIdx.x;
-  const int k = find_block_item(items, nitems, blockIdx.x);
-  const RhItem& it = items[k];
-  const RHGeom& g = it.g;
-  const int top = g.L - 1;
-  const int b = (int)(blockIdx.x - it.block_base);
-  const int K = b % g.gw[top], J = b / g.gw[top];
-  const float4* p = it.nodes + 4 * ((size_t)g.off[top] + (size_t)J * g.gw[top] + K);
-  const float4 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3];
-  if (a0.w == 0.f) return;
-  Node root;
-  root.dmin = {a0.x, a0.y, a0.z};
-  root.dmax = {a1.x, a1.y, a1.z};
-  root.omin = {a2.x, a2.y, a2.z};
-  root.omax = {a3.x, a3.y, a3.z};
-  if (tid == 0) sCnt = 0;
-  __syncthreads();
-  const uint32_t nr = it.nclusters;
-  uint32_t* out = it.clist + (size_t)b * nr;
-  for (uint32_t base = 0; base < nr; base += kThreads) {
-    const uint32_t ri = base + tid;
-    bool ok = false;
-    uint32_t node = 0;
-    if (ri < nr) {
-      node = it.roots[ri];
-      V3 bmin, bmax;
-      uint32_t off, np;
-      bvh_box(it.bvh, node, bmin, bmax, off, np);
-      ok = bvh_bound<NODE>(root, bmin, bmax);
-    }
-    ballot_append(out, &sCnt, ok, node);
-  }
-  __syncthreads();
-  const int cnt = sCnt;
-  const int nu = (cnt + per_unit - 1) / per_unit;
-  if (tid == 0 && nu > 0) sBase = atomicAdd(&ctr[0], (uint32_t)nu);
-  __syncthreads();
-  for (int u = tid; u < nu; u += kThreads)
-    units[sBase + u] = make_uint4((uint32_t)k, (uint32_t)b, (uint32_t)(u * per_unit),
-                                  (uint32_t)min(per_unit, cnt - u * per_unit));
-}
-
 // Appends the (triangle, BVH leaf, hierarchy leaf) pairs of a reached leaf x leaf pair.
 template <int MODE, class SH>
 __device__ __forceinline__ void bvh_emit(SH& sh, const RhItem& it, uint32_t b, uint32_t off,
@@ -851,89 +803,107 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
     }
     __syncthreads();
     if (sh.sV[0] && sh.sTraced > 0) {
-      // the unit's roots passed the bound against the block root: expand them
-      const uint32_t* rl = it.clist + (size_t)b * it.nclusters + unit.z;
-      const int nroot = (int)unit.w;
-      for (int i = tid; i < nroot; i += kThreads) {
-        const uint32_t node = rl[i];
-        V3 bmin, bmax;
-        uint32_t off, np;
-        bvh_box(it.bvh, node, bmin, bmax, off, np);
-        if (top == 0 && np > 0) {
-          bvh_emit<MODE>(sh, it, node, off, np, 0);
-        } else {
-          const int slot = atomicAdd(&sh.sTq, 1);
-          if (slot < kBvhQCap) sh.sT[slot] = node << 7;
-        }
-      }
-      __syncthreads();
-      for (;;) {
-        const int q = min(sh.sTq, kBvhQCap);
-        if (sh.sPq > kPairCap - 8 * kBvhPop * 4 || (q == 0 && sh.sPq > 0)) {
+      // The unit's clusters of frontier roots passed the cluster bound against the block
+      // root.  Four clusters (256 roots) at a time: the roots that pass their own bound seed
+      // the traversal stack, which is drained before the next four.  Pairs are tested when
+      // their queue could not take another round's emissions, and at the end.
+      const uint32_t* cl = it.clist + (size_t)b * it.nclusters + unit.z;
+      const int ncl = (int)unit.w;
+      bool stop = false;
+      for (int k0 = 0; k0 < ncl && !stop; k0 += kThreads / kCluster) {
+        if (sh.sPq > kPairCap - kThreads * 4) {
           bvh_leaf_rays<MODE, NODE, RBMAX, LMAX>(g, it, sh);
           if (MODE == 2 && sh.sOccluded >= sh.sTraced) break;
         }
-        if (q == 0) break;
-        // pop the newest items (bounded stack), room for 8 children each
-        int npop = min(kBvhPop, q);
-        while (npop > 0 && q - npop + 8 * npop > kBvhQCap) --npop;
-        if (npop == 0) {  // cannot happen for subtrees of depth <= 12 (checked on the host)
-          if (tid == 0) atomicOr(&ctr[3], 1u);
-          break;
-        }
-        uint32_t item = 0;
-        const int pi = tid >> 3, c = tid & 7;
-        if (pi < npop) item = sh.sT[q - npop + pi];
-        __syncthreads();
-        if (tid == 0) sh.sTq = q - npop;
-        __syncthreads();
-        bool push = false;
-        uint32_t child = 0;
-        if (pi < npop) {
-          const uint32_t B = item >> 7;
-          const int R = (int)(item & 127u);
-          const int s = sub_level(R);
+        const int k = k0 + tid / kCluster;
+        const uint32_t ri = k < ncl ? cl[k] * kCluster + (uint32_t)(tid % kCluster) : it.nroots;
+        if (ri < it.nroots) {
+          const uint32_t node = it.roots[ri];
           V3 bmin, bmax;
           uint32_t off, np;
-          bvh_box(it.bvh, B, bmin, bmax, off, np);
-          const bool bleaf = np > 0, rleaf = s == top;
-          int rc = -1;
-          uint32_t bc = B;
-          if (!rleaf && !bleaf) {
-            rc = c >> 1;
-            bc = off + (uint32_t)(c & 1);
-          } else if (!rleaf) {
-            rc = c < 4 ? c : -2;
-          } else if (!bleaf) {
-            rc = c < 2 ? -1 : -2;
-            bc = off + (uint32_t)c;
-          } else {
-            rc = -2;  // leaf x leaf items are emitted, never queued
-          }
-          if (rc != -2) {
-            int Rn = R;
-            if (rc >= 0) {
-              const int pl = R - pow4_off(s), pnl = 1 << s;
-              const int dk = pl & (pnl - 1), dj = pl >> s;
-              Rn = pow4_off(s + 1) + (2 * dj + (rc >> 1)) * (2 * pnl) + 2 * dk + (rc & 1);
+          bvh_box(it.bvh, node, bmin, bmax, off, np);
+          if (bvh_bound<NODE>(sh.sN[0], bmin, bmax)) {
+            if (top == 0 && np > 0) {
+              bvh_emit<MODE>(sh, it, node, off, np, 0);
+            } else {
+              const int slot = atomicAdd(&sh.sTq, 1);
+              sh.sT[slot] = node << 7;  // at most 256 seeds on an empty stack
             }
-            if (sh.sV[Rn]) {
-              V3 cmin, cmax;
-              uint32_t coff, cnp;
-              bvh_box(it.bvh, bc, cmin, cmax, coff, cnp);
-              if (bvh_bound<NODE>(sh.sN[Rn], cmin, cmax)) {
-                const int cs = sub_level(Rn);
-                if (cs == top && cnp > 0)
-                  bvh_emit<MODE>(sh, it, bc, coff, cnp, Rn - pow4_off(top));
-                else {
-                  push = true;
-                  child = (bc << 7) | (uint32_t)Rn;
+          }
+        }
+        __syncthreads();
+        for (;;) {
+          const int q = min(sh.sTq, kBvhQCap);
+          if (sh.sPq > kPairCap - kThreads * 4) {
+            bvh_leaf_rays<MODE, NODE, RBMAX, LMAX>(g, it, sh);
+            if (MODE == 2 && sh.sOccluded >= sh.sTraced) {
+              stop = true;
+              break;
+            }
+          }
+          if (q == 0) break;
+          // pop the newest items (bounded stack), room for 8 children each
+          int npop = min(kBvhPop, q);
+          while (npop > 0 && q - npop + 8 * npop > kBvhQCap) --npop;
+          if (npop == 0) {  // cannot happen for subtrees of depth <= 12 (checked on the host)
+            if (tid == 0) atomicOr(&ctr[3], 1u);
+            break;
+          }
+          uint32_t item = 0;
+          const int pi = tid >> 3, c = tid & 7;
+          if (pi < npop) item = sh.sT[q - npop + pi];
+          __syncthreads();
+          if (tid == 0) sh.sTq = q - npop;
+          __syncthreads();
+          bool push = false;
+          uint32_t child = 0;
+          if (pi < npop) {
+            const uint32_t B = item >> 7;
+            const int R = (int)(item & 127u);
+            const int sl = sub_level(R);
+            V3 bmin, bmax;
+            uint32_t off, np;
+            bvh_box(it.bvh, B, bmin, bmax, off, np);
+            const bool bleaf = np > 0, rleaf = sl == top;
+            int rc = -1;
+            uint32_t bc = B;
+            if (!rleaf && !bleaf) {
+              rc = c >> 1;
+              bc = off + (uint32_t)(c & 1);
+            } else if (!rleaf) {
+              rc = c < 4 ? c : -2;
+            } else if (!bleaf) {
+              rc = c < 2 ? -1 : -2;
+              bc = off + (uint32_t)c;
+            } else {
+              rc = -2;  // leaf x leaf items are emitted, never queued
+            }
+            if (rc != -2) {
+              int Rn = R;
+              if (rc >= 0) {
+                const int pl = R - pow4_off(sl), pnl = 1 << sl;
+                const int dk = pl & (pnl - 1), dj = pl >> sl;
+                Rn = pow4_off(sl + 1) + (2 * dj + (rc >> 1)) * (2 * pnl) + 2 * dk + (rc & 1);
+              }
+              if (sh.sV[Rn]) {
+                V3 cmin, cmax;
+                uint32_t coff, cnp;
+                bvh_box(it.bvh, bc, cmin, cmax, coff, cnp);
+                if (bvh_bound<NODE>(sh.sN[Rn], cmin, cmax)) {
+                  if (sub_level(Rn) == top && cnp > 0)
+                    bvh_emit<MODE>(sh, it, bc, coff, cnp, Rn - pow4_off(top));
+                  else {
+                    push = true;
+                    child = (bc << 7) | (uint32_t)Rn;
+                  }
                 }
               }
             }
           }
+          ballot_append(sh.sT, &sh.sTq, push, child);
+          __syncthreads();
         }
-        ballot_append(sh.sT, &sh.sTq, push, child);
+        if (tid == 0) sh.sTq = 0;  // drained, or abandoned once every ray is occluded
         __syncthreads();
       }
       if (sh.sPq > 0 && !(MODE == 2 && sh.sOccluded >= sh.sTraced))
@@ -1374,11 +1344,11 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
   hipLaunchKernelGGL((rh_group_build_kernel<MODE != 0>), dim3(G.nblocks), dim3(kThreads), 0, s,
                      G.items, G.nitems);
   if (G.bvh)
-    hipLaunchKernelGGL((bvh_root_kernel<MODE, NODE>), dim3(G.nblocks), dim3(kThreads), 0, s,
-                       G.items, G.nitems, G.units, G.ctr, G.per_unit);
+    hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE, true>), dim3(G.nblocks), dim3(kThreads), 0,
+                       s, G.items, G.nitems, G.units, G.ctr, G.per_unit);
   else
-    hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE>), dim3(G.nblocks), dim3(kThreads), 0, s,
-                       G.items, G.nitems, G.units, G.ctr, G.per_unit);
+    hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE, false>), dim3(G.nblocks), dim3(kThreads),
+                       0, s, G.items, G.nitems, G.units, G.ctr, G.per_unit);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (t0) {
