@@ -306,7 +306,7 @@ hipError_t timed_launch(Timer& tm, int mode, hipStream_t st, F fn) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-constexpr int kDefaultPer[2] = {16, 4};  // passing clusters per unit: closest, occluded
+constexpr int kDefaultPer[2] = {8, 4};  // passing clusters per unit: closest, occluded
 constexpr int kDefaultGrid = 2048;       // persistent work-groups (8 per CU)
 constexpr int kDefaultRootClustersPerUnit = 8;  // passing root clusters per ray-bvh unit
 

@@ -57,13 +57,21 @@ __device__ __forceinline__ Tri make_tri(V3 a, V3 b, V3 c) {
 
 // Strict Moller-Trumbore of intersectAllLeaves.cl:40-55: det != 0, b1 in (0,1), b2 > 0,
 // b1 + b2 < 1.  Returns false when the barycentric test fails; t/b1/b2 are then undefined.
+// Before the division, b1 = fl(dot(ov, s) * fl(1/det)) is rejected when it is certainly
+// outside (0, 1): a zero or opposite-signed numerator gives b1 <= 0, and
+// |numerator| >= fl(|det| * (1 + 2^-20)) gives b1 > 1 whatever the two roundings (|det| in
+// the normal range, so both roundings are relative).  Only
+// tests the exact sequence would reject are cut short, so the outcome is unchanged.
 __device__ __forceinline__ bool mt_strict(V3 o, V3 d, const Tri& T, float& t, float& b1, float& b2) {
   V3 s = cross(d, T.e2);
   float det = dot(s, T.e1);
   if (det == 0.0f) return false;
-  float inv = 1.0f / det;
   V3 ov = sub(o, T.v1);
-  b1 = dot(ov, s) * inv;
+  const float n1 = dot(ov, s);
+  if (n1 == 0.0f || ((n1 < 0.0f) != (det < 0.0f))) return false;
+  if (fabsf(det) >= 0x1p-100f && fabsf(n1) >= fabsf(det) * (1.0f + 0x1p-20f)) return false;
+  float inv = 1.0f / det;
+  b1 = n1 * inv;
   if (!(b1 > 0.f && b1 < 1.f)) return false;
   V3 s2 = cross(ov, T.e1);
   b2 = dot(d, s2) * inv;
