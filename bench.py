@@ -100,6 +100,14 @@ def bench_uneven(args, rt, scenes, torch):
 
     t_plain = timed(plain_step, "plain")
     t_batch = timed(batch_step, "batched")
+    # per-mode unit-kernel time of one eager batch run
+    b.set_param(rt.RT_PARAM_GRAPH, 0)
+    b.set_timing(True)
+    b.run(stream=stream)
+    stream.synchronize()
+    k_closest = b.kernel_time(rt.RT_CLOSEST_PRIMARY)[0]
+    k_occl = b.kernel_time(2)[0]
+    b.set_timing(False)
     mism_id = sum(int((hits_b[k].id != hits_p[k].id).sum().item()) for k in range(len(jobs)))
     mism_t = sum(int((hits_b[k].t.view(torch.int32) != hits_p[k].t.view(torch.int32)).sum().item())
                  for k in range(len(jobs)))
@@ -121,6 +129,8 @@ def bench_uneven(args, rt, scenes, torch):
                    "leaf": "4x4", "height": 3, "node": "box_dir", "launch_shape": tuned},
         "plain": {"value": round(v_plain, 3), "ms_per_step": round(t_plain * 1e3, 3)},
         "batched_over_plain": round(v_batch / v_plain, 3),
+        "kernels_ms": {"closest_traversal": round(k_closest, 3),
+                       "occlusion_traversal": round(k_occl, 3)},
         "parity": {"id_mismatches": mism_id, "t_mismatches": mism_t,
                    "occlusion_mismatches": mism_occ},
         "roofline": None, "cpu_baseline": None,

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "../../pbrt-v2_amd/host/breadth_first.h"
 #include "../../pbrt-v2_amd/host/pbrt_accel.h"
 
 using namespace pbrt_amd;
@@ -105,6 +106,51 @@ int mirror_run(const char* accel, const char* params, const float* v9, const flo
       const float d6[6] = {du.x, du.y, du.z, dv.x, dv.y, dv.z};
       std::memcpy(dp + 6 * (size_t)i, d6, sizeof d6);
     }
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Breadth-first render of the triangles (one mesh) with accelerator `accel`: image into rgb
+// (3 floats per pixel); stats = {tasks, nx, ny, camera rays, shadow rays, bounce rays}.
+int mirror_render(const char* accel, const char* params, const float* v9, uint32_t ntri,
+                  int xres, int yres, int spp, const float* origin, float fov_deg,
+                  const float* light, float intensity, float albedo, int max_depth,
+                  float* rgb, int64_t* stats, const char* pfm_path) {
+  SetSevereHandler(throw_severe);
+  try {
+    ParamSet ps;
+    std::string p = params ? params : "";
+    if (p.rfind("node=", 0) == 0) ps.AddString("node", p.substr(5));
+    std::vector<TriangleMesh> meshes(1);
+    for (uint32_t t = 0; t < ntri; ++t)
+      for (int k = 0; k < 3; ++k) {
+        meshes[0].vi.push_back((int)meshes[0].p.size());
+        meshes[0].p.emplace_back(v9[9 * t + 3 * k], v9[9 * t + 3 * k + 1], v9[9 * t + 3 * k + 2]);
+      }
+    std::unique_ptr<Aggregate> a(MakeAccelerator(accel, meshes, ps));
+    RenderSettings rs;
+    rs.xres = xres;
+    rs.yres = yres;
+    rs.spp = spp;
+    rs.origin = Point(origin[0], origin[1], origin[2]);
+    rs.fov_deg = fov_deg;
+    rs.light = Point(light[0], light[1], light[2]);
+    rs.intensity = intensity;
+    rs.albedo = albedo;
+    rs.max_depth = max_depth;
+    std::vector<float> img;
+    const RenderStats st = RenderBreadthFirst(*a, meshes, rs, &img);
+    std::memcpy(rgb, img.data(), sizeof(float) * img.size());
+    stats[0] = st.tasks;
+    stats[1] = st.nx;
+    stats[2] = st.ny;
+    stats[3] = (int64_t)st.camera_rays;
+    stats[4] = (int64_t)st.shadow_rays;
+    stats[5] = (int64_t)st.bounce_rays;
+    if (pfm_path && !WritePFM(pfm_path, xres, yres, img)) throw std::runtime_error("PFM write");
     return 0;
   } catch (const std::exception& e) {
     g_err = e.what();
