@@ -623,7 +623,7 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
 // exact-conservative bound of the cull, runs the leaf tests, and evaluates the exact cull
 // chain only for pairs in which some ray passes its leaf test.
 constexpr int kBvhQCap = 3072;   // traversal stack: 256 + 224 * depth <= 3072 for depth <= 12
-constexpr int kPairCap = 2048;   // (triangle, BVH leaf, hierarchy leaf) pairs awaiting tests
+constexpr int kPairCap = 1024;   // (BVH leaf, hierarchy leaf) pairs awaiting leaf tests
 constexpr int kBvhPop = 32;      // traversal items expanded per round (8 children each)
 
 template <int MODE, int RBMAX, int LMAX>
@@ -636,8 +636,9 @@ struct BvhShared {
   uint32_t sAux[RBMAX];  // occlusion flag (shadow)
   uint8_t sRv[RBMAX];
   uint32_t sT[kBvhQCap];   // traversal items: BVH node << 7 | subtree node index
-  uint32_t sPt[kPairCap];  // pair: original triangle id
+  uint32_t sPt[kPairCap];  // pair: first primitive position of the BVH leaf
   uint32_t sPn[kPairCap];  // pair: BVH leaf << 6 | hierarchy leaf
+  uint32_t sPc[kPairCap];  // pair: primitives tested (the leaf's, or 1 for secondary rays)
   uint8_t sPair[kPairCap];
   int sTq, sPq;
   int sTraced, sOccluded;
@@ -711,8 +712,14 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
     const int slot = (int)(sh.sPn[e] & 63u) * G + (item - e * G);
     if (!sh.sRv[slot]) continue;
     if (MODE == 2 && sh.sAux[slot]) continue;
-    float t, b1, b2;
-    if (leaf_hit<TM>(sh, slot, load_tri(v9, sh.sPt[e]), t, b1, b2)) sh.sPair[e] = 1;
+    const uint32_t off = sh.sPt[e], cnt = sh.sPc[e];
+    for (uint32_t f = 0; f < cnt; ++f) {
+      float t, b1, b2;
+      if (leaf_hit<TM>(sh, slot, load_tri(v9, it.prim_order[off + f]), t, b1, b2)) {
+        sh.sPair[e] = 1;
+        break;
+      }
+    }
   }
   __syncthreads();
   int verified = 0;
@@ -728,12 +735,15 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
       if (sh.sPair[e] != 2) continue;
       const int slot = (int)(sh.sPn[e] & 63u) * G + (item - e * G);
       if (!sh.sRv[slot]) continue;
-      const uint32_t tri = sh.sPt[e];
-      float t, b1, b2;
-      if (!leaf_hit<TM>(sh, slot, load_tri(v9, tri), t, b1, b2)) continue;
-      if (MODE == 2) {
-        if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
-      } else {
+      const uint32_t off = sh.sPt[e], cnt = sh.sPc[e];
+      for (uint32_t f = 0; f < cnt; ++f) {
+        const uint32_t tri = it.prim_order[off + f];
+        float t, b1, b2;
+        if (!leaf_hit<TM>(sh, slot, load_tri(v9, tri), t, b1, b2)) continue;
+        if (MODE == 2) {
+          if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
+          break;
+        }
         atomicMin(&sh.sKey[slot], (unsigned long long)hit_key(t, (int32_t)tri));
       }
     }
@@ -743,17 +753,15 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
   __syncthreads();
 }
 
-// Appends the (triangle, BVH leaf, hierarchy leaf) pairs of a reached leaf x leaf pair.
+// Appends the (BVH leaf, hierarchy leaf) pair of a reached leaf x leaf pair; its primitives
+// are tested together (only the first one for secondary rays, intersection6DB_BVH2.cl).
 template <int MODE, class SH>
-__device__ __forceinline__ void bvh_emit(SH& sh, const RhItem& it, uint32_t b, uint32_t off,
-                                         uint32_t np, int li) {
-  const uint32_t cnt = MODE == 1 ? 1u : np;  // secondary rays: first primitive only
-  const int base = atomicAdd(&sh.sPq, (int)cnt);
-  for (uint32_t f = 0; f < cnt; ++f) {
-    if (base + (int)f < kPairCap) {
-      sh.sPt[base + f] = it.prim_order[off + f];
-      sh.sPn[base + f] = (b << 6) | (uint32_t)li;
-    }
+__device__ __forceinline__ void bvh_emit(SH& sh, uint32_t b, uint32_t off, uint32_t np, int li) {
+  const int e = atomicAdd(&sh.sPq, 1);
+  if (e < kPairCap) {  // room for every emission of one round is kept free
+    sh.sPt[e] = off;
+    sh.sPn[e] = (b << 6) | (uint32_t)li;
+    sh.sPc[e] = MODE == 1 ? 1u : np;
   }
 }
 
@@ -811,7 +819,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
       const int ncl = (int)unit.w;
       bool stop = false;
       for (int k0 = 0; k0 < ncl && !stop; k0 += kThreads / kCluster) {
-        if (sh.sPq > kPairCap - kThreads * 4) {
+        if (sh.sPq > kPairCap - kThreads) {
           bvh_leaf_rays<MODE, NODE, RBMAX, LMAX>(g, it, sh);
           if (MODE == 2 && sh.sOccluded >= sh.sTraced) break;
         }
@@ -824,7 +832,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
           bvh_box(it.bvh, node, bmin, bmax, off, np);
           if (bvh_bound<NODE>(sh.sN[0], bmin, bmax)) {
             if (top == 0 && np > 0) {
-              bvh_emit<MODE>(sh, it, node, off, np, 0);
+              bvh_emit<MODE>(sh, node, off, np, 0);
             } else {
               const int slot = atomicAdd(&sh.sTq, 1);
               sh.sT[slot] = node << 7;  // at most 256 seeds on an empty stack
@@ -834,7 +842,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
         __syncthreads();
         for (;;) {
           const int q = min(sh.sTq, kBvhQCap);
-          if (sh.sPq > kPairCap - kThreads * 4) {
+          if (sh.sPq > kPairCap - kThreads) {
             bvh_leaf_rays<MODE, NODE, RBMAX, LMAX>(g, it, sh);
             if (MODE == 2 && sh.sOccluded >= sh.sTraced) {
               stop = true;
@@ -891,7 +899,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
                 bvh_box(it.bvh, bc, cmin, cmax, coff, cnp);
                 if (bvh_bound<NODE>(sh.sN[Rn], cmin, cmax)) {
                   if (sub_level(Rn) == top && cnp > 0)
-                    bvh_emit<MODE>(sh, it, bc, coff, cnp, Rn - pow4_off(top));
+                    bvh_emit<MODE>(sh, bc, coff, cnp, Rn - pow4_off(top));
                   else {
                     push = true;
                     child = (bc << 7) | (uint32_t)Rn;

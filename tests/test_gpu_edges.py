@@ -1,0 +1,144 @@
+"""GPU edge cases through the C ABI, each against the oracle: empty scenes, single-ray and
+all-invalid images, exactly coincident triangles (tie -> smallest id), degenerate triangles,
+rays starting on a surface, repairRun 2, ray-bvh leaf sizes 2-4 with the first-primitive
+rule of secondary rays, BVH heights 1 and 6, and a scene of more than 60,000 triangles (two
+BVH parts)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(rt, r):
+    return rt.DeviceRays.from_numpy(r)
+
+
+def _closest(sc, rt, rays, mode):
+    h = sc.trace_closest(_dev(rt, rays), mode)
+    return h.id.cpu().numpy(), h.t.cpu().numpy()
+
+
+def _ref(orc, sc, v9, rays, p, mode):
+    if sc.desc.kind == 2:
+        nodes, roots, order = sc.handoff
+        return orc.trace_raybvh(v9, nodes, roots, order, rays, p, mode)
+    return orc.trace(v9, rays, p, mode)
+
+
+@pytest.mark.parametrize("accel", ["naive", "rayhierarchy", "ray-bvh"])
+def test_empty_scene_and_tiny_images(rt, scenes, gpu_ctx, accel):
+    empty = np.zeros((0, 9), np.float32)
+    sc = rt.Scene(gpu_ctx, empty, desc=accel)
+    for W, H in ((1, 1), (3, 1), (1, 5), (17, 3)):
+        rays = scenes.pinhole_rays(W, H)
+        ids, ts = _closest(sc, rt, rays, rt.RT_CLOSEST_PRIMARY)
+        assert np.all(ids == -1) and np.all(np.isinf(ts))
+        occ = sc.trace_occluded(_dev(rt, dict(rays, tmax=np.ones(W * H, np.float32)))).cpu()
+        assert int(occ.sum()) == 0
+    sc.close()
+
+
+@pytest.mark.parametrize("accel", ["naive", "rayhierarchy", "ray-bvh"])
+def test_single_ray_and_all_invalid(rt, scenes, orc, gpu_ctx, accel):
+    v9 = scenes.random_triangles(300, seed=4)
+    sc = rt.Scene(gpu_ctx, v9, desc=accel)
+    p = orc.params(accel)
+    one = scenes.pinhole_rays(1, 1, fov_deg=5)
+    ids, _ = _closest(sc, rt, one, rt.RT_CLOSEST_PRIMARY)
+    assert np.array_equal(ids, _ref(orc, sc, v9, one, p, orc.CLOSEST_PRIMARY)["id"])
+    rr = scenes.random_rays(64 * 16, seed=9, lo=-1, hi=1, maxt=np.inf, width=64)
+    rr["valid"] = np.zeros(64 * 16, np.uint8)
+    ids, ts = _closest(sc, rt, rr, rt.RT_CLOSEST_SECONDARY)
+    assert np.all(ids == -1) and np.all(np.isinf(ts))
+    occ = sc.trace_occluded(_dev(rt, dict(rr, tmax=np.full(64 * 16, 5, np.float32)))).cpu()
+    assert int(occ.sum()) == 0
+    sc.close()
+
+
+@pytest.mark.parametrize("accel", ["rayhierarchy", "ray-bvh", "naive"])
+def test_coincident_triangles_tie_to_smallest_id(rt, scenes, orc, gpu_ctx, accel):
+    """Three copies of the same triangle at ids 5, 9 and 200: every hit reports id 5."""
+    v9 = scenes.random_triangles(256, seed=12).copy()
+    big = np.array([-0.6, -0.6, -1.5, 0.7, -0.5, -1.5, 0.0, 0.7, -1.5], np.float32)
+    for i in (5, 9, 200):
+        v9[i] = big
+    rays = scenes.pinhole_rays(64, 64)
+    sc = rt.Scene(gpu_ctx, v9, desc=accel)
+    ids, _ = _closest(sc, rt, rays, rt.RT_CLOSEST_PRIMARY)
+    ref = _ref(orc, sc, v9, rays, orc.params(accel), orc.CLOSEST_PRIMARY)
+    assert np.array_equal(ids, ref["id"])
+    assert int((ids == 5).sum()) > 200 and not np.any((ids == 9) | (ids == 200))
+    sc.close()
+
+
+@pytest.mark.parametrize("accel", ["rayhierarchy", "ray-bvh"])
+def test_degenerate_triangles_and_rays_on_surfaces(rt, scenes, orc, gpu_ctx, accel):
+    """Zero-area triangles (a point, a segment) never hit; secondary rays leaving a surface
+    at tmin = 0 exercise t == tmin."""
+    v9 = scenes.random_triangles(400, seed=21).copy()
+    v9[7] = np.array([0.1, 0.1, 0.1] * 3, np.float32)
+    v9[8] = np.array([0, 0, 0, 0.5, 0.5, 0.5, 1, 1, 1], np.float32)
+    rays = scenes.pinhole_rays(48, 48)
+    sc = rt.Scene(gpu_ctx, v9, desc=accel)
+    p = orc.params(accel)
+    ids, ts = _closest(sc, rt, rays, rt.RT_CLOSEST_PRIMARY)
+    ref = _ref(orc, sc, v9, rays, p, orc.CLOSEST_PRIMARY)
+    assert np.array_equal(ids, ref["id"]) and not np.any((ids == 7) | (ids == 8))
+    hit = ids >= 0
+    t = np.where(hit, ts, 0).astype(np.float32)
+    on = dict(ox=rays["ox"] + rays["dx"] * t, oy=rays["oy"] + rays["dy"] * t,
+              oz=rays["oz"] + rays["dz"] * t, dx=rays["dx"], dy=rays["dy"], dz=rays["dz"],
+              tmin=np.zeros(rays["ox"].size, np.float32),
+              tmax=np.full(rays["ox"].size, np.inf, np.float32), valid=hit.astype(np.uint8),
+              width=48, height=48)
+    on = {k: (v.astype(np.float32) if isinstance(v, np.ndarray) and v.dtype != np.uint8 else v)
+          for k, v in on.items()}
+    ids2, _ = _closest(sc, rt, on, rt.RT_CLOSEST_SECONDARY)
+    assert np.array_equal(ids2, _ref(orc, sc, v9, on, p, orc.CLOSEST_SECONDARY)["id"])
+    sc.close()
+
+
+def test_repair_run_two(rt, scenes, orc, gpu_ctx):
+    v9 = scenes.torus_scene()
+    rays = scenes.subimage(scenes.pinhole_rays(512, 512), 200, 200, 64, 64)
+    desc = rt.default_desc("rayhierarchy", repair_run=2)
+    sc = rt.Scene(gpu_ctx, v9, desc=desc)
+    ids, ts = _closest(sc, rt, rays, rt.RT_CLOSEST_PRIMARY)
+    ref = orc.trace(v9, rays, orc.params("rayhierarchy", repair_run=2), orc.CLOSEST_PRIMARY)
+    assert np.array_equal(ids, ref["id"])
+    np.testing.assert_allclose(ts[ids >= 0], ref["t"][ids >= 0], rtol=1e-5)
+    sc.close()
+
+
+@pytest.mark.parametrize("max_prim,bvh_height", [(2, 3), (4, 3), (1, 1), (1, 6)])
+def test_raybvh_leaf_sizes_and_heights(rt, scenes, orc, gpu_ctx, max_prim, bvh_height):
+    """maxBVHPrim 2-4 (secondary rays test only the first primitive of a BVH leaf,
+    intersection6DB_BVH2.cl) and other BVHheight frontiers."""
+    v9 = scenes.room_scene(nclusters=40)
+    full = scenes.pinhole_rays(256, 256, origin=(0.0, 0.0, -9.0), fov_deg=50)
+    rays = scenes.subimage(full, 96, 96, 64, 64)
+    desc = rt.default_desc("ray-bvh", max_bvh_prim=max_prim, bvh_height=bvh_height)
+    sc = rt.Scene(gpu_ctx, v9, desc=desc)
+    p = orc.params("ray-bvh")
+    ids, _ = _closest(sc, rt, rays, rt.RT_CLOSEST_PRIMARY)
+    assert np.array_equal(ids, _ref(orc, sc, v9, rays, p, orc.CLOSEST_PRIMARY)["id"])
+    rr = scenes.random_rays(64 * 64, seed=31, lo=-4, hi=4, maxt=np.inf, width=64)
+    ids2, _ = _closest(sc, rt, rr, rt.RT_CLOSEST_SECONDARY)
+    assert np.array_equal(ids2, _ref(orc, sc, v9, rr, p, orc.CLOSEST_SECONDARY)["id"])
+    sh = scenes.shadow_rays_np(rays, ids, _closest(sc, rt, rays, 0)[1], (1.0, 2.5, -8.0))
+    occ = sc.trace_occluded(_dev(rt, sh)).cpu().numpy()
+    assert np.array_equal(occ, _ref(orc, sc, v9, sh, p, orc.OCCLUDED)["occ"])
+    sc.close()
+
+
+def test_raybvh_two_parts(rt, scenes, orc, gpu_ctx):
+    """More than 60,000 triangles: the handoff holds two independently built BVH parts."""
+    v9 = scenes.room_scene(nclusters=320)  # 64,012 triangles
+    assert v9.shape[0] > 60000
+    full = scenes.pinhole_rays(256, 256, origin=(0.0, 0.0, -9.0), fov_deg=50)
+    rays = scenes.subimage(full, 112, 112, 32, 32)
+    sc = rt.Scene(gpu_ctx, v9, desc="ray-bvh")
+    ids, _ = _closest(sc, rt, rays, rt.RT_CLOSEST_PRIMARY)
+    assert np.array_equal(ids, _ref(orc, sc, v9, rays, orc.params("ray-bvh"),
+                                    orc.CLOSEST_PRIMARY)["id"])
+    sc.close()
