@@ -112,6 +112,7 @@ struct rt_scene {
   float* d_v9s = nullptr;
   uint32_t* d_perm = nullptr;
   float4* d_cbox = nullptr;
+  float4* d_tbox = nullptr;  // AABB of every sorted triangle
   uint32_t nclusters = 0;
   int clusters_closest = 0, clusters_occluded = 0;  // 0 = built-in choice
   // ray-bvh handoff
@@ -375,6 +376,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
       it.v9s = jb.s->d_v9s;
       it.perm = jb.s->d_perm;
       it.cbox = jb.s->d_cbox;
+      it.tbox = jb.s->d_tbox;
       it.ntri = jb.s->ntri;
       it.nclusters = bvh ? (jb.s->ntri ? jb.s->nrclusters : 0) : jb.s->nclusters;
       if (bvh) {
@@ -557,6 +559,23 @@ int rt_scene_create(rt_ctx* ctx, const float* v9, const float* uv6, uint32_t ntr
     e = upload(s->d_v9s, v9s.data(), v9s.size());
     if (e == hipSuccess) e = upload(s->d_perm, perm.data(), perm.size());
     if (e == hipSuccess) e = upload(s->d_cbox, cbox.data(), cbox.size());
+    if (e == hipSuccess) {
+      // per-triangle AABBs with the device's min/max selects, vertex order v1, v2, v3
+      std::vector<float4> tbox(2 * (size_t)ntri);
+      for (uint32_t i = 0; i < ntri; ++i) {
+        const float* p = &v9s[9 * (size_t)i];
+        float mn[3], mx[3];
+        for (int a = 0; a < 3; ++a) {
+          mn[a] = p[3 + a] < p[a] ? p[3 + a] : p[a];
+          mn[a] = p[6 + a] < mn[a] ? p[6 + a] : mn[a];
+          mx[a] = p[3 + a] > p[a] ? p[3 + a] : p[a];
+          mx[a] = p[6 + a] > mx[a] ? p[6 + a] : mx[a];
+        }
+        tbox[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.f);
+        tbox[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.f);
+      }
+      e = upload(s->d_tbox, tbox.data(), tbox.size());
+    }
     s->nclusters = (uint32_t)(cbox.size() / 2);
   }
   if (e != hipSuccess) {
@@ -689,6 +708,7 @@ int rt_scene_destroy(rt_scene* s) {
   (void)hipFree(s->d_v9s);
   (void)hipFree(s->d_perm);
   (void)hipFree(s->d_cbox);
+  (void)hipFree(s->d_tbox);
   (void)hipFree(s->d_bvh);
   (void)hipFree(s->d_roots);
   (void)hipFree(s->d_prim_order);

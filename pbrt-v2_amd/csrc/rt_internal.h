@@ -60,6 +60,7 @@ struct RhItem {
   const float* v9s;      // Morton-sorted traversal copy, clusters of 64
   const uint32_t* perm;  // sorted slot -> original triangle id
   const float4* cbox;    // cluster AABBs (min, max)
+  const float4* tbox;    // AABB (min, max) of every sorted triangle
   uint32_t ntri;
   uint32_t nclusters;    // clusters of 64 triangles (rayhierarchy) or 64 frontier roots
   // ray-bvh: BVH nodes (2 float4 each), frontier roots (Morton-ordered; cbox holds the boxes

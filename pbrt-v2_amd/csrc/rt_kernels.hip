@@ -282,6 +282,17 @@ __device__ __forceinline__ bool cull_bound(const Node& n, const Tri& T) {
   if (MODE == 0) return prefilter_tri(n, T);
   return prefilter_node6(n, tri_bmin(T), tri_bmax(T));
 }
+// Bound of sorted triangle `slot`: intersectsTri's bound needs the vertices; the others
+// depend on the triangle's AABB only, which is stored per sorted triangle.
+template <int MODE, int NODE>
+__device__ __forceinline__ bool cull_bound_slot(const Node& n, const float* __restrict__ v9s,
+                                                const float4* __restrict__ tbox, uint32_t slot) {
+  if (MODE == 0 && NODE == 0) return prefilter_tri(n, load_tri(v9s, slot));
+  const float4 lo = tbox[2 * (size_t)slot], hi = tbox[2 * (size_t)slot + 1];
+  const V3 bmin = {lo.x, lo.y, lo.z}, bmax = {hi.x, hi.y, hi.z};
+  if (NODE == 1) return cull_ia(n, bmin, bmax);
+  return prefilter_node6(n, bmin, bmax);
+}
 // The reference's cull.
 template <int MODE, int NODE>
 __device__ __forceinline__ bool cull_exact(const Node& n, const Tri& T) {
@@ -418,7 +429,8 @@ __device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restri
 // Phases B and C for the triangles queued in sQ0; empties the queue.
 template <int MODE, int NODE, int RBMAX, int LMAX>
 __device__ __forceinline__ void expand_queue(const RHGeom& g, const float* __restrict__ v9s,
-                             const uint32_t* __restrict__ perm, RhShared<MODE, RBMAX, LMAX>& sh) {
+                             const float4* __restrict__ tbox, const uint32_t* __restrict__ perm,
+                             RhShared<MODE, RBMAX, LMAX>& sh) {
   const int tid = threadIdx.x;
   const int top = g.L - 1;
   if (top == 0) {
@@ -457,7 +469,7 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, const float* __res
         const int ch = (2 * dj + (a >> 1)) * (2 * pnl) + 2 * dk + (a & 1);
         const int ni = pow4_off(s + 1) + ch;
         item = (slot_t << 6) | (uint32_t)ch;
-        if (sh.sV[ni]) ok = cull_bound<MODE, NODE>(sh.sN[ni], load_tri(v9s, slot_t));
+        if (sh.sV[ni]) ok = cull_bound_slot<MODE, NODE>(sh.sN[ni], v9s, tbox, slot_t);
       }
       ballot_append(sh.sQ[s + 1], &sh.sCnt[s + 1], ok, item);
       __syncthreads();
@@ -580,17 +592,17 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
         const int k = k0 + tid / kCluster;
         const uint32_t slot = k < ncl ? cl[k] * kCluster + (uint32_t)(tid % kCluster) : ntri;
         bool pass = false;
-        if (slot < ntri) pass = cull_bound<MODE, NODE>(root, load_tri(v9s, slot));
+        if (slot < ntri) pass = cull_bound_slot<MODE, NODE>(root, v9s, it.tbox, slot);
         ballot_append(sh.sQ0, &sh.sCnt[0], pass, slot << 6);
         __syncthreads();
         const int cnt0 = sh.sCnt[0];
         __syncthreads();  // every thread has read the count before the next round appends
         if (cnt0 > kQ0Cap - kThreads) {
-          expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, perm, sh);
+          expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, it.tbox, perm, sh);
           if (MODE == 2 && sh.sOccluded >= sh.sTraced) done = true;
         }
       }
-      if (!done && sh.sCnt[0] > 0) expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, perm, sh);
+      if (!done && sh.sCnt[0] > 0) expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, it.tbox, perm, sh);
     }
     for (int slot = tid; slot < g.rb; slot += kThreads) {
       int x, y, leaf;
