@@ -246,16 +246,39 @@ def main():
     occ = torch.empty(n, dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
 
+    # path bounces (cosine-weighted, device-generated), each with its own shadow rays
+    depth = wl.get("depth", 0)
+    bounces, bhits, bshadows, boccs = [], [], [], []
+    src, src_hits = prim, hits
+    for dpt in range(1, depth + 1):
+        b = rt.make_bounce_rays(sc, src, src_hits, seed=17, depth=dpt)
+        bh = sc.trace_closest(b, rt.RT_CLOSEST_SECONDARY)
+        bounces.append(b)
+        bhits.append(bh)
+        bshadows.append(rt.make_shadow_rays(b, bh, light))
+        boccs.append(torch.empty(n, dtype=torch.uint8, device="cuda"))
+        src, src_hits = b, bh
+
     def step():
         sc.trace_closest(prim, rt.RT_CLOSEST_PRIMARY, hits=hits, stream=stream)
         rt.make_shadow_rays(prim, hits, light, stream=stream, out=shadow)
         sc.trace_occluded(shadow, out=occ, stream=stream)
+        s_rays, s_hits = prim, hits
+        for k in range(depth):
+            rt.make_bounce_rays(sc, s_rays, s_hits, seed=17, depth=k + 1, stream=stream,
+                                out=bounces[k])
+            sc.trace_closest(bounces[k], rt.RT_CLOSEST_SECONDARY, hits=bhits[k], stream=stream)
+            rt.make_shadow_rays(bounces[k], bhits[k], light, stream=stream, out=bshadows[k])
+            sc.trace_occluded(bshadows[k], out=boccs[k], stream=stream)
+            s_rays, s_hits = bounces[k], bhits[k]
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     n_shadow = int(shadow.valid.sum().item())
-    rays_per_step = n + n_shadow
+    n_bounce = sum(int(b.valid.sum().item()) for b in bounces)
+    n_bshadow = sum(int(b.valid.sum().item()) for b in bshadows)
+    rays_per_step = n + n_shadow + n_bounce + n_bshadow
 
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     sc.set_timing(True)
@@ -266,6 +289,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     closest_ms, nc = sc.kernel_time(rt.RT_CLOSEST_PRIMARY)
+    second_ms, ns = sc.kernel_time(rt.RT_CLOSEST_SECONDARY)
     occl_ms, no = sc.kernel_time(2)
     sc.set_timing(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
@@ -297,15 +321,18 @@ def main():
         "config": {"workload": args.workload, "form": "plain per-scene", "accelerator": accel,
                    "triangles": int(len(v9)),
                    "image": f"{wl['W']}x{wl['H']}x{wl.get('spp', 1)}spp",
-                   "primary_rays": n, "shadow_rays": n_shadow, "leaf": "4x4", "height": 3,
+                   "primary_rays": n, "shadow_rays": n_shadow + n_bshadow,
+                   "bounce_rays": n_bounce, "path_depth": depth, "leaf": "4x4", "height": 3,
                    "node": "box_dir"},
         "kernels_ms": {"closest_traversal": round(closest_ms / max(nc, 1), 4),
                        "occlusion_traversal": round(occl_ms / max(no, 1), 4)},
     }
+    if ns:
+        result["kernels_ms"]["secondary_traversal"] = round(second_ms / ns, 4)
 
     roofline = None
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and depth == 0:
         orc = _load("rt_oracle_py", ROOT / "oracle" / "oracle.py")
         print("[bench] oracle counts + CPU baseline", file=sys.stderr, flush=True)
         p = orc.params(accel)

@@ -113,6 +113,7 @@ struct rt_scene {
   uint32_t* d_perm = nullptr;
   float4* d_cbox = nullptr;
   float4* d_tbox = nullptr;  // AABB of every sorted triangle
+  float4* d_tmt = nullptr;   // ray-bvh: leaf-test operands v1, e1, e2 (3 float4 per triangle)
   uint32_t nclusters = 0;
   int clusters_closest = 0, clusters_occluded = 0;  // 0 = built-in choice
   // ray-bvh handoff
@@ -285,6 +286,21 @@ void sort_clusters(const float* v9, uint32_t ntri, std::vector<float>& v9s,
   }
 }
 
+// Leaf-test operands v1, e1 = v2 - v1, e2 = v3 - v1 (FP32, as the device's make_tri) of the
+// triangles of v9, 3 float4 each.
+std::vector<float4> mt_operands(const float* v9, uint32_t ntri) {
+  std::vector<float4> m(3 * (size_t)ntri);
+  for (uint32_t i = 0; i < ntri; ++i) {
+    const float* p = v9 + 9 * (size_t)i;
+    const float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]};
+    const float e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+    m[3 * (size_t)i] = make_float4(p[0], p[1], p[2], e1[0]);
+    m[3 * (size_t)i + 1] = make_float4(e1[1], e1[2], e2[0], e2[1]);
+    m[3 * (size_t)i + 2] = make_float4(e2[2], 0.f, 0.f, 0.f);
+  }
+  return m;
+}
+
 template <class T>
 hipError_t upload(T*& d, const T* h, size_t n) {
   hipError_t e = hipMalloc((void**)&d, sizeof(T) * std::max<size_t>(n, 1));
@@ -377,6 +393,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
       it.perm = jb.s->d_perm;
       it.cbox = jb.s->d_cbox;
       it.tbox = jb.s->d_tbox;
+      it.tmt = jb.s->d_tmt;
       it.ntri = jb.s->ntri;
       it.nclusters = bvh ? (jb.s->ntri ? jb.s->nrclusters : 0) : jb.s->nclusters;
       if (bvh) {
@@ -578,6 +595,10 @@ int rt_scene_create(rt_ctx* ctx, const float* v9, const float* uv6, uint32_t ntr
     }
     s->nclusters = (uint32_t)(cbox.size() / 2);
   }
+  if (e == hipSuccess && desc->kind == RT_ACCEL_RAYBVH && ntri) {
+    const std::vector<float4> mt = mt_operands(v9, ntri);  // original order
+    e = upload(s->d_tmt, mt.data(), mt.size());
+  }
   if (e != hipSuccess) {
     rt_scene_destroy(s);
     return hip_status(e);
@@ -709,6 +730,7 @@ int rt_scene_destroy(rt_scene* s) {
   (void)hipFree(s->d_perm);
   (void)hipFree(s->d_cbox);
   (void)hipFree(s->d_tbox);
+  (void)hipFree(s->d_tmt);
   (void)hipFree(s->d_bvh);
   (void)hipFree(s->d_roots);
   (void)hipFree(s->d_prim_order);

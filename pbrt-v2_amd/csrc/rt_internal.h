@@ -61,6 +61,8 @@ struct RhItem {
   const uint32_t* perm;  // sorted slot -> original triangle id
   const float4* cbox;    // cluster AABBs (min, max)
   const float4* tbox;    // AABB (min, max) of every sorted triangle
+  const float4* tmt;     // ray-bvh: leaf-test operands (v1, e1, e2) of every triangle, original
+                         // order (null for rayhierarchy)
   uint32_t ntri;
   uint32_t nclusters;    // clusters of 64 triangles (rayhierarchy) or 64 frontier roots
   // ray-bvh: BVH nodes (2 float4 each), frontier roots (Morton-ordered; cbox holds the boxes

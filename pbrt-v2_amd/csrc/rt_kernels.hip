@@ -35,6 +35,20 @@ __device__ __forceinline__ Tri load_tri(const float* __restrict__ v9, uint32_t i
   return make_tri(ld3(p), ld3(p + 3), ld3(p + 6));
 }
 
+// Leaf-test operands of a triangle (v1, e1 = v2 - v1, e2 = v3 - v1, precomputed in FP32 on the
+// host) from 3 float4 per triangle; v2/v3 are not filled (the leaf test does not read them).
+// Used where triangles are reached through an index (ray-bvh leaves): three 16-byte loads in
+// place of nine scalar ones.
+__device__ __forceinline__ Tri load_mt(const float4* __restrict__ tmt, uint32_t i) {
+  const float4 a = tmt[3 * (size_t)i], b = tmt[3 * (size_t)i + 1], c = tmt[3 * (size_t)i + 2];
+  Tri T;
+  T.v1 = {a.x, a.y, a.z};
+  T.e1 = {a.w, b.x, b.y};
+  T.e2 = {b.z, b.w, c.x};
+  T.v2 = T.v3 = T.v1;
+  return T;
+}
+
 __device__ __forceinline__ void node_store(float4* __restrict__ nodes, size_t idx, const float* b,
                                            bool valid) {
   float4* p = nodes + 4 * idx;
@@ -716,7 +730,6 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
   const int total = n * G;
   const bool pow2 = (G & (G - 1)) == 0;
   const int lg = __ffs(G) - 1;
-  const float* __restrict__ v9 = it.v9;
   for (int e = threadIdx.x; e < n; e += kThreads) sh.sPair[e] = 0;
   __syncthreads();
   for (int item = threadIdx.x; item < total; item += kThreads) {
@@ -727,7 +740,7 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
     const uint32_t off = sh.sPt[e], cnt = sh.sPc[e];
     for (uint32_t f = 0; f < cnt; ++f) {
       float t, b1, b2;
-      if (leaf_hit<TM>(sh, slot, load_tri(v9, it.prim_order[off + f]), t, b1, b2)) {
+      if (leaf_hit<TM>(sh, slot, load_mt(it.tmt, it.prim_order[off + f]), t, b1, b2)) {
         sh.sPair[e] = 1;
         break;
       }
@@ -751,7 +764,7 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
       for (uint32_t f = 0; f < cnt; ++f) {
         const uint32_t tri = it.prim_order[off + f];
         float t, b1, b2;
-        if (!leaf_hit<TM>(sh, slot, load_tri(v9, tri), t, b1, b2)) continue;
+        if (!leaf_hit<TM>(sh, slot, load_mt(it.tmt, tri), t, b1, b2)) continue;
         if (MODE == 2) {
           if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
           break;

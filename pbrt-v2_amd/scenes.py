@@ -211,6 +211,11 @@ WORKLOADS = {
     # one task of the 1024x1024 x 4 spp render: 256x256 pixels x 4 spp (1024x256 ray image)
     "room_raybvh_task": dict(scene="room", W=256, H=256, spp=4, seed=3, accel="ray-bvh",
                              origin=(0.0, 0.0, -9.0), fov=50.0, light=(1.0, 2.5, -8.0)),
+    # the same task through the path integrator to depth 3: primary, 3 cosine-weighted
+    # bounces, a shadow ray per hit at every depth
+    "room_raybvh_path3": dict(scene="room", W=256, H=256, spp=4, seed=3, accel="ray-bvh",
+                              origin=(0.0, 0.0, -9.0), fov=50.0, light=(1.0, 2.5, -8.0),
+                              depth=3),
 }
 
 
