@@ -119,6 +119,11 @@ int rt_raybvh_build_handoff(const float *v9, uint32_t ntri, uint32_t max_prims,
                             uint32_t bvh_height, rt_gpu_node **nodes, uint32_t *nnodes,
                             uint32_t **roots, uint32_t *nroots, uint32_t **prim_order);
 int rt_raybvh_free_handoff(rt_gpu_node *nodes, uint32_t *roots, uint32_t *prim_order);
+/* Primitive order of pbrt's BVHAccel over the triangles (SAH, 12 buckets, leaves of at most
+ * min(255, max_prims) primitives; bvh.cpp:209-434): order[i] = input index of the triangle at
+ * BVH position i.  RayHieararchy's "bool sort" parameter swaps its primitives into this order
+ * before flattening them (rayhierarchy.cpp:152-155, parameter at :1332).  HOST arrays. */
+int rt_bvh_order(const float *v9, uint32_t ntri, uint32_t max_prims, uint32_t *order);
 
 /* Batched closest hit: RayHieararchy::Intersect (rayhierarchy.cpp:602-813,816-1140),
  * RayBVH::Intersect (rayBVH.cpp:615-836,839-1138), NaiveAccel::Intersect (naive.cpp:204-338).

@@ -32,7 +32,7 @@ EXPORTS = (
     "rt_accel_desc_default", "rt_ctx_create", "rt_ctx_destroy", "rt_scene_create",
     "rt_scene_set_bvh", "rt_scene_destroy", "rt_trace_closest", "rt_trace_occluded",
     "rt_make_shadow_rays", "rt_scene_set_timing", "rt_scene_kernel_time",
-    "rt_error_string", "rt_raybvh_build_handoff", "rt_raybvh_free_handoff", "rt_scene_set_param",
+    "rt_error_string", "rt_raybvh_build_handoff", "rt_raybvh_free_handoff", "rt_bvh_order", "rt_scene_set_param",
     "rt_batch_create", "rt_batch_destroy", "rt_batch_clear", "rt_batch_add_closest",
     "rt_batch_add_occluded", "rt_batch_run", "rt_batch_set_param", "rt_batch_get_param",
     "rt_batch_autotune", "rt_batch_set_timing", "rt_batch_kernel_time",
@@ -126,6 +126,7 @@ def lib() -> ctypes.CDLL:
                                           P(ctypes.c_void_p), P(ctypes.c_uint32), P(ctypes.c_void_p),
                                           P(ctypes.c_uint32), P(ctypes.c_void_p)]
     L.rt_raybvh_free_handoff.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.rt_bvh_order.argtypes = [_fp, ctypes.c_uint32, ctypes.c_uint32, _fp]
     L.rt_batch_create.argtypes = [ctypes.c_void_p, P(ctypes.c_void_p)]
     L.rt_batch_destroy.argtypes = [ctypes.c_void_p]
     L.rt_batch_clear.argtypes = [ctypes.c_void_p]
@@ -193,6 +194,16 @@ def raybvh_handoff(v9: np.ndarray, max_prims: int = 1, bvh_height: int = 3):
     finally:
         L.rt_raybvh_free_handoff(nodes_p, roots_p, order_p)
     return nodes, roots, order
+
+
+def bvh_order(v9: np.ndarray, max_prims: int = 1) -> np.ndarray:
+    """BVHAccel primitive order over the triangles (position -> input index), the order
+    RayHieararchy's "sort" parameter swaps its triangles into."""
+    v9 = np.ascontiguousarray(v9, dtype=np.float32).reshape(-1)
+    ntri = v9.size // 9
+    order = np.zeros(max(ntri, 1), np.uint32)
+    check(lib().rt_bvh_order(v9.ctypes.data, ntri, max_prims, order.ctypes.data), "rt_bvh_order")
+    return order[:ntri].copy()
 
 
 def _torch():

@@ -293,3 +293,13 @@ extern "C" int rt_raybvh_free_handoff(rt_gpu_node* nodes, uint32_t* roots, uint3
   std::free(prim_order);
   return RT_OK;
 }
+
+extern "C" int rt_bvh_order(const float* v9, uint32_t ntri, uint32_t max_prims, uint32_t* order) {
+  if (ntri && (!v9 || !order)) return RT_ERR_INVALID;
+  if (max_prims == 0) return RT_ERR_INVALID;
+  Builder b(max_prims);  // leaves of at most min(255, max_prims) primitives, as BVHAccel
+  b.build(v9, ntri);
+  const std::vector<uint32_t>& o = b.order();
+  std::copy(o.begin(), o.end(), order);
+  return RT_OK;
+}

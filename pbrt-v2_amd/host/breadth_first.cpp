@@ -70,6 +70,20 @@ RayDifferential camera_ray(const RenderSettings& rs, float tan_half, float aspec
   return RayDifferential(rs.origin, Vector(px * inv, py * inv, pz * inv), 0.f, INFINITY);
 }
 
+// pbrt's perspective camera ray through raster point (fx, fy): screen-window point at z = 1
+// scaled by tan(fov / 2), normalized in camera space, then CameraToWorld (the direction is
+// rotated, not renormalized, as Transform::operator()(Ray) does).
+RayDifferential camera_ray_pbrt(const RenderSettings& rs, float tan_half, float fx, float fy) {
+  const float* sw = rs.screen;
+  const float sx = sw[0] + (fx / (float)rs.xres) * (sw[1] - sw[0]);
+  const float sy = sw[3] - (fy / (float)rs.yres) * (sw[3] - sw[2]);
+  const V d = normalize(V{sx * tan_half, sy * tan_half, 1.0f});
+  const float* M = rs.camera_to_world;
+  const Vector w((M[0] * d.x + M[1] * d.y) + M[2] * d.z, (M[4] * d.x + M[5] * d.y) + M[6] * d.z,
+                 (M[8] * d.x + M[9] * d.y) + M[10] * d.z);
+  return RayDifferential(Point(M[3], M[7], M[11]), w, 0.f, INFINITY);
+}
+
 }  // namespace
 
 void TilesInXY(int count, int dx, int dy, int* nx, int* ny) {
@@ -127,7 +141,9 @@ RenderStats RenderBreadthFirst(Aggregate& accel, const std::vector<TriangleMesh>
         for (int k = 0; k < rs.spp; ++k) {
           const size_t j = ((size_t)(y - y0) * w + (x - x0)) * rs.spp + k;
           const float jx = ((float)k + 0.5f) / (float)rs.spp;
-          rays[j] = camera_ray(rs, tan_half, aspect, (float)x + jx, (float)y + 0.5f);
+          rays[j] = rs.has_camera_to_world
+                        ? camera_ray_pbrt(rs, tan_half, (float)x + jx, (float)y + 0.5f)
+                        : camera_ray(rs, tan_half, aspect, (float)x + jx, (float)y + 0.5f);
           pixel[j] = y * rs.xres + x;
         }
     st.camera_rays += count;
@@ -211,6 +227,60 @@ bool WritePFM(const char* path, int xres, int yres, const std::vector<float>& rg
     ok = std::fwrite(&rgb[3 * (size_t)y * xres], sizeof(float), 3 * (size_t)xres, f) ==
          3 * (size_t)xres;
   return std::fclose(f) == 0 && ok;
+}
+
+bool ReadPFM(const char* path, int* xres, int* yres, std::vector<float>* rgb) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return false;
+  char tag[3] = {0, 0, 0};
+  double scale = 0.0;
+  bool ok = std::fscanf(f, "%2s %d %d %lf", tag, xres, yres, &scale) == 4 &&
+            std::strcmp(tag, "PF") == 0 && *xres > 0 && *yres > 0 && scale != 0.0 &&
+            std::fgetc(f) != EOF;  // the single whitespace byte after the scale
+  if (ok) {
+    const size_t row = 3 * (size_t)*xres;
+    rgb->assign(row * (size_t)*yres, 0.f);
+    // PFM stores the bottom row first
+    for (int y = *yres - 1; y >= 0 && ok; --y)
+      ok = std::fread(&(*rgb)[row * (size_t)y], sizeof(float), row, f) == row;
+    const uint16_t probe = 1;
+    const bool host_little = *(const uint8_t*)&probe == 1;
+    if (ok && (scale < 0.0) != host_little)  // negative scale: little-endian data
+      for (float& v : *rgb) {
+        uint32_t u;
+        std::memcpy(&u, &v, 4);
+        u = __builtin_bswap32(u);
+        std::memcpy(&v, &u, 4);
+      }
+  }
+  std::fclose(f);
+  return ok;
+}
+
+ImageDiff DiffImages(const std::vector<float>& a, const std::vector<float>& b, int xres,
+                     int yres, float tol_percent) {
+  ImageDiff d;
+  const size_t n = 3 * (size_t)xres * (size_t)yres;
+  if (a.size() < n || b.size() < n) Severe("DiffImages: image smaller than xres x yres");
+  double sum1 = 0.0, sum2 = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    if (a[i] == 0.f && b[i] == 0.f) continue;
+    sum1 += a[i];
+    sum2 += b[i];
+    const float rel = std::fabs(a[i] - b[i]) / a[i];
+    d.mse += (double)(a[i] - b[i]) * (a[i] - b[i]);
+    if (rel > .005f) ++d.small;
+    if (rel > .05f) ++d.big;
+  }
+  const double count = 3.0 * xres * yres;
+  d.avg1 = sum1 / count;
+  d.avg2 = sum2 / count;
+  const double lo = std::min(d.avg1, d.avg2);
+  d.avg_delta_percent = lo != 0.0 ? 100.0 * (d.avg1 - d.avg2) / lo : (d.avg1 == d.avg2 ? 0.0 : INFINITY);
+  d.mse /= count;
+  d.differ = tol_percent == 0.f ? (d.big > 0 || d.small > 0)
+                                : std::fabs(d.avg_delta_percent) > tol_percent;
+  return d;
 }
 
 }  // namespace pbrt_amd

@@ -11,8 +11,9 @@
 //   path (src/integrators/path.cpp:43-157): up to maxDepth cosine-weighted bounces, each with
 //     its own shadow rays.
 // Surfaces are Lambertian with reflectance `albedo`; the camera is a pinhole at `origin`
-// looking along +z.  Not the reference's integrator code: a driver that issues the same
-// batched accelerator calls in the same order.
+// looking along +z, or pbrt's perspective camera (see RenderSettings).  Not the reference's
+// integrator code: a driver that issues the same batched accelerator calls in the same
+// order.
 #pragma once
 #include <vector>
 
@@ -22,8 +23,16 @@ namespace pbrt_amd {
 
 struct RenderSettings {
   int xres = 128, yres = 128, spp = 1;
+  // Camera.  By default a pinhole at `origin` looking along +z, `fov_deg` over the image
+  // height.  With has_camera_to_world, pbrt's PerspectiveCamera (perspective.cpp:108-145):
+  // camera-space rays through the screen window `screen` (x0, x1, y0, y1; x right, y up,
+  // z forward, `fov_deg` spanning the window's [-1, 1] extent) are taken to world space by
+  // the row-major 3x4 matrix camera_to_world (CameraToWorld); `origin` is its translation.
   Point origin{0.f, 0.f, -3.f};
   float fov_deg = 60.f;
+  bool has_camera_to_world = false;
+  float camera_to_world[12] = {1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f};
+  float screen[4] = {-1.f, 1.f, -1.f, 1.f};
   Point light{2.f, 2.f, -2.f};
   float intensity = 10.f;
   float albedo = 0.5f;
@@ -47,5 +56,21 @@ RenderStats RenderBreadthFirst(Aggregate& accel, const std::vector<TriangleMesh>
 
 // Writes a little-endian colour PFM.
 bool WritePFM(const char* path, int xres, int yres, const std::vector<float>& rgb);
+// Reads a colour PFM of either byte order into rgb (top row first, as RenderBreadthFirst).
+bool ReadPFM(const char* path, int* xres, int* yres, std::vector<float>* rgb);
+
+// Image comparison with the statistics of pbrt's exrdiff (src/tools/exrdiff.cpp:62-112) over
+// the colour channels of two images of one size: samples where both are 0 are skipped; the
+// relative difference |a - b| / a counts as small above 0.5% and big above 5%; the averages
+// are over 3 * xres * yres samples.  With tol_percent == 0 any small or big difference makes
+// the images differ, otherwise an average delta above tol_percent does.  (exrdiff reads RGBA
+// and skips sample 0 of every pixel as if it were alpha; here all three colour channels count.)
+struct ImageDiff {
+  long long small = 0, big = 0;
+  double avg1 = 0, avg2 = 0, avg_delta_percent = 0, mse = 0;
+  bool differ = false;
+};
+ImageDiff DiffImages(const std::vector<float>& a, const std::vector<float>& b, int xres,
+                     int yres, float tol_percent);
 
 }  // namespace pbrt_amd

@@ -166,12 +166,35 @@ struct Aggregate::Staging {
   }
 };
 
-Aggregate::Aggregate(const std::vector<TriangleMesh>& meshes, const rt_accel_desc& desc)
+Aggregate::Aggregate(const std::vector<TriangleMesh>& meshes, const rt_accel_desc& desc,
+                     uint32_t sort_max_prims)
     : desc_(desc), st_(new Staging) {
   std::vector<float> v9, uv6;
   flatten(meshes, &v9, &uv6, &tri_prim_, &tri_shape_);
   ntri_ = (uint32_t)tri_prim_.size();
   if (ntri_ == 0) Severe("accelerator built over a scene without triangles");
+  if (sort_max_prims > 0) {
+    // RayHieararchy "sort": the primitives are swapped into BVHAccel order before they are
+    // flattened (rayhierarchy.cpp:152-155), so ids, and the smallest-id rule among hits at
+    // equal t, follow that order.
+    tri_orig_.resize(ntri_);
+    check(rt_bvh_order(v9.data(), ntri_, sort_max_prims, tri_orig_.data()), "rt_bvh_order");
+    std::vector<float> v9s(v9.size()), uv6s(uv6.size());
+    std::vector<uint32_t> prim(ntri_);
+    std::vector<int32_t> shape(ntri_);
+    for (uint32_t k = 0; k < ntri_; ++k) {
+      const uint32_t o = tri_orig_[k];
+      std::copy(v9.begin() + 9 * (size_t)o, v9.begin() + 9 * (size_t)o + 9, v9s.begin() + 9 * (size_t)k);
+      std::copy(uv6.begin() + 6 * (size_t)o, uv6.begin() + 6 * (size_t)o + 6,
+                uv6s.begin() + 6 * (size_t)k);
+      prim[k] = tri_prim_[o];
+      shape[k] = tri_shape_[o];
+    }
+    v9.swap(v9s);
+    uv6.swap(uv6s);
+    tri_prim_.swap(prim);
+    tri_shape_.swap(shape);
+  }
   check(rt_scene_create(shared_ctx(), v9.data(), uv6.data(), ntri_, &desc_, &scene_),
         "rt_scene_create");
   if (desc_.kind == RT_ACCEL_RAYBVH) {
@@ -290,7 +313,7 @@ void Aggregate::Closest(const RayDifferential* r, Intersection* in, bool* hit, u
     const float* dv = s.h_hit + 6 * c + 3 * (size_t)i;
     is.dg.dpdu = Vector(du[0], du[1], du[2]);
     is.dg.dpdv = Vector(dv[0], dv[1], dv[2]);
-    is.triangle = id;
+    is.triangle = tri_orig_.empty() ? id : (int32_t)tri_orig_[id];
     is.shapeId = tri_shape_[id];
     is.primitiveId = tri_prim_[id];
     is.rayEpsilon = 1e-3f * t;
@@ -371,10 +394,11 @@ static rt_accel_desc rh_desc(int32_t kind, bool onGPU, int chunkX, int chunkY, i
 }
 
 RayHieararchy::RayHieararchy(const std::vector<TriangleMesh>& meshes, bool onGPU, int chunkX,
-                             int chunkY, int height, const std::string& node, bool,
+                             int chunkY, int height, const std::string& node, bool sortVert,
                              const std::string& sm, int maxBVHPrim, unsigned repairRun)
     : Aggregate(meshes, rh_desc(RT_ACCEL_RAYHIERARCHY, onGPU, chunkX, chunkY, height, node, sm,
-                                maxBVHPrim, 0, repairRun)) {}
+                                maxBVHPrim, 0, repairRun),
+                sortVert ? (uint32_t)std::max(maxBVHPrim, 1) : 0u) {}
 void RayHieararchy::Intersect(const RayDifferential* r, Intersection* in, bool* hit,
                               unsigned count) {
   Closest(r, in, hit, count, false);

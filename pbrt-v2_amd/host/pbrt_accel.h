@@ -98,6 +98,14 @@ class ParamSet {
     auto it = strings_.find(n);
     return it == strings_.end() ? d : it->second;
   }
+  // "type name=value" entries separated by ';', by type then name (for reports and tests).
+  std::string Describe() const {
+    std::string s;
+    for (const auto& kv : ints_) s += "integer " + kv.first + "=" + std::to_string(kv.second) + ";";
+    for (const auto& kv : bools_) s += "bool " + kv.first + "=" + (kv.second ? "true;" : "false;");
+    for (const auto& kv : strings_) s += "string " + kv.first + "=" + kv.second + ";";
+    return s;
+  }
 
  private:
   std::map<std::string, int> ints_;
@@ -133,7 +141,10 @@ class Aggregate {
   uint32_t TriangleCount() const { return ntri_; }
 
  protected:
-  Aggregate(const std::vector<TriangleMesh>& meshes, const rt_accel_desc& desc);
+  // sort_max_prims > 0 reorders the triangles into BVHAccel order with leaves of that many
+  // primitives (RayHieararchy's "sort"); Intersection::triangle stays the input index.
+  Aggregate(const std::vector<TriangleMesh>& meshes, const rt_accel_desc& desc,
+            uint32_t sort_max_prims = 0);
   // Ray image for `count` rays: the preprocessed task image (xResolution * spp) x yResolution,
   // grown by whole rows when the batch is larger; one row for naive or without Preprocess.
   void ImageShape(unsigned count, int* w, int* h) const;
@@ -145,6 +156,7 @@ class Aggregate {
   uint32_t ntri_ = 0;
   std::vector<uint32_t> tri_prim_;   // triangle -> primitiveId
   std::vector<int32_t> tri_shape_;   // triangle -> shapeId
+  std::vector<uint32_t> tri_orig_;   // with "sort": device triangle -> input index
   int xres_ = 0, yres_ = 0;
   unsigned spp_ = 1;
 

@@ -2,7 +2,9 @@
 // (pbrt-v2_amd/libpbrt_accel.so) the way pbrt's breadth-first renderer does: MakeAccelerator
 // from a ParamSet, Preprocess with the task image, then batched Intersect / IntersectP on
 // pbrt rays.  The GPU tests call it through ctypes and compare with the oracle.
+#include <algorithm>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -11,6 +13,7 @@
 
 #include "../../pbrt-v2_amd/host/breadth_first.h"
 #include "../../pbrt-v2_amd/host/pbrt_accel.h"
+#include "../../pbrt-v2_amd/host/pbrt_scene.h"
 
 using namespace pbrt_amd;
 
@@ -151,6 +154,134 @@ int mirror_render(const char* accel, const char* params, const float* v9, uint32
     stats[4] = (int64_t)st.shadow_rays;
     stats[5] = (int64_t)st.bounce_rays;
     if (pfm_path && !WritePFM(pfm_path, xres, yres, img)) throw std::runtime_error("PFM write");
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Parses a pbrt scene file.  v9: room for `cap` triangles, filled with the world-space
+// triangles in the order the accelerators flatten them.  info: "key=value" lines (accelerator,
+// params, integrator, renderer, sampler, film, spp, fov, camera_to_world, screen, light,
+// intensity, max_depth, has_point_light, meshes, skipped, then one "message=" line each).
+// Returns the triangle count (which may exceed cap), or -1 with mirror_last_error().
+int64_t mirror_parse(const char* path, float* v9, uint64_t cap, char* info, uint64_t info_cap) {
+  SetSevereHandler(throw_severe);
+  try {
+    SceneFile sc;
+    ParseSceneFile(path, &sc);
+    uint64_t n = 0;
+    for (const TriangleMesh& m : sc.meshes)
+      for (size_t t = 0; t + 2 < m.vi.size(); t += 3, ++n) {
+        if (n >= cap || !v9) continue;
+        for (int k = 0; k < 3; ++k) {
+          const Point& q = m.p[m.vi[t + k]];
+          v9[9 * n + 3 * k] = q.x;
+          v9[9 * n + 3 * k + 1] = q.y;
+          v9[9 * n + 3 * k + 2] = q.z;
+        }
+      }
+    const RenderSettings& rs = sc.render;
+    std::string s;
+    auto f = [](float x) {
+      char b[32];
+      std::snprintf(b, sizeof b, "%.9g", x);
+      return std::string(b);
+    };
+    s += "accelerator=" + sc.accelerator + "\n";
+    s += "params=" + sc.acceleratorParams.Describe() + "\n";
+    s += "integrator=" + sc.surfaceIntegrator + "\nrenderer=" + sc.renderer + "\n";
+    s += "sampler=" + sc.sampler + "\n";
+    s += "film=" + std::to_string(rs.xres) + "," + std::to_string(rs.yres) + "\n";
+    s += "spp=" + std::to_string(rs.spp) + "\nfov=" + f(rs.fov_deg) + "\n";
+    s += "camera_to_world=";
+    for (int i = 0; i < 12; ++i) s += f(rs.camera_to_world[i]) + (i < 11 ? "," : "\n");
+    s += "screen=";
+    for (int i = 0; i < 4; ++i) s += f(rs.screen[i]) + (i < 3 ? "," : "\n");
+    s += "light=" + f(rs.light.x) + "," + f(rs.light.y) + "," + f(rs.light.z) + "\n";
+    s += "intensity=" + f(rs.intensity) + "\nmax_depth=" + std::to_string(rs.max_depth) + "\n";
+    s += std::string("has_point_light=") + (sc.hasPointLight ? "1" : "0") + "\n";
+    s += "meshes=" + std::to_string(sc.meshes.size()) + "\n";
+    s += "skipped=" + std::to_string(sc.skippedShapes) + "\n";
+    for (const std::string& m : sc.messages) s += "message=" + m + "\n";
+    if (info && info_cap) {
+      const size_t k = std::min<size_t>(s.size(), info_cap - 1);
+      std::memcpy(info, s.data(), k);
+      info[k] = 0;
+    }
+    return (int64_t)n;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// Renders a pbrt scene file with RenderScene (accel_override: "" for the file's Accelerator).
+// rgb: room for rgb_cap floats (3 per pixel of the film).  stats as mirror_render.
+int mirror_render_scene(const char* path, const char* accel_override, float* rgb,
+                        uint64_t rgb_cap, int64_t* stats, const char* pfm_path) {
+  SetSevereHandler(throw_severe);
+  try {
+    SceneFile sc;
+    ParseSceneFile(path, &sc);
+    std::vector<float> img;
+    const RenderStats st = RenderScene(sc, accel_override ? accel_override : "", &img);
+    if (img.size() > rgb_cap) throw std::runtime_error("rgb buffer too small");
+    std::memcpy(rgb, img.data(), sizeof(float) * img.size());
+    stats[0] = st.tasks;
+    stats[1] = st.nx;
+    stats[2] = st.ny;
+    stats[3] = (int64_t)st.camera_rays;
+    stats[4] = (int64_t)st.shadow_rays;
+    stats[5] = (int64_t)st.bounce_rays;
+    if (pfm_path && !WritePFM(pfm_path, sc.render.xres, sc.render.yres, img))
+      throw std::runtime_error("PFM write");
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// WritePFM then ReadPFM of an image: rgb_out gets what was read back; returns 0 when both
+// succeeded with the same size.
+int mirror_pfm_roundtrip(const char* path, int xres, int yres, const float* rgb_in,
+                         float* rgb_out) {
+  SetSevereHandler(throw_severe);
+  try {
+    std::vector<float> img(rgb_in, rgb_in + 3 * (size_t)xres * yres), back;
+    if (!WritePFM(path, xres, yres, img)) throw std::runtime_error("WritePFM failed");
+    int x = 0, y = 0;
+    if (!ReadPFM(path, &x, &y, &back)) throw std::runtime_error("ReadPFM failed");
+    if (x != xres || y != yres) throw std::runtime_error("PFM size changed");
+    std::memcpy(rgb_out, back.data(), sizeof(float) * back.size());
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// ReadPFM into rgb (room for cap floats); returns 0 and the size, or 1.
+int mirror_read_pfm(const char* path, int* xres, int* yres, float* rgb, uint64_t cap) {
+  std::vector<float> img;
+  if (!ReadPFM(path, xres, yres, &img) || img.size() > cap) return 1;
+  std::memcpy(rgb, img.data(), sizeof(float) * img.size());
+  return 0;
+}
+
+// DiffImages: out = {small, big, avg1, avg2, avg_delta_percent, mse, differ}.
+int mirror_diff_images(const float* a, const float* b, int xres, int yres, float tol,
+                       double* out) {
+  SetSevereHandler(throw_severe);
+  try {
+    const size_t n = 3 * (size_t)xres * yres;
+    const ImageDiff d = DiffImages(std::vector<float>(a, a + n), std::vector<float>(b, b + n),
+                                   xres, yres, tol);
+    const double o[7] = {(double)d.small, (double)d.big, d.avg1, d.avg2, d.avg_delta_percent,
+                         d.mse, d.differ ? 1.0 : 0.0};
+    std::memcpy(out, o, sizeof o);
     return 0;
   } catch (const std::exception& e) {
     g_err = e.what();

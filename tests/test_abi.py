@@ -53,3 +53,16 @@ def test_handoff_matches_oracle_builder(rt, orc, scenes):
         b = orc.raybvh_handoff(v9, 1, 3)
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+
+
+def test_bvh_order_matches_oracle_builder(rt, orc, scenes):
+    # the "sort" order of RayHieararchy: BVHAccel's primitive order, against the oracle's own
+    # BVHAccel restatement, at leaf sizes 1 and 4 and over a scene with coincident centroids
+    dup = np.repeat(scenes.random_triangles(200, seed=8), 3, axis=0)
+    for v9 in (scenes.random_triangles(3000, seed=5), scenes.cluster_scene(20000, seed=6), dup):
+        for mp in (1, 4):
+            a = rt.bvh_order(v9, mp)
+            b = orc.bvh_build(v9, max_prims=mp, gpu=False)[2]
+            assert np.array_equal(np.sort(a), np.arange(len(v9)))
+            assert np.array_equal(a, b)
+    assert rt.lib().rt_bvh_order(None, 3, 1, None) == 1

@@ -145,6 +145,28 @@ def test_layer_primary_and_shadow(scenes, orc, accel, node):
 
 
 @pytest.mark.gpu
+def test_layer_sort_param(scenes, orc, rt):
+    """RayHieararchy "bool sort": the triangles are traced in BVHAccel order
+    (rayhierarchy.cpp:152-155), so among hits at equal t the smallest SORTED id wins; the
+    Intersection still names the triangle by its input index.  The scene holds coincident
+    duplicates (input ids i and n + i) so that the tie rule is exercised."""
+    base = scenes.torus_scene()
+    v9 = np.concatenate([base, base[:3000]], axis=0)
+    order = rt.bvh_order(v9, 1)
+    full = scenes.pinhole_rays(512, 512)
+    rays = scenes.subimage(full, 192, 256, 128, 128)
+    rays["width"], rays["height"] = 128, 128
+    out = run("rayhierarchy", "sort=true", v9, rays, 0, pre=(128, 128, 1))
+    p = orc.params("rayhierarchy")
+    ref = orc.trace(v9[order], _closest_rays(rays), p, orc.CLOSEST_PRIMARY)
+    ids = np.where(ref["id"] >= 0, order[np.maximum(ref["id"], 0)].astype(np.int32), -1)
+    plain = orc.trace(v9, _closest_rays(rays), p, orc.CLOSEST_PRIMARY)["id"]
+    print(f"[sort] rays whose winner differs from the unsorted order: "
+          f"{int(np.count_nonzero(ids != plain))}", flush=True)
+    expect_closest(out, ids, ref["t"], rays, orc, v9, None, "rayhierarchy sort")
+
+
+@pytest.mark.gpu
 def test_layer_bounce_rays_and_partial_batch(scenes, orc):
     """Bounce Intersect: hit[] selects the rays; a batch smaller than the preprocessed task
     image is traced as a padded image whose padding rays are invalid."""
