@@ -142,6 +142,152 @@ def bench_uneven(args, rt, scenes, torch):
     print(json.dumps(result))
 
 
+def bench_frame(args, rt, scenes, torch, wl):
+    """A whole frame as the renderer's tasks: the image is cut into tiles (one ray image of
+    (tile width * spp) x tile height each).  Every pass (camera, then per depth the shadow and
+    bounce rays) runs for all tasks either as one batch of jobs (batched form, HIP graph) or
+    task by task as breadth-first.cpp runs them (plain form); both must give the same bits."""
+    v9 = scenes.make_scene(wl["scene"])
+    W, H, spp = wl["W"], wl["H"], wl.get("spp", 1)
+    tx, ty = wl["tiles"]
+    depth = wl.get("depth", 0)
+    light = wl["light"]
+    print(f"[bench] frame {W}x{H}x{spp}spp in {tx * ty} tasks, depth {depth}", file=sys.stderr,
+          flush=True)
+    full = scenes.pinhole_rays(W, H, origin=wl["origin"], fov_deg=wl["fov"], spp=spp,
+                               seed=wl.get("seed"))
+    tw, th = W * spp // tx, H // ty
+    tasks = [scenes.subimage(full, x * tw, y * th, tw, th) for y in range(ty) for x in range(tx)]
+    del full
+    ctx = rt.Context(0)
+    sc = rt.Scene(ctx, v9, desc=wl["accel"])
+    stream = torch.cuda.Stream()
+    T = len(tasks)
+    prim = [rt.DeviceRays.from_numpy(r) for r in tasks]
+    n = sum(p.count for p in prim)
+
+    def buffers():
+        """Device buffers of one form: hits, shadow rays and flags, bounce rays per depth."""
+        hp = [rt.DeviceHits.empty(p.count) for p in prim]
+        sh = [[None] * T for _ in range(depth + 1)]
+        oc = [[torch.empty(p.count, dtype=torch.uint8, device="cuda") for p in prim]
+              for _ in range(depth + 1)]
+        bo = [[None] * T for _ in range(depth)]
+        bh = [[rt.DeviceHits.empty(p.count) for p in prim] for _ in range(depth)]
+        # shapes for every pass: run the chain once eagerly
+        for k in range(T):
+            sc.trace_closest(prim[k], rt.RT_CLOSEST_PRIMARY, hits=hp[k], stream=stream)
+            sh[0][k] = rt.make_shadow_rays(prim[k], hp[k], light, stream=stream)
+            src, srch = prim[k], hp[k]
+            for d in range(depth):
+                bo[d][k] = rt.make_bounce_rays(sc, src, srch, seed=17 + k, depth=d + 1,
+                                               stream=stream)
+                sc.trace_closest(bo[d][k], rt.RT_CLOSEST_SECONDARY, hits=bh[d][k], stream=stream)
+                sh[d + 1][k] = rt.make_shadow_rays(bo[d][k], bh[d][k], light, stream=stream)
+                src, srch = bo[d][k], bh[d][k]
+        return hp, sh, oc, bo, bh
+
+    P = buffers()
+    B = buffers()
+
+    def plain_step():
+        hp, sh, oc, bo, bh = P
+        for k in range(T):  # breadth-first.cpp: one task after the other
+            sc.trace_closest(prim[k], rt.RT_CLOSEST_PRIMARY, hits=hp[k], stream=stream)
+            rt.make_shadow_rays(prim[k], hp[k], light, stream=stream, out=sh[0][k])
+            sc.trace_occluded(sh[0][k], out=oc[0][k], stream=stream)
+            src, srch = prim[k], hp[k]
+            for d in range(depth):
+                rt.make_bounce_rays(sc, src, srch, seed=17 + k, depth=d + 1, stream=stream,
+                                    out=bo[d][k])
+                sc.trace_closest(bo[d][k], rt.RT_CLOSEST_SECONDARY, hits=bh[d][k], stream=stream)
+                rt.make_shadow_rays(bo[d][k], bh[d][k], light, stream=stream, out=sh[d + 1][k])
+                sc.trace_occluded(sh[d + 1][k], out=oc[d + 1][k], stream=stream)
+                src, srch = bo[d][k], bh[d][k]
+
+    # batched: one batch per pass, every task a job of it
+    hp, sh, oc, bo, bh = B
+    b_cam = rt.Batch(ctx)
+    for k in range(T):
+        b_cam.add_closest(sc, prim[k], rt.RT_CLOSEST_PRIMARY, hits=hp[k])
+    b_occ = []
+    for d in range(depth + 1):
+        b = rt.Batch(ctx)
+        for k in range(T):
+            b.add_occluded(sc, sh[d][k], out=oc[d][k])
+        b_occ.append(b)
+    b_sec = []
+    for d in range(depth):
+        b = rt.Batch(ctx)
+        for k in range(T):
+            b.add_closest(sc, bo[d][k], rt.RT_CLOSEST_SECONDARY, hits=bh[d][k])
+        b_sec.append(b)
+    for b in [b_cam] + b_occ + b_sec:
+        b.set_param(rt.RT_PARAM_GRAPH, 1)
+
+    def batch_step():
+        b_cam.run(stream=stream)
+        for k in range(T):
+            rt.make_shadow_rays(prim[k], hp[k], light, stream=stream, out=sh[0][k])
+        b_occ[0].run(stream=stream)
+        for d in range(depth):
+            for k in range(T):
+                src, srch = (prim[k], hp[k]) if d == 0 else (bo[d - 1][k], bh[d - 1][k])
+                rt.make_bounce_rays(sc, src, srch, seed=17 + k, depth=d + 1, stream=stream,
+                                    out=bo[d][k])
+            b_sec[d].run(stream=stream)
+            for k in range(T):
+                rt.make_shadow_rays(bo[d][k], bh[d][k], light, stream=stream, out=sh[d + 1][k])
+            b_occ[d + 1].run(stream=stream)
+
+    def timed(fn, label):
+        for _ in range(args.warmup):
+            fn()
+        stream.synchronize()
+        print(f"[bench] timing {args.steps} {label} steps", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        stream.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    t_plain = timed(plain_step, "plain")
+    t_batch = timed(batch_step, "batched")
+    n_shadow = sum(int(x.valid.sum().item()) for row in B[1] for x in row)
+    n_bounce = sum(int(x.valid.sum().item()) for row in B[3] for x in row)
+    rays_per_step = n + n_shadow + n_bounce
+
+    def neq(a, b):
+        return int((a != b).sum().item())
+    mism_id = sum(neq(B[0][k].id, P[0][k].id) for k in range(T)) + sum(
+        neq(B[4][d][k].id, P[4][d][k].id) for d in range(depth) for k in range(T))
+    mism_t = sum(neq(B[0][k].t.view(torch.int32), P[0][k].t.view(torch.int32)) for k in range(T))
+    mism_occ = sum(neq(B[2][d][k], P[2][d][k]) for d in range(depth + 1) for k in range(T))
+    result = {
+        "metric": "Mrays/s batched vs plain per-scene launch, same hits; p50/p99 request latency",
+        "value": round(rays_per_step / t_batch / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "form": f"batched ({T} task jobs per pass, HIP graph)",
+                   "accelerator": wl["accel"], "triangles": int(len(v9)),
+                   "image": f"{W}x{H}x{spp}spp", "tasks": T, "task_ray_image": f"{tw}x{th}",
+                   "primary_rays": n, "shadow_rays": n_shadow, "bounce_rays": n_bounce,
+                   "path_depth": depth, "leaf": "4x4", "height": 3, "node": "box_dir"},
+        "plain": {"value": round(rays_per_step / t_plain / 1e6, 3),
+                  "ms_per_step": round(t_plain * 1e3, 3)},
+        "batched_over_plain": round(t_plain / t_batch, 3),
+        "parity": {"id_mismatches": mism_id, "t_mismatches": mism_t,
+                   "occlusion_mismatches": mism_occ},
+        "roofline": None, "cpu_baseline": None,
+    }
+    for b in [b_cam] + b_occ + b_sec:
+        b.close()
+    sc.close()
+    ctx.close()
+    print(json.dumps(result))
+
+
 def bench_serving(rt, scenes, torch, rates, nreq, seed0=200):
     """Serving form: 16 resident cluster scenes (triangle counts log-uniform in [1e3, 1e6]),
     Poisson arrivals at each rate, resolutions uniform in {128, 256, 512, 1024} squared,
@@ -205,7 +351,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="torus_rayhierarchy_512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--serve-rates", default="20,100,400",
+    ap.add_argument("--serve-rates", default="10,20,50,100,200,500",
                     help="request rates (req/s) of the serving leg; empty to skip it")
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--autotune", action="store_true",
@@ -228,6 +374,9 @@ def main():
         bench_uneven(args, rt, scenes, torch)
         return
     wl = scenes.WORKLOADS[args.workload]
+    if "tiles" in wl:
+        bench_frame(args, rt, scenes, torch, wl)
+        return
     print(f"[bench] workload {args.workload}: building scene", file=sys.stderr, flush=True)
     v9 = scenes.make_scene(wl["scene"])
     rays_np = scenes.pinhole_rays(wl["W"], wl["H"], origin=wl.get("origin", (0.0, 0.0, -3.0)),
@@ -375,12 +524,21 @@ def main():
             bvh.closest(rays_np, nthreads=threads)
             bvh.occluded(sh_np, nthreads=threads)
             times.append(time.perf_counter() - c0)
+        # one core, for the per-core rate
+        times1 = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            bvh.closest(rays_np, nthreads=1)
+            bvh.occluded(sh_np, nthreads=1)
+            times1.append(time.perf_counter() - c0)
         bvh.close()
         med = statistics.median(times)
-        cpu = {"value": round((n + int(sh_np["valid"].sum())) / med / 1e6, 3), "unit": "Mrays/s",
+        nrays = n + int(sh_np["valid"].sum())
+        cpu = {"value": round(nrays / med / 1e6, 3), "unit": "Mrays/s",
                "cores": threads, "kind": "port",
                "sample": f"full workload (primary + shadow rays), median of 5 runs, "
-                         f"pbrt BVHAccel SAH maxnodeprims 4 restated in oracle/"}
+                         f"pbrt BVHAccel SAH maxnodeprims 4 restated in oracle/",
+               "value_1core": round(nrays / statistics.median(times1) / 1e6, 3)}
     result["roofline"] = roofline
     result["cpu_baseline"] = cpu
     if args.serve_rates:

@@ -382,8 +382,9 @@ def make_bounce_rays(scene: "Scene", rays: DeviceRays, hits: DeviceHits, seed: i
 
 
 class Batch:
-    """Batched form: many rayhierarchy jobs (scenes x ray images, closest and occlusion) run as
-    one launch sequence per (mode, node type) group with a shared persistent traversal kernel.
+    """Batched form: many rayhierarchy or ray-bvh jobs (scenes x ray images, closest and
+    occlusion) run as one launch sequence per (mode, node type, accelerator) group with a
+    shared persistent traversal kernel.
     The jobs' tensors are kept referenced until the batch is cleared."""
 
     def __init__(self, ctx: Context):

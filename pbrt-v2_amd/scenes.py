@@ -216,6 +216,14 @@ WORKLOADS = {
     "room_raybvh_path3": dict(scene="room", W=256, H=256, spp=4, seed=3, accel="ray-bvh",
                               origin=(0.0, 0.0, -9.0), fov=50.0, light=(1.0, 2.5, -8.0),
                               depth=3),
+    # the whole 1024x1024 x 4 spp frame as the renderer's 16 tasks (4 x 4 tiles of 256x256
+    # pixels, each a 1024x256 ray image), direct lighting / path depth 3
+    "room_raybvh_frame": dict(scene="room", W=1024, H=1024, spp=4, seed=3, accel="ray-bvh",
+                              origin=(0.0, 0.0, -9.0), fov=50.0, light=(1.0, 2.5, -8.0),
+                              tiles=(4, 4)),
+    "room_raybvh_frame_path3": dict(scene="room", W=1024, H=1024, spp=4, seed=3,
+                                    accel="ray-bvh", origin=(0.0, 0.0, -9.0), fov=50.0,
+                                    light=(1.0, 2.5, -8.0), tiles=(4, 4), depth=3),
 }
 
 

@@ -148,3 +148,41 @@ def test_batch_graph_replay_and_autotune(rt, scenes, gpu_ctx):
     b.close()
     for s in sc.values():
         s.close()
+
+
+def test_batch_raybvh_task_jobs(rt, scenes, gpu_ctx):
+    """ray-bvh jobs in a batch, as a renderer's tasks of one frame: four task windows of one
+    scene (camera rays, then bounce rays from their hits) and their shadow rays; every job
+    bit-identical to its plain call.  The plain calls are pinned against the oracle by the
+    parity tests."""
+    v9 = scenes.room_scene(nclusters=40)
+    sc = rt.Scene(gpu_ctx, v9, desc="ray-bvh")
+    full = scenes.pinhole_rays(128, 128, origin=(0.0, 0.0, -9.0), fov_deg=50.0)
+    wins = []
+    for y0 in (0, 64):
+        for x0 in (0, 64):
+            w = scenes.subimage(full, x0, y0, 64, 64)
+            w["width"], w["height"] = 64, 64
+            wins.append(rt.DeviceRays.from_numpy(w))
+    plain = [sc.trace_closest(d, rt.RT_CLOSEST_PRIMARY) for d in wins]
+    bounce = [rt.make_bounce_rays(sc, d, h, seed=5 + k, depth=1)
+              for k, (d, h) in enumerate(zip(wins, plain))]
+    plain_b = [sc.trace_closest(d, rt.RT_CLOSEST_SECONDARY) for d in bounce]
+    shadow = [rt.make_shadow_rays(d, h, (1.0, 2.5, -8.0)) for d, h in zip(wins, plain)]
+    plain_o = [sc.trace_occluded(d) for d in shadow]
+    b = rt.Batch(gpu_ctx)
+    outs = [b.add_closest(sc, d, rt.RT_CLOSEST_PRIMARY) for d in wins]
+    outs_b = [b.add_closest(sc, d, rt.RT_CLOSEST_SECONDARY) for d in bounce]
+    occs = [b.add_occluded(sc, d) for d in shadow]
+    b.set_param(rt.RT_PARAM_GRAPH, 1)
+    print(f"[batch] ray-bvh: {len(outs) + len(outs_b)} closest + {len(occs)} occlusion jobs",
+          flush=True)
+    for rep in range(3):  # eager capture run, then graph replays
+        b.run()
+        for k in range(len(wins)):
+            _same(_np(outs[k]), _np(plain[k]), f"rep {rep} task {k} primary")
+            _same(_np(outs_b[k]), _np(plain_b[k]), f"rep {rep} task {k} bounce")
+            assert bool((occs[k] == plain_o[k]).all()), f"rep {rep} task {k} occlusion"
+    assert sum(int((h.id >= 0).sum().item()) for h in plain_b) > 0
+    b.close()
+    sc.close()
