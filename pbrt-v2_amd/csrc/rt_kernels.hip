@@ -14,6 +14,7 @@
 // list, and every ray then runs the leaf test against the list.  The nearest hit is reduced
 // in registers as a (t, id) key, so the result is deterministic (ties -> smallest id).
 #include <algorithm>
+#include <type_traits>
 
 #include "rt_device.h"
 #include "rt_internal.h"
@@ -28,10 +29,33 @@ constexpr uint32_t kNoKey32 = 0xffffffffu;
 
 __device__ __forceinline__ int pow4_off(int s) { return ((1 << (2 * s)) - 1) / 3; }
 
-__device__ __forceinline__ V3 ld3(const float* __restrict__ p) { return {p[0], p[1], p[2]}; }
+// Pointers known to address global memory.  Loads and stores through them compile to global_*
+// instructions; a generic pointer read from a job table compiles to flat_*, whose completion
+// is also counted with the LDS operations, so LDS and memory waits serialize.
+#define RT_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ RT_GLOBAL T* glb(T* p) {
+  return (RT_GLOBAL T*)p;
+}
 
-__device__ __forceinline__ Tri load_tri(const float* __restrict__ v9, uint32_t i) {
-  const float* p = v9 + 9 * (size_t)i;
+// float4 element i through a generic or a global pointer (the global one is read as the
+// native 4-vector, which has no address-space restriction on its copy).
+typedef float rt_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld4(const float4* p, size_t i) { return p[i]; }
+__device__ __forceinline__ float4 ld4(RT_GLOBAL const float4* p, size_t i) {
+  const rt_f4 v = ((RT_GLOBAL const rt_f4*)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 ld4(RT_GLOBAL float4* p, size_t i) {
+  return ld4((RT_GLOBAL const float4*)p, i);
+}
+
+template <class P>
+__device__ __forceinline__ V3 ld3(P p) { return {p[0], p[1], p[2]}; }
+
+template <class P>
+__device__ __forceinline__ Tri load_tri(P v9, uint32_t i) {
+  const P p = v9 + 9 * (size_t)i;
   return make_tri(ld3(p), ld3(p + 3), ld3(p + 6));
 }
 
@@ -39,8 +63,10 @@ __device__ __forceinline__ Tri load_tri(const float* __restrict__ v9, uint32_t i
 // host) from 3 float4 per triangle; v2/v3 are not filled (the leaf test does not read them).
 // Used where triangles are reached through an index (ray-bvh leaves): three 16-byte loads in
 // place of nine scalar ones.
-__device__ __forceinline__ Tri load_mt(const float4* __restrict__ tmt, uint32_t i) {
-  const float4 a = tmt[3 * (size_t)i], b = tmt[3 * (size_t)i + 1], c = tmt[3 * (size_t)i + 2];
+template <class P>
+__device__ __forceinline__ Tri load_mt(P tmt, uint32_t i) {
+  const float4 a = ld4(tmt, 3 * (size_t)i), b = ld4(tmt, 3 * (size_t)i + 1),
+               c = ld4(tmt, 3 * (size_t)i + 2);
   Tri T;
   T.v1 = {a.x, a.y, a.z};
   T.e1 = {a.w, b.x, b.y};
@@ -232,8 +258,9 @@ __global__ __launch_bounds__(256) void rh_init_kernel(const RhItem* __restrict__
 }
 
 // Loads the subtree of block (K, J) into LDS: sub-level s = L-1-l, row-major within it.
-__device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J,
-                                             const float4* __restrict__ nodes, Node* sN, int* sV) {
+template <class P>
+__device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J, P nodes, Node* sN,
+                                             int* sV) {
   const int top = g.L - 1;
   for (int i = threadIdx.x; i < g.nsub; i += kThreads) {
     int s = 0;
@@ -243,8 +270,8 @@ __device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J,
     int valid = 0;
     Node n{};
     if (gk < g.gw[l] && gj < g.gh[l]) {
-      const float4* p = nodes + 4 * ((size_t)g.off[l] + (size_t)gj * g.gw[l] + gk);
-      float4 a = p[0], b = p[1], c = p[2], d = p[3];
+      const P p = nodes + 4 * ((size_t)g.off[l] + (size_t)gj * g.gw[l] + gk);
+      float4 a = ld4(p, 0), b = ld4(p, 1), c = ld4(p, 2), d = ld4(p, 3);
       n.dmin = {a.x, a.y, a.z};
       n.dmax = {b.x, b.y, b.z};
       n.omin = {c.x, c.y, c.z};
@@ -298,11 +325,10 @@ __device__ __forceinline__ bool cull_bound(const Node& n, const Tri& T) {
 }
 // Bound of sorted triangle `slot`: intersectsTri's bound needs the vertices; the others
 // depend on the triangle's AABB only, which is stored per sorted triangle.
-template <int MODE, int NODE>
-__device__ __forceinline__ bool cull_bound_slot(const Node& n, const float* __restrict__ v9s,
-                                                const float4* __restrict__ tbox, uint32_t slot) {
+template <int MODE, int NODE, class PF, class P4>
+__device__ __forceinline__ bool cull_bound_slot(const Node& n, PF v9s, P4 tbox, uint32_t slot) {
   if (MODE == 0 && NODE == 0) return prefilter_tri(n, load_tri(v9s, slot));
-  const float4 lo = tbox[2 * (size_t)slot], hi = tbox[2 * (size_t)slot + 1];
+  const float4 lo = ld4(tbox, 2 * (size_t)slot), hi = ld4(tbox, 2 * (size_t)slot + 1);
   const V3 bmin = {lo.x, lo.y, lo.z}, bmax = {hi.x, hi.y, hi.z};
   if (NODE == 1) return cull_ia(n, bmin, bmax);
   return prefilter_node6(n, bmin, bmax);
@@ -338,6 +364,15 @@ struct Cursors {
   }
 };
 
+// Phase C pair state.  Closest hits: 1 = some ray of the pair passes its leaf test, | kPairOk =
+// the exact cull chain holds.  Occlusion with at most 31 rays per leaf: bit r = ray r of the
+// leaf passes its leaf test, kPairOk as above; the verified bits are applied without running
+// the leaf test again (it is the same deterministic test on the same operands).
+template <int MODE>
+using PairState = typename std::conditional<MODE == 2, uint32_t, uint8_t>::type;
+template <int MODE>
+constexpr PairState<MODE> kPairOk = MODE == 2 ? (PairState<MODE>)0x80000000u : (PairState<MODE>)2;
+
 template <int LMAX>
 struct SubSize {
   static constexpr int value = ((1 << (2 * LMAX)) - 1) / 3;
@@ -352,7 +387,7 @@ struct RhShared {
   unsigned long long sKey[MODE == 2 ? 1 : RBMAX];
   uint32_t sAux[RBMAX];  // repair key (primary) or occlusion flag (shadow)
   uint8_t sRv[RBMAX];    // ray exists and is traced
-  uint8_t sPair[kQ0Cap]; // phase C pair state: 1 = some ray passes, 2 = exact chain verified
+  PairState<MODE> sPair[kQ0Cap];  // phase C pair state
   uint32_t sQ0[kQ0Cap];
   uint32_t sQ[LMAX > 1 ? LMAX : 2][kQCap];  // level s survivors live in sQ[s], s >= 1
   int sCnt[kMaxLevels];
@@ -387,50 +422,68 @@ __device__ __forceinline__ bool leaf_hit(const SH& sh, int slot, const Tri& T, f
 // Phase C over `n` (triangle slot, leaf) pairs.  Item = (pair, ray of its leaf): consecutive
 // lanes take consecutive ray slots of one pair, so the LDS ray reads are conflict-free and
 // the triangle load is a broadcast.
-template <int MODE, int NODE, int RBMAX, int LMAX>
-__device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restrict__ v9s,
-                                          const uint32_t* __restrict__ perm, const uint32_t* q,
+template <int MODE, int NODE, int RBMAX, int LMAX, class PF, class PU>
+__device__ __forceinline__ void leaf_rays(const RHGeom& g, PF v9s, PU perm, const uint32_t* q,
                                           int n, RhShared<MODE, RBMAX, LMAX>& sh) {
   const int G = g.g;
   const int total = n * G;
   const bool pow2 = (G & (G - 1)) == 0;
   const int lg = __ffs(G) - 1;
+  const bool masked = MODE == 2 && G <= 31;
   for (int e = threadIdx.x; e < n; e += kThreads) sh.sPair[e] = 0;
   __syncthreads();
-  // C1: leaf tests; flag pairs with a passing test
+  // C1: leaf tests; flag pairs with a passing test (occlusion: the passing rays).  A ray found
+  // occluded has its sRv cleared, so it takes no further tests.
   for (int item = threadIdx.x; item < total; item += kThreads) {
     const int e = pow2 ? item >> lg : item / G;
     const uint32_t P = q[e];
-    const int slot = (int)(P & 63u) * G + (item - e * G);
+    const int r = item - e * G;
+    const int slot = (int)(P & 63u) * G + r;
     if (!sh.sRv[slot]) continue;
-    if (MODE == 2 && sh.sAux[slot]) continue;
     float t, b1, b2;
-    if (leaf_hit<MODE>(sh, slot, load_tri(v9s, P >> 6), t, b1, b2)) sh.sPair[e] = 1;
+    if (leaf_hit<MODE>(sh, slot, load_tri(v9s, P >> 6), t, b1, b2)) {
+      if constexpr (MODE == 2) {
+        if (masked) atomicOr(&sh.sPair[e], 1u << r);
+        else sh.sPair[e] = 1;
+      } else {
+        sh.sPair[e] = 1;
+      }
+    }
   }
   __syncthreads();
   // C2: exact cull chain of the flagged pairs
   int verified = 0;
   for (int e = threadIdx.x; e < n; e += kThreads) {
-    if (sh.sPair[e] != 1) continue;
+    const PairState<MODE> ps = sh.sPair[e];
+    if (ps == 0) continue;
     const uint32_t P = q[e];
     const bool ok = exact_chain<MODE, NODE, RBMAX, LMAX>(g, sh, load_tri(v9s, P >> 6), (int)(P & 63u));
-    sh.sPair[e] = ok ? 2 : 0;
+    sh.sPair[e] = ok ? (PairState<MODE>)(ps | kPairOk<MODE>) : (PairState<MODE>)0;
     verified |= ok;
   }
   if (!__syncthreads_or(verified)) return;
   // C3: apply the leaf tests of verified pairs
   for (int item = threadIdx.x; item < total; item += kThreads) {
     const int e = pow2 ? item >> lg : item / G;
-    if (sh.sPair[e] != 2) continue;
+    const PairState<MODE> ps = sh.sPair[e];
+    if (!(ps & kPairOk<MODE>)) continue;
     const uint32_t P = q[e];
     const uint32_t slot_t = P >> 6;
-    const int slot = (int)(P & 63u) * G + (item - e * G);
+    const int r = item - e * G;
+    const int slot = (int)(P & 63u) * G + r;
+    if (masked) {  // occlusion: the rays that passed in C1
+      if (!((ps >> r) & 1u)) continue;
+      if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
+      sh.sRv[slot] = 0;
+      continue;
+    }
     if (!sh.sRv[slot]) continue;
     float t, b1, b2;
     if (!leaf_hit<MODE>(sh, slot, load_tri(v9s, slot_t), t, b1, b2)) continue;
     const float tmin = sh.sRo[slot].w;
     if (MODE == 2) {
       if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
+      sh.sRv[slot] = 0;
     } else {
       if (t > tmin && t < __int_as_float(0x7f800000))
         atomicMin(&sh.sKey[slot], (unsigned long long)hit_key(t, (int32_t)perm[slot_t]));
@@ -441,10 +494,9 @@ __device__ __forceinline__ void leaf_rays(const RHGeom& g, const float* __restri
 }
 
 // Phases B and C for the triangles queued in sQ0; empties the queue.
-template <int MODE, int NODE, int RBMAX, int LMAX>
-__device__ __forceinline__ void expand_queue(const RHGeom& g, const float* __restrict__ v9s,
-                             const float4* __restrict__ tbox, const uint32_t* __restrict__ perm,
-                             RhShared<MODE, RBMAX, LMAX>& sh) {
+template <int MODE, int NODE, int RBMAX, int LMAX, class PF, class P4, class PU>
+__device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, PU perm,
+                                             RhShared<MODE, RBMAX, LMAX>& sh) {
   const int tid = threadIdx.x;
   const int top = g.L - 1;
   if (top == 0) {
@@ -566,8 +618,9 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
     const RhItem& it = items[__builtin_amdgcn_readfirstlane(unit.x)];
     const RHGeom& g = it.g;
     const RaysDev& r = it.r;
-    const float* __restrict__ v9s = it.v9s;
-    const uint32_t* __restrict__ perm = it.perm;
+    const auto v9s = glb(it.v9s);
+    const auto perm = glb(it.perm);
+    const auto tbox = glb(it.tbox);
     const uint32_t ntri = it.ntri;
     const int top = g.L - 1;
     const int b = (int)__builtin_amdgcn_readfirstlane(unit.y);
@@ -577,18 +630,20 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
       sh.sTraced = 0;
       sh.sOccluded = 0;
     }
-    load_subtree(g, K, J, it.nodes, sh.sN, sh.sV);
+    load_subtree(g, K, J, glb(it.nodes), sh.sN, sh.sV);
     __syncthreads();
     for (int slot = tid; slot < RBMAX; slot += kThreads) {
       int x = 0, y = 0, leaf = 0;
       bool traced = false;
       if (slot_xy(g, K, J, slot, x, y, leaf)) {
         const size_t idx = (size_t)y * g.W + x;
-        traced = MODE == 0 || r.valid == nullptr || r.valid[idx] != 0;
+        traced = MODE == 0 || r.valid == nullptr || glb(r.valid)[idx] != 0;
         // a ray another unit already found occluded needs no more tests
-        if (MODE == 2 && traced && it.occ[idx]) traced = false;
-        sh.sRo[slot] = make_float4(r.ox[idx], r.oy[idx], r.oz[idx], r.tmin[idx]);
-        sh.sRd[slot] = make_float4(r.dx[idx], r.dy[idx], r.dz[idx], r.tmax[idx]);
+        if (MODE == 2 && traced && glb(it.occ)[idx]) traced = false;
+        sh.sRo[slot] = make_float4(glb(r.ox)[idx], glb(r.oy)[idx], glb(r.oz)[idx],
+                                   glb(r.tmin)[idx]);
+        sh.sRd[slot] = make_float4(glb(r.dx)[idx], glb(r.dy)[idx], glb(r.dz)[idx],
+                                   glb(r.tmax)[idx]);
       }
       sh.sRv[slot] = traced ? 1 : 0;
       if (MODE != 2) sh.sKey[slot] = kMissKey;
@@ -599,31 +654,31 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
     __syncthreads();
     if (sh.sV[0] && sh.sTraced > 0) {
       const Node& root = sh.sN[0];
-      const uint32_t* cl = it.clist + (size_t)b * it.nclusters + unit.z;
+      const auto cl = glb(it.clist) + (size_t)b * it.nclusters + unit.z;
       const int ncl = (int)unit.w;
       bool done = false;
       for (int k0 = 0; k0 < ncl && !done; k0 += kThreads / kCluster) {
         const int k = k0 + tid / kCluster;
         const uint32_t slot = k < ncl ? cl[k] * kCluster + (uint32_t)(tid % kCluster) : ntri;
         bool pass = false;
-        if (slot < ntri) pass = cull_bound_slot<MODE, NODE>(root, v9s, it.tbox, slot);
+        if (slot < ntri) pass = cull_bound_slot<MODE, NODE>(root, v9s, tbox, slot);
         ballot_append(sh.sQ0, &sh.sCnt[0], pass, slot << 6);
         __syncthreads();
         const int cnt0 = sh.sCnt[0];
         __syncthreads();  // every thread has read the count before the next round appends
         if (cnt0 > kQ0Cap - kThreads) {
-          expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, it.tbox, perm, sh);
+          expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, tbox, perm, sh);
           if (MODE == 2 && sh.sOccluded >= sh.sTraced) done = true;
         }
       }
-      if (!done && sh.sCnt[0] > 0) expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, it.tbox, perm, sh);
+      if (!done && sh.sCnt[0] > 0) expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, tbox, perm, sh);
     }
     for (int slot = tid; slot < g.rb; slot += kThreads) {
       int x, y, leaf;
       if (!slot_xy(g, K, J, slot, x, y, leaf)) continue;
       const size_t idx = (size_t)y * g.W + x;
       if (MODE == 2) {
-        if (sh.sAux[slot]) it.occ[idx] = 1;
+        if (sh.sAux[slot]) glb(it.occ)[idx] = 1;
       } else {
         const unsigned long long key = sh.sKey[slot];
         if (key != kMissKey) atomicMin((unsigned long long*)&it.keys[idx], key);
@@ -665,14 +720,15 @@ struct BvhShared {
   uint32_t sPt[kPairCap];  // pair: first primitive position of the BVH leaf
   uint32_t sPn[kPairCap];  // pair: BVH leaf << 6 | hierarchy leaf
   uint32_t sPc[kPairCap];  // pair: primitives tested (the leaf's, or 1 for secondary rays)
-  uint8_t sPair[kPairCap];
+  PairState<MODE> sPair[kPairCap];  // phase C pair state
   int sTq, sPq;
   int sTraced, sOccluded;
 };
 
-__device__ __forceinline__ void bvh_box(const float4* __restrict__ bvh, uint32_t n, V3& bmin,
-                                        V3& bmax, uint32_t& off, uint32_t& np) {
-  const float4 a = bvh[2 * (size_t)n], b = bvh[2 * (size_t)n + 1];
+template <class P>
+__device__ __forceinline__ void bvh_box(P bvh, uint32_t n, V3& bmin, V3& bmax, uint32_t& off,
+                                        uint32_t& np) {
+  const float4 a = ld4(bvh, 2 * (size_t)n), b = ld4(bvh, 2 * (size_t)n + 1);
   bmin = {a.x, a.y, a.z};
   bmax = {a.w, b.x, b.y};
   off = __float_as_uint(b.z);
@@ -700,13 +756,13 @@ __device__ __forceinline__ bool bvh_chain(const RHGeom& g, const SH& sh, const R
                                           uint32_t b, int li) {
   const int top = g.L - 1;
   const int lx = li & (g.nlx - 1), ly = li >> top;
-  const int d = (int)it.bvh_depth[b];
+  const int d = (int)glb(it.bvh_depth)[b];
   uint32_t node = b;
   int dn = d;
   for (int st = max(top, d); st >= 0; --st) {
     const int bd = min(st, d);
     while (dn > bd) {
-      node = it.bvh_parent[node];
+      node = glb(it.bvh_parent)[node];
       --dn;
     }
     const int ss = min(st, top);  // subtree sub-level of the hierarchy node
@@ -714,7 +770,7 @@ __device__ __forceinline__ bool bvh_chain(const RHGeom& g, const SH& sh, const R
     const int idx = pow4_off(ss) + (ly >> sh_) * (1 << ss) + (lx >> sh_);
     V3 bmin, bmax;
     uint32_t off, np;
-    bvh_box(it.bvh, node, bmin, bmax, off, np);
+    bvh_box(glb(it.bvh), node, bmin, bmax, off, np);
     if (!sh.sV[idx] || !bvh_cull<NODE>(sh.sN[idx], bmin, bmax)) return false;
   }
   return true;
@@ -730,18 +786,25 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
   const int total = n * G;
   const bool pow2 = (G & (G - 1)) == 0;
   const int lg = __ffs(G) - 1;
+  const bool masked = MODE == 2 && G <= 31;
   for (int e = threadIdx.x; e < n; e += kThreads) sh.sPair[e] = 0;
   __syncthreads();
   for (int item = threadIdx.x; item < total; item += kThreads) {
     const int e = pow2 ? item >> lg : item / G;
-    const int slot = (int)(sh.sPn[e] & 63u) * G + (item - e * G);
-    if (!sh.sRv[slot]) continue;
-    if (MODE == 2 && sh.sAux[slot]) continue;
+    const int r = item - e * G;
+    const int slot = (int)(sh.sPn[e] & 63u) * G + r;
+    if (!sh.sRv[slot]) continue;  // cleared once the ray is found occluded
     const uint32_t off = sh.sPt[e], cnt = sh.sPc[e];
     for (uint32_t f = 0; f < cnt; ++f) {
       float t, b1, b2;
-      if (leaf_hit<TM>(sh, slot, load_mt(it.tmt, it.prim_order[off + f]), t, b1, b2)) {
-        sh.sPair[e] = 1;
+      if (leaf_hit<TM>(sh, slot, load_mt(glb(it.tmt), glb(it.prim_order)[off + f]), t, b1,
+                       b2)) {
+        if constexpr (MODE == 2) {
+          if (masked) atomicOr(&sh.sPair[e], 1u << r);
+          else sh.sPair[e] = 1;
+        } else {
+          sh.sPair[e] = 1;
+        }
         break;
       }
     }
@@ -749,24 +812,34 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
   __syncthreads();
   int verified = 0;
   for (int e = threadIdx.x; e < n; e += kThreads) {
-    if (sh.sPair[e] != 1) continue;
+    const PairState<MODE> ps = sh.sPair[e];
+    if (ps == 0) continue;
     const bool ok = bvh_chain<NODE>(g, sh, it, sh.sPn[e] >> 6, (int)(sh.sPn[e] & 63u));
-    sh.sPair[e] = ok ? 2 : 0;
+    sh.sPair[e] = ok ? (PairState<MODE>)(ps | kPairOk<MODE>) : (PairState<MODE>)0;
     verified |= ok;
   }
   if (__syncthreads_or(verified)) {
     for (int item = threadIdx.x; item < total; item += kThreads) {
       const int e = pow2 ? item >> lg : item / G;
-      if (sh.sPair[e] != 2) continue;
-      const int slot = (int)(sh.sPn[e] & 63u) * G + (item - e * G);
+      const PairState<MODE> ps = sh.sPair[e];
+      if (!(ps & kPairOk<MODE>)) continue;
+      const int r = item - e * G;
+      const int slot = (int)(sh.sPn[e] & 63u) * G + r;
+      if (masked) {  // occlusion: the rays that passed in C1
+        if (!((ps >> r) & 1u)) continue;
+        if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
+        sh.sRv[slot] = 0;
+        continue;
+      }
       if (!sh.sRv[slot]) continue;
       const uint32_t off = sh.sPt[e], cnt = sh.sPc[e];
       for (uint32_t f = 0; f < cnt; ++f) {
-        const uint32_t tri = it.prim_order[off + f];
+        const uint32_t tri = glb(it.prim_order)[off + f];
         float t, b1, b2;
-        if (!leaf_hit<TM>(sh, slot, load_mt(it.tmt, tri), t, b1, b2)) continue;
+        if (!leaf_hit<TM>(sh, slot, load_mt(glb(it.tmt), tri), t, b1, b2)) continue;
         if (MODE == 2) {
           if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
+          sh.sRv[slot] = 0;
           break;
         }
         atomicMin(&sh.sKey[slot], (unsigned long long)hit_key(t, (int32_t)tri));
@@ -816,17 +889,19 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
       sh.sTraced = 0;
       sh.sOccluded = 0;
     }
-    load_subtree(g, K, J, it.nodes, sh.sN, sh.sV);
+    load_subtree(g, K, J, glb(it.nodes), sh.sN, sh.sV);
     __syncthreads();
     for (int slot = tid; slot < RBMAX; slot += kThreads) {
       int x = 0, y = 0, leaf = 0;
       bool traced = false;
       if (slot_xy(g, K, J, slot, x, y, leaf)) {
         const size_t idx = (size_t)y * g.W + x;
-        traced = MODE == 0 || r.valid == nullptr || r.valid[idx] != 0;
-        if (MODE == 2 && traced && it.occ[idx]) traced = false;
-        sh.sRo[slot] = make_float4(r.ox[idx], r.oy[idx], r.oz[idx], r.tmin[idx]);
-        sh.sRd[slot] = make_float4(r.dx[idx], r.dy[idx], r.dz[idx], r.tmax[idx]);
+        traced = MODE == 0 || r.valid == nullptr || glb(r.valid)[idx] != 0;
+        if (MODE == 2 && traced && glb(it.occ)[idx]) traced = false;
+        sh.sRo[slot] = make_float4(glb(r.ox)[idx], glb(r.oy)[idx], glb(r.oz)[idx],
+                                   glb(r.tmin)[idx]);
+        sh.sRd[slot] = make_float4(glb(r.dx)[idx], glb(r.dy)[idx], glb(r.dz)[idx],
+                                   glb(r.tmax)[idx]);
       }
       sh.sRv[slot] = traced ? 1 : 0;
       if (MODE != 2) sh.sKey[slot] = kMissKey;
@@ -840,7 +915,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
       // root.  Four clusters (256 roots) at a time: the roots that pass their own bound seed
       // the traversal stack, which is drained before the next four.  Pairs are tested when
       // their queue could not take another round's emissions, and at the end.
-      const uint32_t* cl = it.clist + (size_t)b * it.nclusters + unit.z;
+      const auto cl = glb(it.clist) + (size_t)b * it.nclusters + unit.z;
       const int ncl = (int)unit.w;
       bool stop = false;
       for (int k0 = 0; k0 < ncl && !stop; k0 += kThreads / kCluster) {
@@ -851,10 +926,10 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
         const int k = k0 + tid / kCluster;
         const uint32_t ri = k < ncl ? cl[k] * kCluster + (uint32_t)(tid % kCluster) : it.nroots;
         if (ri < it.nroots) {
-          const uint32_t node = it.roots[ri];
+          const uint32_t node = glb(it.roots)[ri];
           V3 bmin, bmax;
           uint32_t off, np;
-          bvh_box(it.bvh, node, bmin, bmax, off, np);
+          bvh_box(glb(it.bvh), node, bmin, bmax, off, np);
           if (bvh_bound<NODE>(sh.sN[0], bmin, bmax)) {
             if (top == 0 && np > 0) {
               bvh_emit<MODE>(sh, node, off, np, 0);
@@ -896,7 +971,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
             const int sl = sub_level(R);
             V3 bmin, bmax;
             uint32_t off, np;
-            bvh_box(it.bvh, B, bmin, bmax, off, np);
+            bvh_box(glb(it.bvh), B, bmin, bmax, off, np);
             const bool bleaf = np > 0, rleaf = sl == top;
             int rc = -1;
             uint32_t bc = B;
@@ -921,7 +996,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
               if (sh.sV[Rn]) {
                 V3 cmin, cmax;
                 uint32_t coff, cnp;
-                bvh_box(it.bvh, bc, cmin, cmax, coff, cnp);
+                bvh_box(glb(it.bvh), bc, cmin, cmax, coff, cnp);
                 if (bvh_bound<NODE>(sh.sN[Rn], cmin, cmax)) {
                   if (sub_level(Rn) == top && cnp > 0)
                     bvh_emit<MODE>(sh, bc, coff, cnp, Rn - pow4_off(top));
@@ -948,7 +1023,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
       if (!slot_xy(g, K, J, slot, x, y, leaf)) continue;
       const size_t idx = (size_t)y * g.W + x;
       if (MODE == 2) {
-        if (sh.sAux[slot]) it.occ[idx] = 1;
+        if (sh.sAux[slot]) glb(it.occ)[idx] = 1;
       } else {
         const unsigned long long key = sh.sKey[slot];
         if (key != kMissKey) atomicMin((unsigned long long*)&it.keys[idx], key);
@@ -1170,7 +1245,7 @@ __global__ __launch_bounds__(256) void rh_repair_kernel(const RhItem* __restrict
       const int l = tid;
       const int gk = x / (g.cx << l), gj = y / (g.cy << l);
       const float4* p = it.nodes + 4 * ((size_t)g.off[l] + (size_t)gj * g.gw[l] + gk);
-      float4 a = p[0], b = p[1], c = p[2], d = p[3];
+      float4 a = ld4(p, 0), b = ld4(p, 1), c = ld4(p, 2), d = ld4(p, 3);
       sChain[l].dmin = {a.x, a.y, a.z};
       sChain[l].dmax = {b.x, b.y, b.z};
       sChain[l].omin = {c.x, c.y, c.z};
