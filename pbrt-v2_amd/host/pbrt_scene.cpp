@@ -174,6 +174,246 @@ Point apply_point(const Mat4& M, float x, float y, float z) {
   return Point(xp * inv, yp * inv, zp * inv);
 }
 
+// -------------------------------------------------------------------- Loop subdivision
+// Loop subdivision with pbrt's rules and order (LoopSubdiv, loopsubdiv.cpp:148-485): even
+// vertices by the one-ring rule (beta = 3/16 for valence 3, 3/(8 valence) otherwise, 1/16 for
+// regular vertices) or the boundary rule (1/8); odd vertices 3/8, 3/8, 1/8, 1/8 (boundary
+// 1/2, 1/2); every face splits into children 0..2 at its corners and child 3 in the middle;
+// after the last level every vertex moves to its limit position (gamma rule, boundary 1/5).
+// The new vertices of a level are the children of the old ones, in order, then the edge
+// vertices in the order faces and edges first reach them; faces are the children of the old
+// faces in order.  Sums are taken in pbrt's order, so the result is bit-identical.
+class LoopMesh {
+ public:
+  LoopMesh(const std::vector<Point>& P, const std::vector<int>& vi) {
+    const size_t nf = vi.size() / 3;
+    V_.resize(P.size());
+    for (size_t i = 0; i < P.size(); ++i) V_[i].p = {P[i].x, P[i].y, P[i].z};
+    F_.resize(nf);
+    for (size_t i = 0; i < nf; ++i)
+      for (int j = 0; j < 3; ++j) {
+        F_[i].v[j] = vi[3 * i + j];
+        V_[vi[3 * i + j]].start = (int)i;  // the last face that holds the vertex
+      }
+    // neighbours across each edge: an edge seen a second time links the two faces
+    std::map<std::pair<int, int>, std::pair<int, int>> open;
+    for (size_t i = 0; i < nf; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const int a = F_[i].v[k], b = F_[i].v[next(k)];
+        const std::pair<int, int> key(std::min(a, b), std::max(a, b));
+        auto it = open.find(key);
+        if (it == open.end()) {
+          open.emplace(key, std::make_pair((int)i, k));
+        } else {
+          F_[it->second.first].f[it->second.second] = (int)i;
+          F_[i].f[k] = it->second.first;
+          open.erase(it);
+        }
+      }
+    for (size_t i = 0; i < V_.size(); ++i) {
+      Vert& v = V_[i];
+      if (v.start < 0) continue;  // a vertex no face uses
+      int f = v.start;
+      do {
+        f = next_face(f, (int)i);
+      } while (f >= 0 && f != v.start);
+      v.boundary = f < 0;
+      const int val = valence((int)i);
+      v.regular = v.boundary ? val == 4 : val == 6;
+    }
+    for (size_t i = 0; i < nf; ++i) faces_.push_back((int)i);
+    for (size_t i = 0; i < V_.size(); ++i) verts_.push_back((int)i);
+  }
+
+  void subdivide(int levels, std::vector<Point>* P, std::vector<int>* vi) {
+    for (int l = 0; l < levels; ++l) level();
+    // limit positions, computed from the last level before any is replaced
+    std::vector<P3> lim(verts_.size());
+    for (size_t i = 0; i < verts_.size(); ++i) {
+      const int v = verts_[i];
+      lim[i] = V_[v].start < 0 ? V_[v].p
+               : V_[v].boundary ? weight_boundary(v, 1.f / 5.f)
+                                : weight_one_ring(v, gamma(valence(v)));
+    }
+    std::vector<int> index(V_.size(), -1);
+    P->clear();
+    for (size_t i = 0; i < verts_.size(); ++i) {
+      index[verts_[i]] = (int)i;
+      P->push_back(Point(lim[i].x, lim[i].y, lim[i].z));
+    }
+    vi->clear();
+    for (int f : faces_)
+      for (int j = 0; j < 3; ++j) vi->push_back(index[F_[f].v[j]]);
+  }
+
+ private:
+  struct P3 {
+    float x, y, z;
+  };
+  struct Vert {
+    P3 p{0.f, 0.f, 0.f};
+    int start = -1, child = -1;
+    bool regular = false, boundary = false;
+  };
+  struct Face {
+    int v[3] = {-1, -1, -1}, f[3] = {-1, -1, -1}, children[4] = {-1, -1, -1, -1};
+  };
+  static int next(int i) { return (i + 1) % 3; }
+  static int prev(int i) { return (i + 2) % 3; }
+  static float beta(int valence) { return valence == 3 ? 3.f / 16.f : 3.f / (8.f * valence); }
+  static float gamma(int valence) { return 1.f / (valence + 3.f / (8.f * beta(valence))); }
+
+  int vnum(int f, int v) const {
+    for (int i = 0; i < 3; ++i)
+      if (F_[f].v[i] == v) return i;
+    Severe("loopsubdiv: inconsistent mesh (a face does not hold its vertex)");
+  }
+  int next_face(int f, int v) const { return F_[f].f[vnum(f, v)]; }
+  int prev_face(int f, int v) const { return F_[f].f[prev(vnum(f, v))]; }
+  int next_vert(int f, int v) const { return F_[f].v[next(vnum(f, v))]; }
+  int prev_vert(int f, int v) const { return F_[f].v[prev(vnum(f, v))]; }
+  int other_vert(int f, int v0, int v1) const {
+    for (int i = 0; i < 3; ++i)
+      if (F_[f].v[i] != v0 && F_[f].v[i] != v1) return F_[f].v[i];
+    Severe("loopsubdiv: inconsistent mesh (degenerate face)");
+  }
+  int valence(int v) const {
+    int f = V_[v].start, nf = 1;
+    if (!V_[v].boundary) {
+      while ((f = next_face(f, v)) != V_[v].start) ++nf;
+      return nf;
+    }
+    while ((f = next_face(f, v)) >= 0) ++nf;
+    f = V_[v].start;
+    while ((f = prev_face(f, v)) >= 0) ++nf;
+    return nf + 1;
+  }
+  void one_ring(int v, std::vector<P3>* ring) const {
+    ring->clear();
+    int f = V_[v].start;
+    if (!V_[v].boundary) {
+      do {
+        ring->push_back(V_[next_vert(f, v)].p);
+        f = next_face(f, v);
+      } while (f != V_[v].start);
+      return;
+    }
+    for (int f2; (f2 = next_face(f, v)) >= 0;) f = f2;
+    ring->push_back(V_[next_vert(f, v)].p);
+    do {
+      ring->push_back(V_[prev_vert(f, v)].p);
+      f = prev_face(f, v);
+    } while (f >= 0);
+  }
+  static P3 scale(float s, const P3& p) { return {s * p.x, s * p.y, s * p.z}; }
+  static void add(P3* a, const P3& b) {
+    a->x += b.x;
+    a->y += b.y;
+    a->z += b.z;
+  }
+  P3 weight_one_ring(int v, float b) const {
+    std::vector<P3> ring;
+    one_ring(v, &ring);
+    const int val = (int)ring.size();
+    P3 r = scale(1 - val * b, V_[v].p);
+    for (const P3& q : ring) add(&r, scale(b, q));
+    return r;
+  }
+  P3 weight_boundary(int v, float b) const {
+    std::vector<P3> ring;
+    one_ring(v, &ring);
+    P3 r = scale(1 - 2 * b, V_[v].p);
+    add(&r, scale(b, ring.front()));
+    add(&r, scale(b, ring.back()));
+    return r;
+  }
+
+  void level() {
+    std::vector<int> nverts, nfaces;
+    for (int v : verts_) {
+      Vert c;
+      c.regular = V_[v].regular;
+      c.boundary = V_[v].boundary;
+      V_[v].child = (int)V_.size();
+      V_.push_back(c);
+      nverts.push_back(V_[v].child);
+    }
+    for (int f : faces_)
+      for (int k = 0; k < 4; ++k) {
+        F_[f].children[k] = (int)F_.size();
+        F_.push_back(Face());
+        nfaces.push_back(F_[f].children[k]);
+      }
+    // even vertices
+    for (int v : verts_) {
+      if (V_[v].start < 0) continue;
+      P3 q;
+      if (!V_[v].boundary)
+        q = weight_one_ring(v, V_[v].regular ? 1.f / 16.f : beta(valence(v)));
+      else
+        q = weight_boundary(v, 1.f / 8.f);
+      V_[V_[v].child].p = q;
+    }
+    // odd vertices, one per edge
+    std::map<std::pair<int, int>, int> edge_vert;
+    for (int f : faces_)
+      for (int k = 0; k < 3; ++k) {
+        const int a = F_[f].v[k], b = F_[f].v[next(k)];
+        const std::pair<int, int> key(std::min(a, b), std::max(a, b));
+        if (edge_vert.count(key)) continue;
+        Vert nv;
+        nv.regular = true;
+        nv.boundary = F_[f].f[k] < 0;
+        nv.start = F_[f].children[3];
+        const P3 &e0 = V_[key.first].p, &e1 = V_[key.second].p;
+        if (nv.boundary) {
+          nv.p = scale(0.5f, e0);
+          add(&nv.p, scale(0.5f, e1));
+        } else {
+          nv.p = scale(3.f / 8.f, e0);
+          add(&nv.p, scale(3.f / 8.f, e1));
+          add(&nv.p, scale(1.f / 8.f, V_[other_vert(f, key.first, key.second)].p));
+          add(&nv.p, scale(1.f / 8.f, V_[other_vert(F_[f].f[k], key.first, key.second)].p));
+        }
+        edge_vert[key] = (int)V_.size();
+        nverts.push_back((int)V_.size());
+        V_.push_back(nv);
+      }
+    // topology of the new level
+    for (int v : verts_)
+      if (V_[v].start >= 0)
+        V_[V_[v].child].start = F_[V_[v].start].children[vnum(V_[v].start, v)];
+    for (int f : faces_) {
+      const Face& F = F_[f];
+      for (int k = 0; k < 3; ++k) {
+        F_[F.children[3]].f[k] = F.children[next(k)];
+        F_[F.children[k]].f[next(k)] = F.children[3];
+        int f2 = F.f[k];
+        F_[F.children[k]].f[k] = f2 >= 0 ? F_[f2].children[vnum(f2, F.v[k])] : -1;
+        f2 = F.f[prev(k)];
+        F_[F.children[k]].f[prev(k)] = f2 >= 0 ? F_[f2].children[vnum(f2, F.v[k])] : -1;
+      }
+    }
+    for (int f : faces_) {
+      const Face F = F_[f];
+      for (int k = 0; k < 3; ++k) {
+        F_[F.children[k]].v[k] = V_[F.v[k]].child;
+        const int a = F.v[k], b = F.v[next(k)];
+        const int ev = edge_vert[std::make_pair(std::min(a, b), std::max(a, b))];
+        F_[F.children[k]].v[next(k)] = ev;
+        F_[F.children[next(k)]].v[k] = ev;
+        F_[F.children[3]].v[k] = ev;
+      }
+    }
+    verts_.swap(nverts);
+    faces_.swap(nfaces);
+  }
+
+  std::vector<Vert> V_;
+  std::vector<Face> F_;
+  std::vector<int> verts_, faces_;  // the current level
+};
+
 // ------------------------------------------------------------------------------------ lexer
 struct Token {
   enum Kind { kEnd, kNum, kStr, kIdent, kLBrack, kRBrack } kind = kEnd;
@@ -830,9 +1070,11 @@ class Parser {
                n == "paraboloid" || n == "hyperboloid") {
       ++out_->skippedShapes;  // not Triangles: the accelerators skip them
       return;
-    } else if (n == "loopsubdiv" || n == "nurbs") {
+    } else if (n == "loopsubdiv") {
+      tri = loop_subdiv(ps, animated ? Xform() : cur_[0], &mesh);
+    } else if (n == "nurbs") {
       ++out_->skippedShapes;
-      msg("Shape \"" + n + "\" refines into triangles in pbrt but is not provided here; skipped");
+      msg("Shape \"nurbs\" refines into triangles in pbrt but is not provided here; skipped");
       return;
     } else {
       msg("Shape \"" + n + "\" unknown.");
@@ -882,6 +1124,34 @@ class Parser {
     const size_t ntri = vi->ints.size() / 3;  // TriangleMesh takes nvi / 3 triangles
     m->vi.assign(vi->ints.begin(), vi->ints.begin() + 3 * ntri);
     if (use_uv) m->uv.assign(uv->num.begin(), uv->num.begin() + 2 * npi);
+    return true;
+  }
+
+  // CreateLoopSubdivShape (loopsubdiv.cpp:487-502): nlevels (3) levels of Loop subdivision of
+  // the control mesh, then a triangle mesh of the limit positions (without uvs).
+  bool loop_subdiv(const Params& ps, const Xform& o2w, TriangleMesh* m) {
+    const Param* vi = ps.find(kInt, "indices");
+    const Param* P = ps.find(kPoint, "P");
+    if (!vi || !P) return false;
+    const int levels = ps.oneInt("nlevels", 3);
+    const size_t npi = P->num.size() / 3;
+    std::vector<int> idx(vi->ints.begin(), vi->ints.begin() + 3 * (vi->ints.size() / 3));
+    for (int v : idx)
+      if (v < 0 || (size_t)v >= npi) {
+        msg("loopsubdiv has out of-bounds vertex index " + std::to_string(v) + "; skipped");
+        return false;
+      }
+    if (levels < 0 || levels > 8) {
+      msg("loopsubdiv nlevels must be in [0, 8]; skipped");
+      return false;
+    }
+    std::vector<Point> cp(npi), lp;
+    for (size_t i = 0; i < npi; ++i) cp[i] = Point(P->num[3 * i], P->num[3 * i + 1], P->num[3 * i + 2]);
+    std::vector<int> lvi;
+    LoopMesh(cp, idx).subdivide(levels, &lp, &lvi);
+    m->p.reserve(lp.size());
+    for (const Point& q : lp) m->p.push_back(apply_point(o2w.m, q.x, q.y, q.z));
+    m->vi = std::move(lvi);
     return true;
   }
 

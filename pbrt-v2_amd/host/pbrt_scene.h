@@ -4,8 +4,9 @@
 // The reader follows pbrt-v2's scene-description front end for the parts that reach the
 // accelerators: the lexer (src/core/pbrtlex.ll), the grammar and parameter typing
 // (src/core/pbrtparse.yy:160-810) and the API state machine (src/core/api.cpp:283-1110).
-//   - Triangles: Shape "trianglemesh" (src/shapes/trianglemesh.cpp:363-440) and "heightfield"
-//     (src/shapes/heightfield.cpp:62-116).  Vertices are taken to world space by the current
+//   - Triangles: Shape "trianglemesh" (src/shapes/trianglemesh.cpp:363-440), "heightfield"
+//     (src/shapes/heightfield.cpp:62-116) and "loopsubdiv" (Loop subdivision to its limit
+//     surface, src/shapes/loopsubdiv.cpp:148-502).  Vertices are taken to world space by the current
 //     transform as TriangleMesh's constructor does, with pbrt's FP32 operation order.  Meshes
 //     are kept in directive order: the order in which the accelerators flatten them
 //     (rayhierarchy.cpp:157-199).
@@ -23,9 +24,9 @@
 //   - non-triangle shapes (sphere, disk, cylinder, cone, paraboloid, hyperboloid);
 //   - shapes inside ObjectBegin / ObjectEnd, and ObjectInstance (TransformedPrimitives);
 //   - shapes under an animated transform (TransformedPrimitive, api.cpp:1010-1040).
-// "loopsubdiv" and "nurbs" refine into triangle meshes in pbrt but are not provided here: they
-// are reported and skipped.  Materials, textures, volumes, filters and area lights do not
-// change the traced geometry and are ignored.
+// "nurbs" refines into a triangle mesh in pbrt but is not provided here: it is reported and
+// skipped.  Materials, textures, volumes, filters and area lights do not change the traced
+// geometry and are ignored.
 //
 // Error behaviour follows pbrt's: a syntax error or an unreadable file is Severe(); what pbrt
 // reports with Error() or Warning() and then ignores is recorded in SceneFile::messages.

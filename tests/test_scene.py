@@ -198,7 +198,7 @@ Shape "trianglemesh" "point P" [0 0 0 1 0 0 0 1 0] "integer indices" [0 1 3]
 Shape "trianglemesh" "point P" [0 0 0 1 0 0 0 1 0] "integer indices" [0 1 2] "float uv" [0 0 1]
 Shape "trianglemesh" "point P" [0 0 0 1 0 0 0 1 0] "integer indices" [0 1 2] "bool x" [1]
 Shape "trianglemesh" "point P" [0 0 0 1 0 0 0 1] "integer indices" [0 1 2]
-Shape "loopsubdiv" "point P" [0 0 0 1 0 0 0 1 0] "integer indices" [0 1 2]
+Shape "nurbs" "integer nu" 2 "integer nv" 2
 WorldEnd
 """
     v9, info, msgs = parse(write(tmp_path, "e.pbrt", text))
@@ -209,11 +209,83 @@ WorldEnd
     assert "Discarding" in joined  # too few uvs: the mesh stays, without uvs
     assert "Expected string parameter value" in joined
     assert "Excess values" in joined  # 8 numbers for "point P"
-    assert "loopsubdiv" in joined
+    assert "nurbs" in joined
     assert info["film"] == "640,480"  # the Film inside the world block was ignored
     # dropped: index 3 of three points, and index 2 of the two points left after the excess
     # value; kept: the mesh whose uvs were discarded and the one with the ignored "bool x"
     assert len(v9) == 2
+
+
+def _mesh_text(P, idx, levels, shape="loopsubdiv"):
+    pts = " ".join(f"{x:.9g}" for x in np.asarray(P, np.float64).reshape(-1))
+    ind = " ".join(str(int(i)) for i in np.asarray(idx).reshape(-1))
+    return (f'WorldBegin\nShape "{shape}" "integer nlevels" [{levels}] "point P" [{pts}] '
+            f'"integer indices" [{ind}]\nWorldEnd\n')
+
+
+def test_loop_subdivision(tmp_path):
+    """Loop subdivision (LoopSubdiv) pinned by what the scheme fixes:
+    - each level splits every face in 4, and adds one vertex per edge (Euler count);
+    - the limit surface of a planar control mesh stays in its plane, exactly;
+    - on a regular planar grid, affine functions are reproduced, so interior vertices keep
+      their place in the plane (to rounding);
+    - a closed regular octahedron subdivides to a surface symmetric under its own symmetries:
+      the limit vertices come in orbits of equal radius;
+    - nlevels 0 gives the limit positions of the control mesh itself."""
+    # octahedron: closed, every vertex of valence 4
+    P = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], F)
+    idx = np.array([[0, 2, 4], [2, 1, 4], [1, 3, 4], [3, 0, 4],
+                    [2, 0, 5], [1, 2, 5], [3, 1, 5], [0, 3, 5]])
+    for levels in (0, 1, 2, 3):
+        v9, info, msgs = parse(write(tmp_path, f"oct{levels}.pbrt", _mesh_text(P, idx, levels)))
+        assert len(v9) == 8 * 4 ** levels, msgs
+        verts = np.unique(v9.reshape(-1, 3), axis=0).astype(np.float64)
+        nv = 6 + sum(12 * 4 ** l for l in range(levels))  # V' = V + E, E = 3F/2 per level
+        assert len(verts) == nv, (levels, len(verts), nv)
+        # the octahedron's symmetries map the limit vertex set onto itself (to rounding: the
+        # one-ring sums start at different neighbours)
+        for T in (lambda v: v * [-1, 1, 1], lambda v: v[:, [1, 0, 2]], lambda v: v[:, [2, 0, 1]]):
+            d = np.abs(T(verts)[:, None, :] - verts[None, :, :]).max(-1).min(1)
+            assert d.max() < 1e-5, (levels, d.max())
+        if levels == 0:
+            # limit rule alone (gamma(4) = 1/8): (1 - 4/8) v + 1/8 (sum of the 4 neighbours),
+            # which is v / 2 here, exactly in FP32
+            want = np.unique(P.astype(np.float64) * 0.5, axis=0)
+            assert np.array_equal(verts, want)
+        else:
+            r = np.sqrt((verts ** 2).sum(1))
+            assert r.min() > 0.4 and r.max() <= 0.5 + 1e-6  # inside the level-0 limit hull
+    # regular tetrahedron, valence 3 (gamma = 1/5): the limit of v is 0.4 v + 0.2 (sum of the
+    # other three) = 0.2 v, since the four vertices sum to 0
+    T = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], F)
+    v9, _, _ = parse(write(tmp_path, "tet.pbrt", _mesh_text(T, [[0, 1, 2], [0, 3, 1], [0, 2, 3],
+                                                                 [1, 3, 2]], 0)))
+    np.testing.assert_allclose(np.unique(v9.reshape(-1, 3), axis=0),
+                               np.unique(T * F(0.2), axis=0), atol=1e-6)
+    # a lone triangle: every vertex on the boundary (rule 1/5): 0.6 v + 0.2 (a + b)
+    v9, _, _ = parse(write(tmp_path, "tri.pbrt", _mesh_text([[0, 0, 0], [1, 0, 0], [0, 1, 0]],
+                                                            [[0, 1, 2]], 0)))
+    np.testing.assert_allclose(v9.reshape(3, 3), [[0.2, 0.2, 0], [0.6, 0.2, 0], [0.2, 0.6, 0]],
+                               atol=1e-6)
+    # planar grid 5 x 5 (with boundary), dyadic coordinates
+    g = 5
+    P = np.array([[x, y, 0] for y in range(g) for x in range(g)], F)
+    tri = []
+    for y in range(g - 1):
+        for x in range(g - 1):
+            a, b, c, d = y * g + x, y * g + x + 1, (y + 1) * g + x + 1, (y + 1) * g + x
+            tri += [[a, b, c], [a, c, d]]
+    v9, info, msgs = parse(write(tmp_path, "grid.pbrt", _mesh_text(P, tri, 2)))
+    assert len(v9) == len(tri) * 16
+    q = v9.reshape(-1, 3)
+    assert np.all(q[:, 2] == 0)  # planar stays planar, exactly
+    inner = (q[:, 0] > 1.01) & (q[:, 0] < 2.99) & (q[:, 1] > 1.01) & (q[:, 1] < 2.99)
+    # the interior of a regular planar grid reproduces the parametrisation: the limit points
+    # lie on the 1/4 lattice (the level-2 vertices), to rounding
+    np.testing.assert_allclose(q[inner, :2] * 4, np.round(q[inner, :2] * 4), atol=2e-5)
+    # out-of-range control index: reported and skipped
+    v9, info, msgs = parse(write(tmp_path, "bad.pbrt", _mesh_text(P, [[0, 1, 99]], 1)))
+    assert len(v9) == 0 and any("out of-bounds" in m for m in msgs)
 
 
 def _index_count(path, seen):
@@ -229,15 +301,23 @@ def _index_count(path, seen):
 
 
 @pytest.mark.skipif(not REF_SCENES.is_dir(), reason="reference scene files not present")
-@pytest.mark.parametrize("name", ["bunny.pbrt", "prt-teapot.pbrt"])
+@pytest.mark.parametrize("name", ["bunny.pbrt", "prt-teapot.pbrt", "killeroo-simple.pbrt"])
 def test_reference_sample_scenes(name):
     """The reference's own sample scenes parse; triangle counts match an independent count of
     their index lists (these two have no non-triangle or skipped meshes)."""
     v9, info, msgs = parse(REF_SCENES / name)
     print(f"[scene] {name}: {len(v9)} triangles, accelerator {info['accelerator']} "
           f"{info['params']}, messages {msgs}")
-    assert len(v9) == _index_count(REF_SCENES / name, set())
     assert np.all(np.isfinite(v9))
+    if name == "killeroo-simple.pbrt":
+        # two quads, and the killeroo twice: its control faces x 4^nlevels (nlevels 1)
+        text = (REF_SCENES / "geometry" / "killeroo.pbrt").read_text()
+        nf = len(re.search(r'"integer indices"\s*\[([^\]]*)\]', text).group(1).split()) // 3
+        lv = int(re.search(r'"integer nlevels"\s*\[\s*(\d+)', text).group(1))
+        assert len(v9) == 4 + 2 * nf * 4 ** lv
+        assert int(info["skipped"]) == 1  # the sphere of the area light
+        return
+    assert len(v9) == _index_count(REF_SCENES / name, set())
     assert info["accelerator"] == "rayhierarchy"
     if name == "bunny.pbrt":
         # the bunny's vertices sit under "Translate 0 -.033 0": y' = fl(y + (-0.033f))
