@@ -303,8 +303,8 @@ __device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J, P no
 //            verified pairs update the ray (LDS atomic min of the (t, id) key / occlusion flag).
 constexpr int kCluster = 64;     // triangles per cluster
 constexpr int kQ0Cap = 1024;     // root survivors held before expansion
-constexpr int kQCap = 256;       // survivors of one expansion round
-constexpr int kExpand = 64;      // parents expanded per round (4 children each)
+constexpr int kQCap = 1024;      // survivors of one expansion round
+constexpr int kExpand = 256;     // parents expanded per round (4 children each, 4 per thread)
 
 // Cluster bound: conservative for every triangle (rayhierarchy) or BVH node box (ray-bvh, whose
 // cull is intersectsNode in every mode) inside [cmin, cmax].
@@ -389,7 +389,7 @@ struct RhShared {
   uint8_t sRv[RBMAX];    // ray exists and is traced
   PairState<MODE> sPair[kQ0Cap];  // phase C pair state
   uint32_t sQ0[kQ0Cap];
-  uint32_t sQ[LMAX > 1 ? LMAX : 2][kQCap];  // level s survivors live in sQ[s], s >= 1
+  uint32_t sQ[LMAX > 1 ? LMAX - 1 : 1][kQCap];  // level s >= 1 survivors live in sQ[s - 1]
   int sCnt[kMaxLevels];
   int sTraced, sOccluded;
 };
@@ -507,7 +507,7 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
     while (true) {
       const int cnt = sh.sCnt[s];
       if (s == top) {
-        leaf_rays<MODE, NODE, RBMAX, LMAX>(g, v9s, perm, sh.sQ[s], cnt, sh);
+        leaf_rays<MODE, NODE, RBMAX, LMAX>(g, v9s, perm, sh.sQ[s - 1], cnt, sh);
         --s;
         continue;
       }
@@ -520,24 +520,29 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
       }
       cur.set(s, c + kExpand);
       const int np = min(kExpand, cnt - c);
-      const uint32_t* src = s == 0 ? sh.sQ0 : sh.sQ[s];
+      const uint32_t* src = s == 0 ? sh.sQ0 : sh.sQ[s - 1];
       __syncthreads();
       if (tid == 0) sh.sCnt[s + 1] = 0;
       __syncthreads();
-      bool ok = false;
-      uint32_t item = 0;
-      if (tid < np * 4) {
-        const uint32_t P = src[c + (tid >> 2)];
-        const uint32_t slot_t = P >> 6;
-        const int pl = (int)(P & 63u), a = tid & 3;
-        const int pnl = 1 << s;
-        const int dk = pl & (pnl - 1), dj = pl >> s;
-        const int ch = (2 * dj + (a >> 1)) * (2 * pnl) + 2 * dk + (a & 1);
-        const int ni = pow4_off(s + 1) + ch;
-        item = (slot_t << 6) | (uint32_t)ch;
-        if (sh.sV[ni]) ok = cull_bound_slot<MODE, NODE>(sh.sN[ni], v9s, tbox, slot_t);
+      // 4 children per parent, every thread takes children until all np * 4 are bounded (a
+      // uniform trip count: the append is a wavefront ballot)
+      for (int cb = 0; cb < np * 4; cb += kThreads) {
+        const int ct = cb + tid;
+        bool ok = false;
+        uint32_t item = 0;
+        if (ct < np * 4) {
+          const uint32_t P = src[c + (ct >> 2)];
+          const uint32_t slot_t = P >> 6;
+          const int pl = (int)(P & 63u), a = ct & 3;
+          const int pnl = 1 << s;
+          const int dk = pl & (pnl - 1), dj = pl >> s;
+          const int ch = (2 * dj + (a >> 1)) * (2 * pnl) + 2 * dk + (a & 1);
+          const int ni = pow4_off(s + 1) + ch;
+          item = (slot_t << 6) | (uint32_t)ch;
+          if (sh.sV[ni]) ok = cull_bound_slot<MODE, NODE>(sh.sN[ni], v9s, tbox, slot_t);
+        }
+        ballot_append(sh.sQ[s], &sh.sCnt[s + 1], ok, item);
       }
-      ballot_append(sh.sQ[s + 1], &sh.sCnt[s + 1], ok, item);
       __syncthreads();
       ++s;
     }
