@@ -169,6 +169,8 @@ class NaiveAccel : public Aggregate {
  public:
   NaiveAccel(const std::vector<TriangleMesh>& meshes, bool onGPU, const std::string& method);
   void Intersect(const RayDifferential* r, Intersection* in, bool* hit, unsigned count) override;
+  // naive.h:53 has one Intersect with bounce = 0 by default: bounce 0 traces camera rays (every
+  // ray), any other value bounce rays (the rays with hit[i] set).
   void Intersect(const RayDifferential* r, Intersection* in, bool* hit, unsigned count,
                  int bounce) override;
   void IntersectP(const Ray* r, char* occluded, size_t count, const bool* hit) override;
@@ -202,5 +204,25 @@ RayBVH* CreateRayBVH(const std::vector<TriangleMesh>& meshes, const ParamSet& ps
 // "naive", "rayhierarchy", "ray-bvh"; anything else is Severe (api.cpp:576-596).
 Aggregate* MakeAccelerator(const std::string& name, const std::vector<TriangleMesh>& meshes,
                            const ParamSet& ps);
+
+// The batched entry points of pbrt's Scene (scene.h:52-99, scene.cpp:55-121): every call is
+// routed to the accelerator by its type (dynamic_cast to RayHieararchy, RayBVH, NaiveAccel);
+// any other aggregate is Severe.  Scene::Intersect(..., bounce) hands NaiveAccel bounce + 1,
+// because NaiveAccel takes bounce 0 for camera rays (scene.cpp:114).
+class Scene {
+ public:
+  explicit Scene(Aggregate* accel) : aggregate(accel) {}
+  void Intersect(const RayDifferential* ray, Intersection* isect, bool* hit, int count,
+                 unsigned samplesPerPixel) const;
+  void Intersect(const RayDifferential* ray, Intersection* isect, bool* hit, int count,
+                 unsigned samplesPerPixel, int bounce) const;
+  void IntersectP(const Ray* ray, char* occluded, size_t count, const bool* hit) const;
+  unsigned MaxRaysPerCall() const;
+  // Scene::Preprocess(camera, spp, nx, ny) with the camera's film resolution.
+  void Preprocess(int xResolution, int yResolution, unsigned samplesPerPixel, int nx,
+                  int ny) const;
+
+  Aggregate* aggregate;
+};
 
 }  // namespace pbrt_amd

@@ -29,7 +29,9 @@ const char* mirror_last_error() { return g_err.c_str(); }
 // accel: "naive", "rayhierarchy" or "ray-bvh"; params as "name=value" pairs separated by ';'
 // (ints, or strings for node/method/splitmethod).  The triangles become two meshes: the first
 // split triangles with primitiveId 7 / shapeId 70, the rest with 8 / 80.
-// mode 0: primary Intersect, 1: bounce Intersect (hit_io selects rays), 2: IntersectP.
+// mode 0: primary Intersect, 1: bounce Intersect (hit_io selects rays), 2: IntersectP,
+// 3: Scene::Intersect, 4: Scene::Intersect with bounce 0, 5: Scene::IntersectP,
+// 6: the accelerator's Intersect(..., bounce) with bounce 0.
 // ray8: n x 8 floats (ox oy oz dx dy dz mint maxt).  Outputs per ray: tri (-1 for no hit),
 // prim, maxt after the call, uv (2), dpdu/dpdv (6), occ ('0'/'1').  calls > 1 repeats the
 // whole batch on fresh copies of the rays (staging reuse), the outputs come from the last.
@@ -78,26 +80,36 @@ int mirror_run(const char* accel, const char* params, const float* v9, const flo
     std::unique_ptr<Aggregate> a(MakeAccelerator(accel, meshes, ps));
     if (xres > 0) a->Preprocess(xres, yres, (unsigned)spp, 1, 1);
     std::vector<RayDifferential> rays(n);
-    std::vector<Ray> shadow(mode == 2 ? n : 0);  // IntersectP takes a Ray array
+    const bool occl = mode == 2 || mode == 5;
+    std::vector<Ray> shadow(occl ? n : 0);  // IntersectP takes a Ray array
     std::vector<Intersection> isect(n);
     std::unique_ptr<bool[]> hit(new bool[n]);
     for (int c = 0; c < calls; ++c) {
       for (uint32_t i = 0; i < n; ++i) {
         const float* q = ray8 + 8 * (size_t)i;
         rays[i] = RayDifferential(Point(q[0], q[1], q[2]), Vector(q[3], q[4], q[5]), q[6], q[7]);
-        if (mode == 2) shadow[i] = Ray(Point(q[0], q[1], q[2]), Vector(q[3], q[4], q[5]), q[6], q[7]);
+        if (occl) shadow[i] = Ray(Point(q[0], q[1], q[2]), Vector(q[3], q[4], q[5]), q[6], q[7]);
         isect[i] = Intersection();
         hit[i] = hit_io[i] != 0;
       }
+      const Scene scene(a.get());
       if (mode == 0)
         a->Intersect(rays.data(), isect.data(), hit.get(), n);
       else if (mode == 1)
         a->Intersect(rays.data(), isect.data(), hit.get(), n, 1);
-      else
+      else if (mode == 2)
         a->IntersectP(shadow.data(), occ, n, hit.get());
+      else if (mode == 3)  // through pbrt's Scene
+        scene.Intersect(rays.data(), isect.data(), hit.get(), (int)n, (unsigned)spp);
+      else if (mode == 4)  // the path integrator's first bounce: Scene bounce 0
+        scene.Intersect(rays.data(), isect.data(), hit.get(), (int)n, (unsigned)spp, 0);
+      else if (mode == 5)
+        scene.IntersectP(shadow.data(), occ, n, hit.get());
+      else  // the accelerator's bounce overload with bounce 0
+        a->Intersect(rays.data(), isect.data(), hit.get(), n, 0);
     }
     for (uint32_t i = 0; i < n; ++i) {
-      if (mode == 2) continue;
+      if (occl) continue;
       hit_io[i] = hit[i] ? 1 : 0;
       tri[i] = hit[i] ? isect[i].triangle : -1;
       prim[i] = hit[i] ? (int32_t)isect[i].primitiveId : -1;
@@ -154,6 +166,28 @@ int mirror_render(const char* accel, const char* params, const float* v9, uint32
     stats[4] = (int64_t)st.shadow_rays;
     stats[5] = (int64_t)st.bounce_rays;
     if (pfm_path && !WritePFM(pfm_path, xres, yres, img)) throw std::runtime_error("PFM write");
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Scene over an aggregate it does not know (none): which = 0 Intersect, 1 Intersect(bounce),
+// 2 IntersectP, 3 MaxRaysPerCall.  Returns 1 with the Severe message, 0 if nothing failed.
+int mirror_scene_unsupported(int which) {
+  SetSevereHandler(throw_severe);
+  try {
+    const Scene scene(nullptr);
+    RayDifferential r;
+    Ray sr;
+    Intersection is;
+    bool hit = true;
+    char occ = '0';
+    if (which == 0) scene.Intersect(&r, &is, &hit, 1, 1);
+    else if (which == 1) scene.Intersect(&r, &is, &hit, 1, 1, 0);
+    else if (which == 2) scene.IntersectP(&sr, &occ, 1, &hit);
+    else (void)scene.MaxRaysPerCall();
     return 0;
   } catch (const std::exception& e) {
     g_err = e.what();

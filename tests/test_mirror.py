@@ -104,7 +104,59 @@ def test_factory_rejects(scenes, accel, params, msg):
         run(accel, params, v9, rays, 0)
 
 
+@pytest.mark.parametrize("which,msg", [
+    (0, "Called Intersect with unsupported aggregate!"),
+    (1, "Called Intersect with unsupported aggregate!"),
+    (2, "Called IntersectP with unsupported aggregate!"),
+    (3, "Called MaxRaysPerCall with unsoppurted aggregate!"),
+])
+def test_scene_unsupported_aggregate(which, msg):
+    """pbrt's Scene dispatches only to the three ray-space accelerators; any other aggregate
+    is Severe() with the reference's messages (scene.cpp:55-120, scene.h:71-99; the misspelling
+    is the reference's)."""
+    lib = _driver()
+    lib.mirror_scene_unsupported.restype = ctypes.c_int
+    assert lib.mirror_scene_unsupported(which) == 1
+    assert lib.mirror_last_error().decode() == msg
+
+
 # ------------------------------------------------------------------------------------ GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize("accel", ["naive", "rayhierarchy", "ray-bvh"])
+def test_scene_dispatch(scenes, accel):
+    """pbrt's Scene forwards to the accelerator (scene.cpp:70-120, scene.h:71-99):
+    Scene::Intersect without a bounce is the primary call, Scene::Intersect(..., bounce) is the bounce call (naive gets
+    bounce + 1, so Scene's bounce 0 is already a secondary call, naive.cpp:215), and
+    Scene::IntersectP is IntersectP.  The accelerator calls themselves are pinned to the oracle
+    by the tests above; here the Scene route must give the same outputs field for field."""
+    v9 = scenes.random_triangles(800, seed=21)
+    n = 64 * 64
+    rays = scenes.random_rays(n, seed=22, lo=-1.5, hi=1.5, maxt=np.inf, width=64)
+    rays["dx"][::11] = np.float32(-6.0)  # naive skips such bounce rays
+    sel = np.random.default_rng(23).random(n) < 0.75
+    params = "node=ia" if accel != "naive" else ""
+    pre = (64, 64, 1)
+
+    def same(a, b, label):
+        for k in ("hit", "tri", "prim", "maxt", "u", "v", "dpdu", "dpdv"):
+            assert np.array_equal(a[k], b[k]), f"{accel} {label}: {k} differs"
+
+    same(run(accel, params, v9, rays, 3, pre=pre), run(accel, params, v9, rays, 0, pre=pre),
+         "Scene::Intersect")
+    same(run(accel, params, v9, rays, 4, hit=sel, pre=pre),
+         run(accel, params, v9, rays, 1, hit=sel, pre=pre), "Scene::Intersect(bounce 0)")
+    if accel == "naive":
+        # the accelerator's own bounce 0 is the camera kernel
+        same(run(accel, params, v9, rays, 6, pre=pre), run(accel, params, v9, rays, 0, pre=pre),
+             "NaiveAccel::Intersect(bounce 0)")
+    sh = dict(rays)
+    sh["tmax"] = np.random.default_rng(24).uniform(0.5, 4.0, n).astype(np.float32)
+    a = run(accel, params, v9, sh, 5, hit=sel, pre=pre)
+    b = run(accel, params, v9, sh, 2, hit=sel, pre=pre)
+    assert np.array_equal(a["occ"], b["occ"]), f"{accel}: Scene::IntersectP differs"
+    assert np.count_nonzero(a["occ"] == "1") > 0
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("accel,node", [("rayhierarchy", "box_dir"), ("rayhierarchy", "ia"),
                                         ("ray-bvh", "box_dir")])
