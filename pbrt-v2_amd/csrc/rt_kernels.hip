@@ -267,10 +267,21 @@ __device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J, P no
     while (pow4_off(s + 1) <= i) ++s;
     int local = i - pow4_off(s), nl = 1 << s, l = top - s;
     int gk = K * nl + local % nl, gj = J * nl + local / nl;
+    // the level tables are read with constant indices (scalar loads issued together) and
+    // selected per lane, not loaded per lane one after another
+    int gw = 0, gh = 0, off = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxLevels; ++q) {
+      if (q == l) {
+        gw = g.gw[q];
+        gh = g.gh[q];
+        off = g.off[q];
+      }
+    }
     int valid = 0;
     Node n{};
-    if (gk < g.gw[l] && gj < g.gh[l]) {
-      const P p = nodes + 4 * ((size_t)g.off[l] + (size_t)gj * g.gw[l] + gk);
+    if (gk < gw && gj < gh) {
+      const P p = nodes + 4 * ((size_t)off + (size_t)gj * gw + gk);
       float4 a = ld4(p, 0), b = ld4(p, 1), c = ld4(p, 2), d = ld4(p, 3);
       n.dmin = {a.x, a.y, a.z};
       n.dmax = {b.x, b.y, b.z};
@@ -302,6 +313,7 @@ __device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J, P no
 //            staged in LDS; pairs with a passing test get the exact cull chain; the tests of
 //            verified pairs update the ray (LDS atomic min of the (t, id) key / occlusion flag).
 constexpr int kCluster = 64;     // triangles per cluster
+constexpr uint32_t kNoUnit = 0xffffffffu;  // no unit left in the queue
 constexpr int kQ0Cap = 1024;     // root survivors held before expansion
 constexpr int kQCap = 1024;      // survivors of one expansion round
 constexpr int kExpand = 256;     // parents expanded per round (4 children each, 4 per thread)
@@ -391,6 +403,7 @@ struct RhShared {
   uint32_t sQ0[kQ0Cap];
   uint32_t sQ[LMAX > 1 ? LMAX - 1 : 1][kQCap];  // level s >= 1 survivors live in sQ[s - 1]
   int sCnt[kMaxLevels];
+  int sSeg[kThreads / 64];  // phase C: flagged pairs of each wavefront's quarter of the list
   int sTraced, sOccluded;
 };
 
@@ -422,9 +435,14 @@ __device__ __forceinline__ bool leaf_hit(const SH& sh, int slot, const Tri& T, f
 // Phase C over `n` (triangle slot, leaf) pairs.  Item = (pair, ray of its leaf): consecutive
 // lanes take consecutive ray slots of one pair, so the LDS ray reads are conflict-free and
 // the triangle load is a broadcast.
+//
+// Between C1 and C2 every wavefront moves the flagged pairs of its quarter of the list to the
+// front of that quarter (in place, in list order), so that the exact cull chain of C2 runs
+// with every lane on a flagged pair and C3 sees flagged pairs only.  Pairs without a passing
+// leaf test cannot change a ray, so dropping them changes nothing.
 template <int MODE, int NODE, int RBMAX, int LMAX, class PF, class PU>
-__device__ __forceinline__ void leaf_rays(const RHGeom& g, PF v9s, PU perm, const uint32_t* q,
-                                          int n, RhShared<MODE, RBMAX, LMAX>& sh) {
+__device__ __forceinline__ void leaf_rays(const RHGeom& g, PF v9s, PU perm, uint32_t* q, int n,
+                                          RhShared<MODE, RBMAX, LMAX>& sh) {
   const int G = g.g;
   const int total = n * G;
   const bool pow2 = (G & (G - 1)) == 0;
@@ -451,25 +469,56 @@ __device__ __forceinline__ void leaf_rays(const RHGeom& g, PF v9s, PU perm, cons
     }
   }
   __syncthreads();
-  // C2: exact cull chain of the flagged pairs
+  // Compaction: wavefront w owns list positions [w * S, min(n, (w + 1) * S)), S a multiple of
+  // 64.  A flagged entry moves to a position at or below its own, and each 64-entry chunk is
+  // read in full before it is written, so no entry is overwritten before it has been read.
+  static_assert(kThreads == 256, "four wavefronts of 64 per work-group");
+  const int lane = __lane_id();
+  const int wv = threadIdx.x >> 6;
+  const int S = ((n + kThreads - 1) / kThreads) * 64;
+  const int s0 = wv * S, s1 = min(n, s0 + S);
+  int cw = 0;
+  for (int base = s0; base < s1; base += 64) {
+    const int e = base + lane;
+    PairState<MODE> ps = 0;
+    uint32_t P = 0;
+    if (e < s1) {
+      ps = sh.sPair[e];
+      P = q[e];
+    }
+    const uint64_t bal = __ballot(ps != 0);
+    if (ps != 0) {
+      const int d = s0 + cw + __popcll(bal & ((1ull << lane) - 1ull));
+      q[d] = P;
+      sh.sPair[d] = ps;
+    }
+    cw += __popcll(bal);
+  }
+  if (lane == 0) sh.sSeg[wv] = cw;
+  // C2: exact cull chain of the wavefront's flagged pairs (written by this wavefront above)
   int verified = 0;
-  for (int e = threadIdx.x; e < n; e += kThreads) {
+  for (int i = lane; i < cw; i += 64) {
+    const int e = s0 + i;
     const PairState<MODE> ps = sh.sPair[e];
-    if (ps == 0) continue;
     const uint32_t P = q[e];
     const bool ok = exact_chain<MODE, NODE, RBMAX, LMAX>(g, sh, load_tri(v9s, P >> 6), (int)(P & 63u));
     sh.sPair[e] = ok ? (PairState<MODE>)(ps | kPairOk<MODE>) : (PairState<MODE>)0;
     verified |= ok;
   }
   if (!__syncthreads_or(verified)) return;
-  // C3: apply the leaf tests of verified pairs
-  for (int item = threadIdx.x; item < total; item += kThreads) {
-    const int e = pow2 ? item >> lg : item / G;
+  // C3: apply the leaf tests of verified pairs.  Flagged pair j (over the four quarters in
+  // order) sits at position w * S + (j - p[w]), p the prefix sums of the quarters' counts.
+  const int p1 = sh.sSeg[0], p2 = p1 + sh.sSeg[1], p3 = p2 + sh.sSeg[2], nf = p3 + sh.sSeg[3];
+  const int ftotal = nf * G;
+  for (int item = threadIdx.x; item < ftotal; item += kThreads) {
+    const int j = pow2 ? item >> lg : item / G;
+    const int r = item - j * G;
+    const int w = (j >= p1) + (j >= p2) + (j >= p3);
+    const int e = w * S + j - (w == 0 ? 0 : w == 1 ? p1 : w == 2 ? p2 : p3);
     const PairState<MODE> ps = sh.sPair[e];
     if (!(ps & kPairOk<MODE>)) continue;
     const uint32_t P = q[e];
     const uint32_t slot_t = P >> 6;
-    const int r = item - e * G;
     const int slot = (int)(P & 63u) * G + r;
     if (masked) {  // occlusion: the rays that passed in C1
       if (!((ps >> r) & 1u)) continue;
@@ -611,16 +660,26 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
                                                       const uint4* __restrict__ units,
                                                       uint32_t* __restrict__ ctr) {
   __shared__ RhShared<MODE, RBMAX, LMAX> sh;
-  __shared__ uint32_t sUnit;
+  __shared__ uint4 sUnit;  // the unit taken next; x = kNoUnit when the queue is empty
   const int tid = threadIdx.x;
   const uint32_t nunits = __builtin_amdgcn_readfirstlane(ctr[0]);
+  if (tid == 0) {
+    const uint32_t u = atomicAdd(&ctr[1], 1u);
+    sUnit = u < nunits ? units[u] : make_uint4(kNoUnit, 0u, 0u, 0u);
+  }
+  if (tid < kMaxLevels) sh.sCnt[tid] = 0;
+  if (tid == 0) {
+    sh.sTraced = 0;
+    sh.sOccluded = 0;
+  }
+  __syncthreads();
   for (;;) {
-    if (tid == 0) sUnit = atomicAdd(&ctr[1], 1u);
-    __syncthreads();
-    const uint32_t u = __builtin_amdgcn_readfirstlane(sUnit);
-    if (u >= nunits) break;
-    const uint4 unit = units[u];
-    const RhItem& it = items[__builtin_amdgcn_readfirstlane(unit.x)];
+    const uint4 unit = make_uint4(__builtin_amdgcn_readfirstlane(sUnit.x),
+                                  __builtin_amdgcn_readfirstlane(sUnit.y),
+                                  __builtin_amdgcn_readfirstlane(sUnit.z),
+                                  __builtin_amdgcn_readfirstlane(sUnit.w));
+    if (unit.x == kNoUnit) break;
+    const RhItem& it = items[unit.x];
     const RHGeom& g = it.g;
     const RaysDev& r = it.r;
     const auto v9s = glb(it.v9s);
@@ -628,23 +687,23 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
     const auto tbox = glb(it.tbox);
     const uint32_t ntri = it.ntri;
     const int top = g.L - 1;
-    const int b = (int)__builtin_amdgcn_readfirstlane(unit.y);
+    const int b = (int)unit.y;
     const int K = b % g.gw[top], J = b / g.gw[top];
-    if (tid < kMaxLevels) sh.sCnt[tid] = 0;
-    if (tid == 0) {
-      sh.sTraced = 0;
-      sh.sOccluded = 0;
-    }
+    const auto cl = glb(it.clist) + (size_t)b * it.nclusters + unit.z;
+    const int ncl = (int)unit.w;
+    // first cluster of this thread, loaded with the staging loads
+    const uint32_t cfirst = tid / kCluster < ncl ? cl[tid / kCluster] : 0u;
+    // subtree, rays, validity and occlusion flags: independent loads, issued together
     load_subtree(g, K, J, glb(it.nodes), sh.sN, sh.sV);
-    __syncthreads();
     for (int slot = tid; slot < RBMAX; slot += kThreads) {
       int x = 0, y = 0, leaf = 0;
       bool traced = false;
       if (slot_xy(g, K, J, slot, x, y, leaf)) {
         const size_t idx = (size_t)y * g.W + x;
-        traced = MODE == 0 || r.valid == nullptr || glb(r.valid)[idx] != 0;
+        const bool valid = MODE == 0 || r.valid == nullptr || glb(r.valid)[idx] != 0;
         // a ray another unit already found occluded needs no more tests
-        if (MODE == 2 && traced && glb(it.occ)[idx]) traced = false;
+        const bool found = MODE == 2 && glb(it.occ)[idx] != 0;
+        traced = valid && !found;
         sh.sRo[slot] = make_float4(glb(r.ox)[idx], glb(r.oy)[idx], glb(r.oz)[idx],
                                    glb(r.tmin)[idx]);
         sh.sRd[slot] = make_float4(glb(r.dx)[idx], glb(r.dy)[idx], glb(r.dz)[idx],
@@ -659,12 +718,11 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
     __syncthreads();
     if (sh.sV[0] && sh.sTraced > 0) {
       const Node& root = sh.sN[0];
-      const auto cl = glb(it.clist) + (size_t)b * it.nclusters + unit.z;
-      const int ncl = (int)unit.w;
       bool done = false;
       for (int k0 = 0; k0 < ncl && !done; k0 += kThreads / kCluster) {
         const int k = k0 + tid / kCluster;
-        const uint32_t slot = k < ncl ? cl[k] * kCluster + (uint32_t)(tid % kCluster) : ntri;
+        const uint32_t c = k0 == 0 ? cfirst : (k < ncl ? cl[k] : 0u);
+        const uint32_t slot = k < ncl ? c * kCluster + (uint32_t)(tid % kCluster) : ntri;
         bool pass = false;
         if (slot < ntri) pass = cull_bound_slot<MODE, NODE>(root, v9s, tbox, slot);
         ballot_append(sh.sQ0, &sh.sCnt[0], pass, slot << 6);
@@ -677,6 +735,14 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
         }
       }
       if (!done && sh.sCnt[0] > 0) expand_queue<MODE, NODE, RBMAX, LMAX>(g, v9s, tbox, perm, sh);
+    }
+    // The next unit is claimed once this one's traversal is over (a claim made earlier would
+    // hold a second unit per work-group when the queue runs dry); the atomic and the
+    // descriptor load overlap the merge below.
+    uint4 ndesc = make_uint4(kNoUnit, 0u, 0u, 0u);
+    if (tid == 0) {
+      const uint32_t next = atomicAdd(&ctr[1], 1u);
+      if (next < nunits) ndesc = units[next];
     }
     for (int slot = tid; slot < g.rb; slot += kThreads) {
       int x, y, leaf;
@@ -693,7 +759,14 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
         }
       }
     }
-    __syncthreads();  // LDS (and sUnit) are rewritten by the next unit
+    __syncthreads();  // LDS is rewritten by the next unit
+    if (tid == 0) {
+      sUnit = ndesc;
+      sh.sTraced = 0;
+      sh.sOccluded = 0;
+    }
+    if (tid < kMaxLevels) sh.sCnt[tid] = 0;
+    __syncthreads();
   }
 }
 
