@@ -782,7 +782,7 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
 // exact-conservative bound of the cull, runs the leaf tests, and evaluates the exact cull
 // chain only for pairs in which some ray passes its leaf test.
 constexpr int kBvhQCap = 3072;   // traversal stack: 256 + 224 * depth <= 3072 for depth <= 12
-constexpr int kPairCap = 1024;   // (BVH leaf, hierarchy leaf) pairs awaiting leaf tests
+constexpr int kPairCap = 512;    // (BVH leaf, hierarchy leaf) pairs awaiting leaf tests
 constexpr int kBvhPop = 32;      // traversal items expanded per round (8 children each)
 
 template <int MODE, int RBMAX, int LMAX>
@@ -792,13 +792,18 @@ struct BvhShared {
   float4 sRo[RBMAX];
   float4 sRd[RBMAX];
   unsigned long long sKey[MODE == 2 ? 1 : RBMAX];
-  uint32_t sAux[RBMAX];  // occlusion flag (shadow)
+  uint32_t sAux[MODE == 2 ? RBMAX : 1];  // occlusion flag (shadow)
   uint8_t sRv[RBMAX];
   uint32_t sT[kBvhQCap];   // traversal items: BVH node << 7 | subtree node index
-  uint32_t sPt[kPairCap];  // pair: first primitive position of the BVH leaf
   uint32_t sPn[kPairCap];  // pair: BVH leaf << 6 | hierarchy leaf
-  uint32_t sPc[kPairCap];  // pair: primitives tested (the leaf's, or 1 for secondary rays)
+  // Occlusion keeps each pair's primitive range (first position, count) in LDS: its leaf
+  // tests are the bulk of the work and an extra dependent load each costs more than the
+  // occupancy they take.  Closest hits read the range from the leaf node, which keeps the
+  // kernel within six work-groups per CU.
+  uint32_t sPt[MODE == 2 ? kPairCap : 1];
+  uint32_t sPc[MODE == 2 ? kPairCap : 1];
   PairState<MODE> sPair[kPairCap];  // phase C pair state
+  int sSeg[kThreads / 64];  // phase C: flagged pairs of each wavefront's quarter of the list
   int sTq, sPq;
   int sTraced, sOccluded;
 };
@@ -812,6 +817,7 @@ __device__ __forceinline__ void bvh_box(P bvh, uint32_t n, V3& bmin, V3& bmax, u
   off = __float_as_uint(b.z);
   np = __float_as_uint(b.w);
 }
+
 
 template <int NODE>
 __device__ __forceinline__ bool bvh_bound(const Node& n, V3 bmin, V3 bmax) {
@@ -854,6 +860,21 @@ __device__ __forceinline__ bool bvh_chain(const RHGeom& g, const SH& sh, const R
   return true;
 }
 
+// First primitive position and number of primitives that pair e tests: its BVH leaf's, or
+// only the first one for secondary rays (intersection6DB_BVH2.cl).
+template <int MODE, class SH>
+__device__ __forceinline__ void pair_prims(const SH& sh, const RhItem& it, int e, uint32_t& off,
+                                           uint32_t& cnt) {
+  if constexpr (MODE == 2) {
+    off = sh.sPt[e];
+    cnt = sh.sPc[e];
+  } else {
+    const float4 b = ld4(glb(it.bvh), 2 * (size_t)(sh.sPn[e] >> 6) + 1);
+    off = __float_as_uint(b.z);
+    cnt = MODE == 1 ? 1u : __float_as_uint(b.w);
+  }
+}
+
 // Leaf tests of the queued pairs (test, exact chain of pairs with a passing test, apply).
 template <int MODE, int NODE, int RBMAX, int LMAX>
 __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
@@ -872,7 +893,8 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
     const int r = item - e * G;
     const int slot = (int)(sh.sPn[e] & 63u) * G + r;
     if (!sh.sRv[slot]) continue;  // cleared once the ray is found occluded
-    const uint32_t off = sh.sPt[e], cnt = sh.sPc[e];
+    uint32_t off, cnt;
+    pair_prims<MODE>(sh, it, e, off, cnt);
     for (uint32_t f = 0; f < cnt; ++f) {
       float t, b1, b2;
       if (leaf_hit<TM>(sh, slot, load_mt(glb(it.tmt), glb(it.prim_order)[off + f]), t, b1,
@@ -888,20 +910,55 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
     }
   }
   __syncthreads();
+  // Compaction of the flagged pairs, per wavefront and in place, as in leaf_rays.
+  const int lane = __lane_id();
+  const int wv = threadIdx.x >> 6;
+  const int S = ((n + kThreads - 1) / kThreads) * 64;
+  const int s0 = wv * S, s1 = min(n, s0 + S);
+  int cw = 0;
+  for (int base = s0; base < s1; base += 64) {
+    const int e = base + lane;
+    PairState<MODE> ps = 0;
+    uint32_t pn = 0, pt = 0, pc = 0;
+    if (e < s1) {
+      ps = sh.sPair[e];
+      pn = sh.sPn[e];
+      if constexpr (MODE == 2) {
+        pt = sh.sPt[e];
+        pc = sh.sPc[e];
+      }
+    }
+    const uint64_t bal = __ballot(ps != 0);
+    if (ps != 0) {
+      const int d = s0 + cw + __popcll(bal & ((1ull << lane) - 1ull));
+      sh.sPair[d] = ps;
+      sh.sPn[d] = pn;
+      if constexpr (MODE == 2) {
+        sh.sPt[d] = pt;
+        sh.sPc[d] = pc;
+      }
+    }
+    cw += __popcll(bal);
+  }
+  if (lane == 0) sh.sSeg[wv] = cw;
   int verified = 0;
-  for (int e = threadIdx.x; e < n; e += kThreads) {
+  for (int i = lane; i < cw; i += 64) {
+    const int e = s0 + i;
     const PairState<MODE> ps = sh.sPair[e];
-    if (ps == 0) continue;
     const bool ok = bvh_chain<NODE>(g, sh, it, sh.sPn[e] >> 6, (int)(sh.sPn[e] & 63u));
     sh.sPair[e] = ok ? (PairState<MODE>)(ps | kPairOk<MODE>) : (PairState<MODE>)0;
     verified |= ok;
   }
   if (__syncthreads_or(verified)) {
-    for (int item = threadIdx.x; item < total; item += kThreads) {
-      const int e = pow2 ? item >> lg : item / G;
+    const int p1 = sh.sSeg[0], p2 = p1 + sh.sSeg[1], p3 = p2 + sh.sSeg[2], nf = p3 + sh.sSeg[3];
+    const int ftotal = nf * G;
+    for (int item = threadIdx.x; item < ftotal; item += kThreads) {
+      const int j = pow2 ? item >> lg : item / G;
+      const int r = item - j * G;
+      const int w = (j >= p1) + (j >= p2) + (j >= p3);
+      const int e = w * S + j - (w == 0 ? 0 : w == 1 ? p1 : w == 2 ? p2 : p3);
       const PairState<MODE> ps = sh.sPair[e];
       if (!(ps & kPairOk<MODE>)) continue;
-      const int r = item - e * G;
       const int slot = (int)(sh.sPn[e] & 63u) * G + r;
       if (masked) {  // occlusion: the rays that passed in C1
         if (!((ps >> r) & 1u)) continue;
@@ -910,7 +967,8 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
         continue;
       }
       if (!sh.sRv[slot]) continue;
-      const uint32_t off = sh.sPt[e], cnt = sh.sPc[e];
+      uint32_t off, cnt;
+      pair_prims<MODE>(sh, it, e, off, cnt);
       for (uint32_t f = 0; f < cnt; ++f) {
         const uint32_t tri = glb(it.prim_order)[off + f];
         float t, b1, b2;
@@ -935,9 +993,11 @@ template <int MODE, class SH>
 __device__ __forceinline__ void bvh_emit(SH& sh, uint32_t b, uint32_t off, uint32_t np, int li) {
   const int e = atomicAdd(&sh.sPq, 1);
   if (e < kPairCap) {  // room for every emission of one round is kept free
-    sh.sPt[e] = off;
     sh.sPn[e] = (b << 6) | (uint32_t)li;
-    sh.sPc[e] = MODE == 1 ? 1u : np;
+    if constexpr (MODE == 2) {
+      sh.sPt[e] = off;
+      sh.sPc[e] = np;
+    }
   }
 }
 
@@ -983,7 +1043,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
       }
       sh.sRv[slot] = traced ? 1 : 0;
       if (MODE != 2) sh.sKey[slot] = kMissKey;
-      sh.sAux[slot] = 0u;
+      if (MODE == 2) sh.sAux[slot] = 0u;
       const uint64_t bal = __ballot(traced);
       if (__lane_id() == 0 && bal) atomicAdd(&sh.sTraced, __popcll(bal));
     }
