@@ -397,7 +397,9 @@ struct RhShared {
   float4 sRo[RBMAX];  // origin, tmin
   float4 sRd[RBMAX];  // direction, tmax
   unsigned long long sKey[MODE == 2 ? 1 : RBMAX];
-  uint32_t sAux[RBMAX];  // repair key (primary) or occlusion flag (shadow)
+  // repair key per ray (primary), or occlusion bit per ray (shadow); a bitmask keeps the
+  // occlusion kernel within the LDS of six work-groups per CU
+  uint32_t sAux[MODE == 0 ? RBMAX : MODE == 2 ? (RBMAX + 31) / 32 : 1];
   uint8_t sRv[RBMAX];    // ray exists and is traced
   PairState<MODE> sPair[kQ0Cap];  // phase C pair state
   uint32_t sQ0[kQ0Cap];
@@ -406,6 +408,13 @@ struct RhShared {
   int sSeg[kThreads / 64];  // phase C: flagged pairs of each wavefront's quarter of the list
   int sTraced, sOccluded;
 };
+
+// Marks ray `slot` occluded; true when it was not marked before.
+template <class SH>
+__device__ __forceinline__ bool mark_occluded(SH& sh, int slot) {
+  const uint32_t bit = 1u << (slot & 31);
+  return (atomicOr(&sh.sAux[slot >> 5], bit) & bit) == 0u;
+}
 
 // The exact cull chain from the block root to leaf `li` (reference candidate test).
 template <int MODE, int NODE, int RBMAX, int LMAX>
@@ -522,7 +531,7 @@ __device__ __forceinline__ void leaf_rays(const RHGeom& g, PF v9s, PU perm, uint
     const int slot = (int)(P & 63u) * G + r;
     if (masked) {  // occlusion: the rays that passed in C1
       if (!((ps >> r) & 1u)) continue;
-      if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
+      if (mark_occluded(sh, slot)) atomicAdd(&sh.sOccluded, 1);
       sh.sRv[slot] = 0;
       continue;
     }
@@ -531,12 +540,14 @@ __device__ __forceinline__ void leaf_rays(const RHGeom& g, PF v9s, PU perm, uint
     if (!leaf_hit<MODE>(sh, slot, load_tri(v9s, slot_t), t, b1, b2)) continue;
     const float tmin = sh.sRo[slot].w;
     if (MODE == 2) {
-      if (atomicOr(&sh.sAux[slot], 1u) == 0u) atomicAdd(&sh.sOccluded, 1);
+      if (mark_occluded(sh, slot)) atomicAdd(&sh.sOccluded, 1);
       sh.sRv[slot] = 0;
     } else {
       if (t > tmin && t < __int_as_float(0x7f800000))
         atomicMin(&sh.sKey[slot], (unsigned long long)hit_key(t, (int32_t)perm[slot_t]));
-      if (MODE == 0 && repair_region(b1, b2) && t >= tmin) atomicMin(&sh.sAux[slot], fkey(t));
+      if constexpr (MODE == 0) {
+        if (repair_region(b1, b2) && t >= tmin) atomicMin(&sh.sAux[slot], fkey(t));
+      }
     }
   }
   __syncthreads();
@@ -711,7 +722,10 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
       }
       sh.sRv[slot] = traced ? 1 : 0;
       if (MODE != 2) sh.sKey[slot] = kMissKey;
-      sh.sAux[slot] = MODE == 0 ? kNoKey32 : 0u;
+      if constexpr (MODE == 0) sh.sAux[slot] = kNoKey32;
+      if constexpr (MODE == 2) {
+        if ((slot & 31) == 0) sh.sAux[slot >> 5] = 0u;
+      }
       const uint64_t bal = __ballot(traced);
       if (__lane_id() == 0 && bal) atomicAdd(&sh.sTraced, __popcll(bal));
     }
@@ -749,13 +763,15 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
       if (!slot_xy(g, K, J, slot, x, y, leaf)) continue;
       const size_t idx = (size_t)y * g.W + x;
       if (MODE == 2) {
-        if (sh.sAux[slot]) glb(it.occ)[idx] = 1;
+        if ((sh.sAux[slot >> 5] >> (slot & 31)) & 1u) glb(it.occ)[idx] = 1;
       } else {
         const unsigned long long key = sh.sKey[slot];
         if (key != kMissKey) atomicMin((unsigned long long*)&it.keys[idx], key);
-        if (MODE == 0 && it.repkeys) {
-          const uint32_t rk = sh.sAux[slot];
-          if (rk != kNoKey32) atomicMin(&it.repkeys[idx], rk);
+        if constexpr (MODE == 0) {
+          if (it.repkeys) {
+            const uint32_t rk = sh.sAux[slot];
+            if (rk != kNoKey32) atomicMin(&it.repkeys[idx], rk);
+          }
         }
       }
     }
