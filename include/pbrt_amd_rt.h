@@ -160,7 +160,8 @@ int rt_make_bounce_rays(rt_scene *scene, const rt_rays *in, const int32_t *id, c
  *   triangles each) per rayhierarchy work unit for closest-hit / occlusion traces; smaller
  *   values split a block's work over more units of the persistent traversal kernel.
  *   RT_PARAM_GRID_CLOSEST / RT_PARAM_GRID_OCCLUDED: work-groups of the persistent traversal
- *   kernel (batches only).
+ *   kernel (batches only); the built-in choice (0) is one work-group per resident slot of the
+ *   kernel on the device (CUs x work-groups per CU).
  *   RT_PARAM_GRAPH: 1 = replay the batch's launch sequence as a HIP graph while its jobs are
  *   unchanged (batches only; needs a non-null stream). */
 enum {

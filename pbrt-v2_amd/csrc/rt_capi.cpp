@@ -324,7 +324,6 @@ hipError_t timed_launch(Timer& tm, int mode, hipStream_t st, F fn) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int kDefaultPer[2] = {8, 4};  // passing clusters per unit: closest, occluded
-constexpr int kDefaultGrid = 2048;       // persistent work-groups (8 per CU)
 constexpr int kDefaultRootClustersPerUnit = 8;  // passing root clusters per ray-bvh unit
 
 // Lays out the jobs as launch groups (one per mode and node type, in job order) in the
@@ -427,8 +426,10 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     if (blocks >= (1ull << 31) || rays >= (1ull << 31)) return RT_ERR_UNSUPPORTED;
     L.nblocks = (uint32_t)blocks;
     L.nrays = (uint32_t)rays;
-    const int gset = prm.grid[mi] > 0 ? prm.grid[mi] : kDefaultGrid;
-    L.grid = (int)std::max<size_t>(1, std::min<size_t>(L.max_units, (size_t)gset));
+    // 0: the launch takes one work-group per resident slot of the traversal kernel
+    L.grid = prm.grid[mi] > 0
+                 ? (int)std::max<size_t>(1, std::min<size_t>(L.max_units, (size_t)prm.grid[mi]))
+                 : 0;
     L.units = total;
     total += align256(sizeof(uint4) * std::max<size_t>(L.max_units, 1));
     L.ctr = total;
@@ -480,6 +481,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     G.replist = gl[gi].repair ? (uint32_t*)(A + gl[gi].replist) : nullptr;
     G.per_unit = gl[gi].per;
     G.grid = gl[gi].grid;
+    G.max_units = (uint32_t)std::min<size_t>(gl[gi].max_units, 0xffffffffu);
     groups.push_back(G);
   }
   const bool same = ws.last_arena == A && ws.last.size() == table.size() &&
@@ -1005,7 +1007,7 @@ int rt_batch_get_param(rt_batch* b, int32_t param, int32_t* value) {
   if (!p) return RT_ERR_INVALID;
   const int mi = (param == RT_PARAM_CLUSTERS_OCCLUDED || param == RT_PARAM_GRID_OCCLUDED) ? 1 : 0;
   const bool grid = param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED;
-  *value = *p > 0 ? *p : (grid ? kDefaultGrid : kDefaultPer[mi]);
+  *value = *p > 0 ? *p : (grid ? 0 : kDefaultPer[mi]);
   return RT_OK;
 }
 
@@ -1016,7 +1018,7 @@ int rt_batch_autotune(rt_batch* b, void* stream, int32_t reps) {
   hipStream_t hs = (hipStream_t)stream;
   if (b->jobs.empty()) return RT_OK;
   static const int kPer[] = {2, 4, 8, 16, 32};
-  static const int kGrid[] = {1024, 2048, 4096};
+  static const int kGrid[] = {0, 1024, 2048};  // 0: one work-group per resident slot
   const bool was_on = b->timer.on;
   const bool was_graph = b->use_graph;
   b->use_graph = false;

@@ -98,7 +98,8 @@ struct RhGroup {
   uint32_t* ctr;      // [0] units, [1] next unit to take, [2] rays flagged for repair
   uint32_t* replist;  // global ray indices flagged for repair (primary with repair only)
   int per_unit;       // passing clusters per work unit
-  int grid;           // work-groups of the persistent unit kernel
+  int grid;           // work-groups of the persistent unit kernel; 0 = one per resident slot
+  uint32_t max_units;  // upper bound of the group's work units
 };
 
 // Launchers (all asynchronous on `s`).

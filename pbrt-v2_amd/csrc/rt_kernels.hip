@@ -14,6 +14,7 @@
 // list, and every ray then runs the leaf test against the list.  The nearest hit is reduced
 // in registers as a (t, id) key, so the result is deterministic (ties -> smallest id).
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "rt_device.h"
@@ -1598,6 +1599,33 @@ __global__ void fill_kernel(T* p, T v, size_t n) {
 
 inline dim3 grid1(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
+// Work-groups of a persistent kernel that are resident at once on the current device: CUs x
+// work-groups per CU (limited by registers and LDS), at most `cap`.  A larger grid only adds
+// work-groups that start when the queue is nearly drained.  The count is kept per kernel and
+// device once known (2048 until then).
+template <class F>
+unsigned resident_grid(F kernel, uint32_t cap) {
+  constexpr int kDevices = 64;
+  static std::atomic<unsigned> known[kDevices];
+  int dev = 0;
+  unsigned n = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kDevices)
+    n = known[dev].load(std::memory_order_relaxed);
+  if (n == 0) {
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) ==
+            hipSuccess &&
+        cus > 0 && per_cu > 0) {
+      n = (unsigned)cus * (unsigned)per_cu;
+      if (dev >= 0 && dev < kDevices) known[dev].store(n, std::memory_order_relaxed);
+    } else {
+      n = 2048;
+    }
+  }
+  return std::max(1u, std::min(n, std::max(cap, 1u)));
+}
+
 // Launch sequence of one rayhierarchy group with the mode and node type fixed.
 template <int MODE, int NODE>
 hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
@@ -1617,19 +1645,18 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
     e = hipEventRecord(t0, s);
     if (e != hipSuccess) return e;
   }
-  const dim3 grid(std::max(G.grid, 1));
+  auto launch = [&](auto kernel) {
+    const dim3 grid(G.grid > 0 ? (unsigned)G.grid : resident_grid(kernel, G.max_units));
+    hipLaunchKernelGGL(kernel, grid, dim3(kThreads), 0, s, G.items, G.units, G.ctr);
+  };
   if (G.bvh && G.big)
-    hipLaunchKernelGGL((bvh_unit_kernel<MODE, NODE, 1024, 4>), grid, dim3(kThreads), 0, s,
-                       G.items, G.units, G.ctr);
+    launch(bvh_unit_kernel<MODE, NODE, 1024, 4>);
   else if (G.bvh)
-    hipLaunchKernelGGL((bvh_unit_kernel<MODE, NODE, 256, 3>), grid, dim3(kThreads), 0, s,
-                       G.items, G.units, G.ctr);
+    launch(bvh_unit_kernel<MODE, NODE, 256, 3>);
   else if (G.big)
-    hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 1024, 4>), grid, dim3(kThreads), 0, s,
-                       G.items, G.units, G.ctr);
+    launch(rh_unit_kernel<MODE, NODE, 1024, 4>);
   else
-    hipLaunchKernelGGL((rh_unit_kernel<MODE, NODE, 256, 3>), grid, dim3(kThreads), 0, s,
-                       G.items, G.units, G.ctr);
+    launch(rh_unit_kernel<MODE, NODE, 256, 3>);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (t1) {
