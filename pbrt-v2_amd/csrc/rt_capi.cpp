@@ -130,8 +130,10 @@ struct rt_scene {
   // scratch of the naive path, grown on demand
   uint64_t* d_keys = nullptr;
   size_t keys_cap = 0;
-  // scratch of the rayhierarchy and ray-bvh paths (a launch group of one job)
-  RhWorkspace ws;
+  // scratch of the rayhierarchy and ray-bvh paths (a launch group of one job), one per trace
+  // mode, so that a renderer alternating camera, shadow and bounce calls on the same buffers
+  // finds its job table already on the device
+  RhWorkspace ws[3];
   Timer timer;
   std::mutex mu;
 };
@@ -767,7 +769,7 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
     RhParams prm;
     prm.per[0] = kind == RT_ACCEL_RAYBVH ? s->roots_per_unit : s->clusters_closest;
     prm.per[1] = kind == RT_ACCEL_RAYBVH ? s->roots_per_unit : s->clusters_occluded;
-    st = rh_prepare(s->ws, jobs, prm, hs, groups, &changed);
+    st = rh_prepare(s->ws[mode], jobs, prm, hs, groups, &changed);
     if (st) return st;
     Timer::Slot* t = s->timer.next(mode);
     RT_TRY(launch_rh_group(groups[0], hs, t ? t->a : nullptr, t ? t->b : nullptr));
