@@ -7,12 +7,21 @@
 //   keys       one uint64 per ray: (order-preserving t bits << 32) | triangle id
 //   bvh        2 float4 per node {ax ay az bx}{by bz primOffset nPrimitives}
 //
-// Work decomposition (one 256-thread work-group per top-level hierarchy node, "block"):
-// the block's rays live in registers, triangles stream through LDS in tiles of 256, each
-// thread evaluates the reference's cull chain for one triangle against the block's subtree,
-// surviving (triangle, leaf-mask) pairs are compacted with a wavefront ballot into an LDS
-// list, and every ray then runs the leaf test against the list.  The nearest hit is reduced
-// in registers as a (t, id) key, so the result is deterministic (ties -> smallest id).
+// Work decomposition of a rayhierarchy / ray-bvh trace (a launch group of one or more jobs):
+//   init      clears the per-ray keys / occlusion flags and the group counters;
+//   build     one 256-thread work-group per top-level hierarchy node ("block") builds the
+//             block's leaves and levels from its rays;
+//   cluster   one work-group per block bounds every cluster of 64 triangles (or of 64 ray-bvh
+//             frontier roots) against the block root and queues work units of a few passing
+//             clusters;
+//   unit      a persistent grid (one work-group per resident slot) takes units from the queue
+//             with an atomic counter, stages the unit's block of rays and subtree in LDS and
+//             traverses (phases below); per-ray results are reduced in LDS and merged with
+//             global atomics;
+//   finalize  turns each ray's (t, id) key into id / t / u / v (+ dpdu, dpdv);
+//   repair    the reference's repairRun pass for the rays finalize flagged.
+// The nearest hit is reduced as a (t, id) key with atomic min, so the result is
+// deterministic (ties -> smallest id).
 #include <algorithm>
 #include <atomic>
 #include <type_traits>
@@ -297,22 +306,24 @@ __device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J, P no
 
 // ---------------------------------------------------------------------------------------
 // Hierarchy x triangle traversal (intersection6DB.cl, intersection6DB2.cl, intersectionP6DB.cl,
-// intersectionIA.cl), breadth first with compaction at every level.  grid.x = blocks,
-// grid.y = cluster chunks.  Triangles are stored Morton-sorted in clusters of 64 with AABBs.
+// intersectionIA.cl), breadth first with compaction at every level, one work unit (a block and
+// a few of its passing clusters) at a time.  Triangles are stored Morton-sorted in clusters of
+// 64 with AABBs.
 //
 // The result of a ray depends only on triangles that pass the whole cull chain (the
 // reference's candidate set) AND pass the leaf test.  The traversal therefore descends with
 // cheap exact-conservative bounds of the culls (never false where the cull is true) and
 // evaluates the exact cull chain only for (triangle, leaf) pairs in which some ray passes the
 // leaf test:
-//   phase A  each thread tests one cluster against the block root (cluster bound); the
-//            triangles of passing clusters get the per-triangle bound; survivors queue in LDS
-//            (ballot compaction);
-//   phase B  queued triangles are expanded 64 at a time into 256 (triangle, child) items,
-//            bounded in parallel and compacted again, level by level down to the leaves;
+//   phase A  each thread takes one triangle of the unit's clusters through the per-triangle
+//            bound against the block root; survivors queue in LDS (ballot compaction);
+//   phase B  queued triangles are expanded kExpand at a time into 4 (triangle, child) items
+//            each, bounded in parallel and compacted again, level by level down to the leaves
+//            (depth first over rounds, so each level needs one queue);
 //   phase C  (triangle, leaf) pairs are expanded into (pair, ray) leaf tests over the rays
-//            staged in LDS; pairs with a passing test get the exact cull chain; the tests of
-//            verified pairs update the ray (LDS atomic min of the (t, id) key / occlusion flag).
+//            staged in LDS; the pairs with a passing test are compacted and get the exact cull
+//            chain; the tests of verified pairs update the ray (LDS atomic min of the (t, id)
+//            key / occlusion bit).
 constexpr int kCluster = 64;     // triangles per cluster
 constexpr uint32_t kNoUnit = 0xffffffffu;  // no unit left in the queue
 constexpr int kQ0Cap = 1024;     // root survivors held before expansion
