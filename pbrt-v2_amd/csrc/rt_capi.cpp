@@ -210,7 +210,7 @@ RaysDev to_dev(const rt_rays* r) {
 
 // Triangle chunks per block so that a launch has >= ~1024 work-groups (4 per CU).
 int pick_tchunks(int nblocks, uint32_t nwork, uint32_t per_tile) {
-  int want = (1024 + nblocks - 1) / std::max(nblocks, 1);
+  int want = (2048 + nblocks - 1) / std::max(nblocks, 1);  // ~8 work-groups per CU
   int maxc = (int)std::max<uint32_t>(1, nwork / std::max<uint32_t>(per_tile, 1));
   return std::max(1, std::min(want, maxc));
 }
@@ -785,7 +785,7 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
     else RT_TRY(launch_fill_u8(occ, 0, n, hs));
   } else if (kind == RT_ACCEL_NAIVE) {
     const int nblk = (int)((n + 255) / 256);
-    const int tchunks = pick_tchunks(nblk, ntri, 256);
+    const int tchunks = pick_tchunks(nblk, ntri, 64);
     if (tchunks > 1) {
       if (closest) RT_TRY(launch_fill_u64(s->d_keys, ~0ull, n, hs));
       else RT_TRY(launch_fill_u8(occ, 0, n, hs));
