@@ -25,6 +25,8 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (with packed FP32 instructions)
+FP32_UNPACKED_TFLOPS = 78.6   # without them; this build compiles packed FP32 off
 
 
 def _load(name, path, search=None):
@@ -515,6 +517,15 @@ def main():
                     "traffic": profiled_traffic(args.workload, name), "bytes_per_launch": int(b),
                     "counts": {"cull_tests": int((os_ if name[0] == "o" else cs).cull_tests),
                                "leaf_pairs": int((os_ if name[0] == "o" else cs).leaf_pairs)}}
+        # Arithmetic side: ~45 FLOP per ray-triangle leaf test over the oracle's leaf tests
+        # (valid rays of every reached (leaf, triangle) pair; occlusion stops testing a ray
+        # once the oracle finds it occluded).  The cull evaluations are not counted.
+        ops = 45 * int((os_ if name[0] == "o" else cs).ray_tests)
+        tflops = ops / (t * 1e-3) / 1e12
+        roofline["valu"] = {"flops_per_launch": ops, "achieved": round(tflops, 3),
+                            "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
+                            "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                            "peak_unpacked": FP32_UNPACKED_TFLOPS}
         # CPU baseline: pbrt's BVHAccel on the same primary + shadow rays, all host threads
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         bvh = orc.CpuBvh(v9, max_prims=4)
