@@ -3,7 +3,8 @@
 The batched form must give, bit for bit, the results of running every job alone: the same
 hit ids, t, u, v and occlusion flags.  Covered: scenes of different sizes and node types in
 one batch, closest primary / secondary and occlusion jobs mixed, jobs of the 256-ray and the
-1024-ray block kernels in one group, HIP-graph replay (several replays with eager work in
+1024-ray block kernels in one group, launch shapes (grid from one work-group to more than
+the resident slots, several unit sizes), HIP-graph replay (several replays with eager work in
 between, and re-capture after the jobs change), and the autotuner (which must not change any
 result).
 """
@@ -101,6 +102,46 @@ def test_batch_equals_plain_and_oracle(rt, scenes, orc, gpu_ctx):
     b.close()
     for s in sc.values():
         s.close()
+
+
+def test_batch_launch_shapes_same_results(rt, scenes, gpu_ctx):
+    """The persistent grid and the unit size change no result: the default (one work-group per
+    resident slot), a single work-group taking every unit of the group, small odd grids and
+    more work-groups than slots, each with several unit sizes."""
+    sc, jobs = _jobs(rt, scenes, gpu_ctx)
+    b = rt.Batch(gpu_ctx)
+    outs, occs, dev = [], [], []
+    for name, rays, mode in jobs:
+        d = rt.DeviceRays.from_numpy(rays)
+        dev.append(d)
+        outs.append(b.add_closest(sc[name], d, mode))
+    b.run()
+    ref = [_np(o) for o in outs]
+    shadows = [rt.DeviceRays.from_numpy(_shadow_of(scenes, rays, ref[i]))
+               for i, (_, rays, _) in enumerate(jobs)]
+    for i, (name, _, _) in enumerate(jobs):
+        occs.append(b.add_occluded(sc[name], shadows[i]))
+    b.run()
+    ref_occ = [o.cpu().numpy() for o in occs]
+    for grid in (1, 3, 257, 4096, 0):
+        for per in (1, 5, 0):
+            for p_ in (rt.RT_PARAM_GRID_CLOSEST, rt.RT_PARAM_GRID_OCCLUDED):
+                b.set_param(p_, grid)
+            for p_ in (rt.RT_PARAM_CLUSTERS_CLOSEST, rt.RT_PARAM_CLUSTERS_OCCLUDED):
+                b.set_param(p_, per)
+            print(f"[batch] grid {grid} clusters/unit {per}", flush=True)
+            for o in outs:
+                o.id.fill_(-7)
+            for o in occs:
+                o.fill_(7)
+            b.run()
+            for i in range(len(jobs)):
+                _same(_np(outs[i]), ref[i], f"grid {grid} per {per} job {i}")
+                assert np.array_equal(occs[i].cpu().numpy(), ref_occ[i]), \
+                    f"grid {grid} per {per} job {i} occlusion"
+    b.close()
+    for s_ in sc.values():
+        s_.close()
 
 
 def test_batch_graph_replay_and_autotune(rt, scenes, gpu_ctx):
