@@ -163,21 +163,31 @@ int rt_make_bounce_rays(rt_scene *scene, const rt_rays *in, const int32_t *id, c
  *   kernel (batches only); the built-in choice (0) is one work-group per resident slot of the
  *   kernel on the device (CUs x work-groups per CU).
  *   RT_PARAM_GRAPH: 1 = replay the batch's launch sequence as a HIP graph while its jobs are
- *   unchanged (batches only; needs a non-null stream). */
+ *   unchanged (batches only; needs a non-null stream).
+ *   RT_PARAM_RAY_CLUSTERS_CLOSEST / RT_PARAM_RAY_CLUSTERS_OCCLUDED: a block (top-level
+ *   hierarchy node) whose root passes at least this many triangle clusters (ray-bvh: clusters
+ *   of frontier roots) is traced ray by ray -- each ray walks a conservative BVH of the
+ *   triangles and the reference's exact cull chain is evaluated for the triangles that pass
+ *   its strict leaf test -- in place of enumerating the block's (leaf, triangle) candidate
+ *   pairs.  1 sends every block that way; a value above the scene's cluster count (at most
+ *   2^30) never does. */
 enum {
   RT_PARAM_CLUSTERS_CLOSEST = 1,
   RT_PARAM_CLUSTERS_OCCLUDED = 2,
   RT_PARAM_GRID_CLOSEST = 3,
   RT_PARAM_GRID_OCCLUDED = 4,
-  RT_PARAM_GRAPH = 5
+  RT_PARAM_GRAPH = 5,
+  RT_PARAM_RAY_CLUSTERS_CLOSEST = 6,
+  RT_PARAM_RAY_CLUSTERS_OCCLUDED = 7
 };
 int rt_scene_set_param(rt_scene *scene, int32_t param, int32_t value);
 
 /* HIP-event timing of the traversal kernels.  rt_scene_set_timing(scene, 1) starts recording
- * one event pair around every traversal launch (and clears earlier records);
- * rt_scene_kernel_time sums the recorded launches of one mode (RT_CLOSEST_PRIMARY,
- * RT_CLOSEST_SECONDARY, or 2 for occlusion), waiting for them to finish.  This is the
- * counterpart of the reference's GPU_TIMES accounting (GPUparallel.h:18-27,528-539). */
+ * events around every traversal launch (and clears earlier records); rt_scene_kernel_time
+ * sums the recorded launches of one mode (RT_CLOSEST_PRIMARY, RT_CLOSEST_SECONDARY, or 2 for
+ * occlusion), waiting for them to finish.  mode + 3 gives only the per-ray kernel of those
+ * launches, mode + 6 only the unit (block) kernel.  This is the counterpart of the reference's
+ * GPU_TIMES accounting (GPUparallel.h:18-27,528-539). */
 int rt_scene_set_timing(rt_scene *scene, int32_t enable);
 int rt_scene_kernel_time(rt_scene *scene, int32_t mode, float *total_ms, int32_t *launches);
 

@@ -93,32 +93,42 @@ def params(kind="rayhierarchy", node="box_dir", cx=4, cy=4, height=3, repair_run
     return OrcParams(k, nd, cx, cy, height, repair_run)
 
 
-def trace(v9, rays: dict, p: OrcParams, mode: int):
-    """Returns dict(id, t, b1, b2) for closest modes or dict(occ) for OCCLUDED, plus stats."""
+NEAR_TIE, ULP_MARGINAL = 1, 2  # bits of the per-ray flags
+
+
+def trace(v9, rays: dict, p: OrcParams, mode: int, flags: bool = False):
+    """Returns dict(id, t, b1, b2) for closest modes or dict(occ) for OCCLUDED, plus stats;
+    with flags=True also "flags", one byte per ray (NEAR_TIE | ULP_MARGINAL bits)."""
     v9 = _f32(v9).reshape(-1)
     ntri = v9.size // 9
     keep = _RayKeep(rays)
     n = keep.n
     st = OrcStats()
+    fl = np.zeros(n, np.uint8) if flags else None
+    flp = _fp(fl.ctypes.data) if flags else None
     if mode == OCCLUDED:
         occ = np.zeros(n, np.uint8)
         rc = lib().orc_trace(_fp(v9.ctypes.data), ctypes.c_uint32(ntri), ctypes.byref(keep.s),
                              ctypes.byref(p), ctypes.c_int32(mode), None, None, None, None,
-                             _fp(occ.ctypes.data), ctypes.byref(st))
+                             _fp(occ.ctypes.data), ctypes.byref(st), flp)
         assert rc == 0
-        return dict(occ=occ, stats=st)
-    out = dict(id=np.zeros(n, np.int32), t=np.zeros(n, np.float32), b1=np.zeros(n, np.float32),
-               b2=np.zeros(n, np.float32))
-    rc = lib().orc_trace(_fp(v9.ctypes.data), ctypes.c_uint32(ntri), ctypes.byref(keep.s),
-                         ctypes.byref(p), ctypes.c_int32(mode), _fp(out["id"].ctypes.data),
-                         _fp(out["t"].ctypes.data), _fp(out["b1"].ctypes.data),
-                         _fp(out["b2"].ctypes.data), None, ctypes.byref(st))
-    assert rc == 0
-    out["stats"] = st
+        out = dict(occ=occ, stats=st)
+    else:
+        out = dict(id=np.zeros(n, np.int32), t=np.zeros(n, np.float32),
+                   b1=np.zeros(n, np.float32), b2=np.zeros(n, np.float32))
+        rc = lib().orc_trace(_fp(v9.ctypes.data), ctypes.c_uint32(ntri), ctypes.byref(keep.s),
+                             ctypes.byref(p), ctypes.c_int32(mode), _fp(out["id"].ctypes.data),
+                             _fp(out["t"].ctypes.data), _fp(out["b1"].ctypes.data),
+                             _fp(out["b2"].ctypes.data), None, ctypes.byref(st), flp)
+        assert rc == 0
+        out["stats"] = st
+    if flags:
+        out["flags"] = fl
     return out
 
 
-def trace_raybvh(v9, nodes, roots, prim_order, rays: dict, p: OrcParams, mode: int):
+def trace_raybvh(v9, nodes, roots, prim_order, rays: dict, p: OrcParams, mode: int,
+                 flags: bool = False):
     v9 = _f32(v9).reshape(-1)
     ntri = v9.size // 9
     nodes = np.ascontiguousarray(nodes, dtype=GPU_NODE_DTYPE)
@@ -130,19 +140,24 @@ def trace_raybvh(v9, nodes, roots, prim_order, rays: dict, p: OrcParams, mode: i
     args = [_fp(v9.ctypes.data), ctypes.c_uint32(ntri), _fp(nodes.ctypes.data),
             ctypes.c_uint32(nodes.size), _fp(roots.ctypes.data), ctypes.c_uint32(roots.size),
             _fp(prim_order.ctypes.data), ctypes.byref(keep.s), ctypes.byref(p), ctypes.c_int32(mode)]
+    fl = np.zeros(n, np.uint8) if flags else None
+    flp = _fp(fl.ctypes.data) if flags else None
     if mode == OCCLUDED:
         occ = np.zeros(n, np.uint8)
         rc = lib().orc_trace_raybvh(*args, None, None, None, None, _fp(occ.ctypes.data),
-                                    ctypes.byref(st))
+                                    ctypes.byref(st), flp)
         assert rc == 0
-        return dict(occ=occ, stats=st)
-    out = dict(id=np.zeros(n, np.int32), t=np.zeros(n, np.float32), b1=np.zeros(n, np.float32),
-               b2=np.zeros(n, np.float32))
-    rc = lib().orc_trace_raybvh(*args, _fp(out["id"].ctypes.data), _fp(out["t"].ctypes.data),
-                                _fp(out["b1"].ctypes.data), _fp(out["b2"].ctypes.data), None,
-                                ctypes.byref(st))
-    assert rc == 0
-    out["stats"] = st
+        out = dict(occ=occ, stats=st)
+    else:
+        out = dict(id=np.zeros(n, np.int32), t=np.zeros(n, np.float32),
+                   b1=np.zeros(n, np.float32), b2=np.zeros(n, np.float32))
+        rc = lib().orc_trace_raybvh(*args, _fp(out["id"].ctypes.data), _fp(out["t"].ctypes.data),
+                                    _fp(out["b1"].ctypes.data), _fp(out["b2"].ctypes.data), None,
+                                    ctypes.byref(st), flp)
+        assert rc == 0
+        out["stats"] = st
+    if flags:
+        out["flags"] = fl
     return out
 
 

@@ -63,6 +63,32 @@ inline V3 cross_d(V3 a, V3 b) {
           (float)((ax * by) - (ay * bx))};
 }
 
+// ---- decision margins (parity bookkeeping) ---------------------------------------------
+// A decision is ulp-marginal when its operand lies within 4 ulp of the threshold it is
+// compared with.  For a comparison with 0 of a sum of products (det, the barycentric
+// numerators) the ulp is that of the largest product; for normalised direction components
+// it is that of 1.  Tracking is on only while a trace asks for per-ray flags.
+thread_local bool g_track = false;
+thread_local bool g_marg = false;
+inline float ulp_of(float x) {
+  x = std::fabs(x);
+  if (!(x >= std::numeric_limits<float>::min())) return std::numeric_limits<float>::denorm_min();
+  int e;
+  (void)std::frexp(x, &e);  // x = m * 2^e, m in [0.5, 1)
+  return std::ldexp(1.0f, e - 24);
+}
+inline bool near_thr(float a, float b) {
+  if (!std::isfinite(a) || !std::isfinite(b)) return false;
+  return std::fabs(a - b) <= 4.f * ulp_of(std::fmax(std::fabs(a), std::fabs(b)));
+}
+inline bool near_zero(float x, float scale) { return std::fabs(x) <= 4.f * ulp_of(scale); }
+inline float dot_scale(V3 a, V3 b) {
+  return std::fmax(std::fmax(std::fabs(a.x * b.x), std::fabs(a.y * b.y)), std::fabs(a.z * b.z));
+}
+inline void mark(bool m) {
+  if (g_track && m) g_marg = true;
+}
+
 struct Tri {
   V3 v1, v2, v3, e1, e2, bmin, bmax;
 };
@@ -93,6 +119,46 @@ inline bool mt_strict(V3 o, V3 d, const Tri& T, float& t, float& b1, float& b2) 
   b2 = dot(d, s2) * inv;
   if (!(b2 > 0.f && (b1 + b2) < 1.f)) return false;
   t = dot(T.e2, s2) * inv;
+  return true;
+}
+// mt_strict, and in `marg` whether its outcome rests on a marginal decision: an accepted test
+// with any marginal comparison, or a rejection made by a marginal comparison.  Unless det is
+// 0, t is computed as the accepted test would, also for a rejection (so that a marginal
+// rejection can be weighed against the ray's result); `t_known` tells.
+inline bool mt_strict_m(V3 o, V3 d, const Tri& T, float& t, float& b1, float& b2, bool& marg,
+                        bool& t_known) {
+  bool m = false;
+  t_known = false;
+  V3 s = cross(d, T.e2);
+  float det = dot(s, T.e1);
+  const bool mdet = near_zero(det, dot_scale(s, T.e1));
+  if (det == 0.0f) {
+    marg = mdet;
+    return false;
+  }
+  m |= mdet;
+  float inv = 1.0f / det;
+  V3 ov = sub(o, T.v1);
+  const float n1 = dot(ov, s);
+  b1 = n1 * inv;
+  V3 s2 = cross(ov, T.e1);
+  const float n2 = dot(d, s2);
+  b2 = n2 * inv;
+  t = dot(T.e2, s2) * inv;
+  t_known = true;
+  const bool m1 = near_zero(n1, dot_scale(ov, s)) || near_thr(b1, 1.f);
+  if (!(b1 > 0.f && b1 < 1.f)) {
+    marg = m1;
+    return false;
+  }
+  m |= m1;
+  const bool m2 = near_zero(n2, dot_scale(d, s2)) || near_thr(b1 + b2, 1.f);
+  if (!(b2 > 0.f && (b1 + b2) < 1.f)) {
+    marg = m2;
+    return false;
+  }
+  m |= m2;
+  marg = m;
   return true;
 }
 // naive.cl:30-46 (TriRay closest: |det| >= EPS, loose barycentric bounds)
@@ -172,6 +238,12 @@ inline bool box_logic(V3 tmin, V3 tmax, const Node& n) {
   // intersectsNode6DB.cl:53-63 / 88-98; the uninitialised `ret` is taken as false.
   V3 f0 = vmax(tmin, n.dmin);
   V3 f1 = vmin(tmax, n.dmax);
+  if (g_track) {
+    mark(near_zero(tmin.x, 1.f) || near_zero(tmax.x, 1.f) || near_zero(tmin.y, 1.f) ||
+         near_zero(tmax.y, 1.f) || near_zero(tmin.z, 1.f) || near_zero(tmax.z, 1.f));
+    mark(near_thr(f0.x, f1.x + kEps) || near_thr(f0.y, f1.y + kEps) ||
+         near_thr(f0.z, f1.z + kEps));
+  }
   bool ret = false;
   if (tmin.x < 0.f && tmax.x > 0.f) ret = (f0.x < f1.x + kEps);
   if (tmin.y < 0.f && tmax.y > 0.f) ret |= (f0.y < f1.y + kEps);
@@ -222,6 +294,9 @@ inline void ia_axis(float lo, float hi, float dlo, float dhi, float& smin, float
 inline bool cull_ia(const Node& n, V3 bmin, V3 bmax) {
   V3 lo = sub(bmin, n.omax);
   V3 hi = sub(bmax, n.omin);
+  if (g_track)
+    mark(near_zero(n.dmin.x, 1.f) || near_zero(n.dmax.x, 1.f) || near_zero(n.dmin.y, 1.f) ||
+         near_zero(n.dmax.y, 1.f) || near_zero(n.dmin.z, 1.f) || near_zero(n.dmax.z, 1.f));
   if ((n.dmin.x <= 0.f && n.dmax.x >= 0.f) || (n.dmin.y <= 0.f && n.dmax.y >= 0.f) ||
       (n.dmin.z <= 0.f && n.dmax.z >= 0.f))
     return true;
@@ -234,6 +309,7 @@ inline bool cull_ia(const Node& n, V3 bmin, V3 bmax) {
   ia_axis(lo.z, hi.z, iz0, iz1, ux, uy);
   sx = fmx(fmx(sx, tx), ux);
   sy = fmn(fmn(sy, ty), uy);
+  if (g_track) mark(near_thr(sx, sy + kEps));
   return sx < sy + kEps;
 }
 
@@ -355,6 +431,23 @@ struct Out {
   uint8_t* occ;
 };
 
+// Per-ray parity bookkeeping, kept while a trace asks for flags:
+//   t1 / t2   the two smallest t of accepted candidates (closest modes), for near ties;
+//   tmarg     the smallest t of a candidate whose outcome rests on a marginal decision (a
+//             marginal leaf-test comparison, or a marginal cull on its chain); -INF when that
+//             t is unknown or (occlusion) the candidate lies in the ray's interval;
+//   robust    occlusion found by a candidate with no marginal decision.
+struct Margins {
+  std::vector<float> t1, t2, tmarg;
+  std::vector<uint8_t> robust;
+  void init(size_t n) {
+    t1.assign(n, kInf);
+    t2.assign(n, kInf);
+    tmarg.assign(n, kInf);
+    robust.assign(n, 0);
+  }
+};
+
 // Leaf-level work shared by the RH and ray-bvh traversals.
 struct LeafCtx {
   const orc_rays* r;
@@ -362,33 +455,74 @@ struct LeafCtx {
   int mode;
   Out out;
   orc_stats* st;
+  Margins* mg;  // null unless flags are asked for
 };
 
-inline void leaf_test(LeafCtx& c, int k, int j, const Tri& T, int32_t tid) {
+// Leaf tests of triangle T for the rays of leaf (k, j).  `shadow`: the pair is not a candidate
+// (a cull on its chain rejected it) but that cull was marginal, so the tests only record
+// marginal candidates; `chain_marg`: some cull on the chain was marginal.
+inline void leaf_test(LeafCtx& c, int k, int j, const Tri& T, int32_t tid, bool shadow = false,
+                      bool chain_marg = false) {
   const orc_rays* r = c.r;
-  c.st->leaf_pairs++;
+  Margins* mg = c.mg;
+  if (!shadow) c.st->leaf_pairs++;
   int x0 = k * c.h->cx, y0 = j * c.h->cy;
   int x1 = std::min(r->W, x0 + c.h->cx), y1 = std::min(r->H, y0 + c.h->cy);
   for (int y = y0; y < y1; ++y)
     for (int x = x0; x < x1; ++x) {
       size_t i = (size_t)y * r->W + x;
       if (c.mode != ORC_CLOSEST_PRIMARY && !ray_valid(r, i)) continue;
-      if (c.mode == ORC_OCCLUDED && c.out.occ[i]) continue;
-      c.st->ray_tests++;
+      if (c.mode == ORC_OCCLUDED && c.out.occ[i] && (!mg || mg->robust[i])) continue;
+      if (!shadow) c.st->ray_tests++;
       float t, b1, b2;
-      if (!mt_strict(ray_o(r, i), ray_d(r, i), T, t, b1, b2)) continue;
+      bool pm = false, t_known = true;
+      const bool ok = mg ? mt_strict_m(ray_o(r, i), ray_d(r, i), T, t, b1, b2, pm, t_known)
+                         : mt_strict(ray_o(r, i), ray_d(r, i), T, t, b1, b2);
+      if (!ok) {
+        if (!pm) continue;
+        // a marginal rejection counts where the accepted test would have counted
+        if (!t_known) {
+          mg->tmarg[i] = -kInf;
+        } else if (c.mode == ORC_OCCLUDED) {
+          if (!(t <= r->tmin[i]) && !(t >= r->tmax[i])) mg->tmarg[i] = -kInf;
+        } else if (t > r->tmin[i]) {
+          mg->tmarg[i] = std::fmin(mg->tmarg[i], t);
+        }
+        continue;
+      }
+      bool m = pm || chain_marg || shadow;
       if (c.mode == ORC_OCCLUDED) {
         // intersectAllLeaves.cl:177-179
-        if (t > r->tmin[i] && t < r->tmax[i]) c.out.occ[i] = 1;
+        const bool in = t > r->tmin[i] && t < r->tmax[i];
+        if (mg) m |= near_thr(t, r->tmin[i]) || near_thr(t, r->tmax[i]);
+        if (in && !shadow) {
+          c.out.occ[i] = 1;
+          if (mg && !m) mg->robust[i] = 1;
+        }
+        if (mg && m && (in || near_thr(t, r->tmin[i]) || near_thr(t, r->tmax[i])))
+          mg->tmarg[i] = -kInf;
       } else {
         // intersectAllLeaves.cl:57-61 (triangles visited in increasing id order, so the
         // strict compare keeps the smallest id among equal t)
-        if (t > r->tmin[i] && t < c.out.t[i]) {
-          c.out.t[i] = t;
-          c.out.id[i] = tid;
-          c.out.b1[i] = b1;
-          c.out.b2[i] = b2;
+        const bool main = t > r->tmin[i];
+        if (mg) m |= near_thr(t, r->tmin[i]);
+        if (main && !shadow) {
+          if (mg) {
+            if (t < mg->t1[i]) {
+              mg->t2[i] = mg->t1[i];
+              mg->t1[i] = t;
+            } else if (t < mg->t2[i]) {
+              mg->t2[i] = t;
+            }
+          }
+          if (t < c.out.t[i]) {
+            c.out.t[i] = t;
+            c.out.id[i] = tid;
+            c.out.b1[i] = b1;
+            c.out.b2[i] = b2;
+          }
         }
+        if (mg && m && (main || near_thr(t, r->tmin[i]))) mg->tmarg[i] = std::fmin(mg->tmarg[i], t);
       }
     }
 }
@@ -401,20 +535,31 @@ inline bool rh_cull(const Node& n, const Tri& T, int mode, int node_kind, orc_st
 }
 
 // DFS of intersection6DB.cl:51-119 (and the IA / secondary / shadow variants) for one triangle.
+// While flags are tracked, a cull that rejects on a marginal decision is descended anyway in
+// shadow mode (no counts, no results; its leaf tests only record marginal candidates).
 template <class F>
 void rh_visit(const RH& h, int l, int k, int j, const Tri& T, int mode, int node_kind,
-              bool use_valid, orc_stats* st, F&& at_leaf) {
+              bool use_valid, orc_stats* st, F&& at_leaf, bool shadow = false,
+              bool chain_marg = false) {
   const Node& n = h.at(l, k, j);
   if (use_valid && !n.valid) return;
-  if (!rh_cull(n, T, mode, node_kind, st)) return;
+  orc_stats scratch{0, 0, 0, 0};
+  g_marg = false;
+  const bool pass = rh_cull(n, T, mode, node_kind, shadow ? &scratch : st);
+  const bool m = g_track && g_marg;
+  if (!pass) {
+    if (!m) return;
+    shadow = true;
+  }
+  chain_marg |= m;
   if (l == 0) {
-    at_leaf(k, j);
+    at_leaf(k, j, shadow, chain_marg);
     return;
   }
   for (int cj = 2 * j; cj <= 2 * j + 1; ++cj)
     for (int ck = 2 * k; ck <= 2 * k + 1; ++ck)
       if (ck < h.gw[l - 1] && cj < h.gh[l - 1])
-        rh_visit(h, l - 1, ck, cj, T, mode, node_kind, use_valid, st, at_leaf);
+        rh_visit(h, l - 1, ck, cj, T, mode, node_kind, use_valid, st, at_leaf, shadow, chain_marg);
 }
 
 void init_out(const orc_rays* r, int mode, Out& o) {
@@ -465,17 +610,19 @@ void trace_naive(const float* v9, uint32_t ntri, const orc_rays* r, int mode, Ou
 }
 
 void trace_rh(const float* v9, uint32_t ntri, const orc_rays* r, const orc_params* p, int mode,
-              Out& o, orc_stats* st) {
+              Out& o, orc_stats* st, Margins* mg) {
   bool use_valid = mode != ORC_CLOSEST_PRIMARY;
   RH h = rh_build(r, p->cx, p->cy, p->height, use_valid);
-  LeafCtx c{r, &h, mode, o, st};
+  LeafCtx c{r, &h, mode, o, st, mg};
   int top = h.L - 1;
   for (uint32_t tid = 0; tid < ntri; ++tid) {
     Tri T = load_tri(v9, tid);
     for (int j = 0; j < h.gh[top]; ++j)
       for (int k = 0; k < h.gw[top]; ++k)
         rh_visit(h, top, k, j, T, mode, p->node, use_valid, st,
-                 [&](int lk, int lj) { leaf_test(c, lk, lj, T, (int32_t)tid); });
+                 [&](int lk, int lj, bool shadow, bool cm) {
+                   leaf_test(c, lk, lj, T, (int32_t)tid, shadow, cm);
+                 });
   }
   if (mode != ORC_CLOSEST_PRIMARY || p->repair_run <= 0) return;
   // Repair pass (yetAnotherIntersection6DB.cl, intersectAllLeaves.cl:188-238), repairRun
@@ -483,12 +630,16 @@ void trace_rh(const float* v9, uint32_t ntri, const orc_rays* r, const orc_param
   size_t n = (size_t)r->W * r->H;
   std::vector<int32_t> id0(o.id, o.id + n);
   orc_stats dummy{0, 0, 0, 0};
+  const bool track = g_track;
+  g_track = false;  // the flags describe the main pass
   for (int run = 0; run < p->repair_run; ++run)
     for (uint32_t tid = 0; tid < ntri; ++tid) {
       Tri T = load_tri(v9, tid);
       for (int j = 0; j < h.gh[top]; ++j)
         for (int k = 0; k < h.gw[top]; ++k)
-          rh_visit(h, top, k, j, T, mode, p->node, false, &dummy, [&](int lk, int lj) {
+          rh_visit(h, top, k, j, T, mode, p->node, false, &dummy, [&](int lk, int lj, bool shadow,
+                                                                      bool) {
+            if (shadow) return;
             int x0 = lk * h.cx, y0 = lj * h.cy;
             int x1 = std::min(r->W, x0 + h.cx), y1 = std::min(r->H, y0 + h.cy);
             for (int y = y0; y < y1; ++y)
@@ -505,6 +656,7 @@ void trace_rh(const float* v9, uint32_t ntri, const orc_rays* r, const orc_param
               }
           });
     }
+  g_track = track;
   for (size_t i = 0; i < n; ++i)
     if (o.id[i] != id0[i]) st->repair_rays++;
 }
@@ -523,34 +675,46 @@ struct BvhCtx {
   orc_stats* st;
 };
 
-void bvh_visit(BvhCtx& c, int l, int k, int j, uint32_t ni) {
+// One step of the lockstep product traversal: the cull of (hierarchy node (l, k, j), BVH node
+// ni).  Returns false when the pair stops the walk; `shadow` / `chain_marg` as in rh_visit.
+inline bool bvh_step(BvhCtx& c, int l, int k, int j, uint32_t ni, bool& shadow, bool& chain_marg) {
   const Node& n = c.h->at(l, k, j);
-  if (c.use_valid && !n.valid) return;
+  if (c.use_valid && !n.valid) return false;
   const orc_gpu_node& b = c.nodes[ni];
   V3 bmin = {b.ax, b.ay, b.az}, bmax = {b.bx, b.by, b.bz};
-  c.st->cull_tests++;
+  if (!shadow) c.st->cull_tests++;
+  g_marg = false;
   bool pass = c.node_kind == 1 ? cull_ia(n, bmin, bmax) : cull_node6(n, bmin, bmax);
-  if (!pass) return;
+  const bool m = g_track && g_marg;
+  if (!pass) {
+    if (!m) return false;
+    shadow = true;
+  }
+  chain_marg |= m;
+  return true;
+}
+
+// Calls emit(k, j, bvh leaf, shadow, chain_marg) for every (hierarchy leaf, BVH leaf) pair the
+// lockstep walk from (l, k, j, ni) reaches.
+template <class F>
+void bvh_walk(BvhCtx& c, int l, int k, int j, uint32_t ni, bool shadow, bool chain_marg, F&& emit) {
+  if (!bvh_step(c, l, k, j, ni, shadow, chain_marg)) return;
+  const orc_gpu_node& b = c.nodes[ni];
   bool rleaf = l == 0, bleaf = b.nPrimitives > 0;
   if (rleaf && bleaf) {
-    uint32_t cnt = c.mode == ORC_CLOSEST_SECONDARY ? 1u : b.nPrimitives;
-    for (uint32_t f = 0; f < cnt; ++f) {
-      uint32_t orig = c.prim_order[b.primOffset + f];
-      Tri T = load_tri(c.v9, orig);
-      leaf_test(*c.leaf, k, j, T, (int32_t)orig);
-    }
+    emit(k, j, ni, shadow, chain_marg);
   } else if (rleaf) {
-    bvh_visit(c, l, k, j, b.primOffset);
-    bvh_visit(c, l, k, j, b.primOffset + 1);
+    bvh_walk(c, l, k, j, b.primOffset, shadow, chain_marg, emit);
+    bvh_walk(c, l, k, j, b.primOffset + 1, shadow, chain_marg, emit);
   } else {
     for (int cj = 2 * j; cj <= 2 * j + 1; ++cj)
       for (int ck = 2 * k; ck <= 2 * k + 1; ++ck) {
         if (ck >= c.h->gw[l - 1] || cj >= c.h->gh[l - 1]) continue;
         if (bleaf) {
-          bvh_visit(c, l - 1, ck, cj, ni);
+          bvh_walk(c, l - 1, ck, cj, ni, shadow, chain_marg, emit);
         } else {
-          bvh_visit(c, l - 1, ck, cj, b.primOffset);
-          bvh_visit(c, l - 1, ck, cj, b.primOffset + 1);
+          bvh_walk(c, l - 1, ck, cj, b.primOffset, shadow, chain_marg, emit);
+          bvh_walk(c, l - 1, ck, cj, b.primOffset + 1, shadow, chain_marg, emit);
         }
       }
   }
@@ -954,18 +1118,51 @@ int64_t orc_rh_build(const orc_rays* r, int32_t cx, int32_t cy, int32_t height, 
   return (int64_t)h.nodes.size();
 }
 
+// Per-ray flags from the bookkeeping of a finished trace: bit 0 near tie (closest modes: the
+// two nearest candidates within 1e-6 * t), bit 1 ulp-marginal (a candidate whose outcome rests
+// on a marginal decision could change the result: closest modes, one no farther than the
+// result; occlusion, one in the ray's interval while no candidate occludes it robustly).
+void finish_flags(const orc_rays* r, int mode, const Out& o, const Margins& mg, uint8_t* flags) {
+  const size_t n = (size_t)r->W * r->H;
+  for (size_t i = 0; i < n; ++i) {
+    uint8_t f = 0;
+    if (mode == ORC_OCCLUDED) {
+      if (mg.tmarg[i] == -kInf && !mg.robust[i]) f |= 2;
+    } else {
+      if (std::isfinite(mg.t1[i]) && std::isfinite(mg.t2[i]) &&
+          mg.t2[i] - mg.t1[i] <= 1e-6f * mg.t1[i])
+        f |= 1;
+      const float tr = o.t[i];
+      if (mg.tmarg[i] == -kInf || (mg.tmarg[i] < kInf && (!std::isfinite(tr) ||
+                                                          mg.tmarg[i] <= tr * (1.f + 1e-6f))))
+        f |= 2;
+    }
+    flags[i] = f;
+  }
+}
+
 int orc_trace(const float* v9, uint32_t ntri, const orc_rays* r, const orc_params* p, int32_t mode,
-              int32_t* id, float* t, float* b1, float* b2, uint8_t* occ, orc_stats* st) {
+              int32_t* id, float* t, float* b1, float* b2, uint8_t* occ, orc_stats* st,
+              uint8_t* flags) {
   orc_stats local{0, 0, 0, 0};
   if (!st) st = &local;
   Out o{id, t, b1, b2, occ};
   init_out(r, mode, o);
+  Margins mg;
+  if (flags) mg.init((size_t)r->W * r->H);
+  g_track = flags != nullptr;
   if (p->kind == 0) {
     trace_naive(v9, ntri, r, mode, o, st);
   } else if (p->kind == 1) {
-    trace_rh(v9, ntri, r, p, mode, o, st);
+    trace_rh(v9, ntri, r, p, mode, o, st, flags ? &mg : nullptr);
   } else {
+    g_track = false;
     return 1;
+  }
+  g_track = false;
+  if (flags) {
+    if (p->kind == 0) std::memset(flags, 0, (size_t)r->W * r->H);
+    else finish_flags(r, mode, o, mg, flags);
   }
   return 0;
 }
@@ -973,70 +1170,50 @@ int orc_trace(const float* v9, uint32_t ntri, const orc_rays* r, const orc_param
 int orc_trace_raybvh(const float* v9, uint32_t ntri, const orc_gpu_node* nodes, uint32_t nnodes,
                      const uint32_t* roots, uint32_t nroots, const uint32_t* prim_order,
                      const orc_rays* r, const orc_params* p, int32_t mode, int32_t* id, float* t,
-                     float* b1, float* b2, uint8_t* occ, orc_stats* st) {
+                     float* b1, float* b2, uint8_t* occ, orc_stats* st, uint8_t* flags) {
   (void)ntri;
   (void)nnodes;
   orc_stats local{0, 0, 0, 0};
   if (!st) st = &local;
   Out o{id, t, b1, b2, occ};
   init_out(r, mode, o);
+  Margins mg;
+  if (flags) mg.init((size_t)r->W * r->H);
+  g_track = flags != nullptr;
   bool use_valid = mode != ORC_CLOSEST_PRIMARY;
   RH h = rh_build(r, p->cx, p->cy, p->height, use_valid);
-  LeafCtx lc{r, &h, mode, o, st};
+  LeafCtx lc{r, &h, mode, o, st, flags ? &mg : nullptr};
   BvhCtx c{&h, nodes, prim_order, v9, mode, p->node, use_valid, &lc, st};
   int top = h.L - 1;
-  // The candidate set does not depend on visit order; for the closest modes the per-ray
-  // reduction must still keep the smallest original id among equal t, which leaf_test's
-  // strict compare does only for increasing ids, so closest results are re-reduced below.
-  if (mode == ORC_OCCLUDED) {
-    for (uint32_t f = 0; f < nroots; ++f)
-      for (int j = 0; j < h.gh[top]; ++j)
-        for (int k = 0; k < h.gw[top]; ++k) bvh_visit(c, top, k, j, roots[f]);
-    return 0;
-  }
-  // Collect (leaf, original id) candidate pairs, then test them in increasing id order.
+  // The candidate set does not depend on visit order, but for the closest modes the per-ray
+  // reduction must keep the smallest original id among equal t, which leaf_test's strict
+  // compare does only for increasing ids: the reached (leaf, primitive) pairs are collected
+  // and tested in increasing id order (occlusion tests them as they are reached).  Secondary
+  // rays test only the first primitive of a BVH leaf (intersection6DB_BVH2.cl:88).
   struct Pair {
     int32_t tid;
     int k, j;
+    bool shadow, cm;
   };
   std::vector<Pair> pairs;
-  struct Collector {
-    static void visit(BvhCtx& c, std::vector<Pair>& out, int l, int k, int j, uint32_t ni) {
-      const Node& n = c.h->at(l, k, j);
-      if (c.use_valid && !n.valid) return;
-      const orc_gpu_node& b = c.nodes[ni];
-      V3 bmin = {b.ax, b.ay, b.az}, bmax = {b.bx, b.by, b.bz};
-      c.st->cull_tests++;
-      bool pass = c.node_kind == 1 ? cull_ia(n, bmin, bmax) : cull_node6(n, bmin, bmax);
-      if (!pass) return;
-      bool rleaf = l == 0, bleaf = b.nPrimitives > 0;
-      if (rleaf && bleaf) {
-        uint32_t cnt = c.mode == ORC_CLOSEST_SECONDARY ? 1u : b.nPrimitives;
-        for (uint32_t f = 0; f < cnt; ++f)
-          out.push_back({(int32_t)c.prim_order[b.primOffset + f], k, j});
-      } else if (rleaf) {
-        visit(c, out, l, k, j, b.primOffset);
-        visit(c, out, l, k, j, b.primOffset + 1);
-      } else {
-        for (int cj = 2 * j; cj <= 2 * j + 1; ++cj)
-          for (int ck = 2 * k; ck <= 2 * k + 1; ++ck) {
-            if (ck >= c.h->gw[l - 1] || cj >= c.h->gh[l - 1]) continue;
-            if (bleaf) {
-              visit(c, out, l - 1, ck, cj, ni);
-            } else {
-              visit(c, out, l - 1, ck, cj, b.primOffset);
-              visit(c, out, l - 1, ck, cj, b.primOffset + 1);
-            }
-          }
-      }
+  auto emit = [&](int k, int j, uint32_t ni, bool shadow, bool cm) {
+    const orc_gpu_node& b = nodes[ni];
+    uint32_t cnt = mode == ORC_CLOSEST_SECONDARY ? 1u : b.nPrimitives;
+    for (uint32_t f = 0; f < cnt; ++f) {
+      const uint32_t orig = prim_order[b.primOffset + f];
+      if (mode == ORC_OCCLUDED) leaf_test(lc, k, j, load_tri(v9, orig), (int32_t)orig, shadow, cm);
+      else pairs.push_back({(int32_t)orig, k, j, shadow, cm});
     }
   };
   for (uint32_t f = 0; f < nroots; ++f)
     for (int j = 0; j < h.gh[top]; ++j)
-      for (int k = 0; k < h.gw[top]; ++k) Collector::visit(c, pairs, top, k, j, roots[f]);
+      for (int k = 0; k < h.gw[top]; ++k) bvh_walk(c, top, k, j, roots[f], false, false, emit);
   std::stable_sort(pairs.begin(), pairs.end(),
                    [](const Pair& a, const Pair& b) { return a.tid < b.tid; });
-  for (const Pair& pr : pairs) leaf_test(lc, pr.k, pr.j, load_tri(v9, (uint32_t)pr.tid), pr.tid);
+  for (const Pair& pr : pairs)
+    leaf_test(lc, pr.k, pr.j, load_tri(v9, (uint32_t)pr.tid), pr.tid, pr.shadow, pr.cm);
+  g_track = false;
+  if (flags) finish_flags(r, mode, o, mg, flags);
   return 0;
 }
 

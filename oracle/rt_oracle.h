@@ -81,14 +81,19 @@ int64_t orc_rh_build(const orc_rays *r, int32_t cx, int32_t cy, int32_t height, 
                      float *nodes);
 
 /* ---- traces ---- */
-/* closest modes write id/t/b1/b2 (miss: id -1, t INF); occluded writes occ (0/1) */
+/* closest modes write id/t/b1/b2 (miss: id -1, t INF); occluded writes occ (0/1).
+ * flags (optional, one byte per ray): bit 0 near tie -- the two nearest candidates lie
+ * within 1e-6*t; bit 1 ulp-marginal -- a candidate whose leaf test or chain of culls rests on
+ * an operand within 4 ulp of its threshold could change the ray's result (rayhierarchy and
+ * ray-bvh; naive writes 0). */
 int orc_trace(const float *v9, uint32_t ntri, const orc_rays *r, const orc_params *p, int32_t mode,
-              int32_t *id, float *t, float *b1, float *b2, uint8_t *occ, orc_stats *st);
+              int32_t *id, float *t, float *b1, float *b2, uint8_t *occ, orc_stats *st,
+              uint8_t *flags);
 /* ray-bvh: triangles in original order, BVH nodes of all parts concatenated, roots = frontier */
 int orc_trace_raybvh(const float *v9, uint32_t ntri, const orc_gpu_node *nodes, uint32_t nnodes,
                      const uint32_t *roots, uint32_t nroots, const uint32_t *prim_order,
                      const orc_rays *r, const orc_params *p, int32_t mode, int32_t *id, float *t,
-                     float *b1, float *b2, uint8_t *occ, orc_stats *st);
+                     float *b1, float *b2, uint8_t *occ, orc_stats *st, uint8_t *flags);
 
 /* shading record of computeDpTuTv.cl:8-91 for hits (id>=0): uv (2/ray), dpdu, dpdv (3/ray) */
 void orc_shading(const float *v9, const float *uv6, uint32_t ntri, const orc_rays *r,

@@ -40,7 +40,13 @@ EXPORTS = (
     "rt_server_destroy", "rt_make_bounce_rays",
 )
 (RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED, RT_PARAM_GRID_CLOSEST,
- RT_PARAM_GRID_OCCLUDED, RT_PARAM_GRAPH) = 1, 2, 3, 4, 5
+ RT_PARAM_GRID_OCCLUDED, RT_PARAM_GRAPH, RT_PARAM_RAY_CLUSTERS_CLOSEST,
+ RT_PARAM_RAY_CLUSTERS_OCCLUDED) = 1, 2, 3, 4, 5, 6, 7
+# a per-ray threshold above any cluster count: every block enumerates its candidate pairs
+RAY_PATH_OFF = 1 << 30
+# kernel_time modes: + RT_TIME_PER_RAY = only the per-ray kernel, + RT_TIME_UNIT = only the
+# unit (block) kernel of the launches of a mode
+RT_TIME_PER_RAY, RT_TIME_UNIT = 3, 6
 
 
 class RtError(RuntimeError):
@@ -349,7 +355,8 @@ class Scene:
 
     def kernel_time(self, mode: int):
         """(total ms, launches) of the traversal kernels of one mode recorded since
-        set_timing(True); mode 0/1 closest primary/secondary, 2 occlusion."""
+        set_timing(True); mode 0/1 closest primary/secondary, 2 occlusion (+ RT_TIME_PER_RAY:
+        the per-ray kernel only, + RT_TIME_UNIT: the unit kernel only)."""
         ms, n = ctypes.c_float(), ctypes.c_int32()
         check(lib().rt_scene_kernel_time(self.handle, mode, ctypes.byref(ms), ctypes.byref(n)),
               "rt_scene_kernel_time")

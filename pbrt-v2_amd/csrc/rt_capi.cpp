@@ -17,11 +17,13 @@
 
 using namespace rti;
 
-// HIP-event timing of the traversal kernels: one event pair per launch while enabled.
+// HIP-event timing of the traversal kernels: one event triple per launch while enabled
+// (before the unit kernel, between it and the per-ray kernel, after the per-ray kernel).
 struct Timer {
   struct Slot {
     int mode;
-    hipEvent_t a, b;
+    bool split;  // m was recorded (rayhierarchy / ray-bvh launches)
+    hipEvent_t a, m, b;
   };
   bool on = false;
   std::vector<Slot> slots;
@@ -29,26 +31,34 @@ struct Timer {
   ~Timer() {
     for (auto& t : slots) {
       (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.m);
       (void)hipEventDestroy(t.b);
     }
   }
-  // The event pair for the next launch of `mode`, or null when timing is off.
-  Slot* next(int mode) {
+  // The events for the next launch of `mode`, or null when timing is off.
+  Slot* next(int mode, bool split) {
     if (!on) return nullptr;
     if (used == slots.size()) {
-      Slot t{mode, nullptr, nullptr};
+      Slot t{mode, split, nullptr, nullptr, nullptr};
       if (hipEventCreate(&t.a) != hipSuccess) return nullptr;
+      if (hipEventCreate(&t.m) != hipSuccess) {
+        (void)hipEventDestroy(t.a);
+        return nullptr;
+      }
       if (hipEventCreate(&t.b) != hipSuccess) {
         (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.m);
         return nullptr;
       }
       slots.push_back(t);
     }
     Slot& t = slots[used++];
     t.mode = mode;
+    t.split = split;
     return &t;
   }
-  int total(int mode, float* ms_out, int32_t* n_out) {
+  // part 0: whole traversal, 1: the unit (block) kernel, 2: the per-ray kernel.
+  int total(int mode, int part, float* ms_out, int32_t* n_out) {
     float sum = 0.f;
     int32_t n = 0;
     for (size_t i = 0; i < used; ++i) {
@@ -56,7 +66,13 @@ struct Timer {
       if (t.mode != mode) continue;
       if (hipEventSynchronize(t.b) != hipSuccess) return RT_ERR_HIP;
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) return RT_ERR_HIP;
+      hipEvent_t from = t.a, to = t.b;
+      if (part == 1 && t.split) to = t.m;
+      if (part == 2) {
+        if (!t.split) continue;
+        from = t.m;
+      }
+      if (hipEventElapsedTime(&ms, from, to) != hipSuccess) return RT_ERR_HIP;
       sum += ms;
       ++n;
     }
@@ -64,6 +80,7 @@ struct Timer {
     *n_out = n;
     return RT_OK;
   }
+  int total(int mode, float* ms_out, int32_t* n_out) { return total(mode, 0, ms_out, n_out); }
 };
 
 // Scratch of rayhierarchy launch groups: one device arena (job tables, per-job hierarchy,
@@ -100,6 +117,7 @@ struct RhJob {
 struct RhParams {
   int per[2] = {0, 0};
   int grid[2] = {0, 0};
+  int ray[2] = {0, 0};  // passing clusters from which a block takes the per-ray path
 };
 
 struct rt_scene {
@@ -114,8 +132,13 @@ struct rt_scene {
   float4* d_cbox = nullptr;
   float4* d_tbox = nullptr;  // AABB of every sorted triangle
   float4* d_tmt = nullptr;   // ray-bvh: leaf-test operands v1, e1, e2 (3 float4 per triangle)
+  // per-ray path: search BVH over the sorted triangles and their leaf-test operands
+  float4* d_snode = nullptr;
+  float4* d_tmts = nullptr;
+  uint32_t* d_tleaf = nullptr;  // ray-bvh: reference BVH leaf of every original triangle
   uint32_t nclusters = 0;
   int clusters_closest = 0, clusters_occluded = 0;  // 0 = built-in choice
+  int ray_closest = 0, ray_occluded = 0;            // per-ray path thresholds; 0 = built-in
   // ray-bvh handoff
   float4* d_bvh = nullptr;
   uint32_t* d_roots = nullptr;
@@ -252,7 +275,8 @@ uint32_t morton3(const float* p, const float* lo, const float* hi) {
 // of kClusterSize with their vertex AABBs.  Only the traversal order changes: hits carry the
 // original ids through perm.
 void sort_clusters(const float* v9, uint32_t ntri, std::vector<float>& v9s,
-                   std::vector<uint32_t>& perm, std::vector<float4>& cbox) {
+                   std::vector<uint32_t>& perm, std::vector<float4>& cbox,
+                   std::vector<uint32_t>& scode) {
   std::vector<float> c(3 * (size_t)ntri);
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (uint32_t i = 0; i < ntri; ++i)
@@ -269,6 +293,8 @@ void sort_clusters(const float* v9, uint32_t ntri, std::vector<float>& v9s,
   for (uint32_t i = 0; i < ntri; ++i) perm[i] = i;
   std::stable_sort(perm.begin(), perm.end(),
                    [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+  scode.resize(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) scode[i] = code[perm[i]];
   v9s.resize(9 * (size_t)ntri);
   for (uint32_t i = 0; i < ntri; ++i)
     std::memcpy(&v9s[9 * (size_t)i], v9 + 9 * (size_t)perm[i], 9 * sizeof(float));
@@ -313,7 +339,7 @@ hipError_t upload(T*& d, const T* h, size_t n) {
 // Launches `fn` bracketed by the timer's next event pair for `mode` (if timing is on).
 template <class F>
 hipError_t timed_launch(Timer& tm, int mode, hipStream_t st, F fn) {
-  Timer::Slot* t = tm.next(mode);
+  Timer::Slot* t = tm.next(mode, false);
   if (t) {
     hipError_t e = hipEventRecord(t->a, st);
     if (e != hipSuccess) return e;
@@ -326,6 +352,9 @@ hipError_t timed_launch(Timer& tm, int mode, hipStream_t st, F fn) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int kDefaultPer[2] = {8, 4};  // passing clusters per unit: closest, occluded
+// Blocks whose root passes at least this many clusters (64 triangles or 64 frontier roots
+// each) are traced ray by ray: closest, occluded.
+constexpr int kDefaultRay[2] = {64, 64};
 constexpr int kDefaultRootClustersPerUnit = 8;  // passing root clusters per ray-bvh unit
 
 // Lays out the jobs as launch groups (one per mode and node type, in job order) in the
@@ -357,7 +386,8 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
   const size_t tables = align256(sizeof(RhItem) * std::max<size_t>(jobs.size(), 1));
   total += tables;
   struct GroupLayout {
-    size_t units, ctr, replist, max_units;
+    size_t units, rq, ctr, replist, max_units, max_chunks;
+    uint32_t ray;
     uint32_t nblocks, nrays;
     bool big, repair;
     int per, grid;
@@ -370,7 +400,9 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     GroupLayout& L = gl[gi];
     // work units: passing clusters of triangles (rayhierarchy) or of frontier roots (ray-bvh)
     L.per = prm.per[mi] > 0 ? prm.per[mi] : (bvh ? kDefaultRootClustersPerUnit : kDefaultPer[mi]);
+    L.ray = (uint32_t)(prm.ray[mi] > 0 ? prm.ray[mi] : kDefaultRay[mi]);
     L.max_units = 0;
+    L.max_chunks = 0;
     L.nblocks = 0;
     L.nrays = 0;
     L.big = false;
@@ -395,6 +427,9 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
       it.cbox = jb.s->d_cbox;
       it.tbox = jb.s->d_tbox;
       it.tmt = jb.s->d_tmt;
+      it.snode = jb.s->d_snode;
+      it.tmts = jb.s->d_tmts;
+      it.tleaf = jb.s->d_tleaf;
       it.ntri = jb.s->ntri;
       it.nclusters = bvh ? (jb.s->ntri ? jb.s->nrclusters : 0) : jb.s->nclusters;
       if (bvh) {
@@ -416,6 +451,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
       if (it.g.rb > 256 || it.g.L > 3) L.big = true;
       if (it.repair_run > 0) L.repair = true;
       L.max_units += (size_t)it.g.ntop * ((it.nclusters + L.per - 1) / L.per);
+      L.max_chunks += (size_t)it.g.ntop * (size_t)((it.g.rb + 63) / 64);
       off_nodes[j] = total;
       total += align256(sizeof(float4) * 4 * (size_t)it.g.nnodes);
       off_clist[j] = total;
@@ -434,6 +470,8 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
                  : 0;
     L.units = total;
     total += align256(sizeof(uint4) * std::max<size_t>(L.max_units, 1));
+    L.rq = total;
+    total += align256(sizeof(uint2) * std::max<size_t>(L.max_chunks, 1));
     L.ctr = total;
     total += 256;
     L.replist = total;
@@ -479,6 +517,9 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     G.nblocks = gl[gi].nblocks;
     G.nrays = gl[gi].nrays;
     G.units = (uint4*)(A + gl[gi].units);
+    G.rq = (uint2*)(A + gl[gi].rq);
+    G.max_chunks = (uint32_t)std::min<size_t>(gl[gi].max_chunks, 0xffffffffu);
+    G.ray_clusters = gl[gi].ray;
     G.ctr = (uint32_t*)(A + gl[gi].ctr);
     G.replist = gl[gi].repair ? (uint32_t*)(A + gl[gi].replist) : nullptr;
     G.per_unit = gl[gi].per;
@@ -513,7 +554,19 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
 }  // namespace
 
 namespace rti {
-uint32_t scene_nclusters(const rt_scene* s) { return s->nclusters; }
+uint32_t scene_nclusters(const rt_scene* s) {
+  return s->desc.kind == RT_ACCEL_RAYBVH ? s->nrclusters : s->nclusters;
+}
+int scene_check_image(const rt_scene* s, int W, int H) {
+  if (s->desc.kind != RT_ACCEL_RAYHIERARCHY && s->desc.kind != RT_ACCEL_RAYBVH)
+    return RT_ERR_UNSUPPORTED;
+  if (s->desc.kind == RT_ACCEL_RAYBVH && s->ntri > 0 && !s->has_bvh) return RT_ERR_STATE;
+  RHGeom g;
+  if (rh_geometry(W, H, s->desc.chunk_x, s->desc.chunk_y, s->desc.height, &g))
+    return RT_ERR_UNSUPPORTED;
+  if (g.rb > 1024 || s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
+  return RT_OK;
+}
 }  // namespace rti
 
 extern "C" {
@@ -572,32 +625,45 @@ int rt_scene_create(rt_ctx* ctx, const float* v9, const float* uv6, uint32_t ntr
         e = hipMemcpy(s->d_uv6, uv6, sizeof(float) * 6 * (size_t)ntri, hipMemcpyHostToDevice);
     }
   }
-  if (e == hipSuccess && desc->kind == RT_ACCEL_RAYHIERARCHY && ntri) {
+  if (e == hipSuccess && desc->kind != RT_ACCEL_NAIVE && ntri) {
+    // Morton-sorted traversal copy: clusters of 64 with AABBs (rayhierarchy block path) and
+    // the search BVH with sorted leaf-test operands (per-ray path of both accelerators)
     std::vector<float> v9s;
-    std::vector<uint32_t> perm;
+    std::vector<uint32_t> perm, scode;
     std::vector<float4> cbox;
-    sort_clusters(v9, ntri, v9s, perm, cbox);
+    sort_clusters(v9, ntri, v9s, perm, cbox, scode);
     e = upload(s->d_v9s, v9s.data(), v9s.size());
     if (e == hipSuccess) e = upload(s->d_perm, perm.data(), perm.size());
-    if (e == hipSuccess) e = upload(s->d_cbox, cbox.data(), cbox.size());
     if (e == hipSuccess) {
-      // per-triangle AABBs with the device's min/max selects, vertex order v1, v2, v3
-      std::vector<float4> tbox(2 * (size_t)ntri);
-      for (uint32_t i = 0; i < ntri; ++i) {
-        const float* p = &v9s[9 * (size_t)i];
-        float mn[3], mx[3];
-        for (int a = 0; a < 3; ++a) {
-          mn[a] = p[3 + a] < p[a] ? p[3 + a] : p[a];
-          mn[a] = p[6 + a] < mn[a] ? p[6 + a] : mn[a];
-          mx[a] = p[3 + a] > p[a] ? p[3 + a] : p[a];
-          mx[a] = p[6 + a] > mx[a] ? p[6 + a] : mx[a];
-        }
-        tbox[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.f);
-        tbox[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.f);
-      }
-      e = upload(s->d_tbox, tbox.data(), tbox.size());
+      std::vector<float4> snode;
+      build_search_bvh(v9s.data(), scode.data(), ntri, snode);
+      e = upload(s->d_snode, snode.data(), snode.size());
     }
-    s->nclusters = (uint32_t)(cbox.size() / 2);
+    if (e == hipSuccess) {
+      const std::vector<float4> mts = mt_operands(v9s.data(), ntri);
+      e = upload(s->d_tmts, mts.data(), mts.size());
+    }
+    if (e == hipSuccess && desc->kind == RT_ACCEL_RAYHIERARCHY) {
+      e = upload(s->d_cbox, cbox.data(), cbox.size());
+      if (e == hipSuccess) {
+        // per-triangle AABBs with the device's min/max selects, vertex order v1, v2, v3
+        std::vector<float4> tbox(2 * (size_t)ntri);
+        for (uint32_t i = 0; i < ntri; ++i) {
+          const float* p = &v9s[9 * (size_t)i];
+          float mn[3], mx[3];
+          for (int a = 0; a < 3; ++a) {
+            mn[a] = p[3 + a] < p[a] ? p[3 + a] : p[a];
+            mn[a] = p[6 + a] < mn[a] ? p[6 + a] : mn[a];
+            mx[a] = p[3 + a] > p[a] ? p[3 + a] : p[a];
+            mx[a] = p[6 + a] > mx[a] ? p[6 + a] : mx[a];
+          }
+          tbox[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.f);
+          tbox[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.f);
+        }
+        e = upload(s->d_tbox, tbox.data(), tbox.size());
+      }
+      s->nclusters = (uint32_t)(cbox.size() / 2);
+    }
   }
   if (e == hipSuccess && desc->kind == RT_ACCEL_RAYBVH && ntri) {
     const std::vector<float4> mt = mt_operands(v9, ntri);  // original order
@@ -632,8 +698,14 @@ int rt_scene_set_bvh(rt_scene* s, const rt_gpu_node* nodes, uint32_t nnodes, con
     packed[2 * (size_t)i] = make_float4(n.ax, n.ay, n.az, n.bx);
     packed[2 * (size_t)i + 1] = make_float4(n.by, n.bz, fpo, fnp);
   }
-  for (uint32_t i = 0; i < s->ntri; ++i)
-    if (prim_order[i] >= s->ntri) return RT_ERR_INVALID;
+  // prim_order must be a permutation: every triangle sits at exactly one BVH position
+  {
+    std::vector<uint8_t> used(s->ntri, 0);
+    for (uint32_t i = 0; i < s->ntri; ++i) {
+      if (prim_order[i] >= s->ntri || used[prim_order[i]]) return RT_ERR_INVALID;
+      used[prim_order[i]] = 1;
+    }
+  }
   // Every node below a frontier root gets its parent and its depth below that root (the
   // device rebuilds the exact lockstep cull chain of a hit from them).  Subtrees deeper than
   // 12 levels, nodes shared by two roots and cycles are rejected: the device traversal's
@@ -659,6 +731,15 @@ int rt_scene_set_bvh(rt_scene* s, const rt_gpu_node* nodes, uint32_t nnodes, con
         stack.push_back(c);
       }
     }
+  }
+  // Reference BVH leaf of every triangle below a frontier root, for the per-ray path: the leaf
+  // node index, bit 31 set when the triangle is not the leaf's first primitive (secondary
+  // rays test only the first, intersection6DB_BVH2.cl); ~0u for a triangle in no such leaf.
+  std::vector<uint32_t> tleaf(s->ntri, 0xffffffffu);
+  for (uint32_t n = 0; n < nnodes; ++n) {
+    if (!seen[n] || !nodes[n].n_primitives) continue;
+    for (uint32_t f = 0; f < nodes[n].n_primitives; ++f)
+      tleaf[prim_order[nodes[n].prim_offset + f]] = n | (f > 0 ? 0x80000000u : 0u);
   }
   // Frontier roots in Morton order of their box centres, cut into clusters of 64 with the
   // union of their boxes (the traversal order does not change any result).
@@ -710,6 +791,8 @@ int rt_scene_set_bvh(rt_scene* s, const rt_gpu_node* nodes, uint32_t nnodes, con
   if (s->d_prim_order) (void)hipFree(s->d_prim_order);
   if (s->d_bvh_parent) (void)hipFree(s->d_bvh_parent);
   if (s->d_bvh_depth) (void)hipFree(s->d_bvh_depth);
+  if (s->d_tleaf) (void)hipFree(s->d_tleaf);
+  s->d_tleaf = nullptr;
   s->d_bvh = nullptr;
   s->d_roots = s->d_prim_order = s->d_bvh_parent = nullptr;
   s->d_bvh_depth = nullptr;
@@ -720,6 +803,7 @@ int rt_scene_set_bvh(rt_scene* s, const rt_gpu_node* nodes, uint32_t nnodes, con
   RT_TRY(upload(s->d_prim_order, prim_order, s->ntri));
   RT_TRY(upload(s->d_bvh_parent, parent.data(), parent.size()));
   RT_TRY(upload(s->d_bvh_depth, depth.data(), depth.size()));
+  RT_TRY(upload(s->d_tleaf, tleaf.data(), tleaf.size()));
   s->nroots = nroots;
   s->has_bvh = true;
   return RT_OK;
@@ -735,6 +819,9 @@ int rt_scene_destroy(rt_scene* s) {
   (void)hipFree(s->d_cbox);
   (void)hipFree(s->d_tbox);
   (void)hipFree(s->d_tmt);
+  (void)hipFree(s->d_snode);
+  (void)hipFree(s->d_tmts);
+  (void)hipFree(s->d_tleaf);
   (void)hipFree(s->d_bvh);
   (void)hipFree(s->d_roots);
   (void)hipFree(s->d_prim_order);
@@ -769,10 +856,13 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
     RhParams prm;
     prm.per[0] = kind == RT_ACCEL_RAYBVH ? s->roots_per_unit : s->clusters_closest;
     prm.per[1] = kind == RT_ACCEL_RAYBVH ? s->roots_per_unit : s->clusters_occluded;
+    prm.ray[0] = s->ray_closest;
+    prm.ray[1] = s->ray_occluded;
     st = rh_prepare(s->ws[mode], jobs, prm, hs, groups, &changed);
     if (st) return st;
-    Timer::Slot* t = s->timer.next(mode);
-    RT_TRY(launch_rh_group(groups[0], hs, t ? t->a : nullptr, t ? t->b : nullptr));
+    Timer::Slot* t = s->timer.next(mode, true);
+    RT_TRY(launch_rh_group(groups[0], hs, t ? t->a : nullptr, t ? t->m : nullptr,
+                           t ? t->b : nullptr));
     return RT_OK;
   }
   if (closest) {
@@ -849,6 +939,8 @@ int rt_scene_set_param(rt_scene* s, int32_t param, int32_t value) {
     s->roots_per_unit = value;
   else if (param == RT_PARAM_CLUSTERS_CLOSEST) s->clusters_closest = value;
   else if (param == RT_PARAM_CLUSTERS_OCCLUDED) s->clusters_occluded = value;
+  else if (param == RT_PARAM_RAY_CLUSTERS_CLOSEST) s->ray_closest = value;
+  else if (param == RT_PARAM_RAY_CLUSTERS_OCCLUDED) s->ray_occluded = value;
   else return RT_ERR_INVALID;
   return RT_OK;
 }
@@ -862,9 +954,9 @@ int rt_scene_set_timing(rt_scene* s, int32_t enable) {
 }
 
 int rt_scene_kernel_time(rt_scene* s, int32_t mode, float* total_ms, int32_t* launches) {
-  if (!s || !total_ms || !launches) return RT_ERR_INVALID;
+  if (!s || !total_ms || !launches || mode < 0 || mode > 8) return RT_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);
-  return s->timer.total(mode, total_ms, launches);
+  return s->timer.total(mode % 3, mode < 3 ? 0 : mode < 6 ? 2 : 1, total_ms, launches);
 }
 
 int rt_batch_create(rt_ctx* ctx, rt_batch** out) {
@@ -895,15 +987,10 @@ static int batch_add(rt_batch* b, rt_scene* s, const rt_rays* rays, int mode, co
                      uint8_t* occ) {
   if (!b || !s) return RT_ERR_INVALID;
   if (s->ctx->device != b->ctx->device) return RT_ERR_INVALID;
-  if (s->desc.kind != RT_ACCEL_RAYHIERARCHY && s->desc.kind != RT_ACCEL_RAYBVH)
-    return RT_ERR_UNSUPPORTED;
-  if (s->desc.kind == RT_ACCEL_RAYBVH && s->ntri > 0 && !s->has_bvh) return RT_ERR_STATE;
   int st = check_rays(rays, true);
   if (st) return st;
-  RHGeom g;
-  if (rh_geometry(rays->width, rays->height, s->desc.chunk_x, s->desc.chunk_y, s->desc.height, &g))
-    return RT_ERR_UNSUPPORTED;
-  if (g.rb > 1024 || s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
+  st = rti::scene_check_image(s, rays->width, rays->height);
+  if (st) return st;
   std::lock_guard<std::mutex> lk(b->mu);
   const HitOut out = hits ? HitOut{hits->id, hits->t, hits->u, hits->v, hits->dpdu, hits->dpdv}
                           : HitOut{};
@@ -927,8 +1014,8 @@ int rt_batch_add_occluded(rt_batch* b, rt_scene* s, const rt_rays* rays, uint8_t
 static int batch_launch(rt_batch* b, const std::vector<RhGroup>& groups, hipStream_t hs,
                         bool timed) {
   for (const RhGroup& G : groups) {
-    Timer::Slot* t = timed ? b->timer.next(G.mode) : nullptr;
-    RT_TRY(launch_rh_group(G, hs, t ? t->a : nullptr, t ? t->b : nullptr));
+    Timer::Slot* t = timed ? b->timer.next(G.mode, true) : nullptr;
+    RT_TRY(launch_rh_group(G, hs, t ? t->a : nullptr, t ? t->m : nullptr, t ? t->b : nullptr));
   }
   return RT_OK;
 }
@@ -974,6 +1061,8 @@ static int* batch_param(rt_batch* b, int32_t param) {
     case RT_PARAM_CLUSTERS_OCCLUDED: return &b->prm.per[1];
     case RT_PARAM_GRID_CLOSEST: return &b->prm.grid[0];
     case RT_PARAM_GRID_OCCLUDED: return &b->prm.grid[1];
+    case RT_PARAM_RAY_CLUSTERS_CLOSEST: return &b->prm.ray[0];
+    case RT_PARAM_RAY_CLUSTERS_OCCLUDED: return &b->prm.ray[1];
     default: return nullptr;
   }
 }
@@ -988,8 +1077,9 @@ int rt_batch_set_param(rt_batch* b, int32_t param, int32_t value) {
   }
   int* p = batch_param(b, param);
   if (!p) return RT_ERR_INVALID;
-  if (param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED) {
-    if (value > (1 << 20)) return RT_ERR_INVALID;
+  if (param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED ||
+      param == RT_PARAM_RAY_CLUSTERS_CLOSEST || param == RT_PARAM_RAY_CLUSTERS_OCCLUDED) {
+    if (value > (1 << 30)) return RT_ERR_INVALID;
   } else if (value > (1 << 16)) {
     return RT_ERR_INVALID;
   }
@@ -1007,9 +1097,13 @@ int rt_batch_get_param(rt_batch* b, int32_t param, int32_t* value) {
   }
   int* p = batch_param(b, param);
   if (!p) return RT_ERR_INVALID;
-  const int mi = (param == RT_PARAM_CLUSTERS_OCCLUDED || param == RT_PARAM_GRID_OCCLUDED) ? 1 : 0;
+  const int mi = (param == RT_PARAM_CLUSTERS_OCCLUDED || param == RT_PARAM_GRID_OCCLUDED ||
+                  param == RT_PARAM_RAY_CLUSTERS_OCCLUDED)
+                     ? 1
+                     : 0;
   const bool grid = param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED;
-  *value = *p > 0 ? *p : (grid ? 0 : kDefaultPer[mi]);
+  const bool ray = param == RT_PARAM_RAY_CLUSTERS_CLOSEST || param == RT_PARAM_RAY_CLUSTERS_OCCLUDED;
+  *value = *p > 0 ? *p : (grid ? 0 : ray ? kDefaultRay[mi] : kDefaultPer[mi]);
   return RT_OK;
 }
 
@@ -1074,9 +1168,9 @@ int rt_batch_set_timing(rt_batch* b, int32_t enable) {
 }
 
 int rt_batch_kernel_time(rt_batch* b, int32_t mode, float* total_ms, int32_t* launches) {
-  if (!b || !total_ms || !launches) return RT_ERR_INVALID;
+  if (!b || !total_ms || !launches || mode < 0 || mode > 8) return RT_ERR_INVALID;
   std::lock_guard<std::mutex> lk(b->mu);
-  return b->timer.total(mode, total_ms, launches);
+  return b->timer.total(mode % 3, mode < 3 ? 0 : mode < 6 ? 2 : 1, total_ms, launches);
 }
 
 const char* rt_error_string(int status) {

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 struct rt_ctx {
   int device;
 };
@@ -10,8 +12,13 @@ struct rt_scene;
 
 namespace rti {
 
-// Morton clusters of a rayhierarchy scene (0 for other accelerators).
+// Clusters a block's cluster pass tests: Morton clusters of triangles (rayhierarchy) or of
+// frontier roots (ray-bvh); 0 for naive scenes.
 uint32_t scene_nclusters(const rt_scene* s);
+// RT_OK when `s` can trace a W x H ray image through the batched form (a rayhierarchy or
+// ray-bvh scene with its BVH, a hierarchy shape the kernels support), else the status the
+// batch would return.
+int scene_check_image(const rt_scene* s, int W, int H);
 
 constexpr int kMaxLevels = 4;   // leaves per block <= 64 (uint64 masks)
 constexpr int kMaxSub = 85;     // 1 + 4 + 16 + 64 nodes in a block subtree
@@ -63,6 +70,12 @@ struct RhItem {
   const float4* tbox;    // AABB (min, max) of every sorted triangle
   const float4* tmt;     // ray-bvh: leaf-test operands (v1, e1, e2) of every triangle, original
                          // order (null for rayhierarchy)
+  // per-ray path: padded search BVH over the sorted triangles (4 float4 per node), their
+  // leaf-test operands in sorted order, and for ray-bvh the reference BVH leaf of every
+  // original triangle (bit 31 set: not the first primitive of its leaf; ~0u: in no leaf)
+  const float4* snode;
+  const float4* tmts;
+  const uint32_t* tleaf;
   uint32_t ntri;
   uint32_t nclusters;    // clusters of 64 triangles (rayhierarchy) or 64 frontier roots
   // ray-bvh: BVH nodes (2 float4 each), frontier roots (Morton-ordered; cbox holds the boxes
@@ -95,20 +108,40 @@ struct RhGroup {
   int bvh;   // ray-bvh jobs (product traversal with the scene's BVH)
   uint32_t nblocks, nrays;
   uint4* units;       // {item, block, first cluster, count}
-  uint32_t* ctr;      // [0] units, [1] next unit to take, [2] rays flagged for repair
+  uint2* rq;          // per-ray path: {item, block << 4 | 64-ray chunk of the block}
+  uint32_t* ctr;      // [0] units, [1] next unit to take, [2] rays flagged for repair,
+                      // [3] traversal stack overflow (ray-bvh), [4] per-ray chunks,
+                      // [5] next per-ray chunk to take
   uint32_t* replist;  // global ray indices flagged for repair (primary with repair only)
   int per_unit;       // passing clusters per work unit
   int grid;           // work-groups of the persistent unit kernel; 0 = one per resident slot
   uint32_t max_units;  // upper bound of the group's work units
+  uint32_t max_chunks;  // upper bound of the group's per-ray chunks
+  // blocks whose root passes at least this many clusters are traced ray by ray
+  uint32_t ray_clusters;
 };
+
+// Search BVH of the per-ray path over triangles in sorted order (see rt_search_bvh.cpp).
+// Node = 4 float4: child 0 box (lo.xyz, hi.x), (hi.yz, child 1 lo.xy), (lo.z, hi.xyz),
+// (ref0, ref1 as bits, 0, 0).  A ref with bit 31 clear is an interior node index; with bit
+// 31 set it is a leaf of (bits 27..30) + 1 triangles starting at sorted slot (bits 0..26).
+// Boxes are grown outward by the scene's padding.  The depth is at most kSearchDepth.
+constexpr int kSearchDepth = 30;
+constexpr int kSearchLeaf = 4;
+// Builds the search BVH of `ntri` triangles (v9 in the order the leaves index) whose 30-bit
+// Morton codes `code` are sorted ascending.  Returns the node array (4 float4 per node).
+void build_search_bvh(const float* v9, const uint32_t* code, uint32_t ntri,
+                      std::vector<float4>& nodes);
 
 // Launchers (all asynchronous on `s`).
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 // rayhierarchy / ray-bvh group: init, fused build, cluster (root) pass, persistent unit
-// kernel (bracketed by the events t0 / t1 when they are given) and, for closest hits,
-// finalize and (rayhierarchy) repair.
-hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t t1);
+// kernel, per-ray kernel and, for closest hits, finalize and (rayhierarchy) repair.  When
+// given, t0 is recorded before the unit kernel, tm between the unit and per-ray kernels and
+// t1 after the per-ray kernel.
+hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t tm,
+                           hipEvent_t t1);
 hipError_t launch_naive_trace(const RaysDev& r, int mode, const float* v9, uint32_t ntri,
                               int tchunks, uint64_t* keys, uint8_t* occ, hipStream_t s);
 // Hit records of the naive path from the reduction keys.

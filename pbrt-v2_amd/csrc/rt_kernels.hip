@@ -24,7 +24,10 @@
 // deterministic (ties -> smallest id).
 #include <algorithm>
 #include <atomic>
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "rt_device.h"
 #include "rt_internal.h"
@@ -255,7 +258,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void rh_init_kernel(const RhItem* __restrict__ items, int nitems,
                                                       uint32_t nrays, uint32_t* __restrict__ ctr) {
   const uint32_t gi = blockIdx.x * kThreads + threadIdx.x;
-  if (gi < 4) ctr[gi] = 0u;
+  if (gi < 8) ctr[gi] = 0u;
   if (gi >= nrays) return;
   const RhItem& it = items[find_ray_item(items, nitems, gi)];
   const uint32_t i = gi - it.ray_base;
@@ -627,11 +630,16 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
 // triangles, or of ray-bvh frontier roots) against the block root (cluster bound), stores the
 // passing cluster ids in the job's clist[block * nclusters + ...] and appends work units
 // {job, block, first, count} of at most `per_unit` passing clusters.
+//
+// A block whose root passes at least `ray_clusters` clusters is an incoherent ray group: its
+// candidate set holds most (leaf, triangle) pairs, so it is queued for the per-ray path
+// instead, as 64-ray chunks {job, block << 4 | chunk}.
 template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
                                                          int nitems, uint4* __restrict__ units,
+                                                         uint2* __restrict__ rq,
                                                          uint32_t* __restrict__ ctr,
-                                                         int per_unit) {
+                                                         int per_unit, uint32_t ray_clusters) {
   __shared__ int sCnt;
   __shared__ uint32_t sBase;
   const int tid = threadIdx.x;
@@ -665,6 +673,14 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   }
   __syncthreads();
   const int cnt = sCnt;
+  if ((uint32_t)cnt >= ray_clusters) {
+    const int nch = (g.rb + 63) / 64;
+    if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
+    __syncthreads();
+    for (int q = tid; q < nch; q += kThreads)
+      rq[sBase + q] = make_uint2((uint32_t)k, ((uint32_t)b << 4) | (uint32_t)q);
+    return;
+  }
   const int nu = (cnt + per_unit - 1) / per_unit;
   if (tid == 0 && nu > 0) sBase = atomicAdd(&ctr[0], (uint32_t)nu);
   __syncthreads();
@@ -1199,6 +1215,202 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Per-ray path (blocks the cluster pass queued as incoherent).  A ray's result is the argmin
+// over the triangles that pass its strict leaf test AND the exact cull chain of its leaf (for
+// ray-bvh: that are tested at a (BVH leaf, hierarchy leaf) pair whose lockstep chain passes).
+// Every such triangle passes the strict leaf test, so each ray walks the scene's search BVH
+// (padded, conservative boxes), runs the strict leaf test on the triangles of the leaves it
+// reaches, and evaluates the exact chain only for those that pass it and could change the
+// ray.  The (t, id) key, the repair-region key and the occlusion flag are the block path's,
+// so the outcome is the same; every ray of a block is handled by exactly one lane, which
+// writes its results directly.
+constexpr uint32_t kSearchLeafBit = 0x80000000u;
+constexpr float kSlack = 0x1p-16f;  // relative slack of the slab-test parameters
+
+// Slab test of the ray against a padded box: passes when [tn, tf] (tf widened by kSlack)
+// meets [tlo, thi]; tn is the entry parameter used to order the children.
+__device__ __forceinline__ bool slab(V3 o, V3 inv, V3 lo, V3 hi, float tlo, float thi, float& tn) {
+  const float ax = (lo.x - o.x) * inv.x, bx = (hi.x - o.x) * inv.x;
+  const float ay = (lo.y - o.y) * inv.y, by = (hi.y - o.y) * inv.y;
+  const float az = (lo.z - o.z) * inv.z, bz = (hi.z - o.z) * inv.z;
+  tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+  float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+  tf = tf + fabsf(tf) * kSlack;
+  return tn <= tf && tn <= thi && tf >= tlo;
+}
+
+// Hierarchy node of level l above image position (x, y), from the job's node array.
+template <class P>
+__device__ __forceinline__ bool chain_node(const RHGeom& g, P nodes, int x, int y, int l, Node& n) {
+  int gw = 0, off = 0, cxl = 0, cyl = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxLevels; ++q) {
+    if (q == l) {
+      gw = g.gw[q];
+      off = g.off[q];
+    }
+  }
+  cxl = g.cx << l;
+  cyl = g.cy << l;
+  const P p = nodes + 4 * ((size_t)off + (size_t)(y / cyl) * gw + (x / cxl));
+  const float4 a = ld4(p, 0), b = ld4(p, 1), c = ld4(p, 2), d = ld4(p, 3);
+  n.dmin = {a.x, a.y, a.z};
+  n.dmax = {b.x, b.y, b.z};
+  n.omin = {c.x, c.y, c.z};
+  n.omax = {d.x, d.y, d.z};
+  return a.w != 0.f;
+}
+
+// rayhierarchy: the exact cull chain of the ray's leaf for sorted triangle `slot`
+// (intersection6DB.cl:51-119 and variants: every level from the block root to the leaf).
+template <int MODE, int NODE>
+__device__ __forceinline__ bool ray_chain_rh(const RhItem& it, int x, int y, uint32_t slot) {
+  const Tri T = load_tri(glb(it.v9s), slot);
+  for (int l = it.g.L - 1; l >= 0; --l) {
+    Node n;
+    if (!chain_node(it.g, glb(it.nodes), x, y, l, n)) return false;
+    if (!cull_exact<MODE, NODE>(n, T)) return false;
+  }
+  return true;
+}
+
+// ray-bvh: the exact lockstep chain of (BVH leaf b, the ray's hierarchy leaf): step s visits
+// the hierarchy node of level max(top - s, 0) and the BVH node of depth min(s, d) above b.
+template <int NODE>
+__device__ __forceinline__ bool ray_chain_bvh(const RhItem& it, int x, int y, uint32_t b) {
+  const int top = it.g.L - 1;
+  const int d = (int)glb(it.bvh_depth)[b];
+  uint32_t node = b;
+  int dn = d;
+  for (int st = max(top, d); st >= 0; --st) {
+    const int bd = min(st, d);
+    while (dn > bd) {
+      node = glb(it.bvh_parent)[node];
+      --dn;
+    }
+    Node n;
+    if (!chain_node(it.g, glb(it.nodes), x, y, top - min(st, top), n)) return false;
+    V3 bmin, bmax;
+    uint32_t off, np;
+    bvh_box(glb(it.bvh), node, bmin, bmax, off, np);
+    if (!bvh_cull<NODE>(n, bmin, bmax)) return false;
+  }
+  return true;
+}
+
+template <int MODE, int NODE, bool BVH>
+__global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__ items,
+                                                      const uint2* __restrict__ rq,
+                                                      uint32_t* __restrict__ ctr) {
+  __shared__ uint32_t sStk[kSearchDepth][64];
+  const int lane = threadIdx.x;
+  const uint32_t nchunks = __builtin_amdgcn_readfirstlane(ctr[4]);
+  for (;;) {
+    uint32_t e = 0;
+    if (lane == 0) e = atomicAdd(&ctr[5], 1u);
+    e = __builtin_amdgcn_readfirstlane(e);
+    if (e >= nchunks) break;
+    const uint2 w = rq[e];
+    const RhItem& it = items[__builtin_amdgcn_readfirstlane(w.x)];
+    const RHGeom& g = it.g;
+    const RaysDev& r = it.r;
+    const uint32_t wy = __builtin_amdgcn_readfirstlane(w.y);
+    const int top = g.L - 1;
+    const int b = (int)(wy >> 4);
+    const int K = b % g.gw[top], J = b / g.gw[top];
+    int x = 0, y = 0, leaf = 0;
+    bool have = slot_xy(g, K, J, (int)(wy & 15u) * 64 + lane, x, y, leaf) && it.ntri > 0;
+    const size_t idx = (size_t)y * g.W + x;
+    if (have && MODE != 0 && r.valid != nullptr && glb(r.valid)[idx] == 0) have = false;
+    if (!have) continue;
+    const V3 o = {glb(r.ox)[idx], glb(r.oy)[idx], glb(r.oz)[idx]};
+    const V3 d = {glb(r.dx)[idx], glb(r.dy)[idx], glb(r.dz)[idx]};
+    const float tmin = glb(r.tmin)[idx], tmax = glb(r.tmax)[idx];
+    // inverse direction for the slab test; zero components become tiny ones, so the slab of
+    // that axis is (almost) the whole line when the origin is inside it and empty otherwise
+    auto safe = [](float v) { return fabsf(v) < 0x1p-80f ? (v < 0.f ? -0x1p-80f : 0x1p-80f) : v; };
+    const V3 inv = {1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z)};
+    const float tlo = tmin - fabsf(tmin) * kSlack;
+    const auto snode = glb(it.snode);
+    const auto tmts = glb(it.tmts);
+    const auto perm = glb(it.perm);
+    const bool track_rep = MODE == 0 && !BVH && it.repkeys != nullptr;
+    uint64_t best = kMissKey;
+    uint32_t rep = kNoKey32;
+    bool occluded = false;
+    int sp = 0;
+    uint32_t cur = 0;  // interior node 0 (the root)
+    for (;;) {
+      if (!(cur & kSearchLeafBit)) {
+        const float4 n0 = ld4(snode, 4 * (size_t)cur), n1 = ld4(snode, 4 * (size_t)cur + 1),
+                     n2 = ld4(snode, 4 * (size_t)cur + 2), n3 = ld4(snode, 4 * (size_t)cur + 3);
+        float thi = MODE == 2 ? tmax : (best == kMissKey ? __int_as_float(0x7f800000)
+                                                         : fkey_inv((uint32_t)(best >> 32)));
+        thi = thi + fabsf(thi) * kSlack;
+        float tn0, tn1;
+        const bool h0 = slab(o, inv, V3{n0.x, n0.y, n0.z}, V3{n0.w, n1.x, n1.y}, tlo, thi, tn0);
+        const bool h1 = slab(o, inv, V3{n1.z, n1.w, n2.x}, V3{n2.y, n2.z, n2.w}, tlo, thi, tn1);
+        const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
+        if (h0 && h1) {
+          const bool first1 = tn1 < tn0;
+          sStk[sp++][lane] = first1 ? r0 : r1;
+          cur = first1 ? r1 : r0;
+          continue;
+        }
+        if (h0 || h1) {
+          cur = h0 ? r0 : r1;
+          continue;
+        }
+      } else {
+        const uint32_t first = cur & ((1u << 27) - 1u), cnt = ((cur >> 27) & 15u) + 1u;
+        for (uint32_t f = 0; f < cnt && !occluded; ++f) {
+          const uint32_t slot = first + f;
+          const uint32_t id = perm[slot];
+          uint32_t bl = 0;
+          if constexpr (BVH) {
+            bl = glb(it.tleaf)[id];
+            if (bl == 0xffffffffu) continue;
+            if (MODE == 1 && (bl & 0x80000000u)) continue;  // first primitive only
+            bl &= 0x7fffffffu;
+          }
+          float t, b1, b2;
+          if (!mt_strict(o, d, load_mt(tmts, slot), t, b1, b2)) continue;
+          if constexpr (MODE == 2) {
+            if (!(t > tmin && t < tmax)) continue;  // intersectAllLeaves.cl:177-179
+          } else {
+            const bool main = t > tmin && t < __int_as_float(0x7f800000);
+            const bool rp = track_rep && repair_region(b1, b2) && t >= tmin;
+            const uint64_t key = hit_key(t, (int32_t)id);
+            const bool um = main && key < best, ur = rp && fkey(t) < rep;
+            if (!um && !ur) continue;
+            bool ok;
+            if constexpr (BVH) ok = ray_chain_bvh<NODE>(it, x, y, bl);
+            else ok = ray_chain_rh<MODE, NODE>(it, x, y, slot);
+            if (!ok) continue;
+            if (um) best = key;
+            if (ur) rep = fkey(t);
+            continue;
+          }
+          bool ok;
+          if constexpr (BVH) ok = ray_chain_bvh<NODE>(it, x, y, bl);
+          else ok = ray_chain_rh<MODE, NODE>(it, x, y, slot);
+          occluded = ok;
+        }
+        if (occluded) break;
+      }
+      if (sp == 0) break;
+      cur = sStk[--sp][lane];
+    }
+    if constexpr (MODE == 2) {
+      if (occluded) glb(it.occ)[idx] = 1;
+    } else {
+      if (best != kMissKey) glb(it.keys)[idx] = best;
+      if (track_rep && rep != kNoKey32) glb(it.repkeys)[idx] = rep;
+    }
+  }
+}
+
 // Triangle staged in LDS as v1 and the two edges (the leaf test's operands).
 __device__ __forceinline__ void stage_tri(float4 (*sT)[3], int slot, const Tri& T) {
   sT[slot][0] = make_float4(T.v1.x, T.v1.y, T.v1.z, T.e1.x);
@@ -1611,25 +1823,32 @@ __global__ void fill_kernel(T* p, T v, size_t n) {
 inline dim3 grid1(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
 // Work-groups of a persistent kernel that are resident at once on the current device: CUs x
-// work-groups per CU (limited by registers and LDS), at most `cap`.  A larger grid only adds
-// work-groups that start when the queue is nearly drained.  The count is kept per kernel and
-// device once known (2048 until then).
+// work-groups per CU (limited by registers and LDS) for `threads`-wide work-groups, at most
+// `cap`.  A larger grid only adds work-groups that start when the queue is nearly drained.
+// The count is kept per (device, kernel) once known (2048 until then).
 template <class F>
-unsigned resident_grid(F kernel, uint32_t cap) {
-  constexpr int kDevices = 64;
-  static std::atomic<unsigned> known[kDevices];
+unsigned resident_grid(F kernel, uint32_t cap, int threads = kThreads) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, unsigned> known;
   int dev = 0;
   unsigned n = 0;
-  if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kDevices)
-    n = known[dev].load(std::memory_order_relaxed);
+  const bool have_dev = hipGetDevice(&dev) == hipSuccess;
+  const std::pair<int, const void*> key{dev, reinterpret_cast<const void*>(kernel)};
+  if (have_dev) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto f = known.find(key);
+    if (f != known.end()) n = f->second;
+  }
   if (n == 0) {
     int cus = 0, per_cu = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) ==
+    if (have_dev &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) ==
             hipSuccess &&
         cus > 0 && per_cu > 0) {
       n = (unsigned)cus * (unsigned)per_cu;
-      if (dev >= 0 && dev < kDevices) known[dev].store(n, std::memory_order_relaxed);
+      std::lock_guard<std::mutex> lk(mu);
+      known[key] = n;
     } else {
       n = 2048;
     }
@@ -1639,17 +1858,19 @@ unsigned resident_grid(F kernel, uint32_t cap) {
 
 // Launch sequence of one rayhierarchy group with the mode and node type fixed.
 template <int MODE, int NODE>
-hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t tm,
+                             hipEvent_t t1) {
   hipLaunchKernelGGL((rh_init_kernel<MODE>), grid1(std::max<uint32_t>(G.nrays, 4)),
                      dim3(kThreads), 0, s, G.items, G.nitems, G.nrays, G.ctr);
   hipLaunchKernelGGL((rh_group_build_kernel<MODE != 0>), dim3(G.nblocks), dim3(kThreads), 0, s,
                      G.items, G.nitems);
   if (G.bvh)
     hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE, true>), dim3(G.nblocks), dim3(kThreads), 0,
-                       s, G.items, G.nitems, G.units, G.ctr, G.per_unit);
+                       s, G.items, G.nitems, G.units, G.rq, G.ctr, G.per_unit, G.ray_clusters);
   else
     hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE, false>), dim3(G.nblocks), dim3(kThreads),
-                       0, s, G.items, G.nitems, G.units, G.ctr, G.per_unit);
+                       0, s, G.items, G.nitems, G.units, G.rq, G.ctr, G.per_unit,
+                       G.ray_clusters);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (t0) {
@@ -1668,6 +1889,20 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
     launch(rh_unit_kernel<MODE, NODE, 1024, 4>);
   else
     launch(rh_unit_kernel<MODE, NODE, 256, 3>);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (tm) {
+    e = hipEventRecord(tm, s);
+    if (e != hipSuccess) return e;
+  }
+  auto launch_rays = [&](auto kernel) {
+    const dim3 grid(resident_grid(kernel, G.max_chunks, 64));
+    hipLaunchKernelGGL(kernel, grid, dim3(64), 0, s, G.items, G.rq, G.ctr);
+  };
+  if (G.bvh)
+    launch_rays(ray_path_kernel<MODE, NODE, true>);
+  else
+    launch_rays(ray_path_kernel<MODE, NODE, false>);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (t1) {
@@ -1735,9 +1970,10 @@ hipError_t launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEvent_t tm,
+                           hipEvent_t t1) {
   if (G.nitems <= 0 || G.nrays == 0) return hipSuccess;
-#define RT_CALL(M, N) rh_group_dispatch<M, N>(G, s, t0, t1)
+#define RT_CALL(M, N) rh_group_dispatch<M, N>(G, s, t0, tm, t1)
   if (G.node == 1) {
     if (G.mode == 0) return RT_CALL(0, 1);
     if (G.mode == 1) return RT_CALL(1, 1);

@@ -445,3 +445,414 @@ def test_cpu_bvh_matches_exhaustive(orc, scenes):
     np.testing.assert_array_equal(got["t"], ex["t"])
     assert (got["id"] == ex["id"]).mean() > 0.999
     bvh.close()
+
+
+# ---------------------------------------------------------------- ray-bvh traversal pins
+def _strict_occ_brute(v9, rays):
+    n = rays["width"] * rays["height"]
+    exp = np.zeros(n, np.uint8)
+    for i in range(n):
+        if rays.get("valid") is not None and not rays["valid"][i]:
+            continue
+        o = (rays["ox"][i], rays["oy"][i], rays["oz"][i])
+        d = (rays["dx"][i], rays["dy"][i], rays["dz"][i])
+        for k in range(len(v9)):
+            ok, (t, _, _) = orc_mt_strict(o, d, v9[k])
+            if ok and rays["tmin"][i] < t < rays["tmax"][i]:
+                exp[i] = 1
+                break
+    return exp
+
+
+def orc_mt_strict(o, d, v):
+    from conftest import load_oracle
+    return load_oracle().mt("strict", o, d, v)
+
+
+def test_raybvh_ia_equals_brute_force(orc, scenes):
+    # With node "ia" every cull of the lockstep chain is the conservative interval test
+    # against a BVH box that contains its triangles, so ray-bvh must return the exhaustive
+    # strict nearest hit (primary, and secondary with one primitive per leaf) and the
+    # exhaustive occlusion.
+    v9 = scenes.random_triangles(60, seed=41, extent=1.0, size=0.3)
+    nodes, roots, order = orc.raybvh_handoff(v9, max_prims=1, bvh_height=3)
+    p = orc.params("ray-bvh", node="ia")
+    rays = scenes.pinhole_rays(16, 16)
+    got = orc.trace_raybvh(v9, nodes, roots, order, rays, p, orc.CLOSEST_PRIMARY)
+    ids, ts = _strict_brute(v9, rays)
+    np.testing.assert_array_equal(got["id"], ids)
+    np.testing.assert_array_equal(got["t"], ts)
+    assert (ids >= 0).sum() > 20
+    sec = scenes.random_rays(256, seed=42, lo=-1.5, hi=1.5, maxt=np.inf, width=16)
+    sec["valid"] = (np.arange(256) % 5 != 0).astype(np.uint8)
+    got = orc.trace_raybvh(v9, nodes, roots, order, sec, p, orc.CLOSEST_SECONDARY)
+    ids, ts = _strict_brute(v9, sec, valid_only=True)
+    np.testing.assert_array_equal(got["id"], ids)
+    assert (ids >= 0).sum() > 10
+    sh = scenes.random_rays(256, seed=43, lo=-1.5, hi=1.5, maxt=2.0, width=16)
+    got = orc.trace_raybvh(v9, nodes, roots, order, sh, p, orc.OCCLUDED)
+    exp = _strict_occ_brute(v9, sh)
+    np.testing.assert_array_equal(got["occ"], exp)
+    assert 10 < exp.sum() < 246
+
+
+def _paired_scene(n, seed):
+    """n triangles plus, for each, its point reflection through its bounding-box centre: both
+    have the same BVH centroid bit for bit (dyadic coordinates), so SAH leaves hold pairs."""
+    g = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        c = g.integers(-256, 256, size=3) / 128.0
+        off = g.integers(-24, 25, size=(3, 3)) / 256.0
+        off -= (off.min(0) + off.max(0)) / 2  # bounding-box centre at c exactly
+        a = c + off
+        b = c - off
+        out.append(a.reshape(-1))
+        out.append(b.reshape(-1))
+    return np.array(out, np.float32)
+
+
+def test_raybvh_secondary_first_primitive_rule(orc, scenes):
+    # intersection6DB_BVH2.cl tests only the first primitive of each BVH leaf for secondary
+    # rays.  With maxBVHPrim 4 and coincident centroids the leaves hold two triangles: the
+    # secondary result is the exhaustive strict nearest hit over first primitives only (IA
+    # keeps every candidate), while primary rays see every primitive.
+    v9 = _paired_scene(40, seed=44)
+    nodes, roots, order = orc.raybvh_handoff(v9, max_prims=4, bvh_height=3)
+    leaves = nodes[nodes["n_primitives"] > 0]
+    assert (leaves["n_primitives"] >= 2).sum() >= 20
+    first = np.array(sorted(int(order[lf["prim_offset"]]) for lf in leaves))
+    p = orc.params("ray-bvh", node="ia")
+    rays = scenes.random_rays(256, seed=45, lo=-2.0, hi=2.0, maxt=np.inf, width=16)
+    got = orc.trace_raybvh(v9, nodes, roots, order, rays, p, orc.CLOSEST_SECONDARY)
+    sub_ids, sub_t = _strict_brute(v9[first], rays, valid_only=True)
+    exp = np.where(sub_ids >= 0, first[np.maximum(sub_ids, 0)], -1)
+    np.testing.assert_array_equal(got["id"], exp)
+    all_ids, _ = _strict_brute(v9, rays)
+    assert (all_ids != exp).sum() > 0, "the rule must change some result here"
+    got0 = orc.trace_raybvh(v9, nodes, roots, order, rays, p, orc.CLOSEST_PRIMARY)
+    np.testing.assert_array_equal(got0["id"], all_ids)
+
+
+def _rays_of(o, d, W, H, tmax=np.inf):
+    o = np.asarray(o, np.float32).reshape(-1, 3)
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    n = W * H
+    return dict(ox=o[:, 0].copy(), oy=o[:, 1].copy(), oz=o[:, 2].copy(), dx=d[:, 0].copy(),
+                dy=d[:, 1].copy(), dz=d[:, 2].copy(), tmin=np.zeros(n, np.float32),
+                tmax=np.full(n, tmax, np.float32), valid=None, width=W, height=H)
+
+
+def test_rh_chain_root_cull_rejects_leaf_pass(orc):
+    # Hand-built: 2x2 rays, 1x1 leaves, height 2 (four leaves under one root).  Ray 0 starts at
+    # the origin and points along +z at triangle A (z = 5 around the axis): its leaf accepts A
+    # (the vertex directions straddle x and y = 0).  The other three rays start at
+    # (100, 100, 0), so the root's origin box is [0,100]^2 x {0} with centre (50, 50, 0);
+    # seen from there A's vertex directions have x, y in (-0.72, -0.69) -- no straddling axis
+    # and no overlap with the root's direction box (0, 0, 1) -- so intersectsTri rejects A at
+    # the root and ray 0 misses although it hits A (intersection6DB.cl:51-119 culls at every
+    # level from the top).
+    A = tri((-1, -1, 5), (1, -1, 5), (0, 1, 5))
+    v = np.array([A], np.float32)
+    o = [(0, 0, 0), (100, 100, 0), (100, 100, 0), (100, 100, 0)]
+    d = [(0, 0, 1)] * 4
+    rays = _rays_of(o, d, 2, 2)
+    assert orc.mt("strict", (0, 0, 0), (0, 0, 1), A)[0]
+    leaf = node16((0, 0, 1), (0, 0, 1), (0, 0, 0), (0, 0, 0))
+    root = node16((0, 0, 1), (0, 0, 1), (0, 0, 0), (100, 100, 0))
+    assert orc.cull("tri", leaf, v9=A)
+    assert not orc.cull("tri", root, v9=A)
+    p = orc.params("rayhierarchy", cx=1, cy=1, height=2, repair_run=0)
+    got = orc.trace(v, rays, p, orc.CLOSEST_PRIMARY)
+    assert got["id"][0] == -1
+    # the same ray alone (its own root) finds A
+    alone = _rays_of([(0, 0, 0)], [(0, 0, 1)], 1, 1)
+    assert orc.trace(v, alone, p, orc.CLOSEST_PRIMARY)["id"][0] == 0
+
+
+def test_rh_chain_leaf_cull_rejects_root_pass(orc):
+    # Hand-built: the non-conservative example of intersectsTri at the leaf.  Ray 0 points at
+    # the interior point (0.6, 0.45, 0.45) of a triangle in the plane x = 0.6; its direction's
+    # x component 0.687 lies above every vertex direction's (at most 0.6), so its 1x1 leaf
+    # rejects the triangle.  The other rays point at the vertices, so the root's direction box
+    # covers the triangle's and the root accepts it: the ray misses because of the leaf level.
+    B = tri((0.6, 0.8, 0.0), (0.6, 0.0, 0.8), (0.6, 0.8, 0.8))
+    v = np.array([B], np.float32)
+    dirs = np.array([(0.6, 0.45, 0.45), (0.6, 0.8, 0.0), (0.6, 0.0, 0.8), (0.6, 0.8, 0.8)],
+                    np.float64)
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    rays = _rays_of([(0, 0, 0)] * 4, dirs, 2, 2)
+    assert orc.mt("strict", (0, 0, 0), dirs[0], B)[0]
+    lo, hi = dirs.min(0), dirs.max(0)
+    assert orc.cull("tri", node16(lo, hi, (0, 0, 0), (0, 0, 0)), v9=B)
+    assert not orc.cull("tri", node16(dirs[0], dirs[0], (0, 0, 0), (0, 0, 0)), v9=B)
+    p = orc.params("rayhierarchy", cx=1, cy=1, height=2, repair_run=0)
+    got = orc.trace(v, rays, p, orc.CLOSEST_PRIMARY)
+    assert got["id"][0] == -1
+    # with 2x2 leaves (one leaf holding all four rays) the leaf is the root: the ray hits
+    p1 = orc.params("rayhierarchy", cx=2, cy=2, height=1, repair_run=0)
+    assert orc.trace(v, rays, p1, orc.CLOSEST_PRIMARY)["id"][0] == 0
+
+
+def _node_at(levels, nodes, l, k, j):
+    off = sum(w * h for w, h in levels[:l])
+    return nodes[off + j * levels[l][0] + k]
+
+
+def _flat_result(orc, v9, rays, mode, cx, cy, candidate):
+    """Nearest strict hit / occlusion over an explicit candidate set candidate[(k, j, tri)]."""
+    W, H = rays["width"], rays["height"]
+    n = W * H
+    ids = np.full(n, -1, np.int32)
+    ts = np.full(n, np.inf, np.float32)
+    occ = np.zeros(n, np.uint8)
+    for i in range(n):
+        if mode != orc.CLOSEST_PRIMARY and rays.get("valid") is not None and not rays["valid"][i]:
+            continue
+        x, y = i % W, i // W
+        o = (rays["ox"][i], rays["oy"][i], rays["oz"][i])
+        dd = (rays["dx"][i], rays["dy"][i], rays["dz"][i])
+        for k in range(len(v9)):
+            if not candidate.get((x // cx, y // cy, k), False):
+                continue
+            ok, (t, _, _) = orc.mt("strict", o, dd, v9[k])
+            if not ok:
+                continue
+            if mode == orc.OCCLUDED:
+                if rays["tmin"][i] < t < rays["tmax"][i]:
+                    occ[i] = 1
+            elif t > rays["tmin"][i] and t < ts[i]:
+                ts[i], ids[i] = t, k
+    return ids, ts, occ
+
+
+def _wide_rays(W, H, seed):
+    # origins spread over a box and directions over a cone, so that nodes of different levels
+    # see a triangle from different origin-box centres
+    g = np.random.default_rng(seed)
+    n = W * H
+    o = g.uniform(-1.0, 1.0, size=(n, 3)) * [1.0, 1.0, 0.2] + [0, 0, -3]
+    d = g.normal(size=(n, 3)) * 0.35 + [0, 0, 1]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return _rays_of(o.astype(np.float32), d.astype(np.float32), W, H)
+
+
+def _with_valid(rays, seed):
+    rays["valid"] = (np.random.default_rng(seed).random(rays["width"] * rays["height"])
+                     < 0.8).astype(np.uint8)
+    return rays
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_rh_boxdir_chain_flat_restatement(orc, mode):
+    # rayhierarchy box_dir: a triangle is tested at a leaf iff every node from the top level
+    # down to that leaf accepts it (intersectsTri for primary rays, intersectsNode otherwise;
+    # intersection6DB.cl:51-119 and variants), and the traversal evaluates a node's cull iff
+    # every ancestor accepted (and, for secondary and shadow rays, the node holds a valid
+    # ray).  Both are evaluated here flat, per (triangle, node), with the single-node
+    # predicates pinned above: the oracle's depth-first traversal must give the same hits and
+    # the same cull and leaf-pair counts.  Primary rays with spread origins must also give
+    # pairs that only an interior level rejects and pairs that only the leaf rejects.
+    v9 = np.array([tri((-1, -1, 5), (1, -1, 5), (0, 1, 5))], np.float32)
+    g = np.random.default_rng(46)
+    v9 = np.concatenate([v9, (g.uniform(-1.2, 1.2, size=(39, 1, 3)) + [0, 0, 1.5]
+                              + g.uniform(-0.4, 0.4, size=(39, 3, 3))).reshape(-1, 9)
+                        .astype(np.float32)])
+    W = H = 8
+    rays = _wide_rays(W, H, seed=47)
+    if mode != 0:
+        rays = _with_valid(rays, 7)
+    if mode == 2:
+        rays["tmax"] = np.full(W * H, 4.0, np.float32)
+    cx = cy = 2
+    use_valid = mode != 0
+    levels, nodes = orc.rh_build(rays, cx=cx, cy=cy, height=3, use_valid=use_valid)
+    L = len(levels)
+
+    def cull(nd, k):
+        if mode == 0:
+            return orc.cull("tri", nd, v9=v9[k])
+        t = v9[k].reshape(3, 3)
+        return orc.cull("node6", nd, bmin=t.min(0), bmax=t.max(0))
+
+    culls = pairs = interior_only = leaf_only = 0
+    candidate = {}
+    for k in range(len(v9)):
+        passed = {}
+        for l in range(L - 1, -1, -1):
+            gw, gh = levels[l]
+            for j in range(gh):
+                for kk in range(gw):
+                    above = l == L - 1 or passed.get((l + 1, kk >> 1, j >> 1), False)
+                    nd = _node_at(levels, nodes, l, kk, j)
+                    if not above or (use_valid and nd[3] == 0):
+                        continue
+                    culls += 1
+                    passed[(l, kk, j)] = cull(nd, k)
+        for j in range(levels[0][1]):
+            for kk in range(levels[0][0]):
+                ok = passed.get((0, kk, j), False)
+                candidate[(kk, j, k)] = ok
+                pairs += ok
+                nd0 = _node_at(levels, nodes, 0, kk, j)
+                if use_valid and nd0[3] == 0:
+                    continue
+                chain = [cull(_node_at(levels, nodes, l, kk >> l, j >> l), k) for l in range(L)]
+                interior_only += chain[0] and not all(chain[1:])
+                leaf_only += (not chain[0]) and all(chain[1:])
+    ids, ts, occ = _flat_result(orc, v9, rays, mode, cx, cy, candidate)
+    p = orc.params("rayhierarchy", cx=cx, cy=cy, height=3, repair_run=0)
+    got = orc.trace(v9, rays, p, mode)
+    if mode == 2:
+        np.testing.assert_array_equal(got["occ"], occ)
+    else:
+        np.testing.assert_array_equal(got["id"], ids)
+        np.testing.assert_array_equal(got["t"], ts)
+    assert got["stats"].cull_tests == culls
+    assert got["stats"].leaf_pairs == pairs
+    if mode == 0:
+        assert interior_only > 0 and leaf_only > 0, (interior_only, leaf_only)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_raybvh_lockstep_chain_flat_restatement(orc, scenes, mode):
+    # ray-bvh box_dir: the product traversal (intersection6DB_BVH.cl:51-223) visits the pairs
+    # (hierarchy node R, BVH node B) step by step from (top node, frontier root): step s pairs
+    # the hierarchy node of level max(top - s, 0) with the BVH node of depth min(s, d) (d the
+    # depth of B's leaf below its root), so a pair's parent is the pair one step up, and a pair
+    # is visited (and culled with intersectsNode) iff its parent was visited and accepted (and
+    # R holds a valid ray for secondary and shadow rays).  Here the visited set is grown
+    # breadth first from those rules and the (hierarchy leaf, BVH leaf) pairs reached set the
+    # candidates; the oracle's recursive traversal must give the same hits and counts.
+    v9 = scenes.random_triangles(48, seed=48, extent=1.0, size=0.4)
+    nodes, roots, order = orc.raybvh_handoff(v9, max_prims=1, bvh_height=3)
+    W = H = 8
+    rays = _wide_rays(W, H, seed=49)
+    if mode != 0:
+        rays = _with_valid(rays, 8)
+    if mode == 2:
+        rays["tmax"] = np.full(W * H, 4.0, np.float32)
+    cx = cy = 2
+    use_valid = mode != 0
+    levels, rnodes = orc.rh_build(rays, cx=cx, cy=cy, height=3, use_valid=use_valid)
+    top = len(levels) - 1
+
+    def children(l, k, j, b):
+        """Pairs one step below (l, k, j, b)."""
+        nb = nodes[b]
+        rk = [(l - 1, ck, cj) for cj in (2 * j, 2 * j + 1) for ck in (2 * k, 2 * k + 1)
+              if l > 0 and ck < levels[l - 1][0] and cj < levels[l - 1][1]] if l > 0 else [(l, k, j)]
+        bk = [b] if nb["n_primitives"] else [int(nb["prim_offset"]), int(nb["prim_offset"]) + 1]
+        return [(r[0], r[1], r[2], c) for r in rk for c in bk]
+
+    culls = 0
+    candidate = {}
+    frontier = [(top, k, j, int(r)) for r in roots for j in range(levels[top][1])
+                for k in range(levels[top][0])]
+    while frontier:
+        nxt = []
+        for (l, k, j, b) in frontier:
+            nd = _node_at(levels, rnodes, l, k, j)
+            if use_valid and nd[3] == 0:
+                continue
+            culls += 1
+            box = nodes[b]
+            if not orc.cull("node6", nd, bmin=(box["ax"], box["ay"], box["az"]),
+                            bmax=(box["bx"], box["by"], box["bz"])):
+                continue
+            if l == 0 and box["n_primitives"]:
+                cnt = 1 if mode == 1 else int(box["n_primitives"])
+                for f in range(cnt):
+                    candidate[(k, j, int(order[box["prim_offset"] + f]))] = True
+                continue
+            nxt.extend(children(l, k, j, b))
+        frontier = nxt
+    ids, ts, occ = _flat_result(orc, v9, rays, mode, cx, cy, candidate)
+    p = orc.params("ray-bvh", cx=cx, cy=cy, height=3)
+    got = orc.trace_raybvh(v9, nodes, roots, order, rays, p, mode)
+    if mode == 2:
+        np.testing.assert_array_equal(got["occ"], occ)
+    else:
+        np.testing.assert_array_equal(got["id"], ids)
+        np.testing.assert_array_equal(got["t"], ts)
+    assert got["stats"].cull_tests == culls
+    assert (ids >= 0).sum() > 5 if mode != 2 else occ.sum() > 5
+
+
+# ---------------------------------------------------------------- parity bookkeeping flags
+def test_flags_near_tie(orc):
+    # two triangles parallel to T0 at z = 2 and z = 2 * (1 + d): the ray along +z from z = 0
+    # sees both; d = 2^-21 (5e-7) is a near tie (within 1e-6 * t), d = 2^-14 is not
+    def scene(dz):
+        far = tri((0, 0, 2 + 2 * dz), (4, 0, 2 + 2 * dz), (0, 4, 2 + 2 * dz))
+        return np.array([T0, far], np.float32)
+    rays = _rays_of([(1, 1, 0)], [(0, 0, 1)], 1, 1)
+    p = orc.params("rayhierarchy", height=1)
+    got = orc.trace(scene(2.0 ** -21), rays, p, orc.CLOSEST_PRIMARY, flags=True)
+    assert got["id"][0] == 0 and got["flags"][0] & orc.NEAR_TIE
+    got = orc.trace(scene(2.0 ** -14), rays, p, orc.CLOSEST_PRIMARY, flags=True)
+    assert got["id"][0] == 0 and not got["flags"][0] & orc.NEAR_TIE
+    # identical triangles: an exact tie
+    got = orc.trace(np.array([T0, T0], np.float32), rays, p, orc.CLOSEST_PRIMARY, flags=True)
+    assert got["id"][0] == 0 and got["flags"][0] & orc.NEAR_TIE
+
+
+def test_flags_ulp_marginal(orc):
+    # Direction (1/16, 1/16, 1) keeps every operand dyadic.  The ray from (-1/8, 1, 0) meets
+    # T0 at (0, 9/8, 2), on the edge b1 == 0: the rejection rests on a numerator that is
+    # exactly 0, so the ray is marginal (any other rounding could accept it).  The ray from
+    # (1, 1, 0) meets the interior (b1 = b2 = 9/32): not marginal; neither is the edge ray
+    # behind a robust nearer hit.
+    d = (1 / 16, 1 / 16, 1)
+    p = orc.params("rayhierarchy", node="ia", height=1)
+    edge = _rays_of([(-1 / 8, 1, 0)], [d], 1, 1)
+    assert not orc.mt("strict", (-1 / 8, 1, 0), d, T0)[0]
+    got = orc.trace(np.array([T0], np.float32), edge, p, orc.CLOSEST_PRIMARY, flags=True)
+    assert got["id"][0] == -1 and got["flags"][0] & orc.ULP_MARGINAL
+    inner = _rays_of([(1, 1, 0)], [d], 1, 1)
+    got = orc.trace(np.array([T0], np.float32), inner, p, orc.CLOSEST_PRIMARY, flags=True)
+    assert got["id"][0] == 0 and got["flags"][0] == 0
+    near = tri((-8, -8, 1), (8, -8, 1), (0, 8, 1))  # robust hit at t = 1 in front of T0
+    got = orc.trace(np.array([near, T0], np.float32), edge, p, orc.CLOSEST_PRIMARY, flags=True)
+    assert got["id"][0] == 0 and got["flags"][0] == 0
+    # occlusion: the edge ray is marginal unless something else occludes it robustly
+    sh = _rays_of([(-1 / 8, 1, 0)], [d], 1, 1, tmax=10.0)
+    got = orc.trace(np.array([T0], np.float32), sh, p, orc.OCCLUDED, flags=True)
+    assert got["occ"][0] == 0 and got["flags"][0] & orc.ULP_MARGINAL
+    got = orc.trace(np.array([near, T0], np.float32), sh, p, orc.OCCLUDED, flags=True)
+    assert got["occ"][0] == 1 and got["flags"][0] == 0
+    # a hit exactly at tmax is a marginal occlusion decision
+    at = _rays_of([(1, 1, 0)], [d], 1, 1, tmax=2.0)
+    got = orc.trace(np.array([T0], np.float32), at, p, orc.OCCLUDED, flags=True)
+    assert got["occ"][0] == 0 and got["flags"][0] & orc.ULP_MARGINAL
+
+
+def test_flags_marginal_cull_descends(orc):
+    # The IA cull of a node whose direction interval touches 0 exactly (d.x == 0) is decided
+    # on its threshold: a hit below it is marginal even though the cull accepts.  With d.x =
+    # 1/16 the straddle test is robust.
+    v = np.array([T0], np.float32)
+    p = orc.params("rayhierarchy", node="ia", height=1)
+    r0 = _rays_of([(1, 1, 0)], [(1 / 16, 1 / 4, 1)], 1, 1)
+    got = orc.trace(v, r0, p, orc.CLOSEST_PRIMARY, flags=True)
+    assert got["id"][0] == 0 and got["flags"][0] == 0
+    r1 = _rays_of([(1, 1, 0)], [(0, 1 / 4, 1)], 1, 1)
+    got = orc.trace(v, r1, p, orc.CLOSEST_PRIMARY, flags=True)
+    assert got["id"][0] == 0 and got["flags"][0] & orc.ULP_MARGINAL
+
+
+def test_flags_do_not_change_results(orc, scenes):
+    v9 = scenes.random_triangles(200, seed=50, extent=1.0, size=0.3)
+    rays = scenes.pinhole_rays(24, 24)
+    for accel in ("rayhierarchy", "ray-bvh"):
+        p = orc.params(accel)
+        if accel == "ray-bvh":
+            nodes, roots, order = orc.raybvh_handoff(v9)
+            run = lambda r, m, f: orc.trace_raybvh(v9, nodes, roots, order, r, p, m, flags=f)
+        else:
+            run = lambda r, m, f: orc.trace(v9, r, p, m, flags=f)
+        a, b = run(rays, orc.CLOSEST_PRIMARY, False), run(rays, orc.CLOSEST_PRIMARY, True)
+        np.testing.assert_array_equal(a["id"], b["id"])
+        np.testing.assert_array_equal(a["t"], b["t"])
+        assert a["stats"].cull_tests == b["stats"].cull_tests
+        assert a["stats"].leaf_pairs == b["stats"].leaf_pairs
