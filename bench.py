@@ -109,6 +109,26 @@ def bench_uneven(args, rt, scenes, torch):
     stream.synchronize()
     k_closest = b.kernel_time(rt.RT_CLOSEST_PRIMARY)[0]
     k_occl = b.kernel_time(2)[0]
+    split = {"closest_unit": round(b.kernel_time(rt.RT_CLOSEST_PRIMARY + rt.RT_TIME_UNIT)[0], 3),
+             "closest_per_ray": round(b.kernel_time(rt.RT_CLOSEST_PRIMARY + rt.RT_TIME_PER_RAY)[0], 3),
+             "occlusion_unit": round(b.kernel_time(2 + rt.RT_TIME_UNIT)[0], 3),
+             "occlusion_per_ray": round(b.kernel_time(2 + rt.RT_TIME_PER_RAY)[0], 3)}
+    sweep = {}
+    for thr in args.ray_sweep:
+        b.set_param(rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, thr)
+        b.set_param(rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED, thr)
+        b.run(stream=stream)
+        b.run(stream=stream)
+        stream.synchronize()
+        b.set_timing(True)
+        b.run(stream=stream)
+        stream.synchronize()
+        sweep[thr] = {k: round(b.kernel_time(m)[0], 3) for k, m in (
+            ("closest", 0), ("closest_per_ray", 0 + rt.RT_TIME_PER_RAY),
+            ("occlusion", 2), ("occlusion_per_ray", 2 + rt.RT_TIME_PER_RAY))}
+        b.set_timing(False)
+    b.set_param(rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, 0)
+    b.set_param(rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED, 0)
     b.set_timing(False)
     mism_id = sum(int((hits_b[k].id != hits_p[k].id).sum().item()) for k in range(len(jobs)))
     mism_t = sum(int((hits_b[k].t.view(torch.int32) != hits_p[k].t.view(torch.int32)).sum().item())
@@ -132,7 +152,8 @@ def bench_uneven(args, rt, scenes, torch):
         "plain": {"value": round(v_plain, 3), "ms_per_step": round(t_plain * 1e3, 3)},
         "batched_over_plain": round(v_batch / v_plain, 3),
         "kernels_ms": {"closest_traversal": round(k_closest, 3),
-                       "occlusion_traversal": round(k_occl, 3)},
+                       "occlusion_traversal": round(k_occl, 3), "split": split,
+                       "ray_threshold_sweep": sweep},
         "parity": {"id_mismatches": mism_id, "t_mismatches": mism_t,
                    "occlusion_mismatches": mism_occ},
         "roofline": None, "cpu_baseline": None,
@@ -358,6 +379,12 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--autotune", action="store_true",
                     help="uneven_batch_64: autotune the batch's launch shape before timing")
+    ap.add_argument("--ray-closest", type=int, default=0,
+                    help="per-ray threshold (passing clusters) of closest-hit traces; 0 = built-in")
+    ap.add_argument("--ray-occluded", type=int, default=0,
+                    help="per-ray threshold of occlusion traces; 0 = built-in")
+    ap.add_argument("--ray-sweep", type=lambda v: [int(x) for x in v.split(",") if x],
+                    default=[], help="uneven_batch_64: per-ray thresholds to time")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed steps, run one plain device copy of this many MB "
                          "(calibrates profiler byte counters against a kernel of known traffic)")
@@ -389,6 +416,10 @@ def main():
 
     ctx = rt.Context(0)
     sc = rt.Scene(ctx, v9, desc=accel)
+    if args.ray_closest:
+        sc.set_param(rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, args.ray_closest)
+    if args.ray_occluded:
+        sc.set_param(rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED, args.ray_occluded)
     prim = rt.DeviceRays.from_numpy(rays_np)
     n = prim.count
     hits = rt.DeviceHits.empty(n)

@@ -166,11 +166,13 @@ int rt_make_bounce_rays(rt_scene *scene, const rt_rays *in, const int32_t *id, c
  *   unchanged (batches only; needs a non-null stream).
  *   RT_PARAM_RAY_CLUSTERS_CLOSEST / RT_PARAM_RAY_CLUSTERS_OCCLUDED: a block (top-level
  *   hierarchy node) whose root passes at least this many triangle clusters (ray-bvh: clusters
- *   of frontier roots) is traced ray by ray -- each ray walks a conservative BVH of the
- *   triangles and the reference's exact cull chain is evaluated for the triangles that pass
- *   its strict leaf test -- in place of enumerating the block's (leaf, triangle) candidate
- *   pairs.  1 sends every block that way; a value above the scene's cluster count (at most
- *   2^30) never does. */
+ *   of frontier roots), or more than half of the scene's clusters, is traced ray by ray --
+ *   each ray walks a conservative BVH of the triangles and the reference's exact cull chain
+ *   is evaluated for the triangles that pass its strict leaf test -- in place of enumerating
+ *   the block's (leaf, triangle) candidate pairs (built-in: 256).  1 sends every block with
+ *   work that way; a value above 2^29 (at most 2^30) never does.
+ *   RT_PARAM_RAY_GRID_CLOSEST / RT_PARAM_RAY_GRID_OCCLUDED: work-groups (one wavefront each)
+ *   of the per-ray kernel (batches only); 0 = one per resident slot. */
 enum {
   RT_PARAM_CLUSTERS_CLOSEST = 1,
   RT_PARAM_CLUSTERS_OCCLUDED = 2,
@@ -178,7 +180,9 @@ enum {
   RT_PARAM_GRID_OCCLUDED = 4,
   RT_PARAM_GRAPH = 5,
   RT_PARAM_RAY_CLUSTERS_CLOSEST = 6,
-  RT_PARAM_RAY_CLUSTERS_OCCLUDED = 7
+  RT_PARAM_RAY_CLUSTERS_OCCLUDED = 7,
+  RT_PARAM_RAY_GRID_CLOSEST = 8,
+  RT_PARAM_RAY_GRID_OCCLUDED = 9
 };
 int rt_scene_set_param(rt_scene *scene, int32_t param, int32_t value);
 
@@ -213,8 +217,10 @@ int rt_batch_add_occluded(rt_batch *batch, rt_scene *scene, const rt_rays *rays,
 int rt_batch_run(rt_batch *batch, void *stream);
 int rt_batch_set_param(rt_batch *batch, int32_t param, int32_t value);
 int rt_batch_get_param(rt_batch *batch, int32_t param, int32_t *value);
-/* Autotuner: times the batch's current jobs over the launch-shape candidates (clusters per
- * work unit x persistent grid, per mode), `reps` runs each, and keeps the fastest. */
+/* Autotuner: times the batch's current jobs over the launch-shape candidates, `reps` runs
+ * each, and keeps the fastest per mode: first the per-ray routing threshold x per-ray grid,
+ * then clusters per work unit x persistent grid of the unit kernel.  No candidate changes a
+ * result. */
 int rt_batch_autotune(rt_batch *batch, void *stream, int32_t reps);
 /* Event timing of the batch's traversal kernels, as rt_scene_set_timing / kernel_time. */
 int rt_batch_set_timing(rt_batch *batch, int32_t enable);
@@ -224,10 +230,12 @@ int rt_batch_kernel_time(rt_batch *batch, int32_t mode, float *total_ms, int32_t
  * camera image (width*spp x height rays, the spp samples of a pixel adjacent in x; pixel
  * centres for spp = 1, hashed jitter from `seed` otherwise) with one shadow ray per hit toward
  * a point light.  Requests are admitted in arrival order while their device memory fits the
- * budget and fewer than max_active are in flight; every tick of the loop runs ONE batch that
- * holds the primary rays of newly admitted requests and the shadow rays of requests whose
- * primaries finished in the previous tick (continuous batching), so work of requests that
- * arrive at different times overlaps on the device.  Each request's result is the same
+ * budget and fewer than max_active are in flight; every tick of the loop launches ONE batch
+ * that holds the primary rays of newly admitted requests and the shadow rays of requests whose
+ * primaries ran in the previous tick (continuous batching), so work of requests that arrive
+ * at different times overlaps on the device.  The loop prepares the next tick (admission,
+ * device blocks from a pool allocated at creation, camera rays, job tables) while the current
+ * one runs, and allocates no device memory for a request.  Each request's result is the same
  * alone and in any mix.  No counterpart in the reference (the breadth-first renderer traces
  * one image task at a time, breadth-first.cpp:134-153); it is for render services. */
 typedef struct rt_server rt_server;
@@ -248,11 +256,16 @@ typedef struct rt_request_stats {
   uint64_t primary_rays;             /* camera rays; one shadow ray slot per camera ray */
   int32_t status;                   /* rt_status of the request */
 } rt_request_stats;
-/* scenes: rayhierarchy scenes that stay resident for the server's lifetime.
- * mem_budget: device bytes the in-flight requests may hold (0 = half the free memory). */
+/* scenes: rayhierarchy or ray-bvh scenes (with their BVH) that stay resident for the
+ * server's lifetime; any other scene gives RT_ERR_UNSUPPORTED.
+ * mem_budget: device bytes the in-flight requests may hold (0 = half the free memory, at most
+ * 32 GiB); the server allocates a pool of that size for the requests' ray and result
+ * buffers. */
 int rt_server_create(rt_ctx *ctx, rt_scene *const *scenes, int32_t nscenes, uint64_t mem_budget,
                      int32_t max_active, rt_server **out);
-/* Queues a request (thread-safe, returns at once). */
+/* Queues a request (thread-safe, returns at once).  A request its scene cannot trace (the
+ * ray image's hierarchy shape) fails here with that status, and so does one whose buffers
+ * exceed the pool (RT_ERR_OOM); a queued request cannot fail because of another. */
 int rt_server_submit(rt_server *server, const rt_request *req, uint64_t *ticket);
 /* Waits for a request (timeout_s < 0: no limit); RT_ERR_STATE on time-out. */
 int rt_server_wait(rt_server *server, uint64_t ticket, double timeout_s, rt_request_stats *stats);

@@ -283,3 +283,47 @@ def exact_closest(v9, rays: dict):
     lib().orc_exact_closest(_fp(v9.ctypes.data), ctypes.c_uint32(v9.size // 9), ctypes.byref(keep.s),
                             _fp(ids.ctypes.data), _fp(ts.ctypes.data))
     return dict(id=ids, t=ts)
+
+
+class OrcSearchStats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_uint64), ("visits", ctypes.c_uint64),
+                ("tests", ctypes.c_uint64), ("culls", ctypes.c_uint64),
+                ("nodes", ctypes.c_uint64)]
+
+
+def search_trace(v9, rays: dict, p: OrcParams, mode: int, handoff=None, mask=None, nthreads=0):
+    """The library's per-ray path restated on the CPU (search BVH traversal + strict leaf test
+    + exact cull chain).  handoff = (nodes, roots, prim_order) for ray-bvh.  Returns the trace
+    outputs and stats (rays, visits, tests, culls, nodes)."""
+    v9 = _f32(v9).reshape(-1)
+    ntri = v9.size // 9
+    keep = _RayKeep(rays)
+    n = keep.n
+    st = OrcSearchStats()
+    if handoff is not None:
+        nodes = np.ascontiguousarray(handoff[0], dtype=GPU_NODE_DTYPE)
+        roots = np.ascontiguousarray(handoff[1], dtype=np.uint32)
+        order = np.ascontiguousarray(handoff[2], dtype=np.uint32)
+        hp = (_fp(nodes.ctypes.data), ctypes.c_uint32(nodes.size), _fp(roots.ctypes.data),
+              ctypes.c_uint32(roots.size), _fp(order.ctypes.data))
+    else:
+        hp = (None, ctypes.c_uint32(0), None, ctypes.c_uint32(0), None)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    out = dict(id=np.zeros(n, np.int32), t=np.zeros(n, np.float32), b1=np.zeros(n, np.float32),
+               b2=np.zeros(n, np.float32), occ=np.zeros(n, np.uint8))
+    closest = mode != OCCLUDED
+    rc = lib().orc_search_trace(
+        _fp(v9.ctypes.data), ctypes.c_uint32(ntri), *hp, ctypes.byref(keep.s), ctypes.byref(p),
+        ctypes.c_int32(mode), None if m is None else _fp(m.ctypes.data), ctypes.c_int32(nthreads),
+        _fp(out["id"].ctypes.data) if closest else None,
+        _fp(out["t"].ctypes.data) if closest else None,
+        _fp(out["b1"].ctypes.data) if closest else None,
+        _fp(out["b2"].ctypes.data) if closest else None,
+        None if closest else _fp(out["occ"].ctypes.data), ctypes.byref(st))
+    assert rc == 0
+    if closest:
+        del out["occ"]
+    else:
+        out = dict(occ=out["occ"])
+    out["stats"] = st
+    return out

@@ -1057,6 +1057,303 @@ void parallel_for(size_t n, int nthreads, F&& fn) {
   for (auto& t : th) t.join();
 }
 
+// ---- per-ray path restatement ------------------------------------------------------------
+// The library traces incoherent ray groups ray by ray: each ray walks a search BVH of the
+// triangles (Morton order of centroids, radix splits, leaves of at most 4, boxes grown by
+// 2^-11 * max(1, largest |coordinate|)), runs the strict leaf test on the triangles of the
+// leaves it reaches and evaluates the reference's cull chain for the ones that pass.  This is
+// a CPU restatement of that traversal (not of the reference): its results must equal the
+// reference-semantics traces above, and its per-ray counts (interior nodes loaded, leaf
+// tests, chain culls) are the work model of the per-ray kernel.
+constexpr int kSDepth = 30;
+constexpr int kSLeaf = 4;
+
+uint32_t spread10(uint32_t x) {
+  x &= 1023u;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+struct SBoxC {
+  float lo[3], hi[3];
+};
+struct SNodeC {
+  SBoxC b[2];
+  uint32_t ref[2];
+};
+
+struct SearchTree {
+  std::vector<uint32_t> perm;  // sorted slot -> triangle id
+  std::vector<Tri> tri;        // sorted order
+  std::vector<SNodeC> nodes;
+};
+
+int s_levels(uint32_t n) {
+  int l = 0;
+  while (n > (uint32_t)kSLeaf) {
+    n = (n + 1) / 2;
+    ++l;
+  }
+  return l;
+}
+
+void s_tribox(const SearchTree& T, const float* v9, uint32_t a, uint32_t b, SBoxC& box) {
+  for (int k = 0; k < 3; ++k) {
+    box.lo[k] = kInf;
+    box.hi[k] = -kInf;
+  }
+  for (uint32_t i = a; i < b; ++i) {
+    const float* p = v9 + 9 * (size_t)T.perm[i];
+    for (int v = 0; v < 3; ++v)
+      for (int k = 0; k < 3; ++k) {
+        box.lo[k] = std::min(box.lo[k], p[3 * v + k]);
+        box.hi[k] = std::max(box.hi[k], p[3 * v + k]);
+      }
+  }
+}
+
+uint32_t s_build(SearchTree& T, const float* v9, const std::vector<uint32_t>& code, uint32_t a,
+                 uint32_t b, int depth, float pad, SBoxC& box) {
+  const uint32_t n = b - a;
+  if (n <= (uint32_t)kSLeaf) {
+    s_tribox(T, v9, a, b, box);
+    return 0x80000000u | ((n - 1) << 27) | a;
+  }
+  const uint32_t node = (uint32_t)T.nodes.size();
+  T.nodes.emplace_back();
+  uint32_t m = a + n / 2;
+  if (code[a] != code[b - 1]) {
+    const int bit = 31 - __builtin_clz(code[a] ^ code[b - 1]);
+    uint32_t lo = a, hi = b - 1;
+    while (lo < hi) {
+      const uint32_t md = lo + (hi - lo) / 2;
+      if (code[md] & (1u << bit)) hi = md;
+      else lo = md + 1;
+    }
+    if (depth + s_levels(lo - a) <= kSDepth && depth + s_levels(b - lo) <= kSDepth) m = lo;
+  }
+  SBoxC b0, b1;
+  const uint32_t r0 = s_build(T, v9, code, a, m, depth + 1, pad, b0);
+  const uint32_t r1 = s_build(T, v9, code, m, b, depth + 1, pad, b1);
+  SNodeC& nd = T.nodes[node];
+  for (int k = 0; k < 3; ++k) {
+    nd.b[0].lo[k] = b0.lo[k] - pad;
+    nd.b[0].hi[k] = b0.hi[k] + pad;
+    nd.b[1].lo[k] = b1.lo[k] - pad;
+    nd.b[1].hi[k] = b1.hi[k] + pad;
+    box.lo[k] = std::min(b0.lo[k], b1.lo[k]);
+    box.hi[k] = std::max(b0.hi[k], b1.hi[k]);
+  }
+  nd.ref[0] = r0;
+  nd.ref[1] = r1;
+  return node;
+}
+
+SearchTree search_tree(const float* v9, uint32_t ntri) {
+  SearchTree T;
+  std::vector<float> c(3 * (size_t)ntri);
+  float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+  for (uint32_t i = 0; i < ntri; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const float* p = v9 + 9 * (size_t)i;
+      const float v = (p[a] + p[3 + a] + p[6 + a]) * (1.0f / 3.0f);
+      c[3 * (size_t)i + a] = v;
+      lo[a] = std::min(lo[a], v);
+      hi[a] = std::max(hi[a], v);
+    }
+  std::vector<uint32_t> code(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) {
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) {
+      const float ext = hi[a] - lo[a];
+      float f = ext > 0.f ? (c[3 * (size_t)i + a] - lo[a]) / ext : 0.f;
+      f = std::min(std::max(f, 0.f), 1.f);
+      q[a] = (uint32_t)(f * 1023.f);
+    }
+    code[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  }
+  T.perm.resize(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) T.perm[i] = i;
+  std::stable_sort(T.perm.begin(), T.perm.end(),
+                   [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+  std::vector<uint32_t> scode(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) scode[i] = code[T.perm[i]];
+  T.tri.resize(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) T.tri[i] = load_tri(v9, T.perm[i]);
+  float big = 1.f;
+  for (size_t i = 0; i < 9 * (size_t)ntri; ++i) big = std::max(big, std::fabs(v9[i]));
+  const float pad = std::ldexp(big, -11);
+  if (ntri == 0) return T;
+  if (ntri <= (uint32_t)kSLeaf) {
+    const uint32_t m = ntri == 1 ? 1 : ntri / 2;
+    SNodeC nd;
+    s_tribox(T, v9, 0, m, nd.b[0]);
+    nd.ref[0] = 0x80000000u | ((m - 1) << 27);
+    nd.b[1] = nd.b[0];
+    nd.ref[1] = nd.ref[0];
+    if (ntri > 1) {
+      s_tribox(T, v9, m, ntri, nd.b[1]);
+      nd.ref[1] = 0x80000000u | ((ntri - m - 1) << 27) | m;
+    }
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 3; ++k) {
+        nd.b[j].lo[k] -= pad;
+        nd.b[j].hi[k] += pad;
+      }
+    T.nodes.push_back(nd);
+    return T;
+  }
+  SBoxC box;
+  s_build(T, v9, scode, 0, ntri, 1, pad, box);
+  return T;
+}
+
+constexpr float kSlackC = 0x1p-16f;
+inline bool s_slab(V3 o, V3 inv, const SBoxC& b, float tlo, float thi, float& tn) {
+  const float ax = (b.lo[0] - o.x) * inv.x, bx = (b.hi[0] - o.x) * inv.x;
+  const float ay = (b.lo[1] - o.y) * inv.y, by = (b.hi[1] - o.y) * inv.y;
+  const float az = (b.lo[2] - o.z) * inv.z, bz = (b.hi[2] - o.z) * inv.z;
+  tn = std::fmax(std::fmax(std::fmin(ax, bx), std::fmin(ay, by)), std::fmin(az, bz));
+  float tf = std::fmin(std::fmin(std::fmax(ax, bx), std::fmax(ay, by)), std::fmax(az, bz));
+  tf = tf + std::fabs(tf) * kSlackC;
+  return tn <= tf && tn <= thi && tf >= tlo;
+}
+
+struct SearchCtx {
+  const SearchTree* T;
+  const RH* h;
+  int mode, node_kind, kind;
+  bool repair;
+  // ray-bvh: the reference BVH, its leaf of every triangle (bit 31: not first), parents, depths
+  const orc_gpu_node* nodes;
+  std::vector<uint32_t> tleaf, parent;
+  std::vector<uint8_t> depth;
+};
+
+// The reference's chain for (the ray's leaf at image position (x, y), triangle or BVH leaf).
+bool s_chain(const SearchCtx& c, int x, int y, uint32_t tri, uint32_t leaf, uint64_t& culls) {
+  const RH& h = *c.h;
+  const int top = h.L - 1;
+  auto rnode = [&](int l) -> const Node& {
+    return h.at(l, x / (h.cx << l), y / (h.cy << l));
+  };
+  if (c.kind == 1) {
+    const Tri& T = c.T->tri[tri];
+    for (int l = top; l >= 0; --l) {
+      const Node& n = rnode(l);
+      ++culls;
+      if (!n.valid) return false;
+      bool pass = c.node_kind == 1 ? cull_ia(n, T.bmin, T.bmax)
+                                   : (c.mode == ORC_CLOSEST_PRIMARY ? cull_tri(n, T)
+                                                                    : cull_node6(n, T.bmin, T.bmax));
+      if (!pass) return false;
+    }
+    return true;
+  }
+  const int d = c.depth[leaf];
+  uint32_t node = leaf;
+  int dn = d;
+  for (int st = std::max(top, d); st >= 0; --st) {
+    const int bd = std::min(st, d);
+    while (dn > bd) {
+      node = c.parent[node];
+      --dn;
+    }
+    const Node& n = rnode(top - std::min(st, top));
+    ++culls;
+    if (!n.valid) return false;
+    const orc_gpu_node& b = c.nodes[node];
+    const V3 bmin = {b.ax, b.ay, b.az}, bmax = {b.bx, b.by, b.bz};
+    if (!(c.node_kind == 1 ? cull_ia(n, bmin, bmax) : cull_node6(n, bmin, bmax))) return false;
+  }
+  return true;
+}
+
+struct SCount {
+  uint64_t rays = 0, visits = 0, tests = 0, culls = 0;
+};
+
+void s_trace_ray(const SearchCtx& c, const orc_rays* r, size_t i, Out& o, SCount& cnt) {
+  const int x = (int)(i % (size_t)r->W), y = (int)(i / (size_t)r->W);
+  const V3 org = ray_o(r, i), d = ray_d(r, i);
+  const float tmin = r->tmin[i], tmax = r->tmax[i];
+  auto safe = [](float v) { return std::fabs(v) < 0x1p-80f ? (v < 0.f ? -0x1p-80f : 0x1p-80f) : v; };
+  const V3 inv = {1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z)};
+  const float tlo = tmin - std::fabs(tmin) * kSlackC;
+  ++cnt.rays;
+  float best_t = kInf;
+  int32_t best_id = -1;
+  float bb1 = 0.f, bb2 = 0.f;
+  bool occluded = false;
+  uint32_t stack[kSDepth + 2];
+  int sp = 0;
+  uint32_t cur = 0;
+  const SearchTree& T = *c.T;
+  for (;;) {
+    if (!(cur & 0x80000000u)) {
+      ++cnt.visits;
+      const SNodeC& nd = T.nodes[cur];
+      float thi = c.mode == ORC_OCCLUDED ? tmax : best_t;
+      thi = thi + std::fabs(thi) * kSlackC;
+      float tn0, tn1;
+      const bool h0 = s_slab(org, inv, nd.b[0], tlo, thi, tn0);
+      const bool h1 = s_slab(org, inv, nd.b[1], tlo, thi, tn1);
+      if (h0 && h1) {
+        const bool first1 = tn1 < tn0;
+        stack[sp++] = first1 ? nd.ref[0] : nd.ref[1];
+        cur = first1 ? nd.ref[1] : nd.ref[0];
+        continue;
+      }
+      if (h0 || h1) {
+        cur = h0 ? nd.ref[0] : nd.ref[1];
+        continue;
+      }
+    } else {
+      const uint32_t first = cur & ((1u << 27) - 1u), n = ((cur >> 27) & 15u) + 1u;
+      for (uint32_t f = 0; f < n && !occluded; ++f) {
+        const uint32_t slot = first + f;
+        const uint32_t id = T.perm[slot];
+        uint32_t leaf = 0;
+        if (c.kind == 2) {
+          leaf = c.tleaf[id];
+          if (leaf == 0xffffffffu) continue;
+          if (c.mode == ORC_CLOSEST_SECONDARY && (leaf & 0x80000000u)) continue;
+          leaf &= 0x7fffffffu;
+        }
+        ++cnt.tests;
+        float t, b1, b2;
+        if (!mt_strict(org, d, T.tri[slot], t, b1, b2)) continue;
+        if (c.mode == ORC_OCCLUDED) {
+          if (!(t > tmin && t < tmax)) continue;
+          occluded = s_chain(c, x, y, slot, leaf, cnt.culls);
+          continue;
+        }
+        if (!(t > tmin)) continue;
+        if (!(t < best_t || (t == best_t && (int32_t)id < best_id))) continue;
+        if (!s_chain(c, x, y, slot, leaf, cnt.culls)) continue;
+        best_t = t;
+        best_id = (int32_t)id;
+        bb1 = b1;
+        bb2 = b2;
+      }
+      if (occluded) break;
+    }
+    if (sp == 0) break;
+    cur = stack[--sp];
+  }
+  if (c.mode == ORC_OCCLUDED) {
+    o.occ[i] = occluded ? 1 : 0;
+  } else {
+    o.id[i] = best_id;
+    o.t[i] = best_t;
+    o.b1[i] = bb1;
+    o.b2[i] = bb2;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1214,6 +1511,71 @@ int orc_trace_raybvh(const float* v9, uint32_t ntri, const orc_gpu_node* nodes, 
     leaf_test(lc, pr.k, pr.j, load_tri(v9, (uint32_t)pr.tid), pr.tid, pr.shadow, pr.cm);
   g_track = false;
   if (flags) finish_flags(r, mode, o, mg, flags);
+  return 0;
+}
+
+int orc_search_trace(const float* v9, uint32_t ntri, const orc_gpu_node* nodes, uint32_t nnodes,
+                     const uint32_t* roots, uint32_t nroots, const uint32_t* prim_order,
+                     const orc_rays* r, const orc_params* p, int32_t mode, const uint8_t* ray_mask,
+                     int32_t nthreads, int32_t* id, float* t, float* b1, float* b2, uint8_t* occ,
+                     orc_search_stats* st) {
+  if (p->kind != 1 && p->kind != 2) return 1;
+  if (p->kind == 2 && (!nodes || !roots || !prim_order)) return 1;
+  Out o{id, t, b1, b2, occ};
+  init_out(r, mode, o);
+  const SearchTree T = search_tree(v9, ntri);
+  const bool use_valid = mode != ORC_CLOSEST_PRIMARY;
+  const RH h = rh_build(r, p->cx, p->cy, p->height, use_valid);
+  SearchCtx c;
+  c.T = &T;
+  c.h = &h;
+  c.mode = mode;
+  c.node_kind = p->node;
+  c.kind = p->kind;
+  c.repair = false;
+  c.nodes = nodes;
+  if (p->kind == 2) {
+    c.tleaf.assign(ntri, 0xffffffffu);
+    c.parent.assign(nnodes, 0xffffffffu);
+    c.depth.assign(nnodes, 0);
+    for (uint32_t f = 0; f < nroots; ++f) {
+      std::vector<uint32_t> stk{roots[f]};
+      c.parent[roots[f]] = roots[f];
+      while (!stk.empty()) {
+        const uint32_t n = stk.back();
+        stk.pop_back();
+        const orc_gpu_node& b = nodes[n];
+        if (b.nPrimitives) {
+          for (uint32_t k = 0; k < b.nPrimitives; ++k)
+            c.tleaf[prim_order[b.primOffset + k]] = n | (k > 0 ? 0x80000000u : 0u);
+          continue;
+        }
+        for (uint32_t ch = b.primOffset; ch <= b.primOffset + 1; ++ch) {
+          c.parent[ch] = n;
+          c.depth[ch] = (uint8_t)(c.depth[n] + 1);
+          stk.push_back(ch);
+        }
+      }
+    }
+  }
+  const size_t n = (size_t)r->W * r->H;
+  std::vector<SCount> per(n);
+  if (ntri > 0)
+    parallel_for(n, nthreads, [&](size_t i) {
+      if (ray_mask && !ray_mask[i]) return;
+      if (mode != ORC_CLOSEST_PRIMARY && !ray_valid(r, i)) return;
+      s_trace_ray(c, r, i, o, per[i]);
+    });
+  if (st) {
+    *st = orc_search_stats{0, 0, 0, 0, 0};
+    st->nodes = T.nodes.size();
+    for (const SCount& q : per) {
+      st->rays += q.rays;
+      st->visits += q.visits;
+      st->tests += q.tests;
+      st->culls += q.culls;
+    }
+  }
   return 0;
 }
 

@@ -95,6 +95,26 @@ int orc_trace_raybvh(const float *v9, uint32_t ntri, const orc_gpu_node *nodes, 
                      const orc_rays *r, const orc_params *p, int32_t mode, int32_t *id, float *t,
                      float *b1, float *b2, uint8_t *occ, orc_stats *st, uint8_t *flags);
 
+/* ---- the library's per-ray path, restated ---- */
+typedef struct orc_search_stats {
+  uint64_t rays;   /* rays traced */
+  uint64_t visits; /* interior search-BVH nodes loaded */
+  uint64_t tests;  /* strict leaf tests */
+  uint64_t culls;  /* chain cull evaluations (hierarchy node x triangle or BVH node) */
+  uint64_t nodes;  /* interior nodes of the search BVH */
+} orc_search_stats;
+/* Per-ray traversal of the search BVH (Morton radix splits, leaves of <= 4, boxes grown by
+ * 2^-11 max(1, max |coordinate|)) with the strict leaf test and the reference's exact cull
+ * chain for passing triangles.  kind 1 (rayhierarchy) or 2 (ray-bvh, with its handoff).
+ * ray_mask (optional) selects the rays to trace; the others keep the miss / 0 outputs.
+ * Results must equal orc_trace / orc_trace_raybvh without repair; counts feed the work model
+ * of the per-ray kernel. */
+int orc_search_trace(const float *v9, uint32_t ntri, const orc_gpu_node *nodes, uint32_t nnodes,
+                     const uint32_t *roots, uint32_t nroots, const uint32_t *prim_order,
+                     const orc_rays *r, const orc_params *p, int32_t mode, const uint8_t *ray_mask,
+                     int32_t nthreads, int32_t *id, float *t, float *b1, float *b2, uint8_t *occ,
+                     orc_search_stats *st);
+
 /* shading record of computeDpTuTv.cl:8-91 for hits (id>=0): uv (2/ray), dpdu, dpdv (3/ray) */
 void orc_shading(const float *v9, const float *uv6, uint32_t ntri, const orc_rays *r,
                  const int32_t *id, float *u, float *v, float *dpdu, float *dpdv);

@@ -41,7 +41,8 @@ EXPORTS = (
 )
 (RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED, RT_PARAM_GRID_CLOSEST,
  RT_PARAM_GRID_OCCLUDED, RT_PARAM_GRAPH, RT_PARAM_RAY_CLUSTERS_CLOSEST,
- RT_PARAM_RAY_CLUSTERS_OCCLUDED) = 1, 2, 3, 4, 5, 6, 7
+ RT_PARAM_RAY_CLUSTERS_OCCLUDED, RT_PARAM_RAY_GRID_CLOSEST,
+ RT_PARAM_RAY_GRID_OCCLUDED) = 1, 2, 3, 4, 5, 6, 7, 8, 9
 # a per-ray threshold above any cluster count: every block enumerates its candidate pairs
 RAY_PATH_OFF = 1 << 30
 # kernel_time modes: + RT_TIME_PER_RAY = only the per-ray kernel, + RT_TIME_UNIT = only the
@@ -455,7 +456,7 @@ class Batch:
 
 
 class Server:
-    """Serving form: a request loop over resident rayhierarchy scenes with continuous
+    """Serving form: a request loop over resident rayhierarchy / ray-bvh scenes with continuous
     batching.  submit() returns a ticket at once; wait() returns the request's stats and, if
     outputs were asked for, numpy arrays of hit ids, t and occlusion flags."""
 

@@ -118,6 +118,7 @@ struct RhParams {
   int per[2] = {0, 0};
   int grid[2] = {0, 0};
   int ray[2] = {0, 0};  // passing clusters from which a block takes the per-ray path
+  int rgrid[2] = {0, 0};  // work-groups of the per-ray kernel
 };
 
 struct rt_scene {
@@ -353,8 +354,8 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int kDefaultPer[2] = {8, 4};  // passing clusters per unit: closest, occluded
 // Blocks whose root passes at least this many clusters (64 triangles or 64 frontier roots
-// each) are traced ray by ray: closest, occluded.
-constexpr int kDefaultRay[2] = {64, 64};
+// each), or more than half of the scene's, are traced ray by ray: closest, occluded.
+constexpr int kDefaultRay[2] = {256, 256};
 constexpr int kDefaultRootClustersPerUnit = 8;  // passing root clusters per ray-bvh unit
 
 // Lays out the jobs as launch groups (one per mode and node type, in job order) in the
@@ -388,6 +389,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
   struct GroupLayout {
     size_t units, rq, ctr, replist, max_units, max_chunks;
     uint32_t ray;
+    int rgrid;
     uint32_t nblocks, nrays;
     bool big, repair;
     int per, grid;
@@ -401,6 +403,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     // work units: passing clusters of triangles (rayhierarchy) or of frontier roots (ray-bvh)
     L.per = prm.per[mi] > 0 ? prm.per[mi] : (bvh ? kDefaultRootClustersPerUnit : kDefaultPer[mi]);
     L.ray = (uint32_t)(prm.ray[mi] > 0 ? prm.ray[mi] : kDefaultRay[mi]);
+    L.rgrid = prm.rgrid[mi];
     L.max_units = 0;
     L.max_chunks = 0;
     L.nblocks = 0;
@@ -520,6 +523,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     G.rq = (uint2*)(A + gl[gi].rq);
     G.max_chunks = (uint32_t)std::min<size_t>(gl[gi].max_chunks, 0xffffffffu);
     G.ray_clusters = gl[gi].ray;
+    G.ray_grid = gl[gi].rgrid;
     G.ctr = (uint32_t*)(A + gl[gi].ctr);
     G.replist = gl[gi].repair ? (uint32_t*)(A + gl[gi].replist) : nullptr;
     G.per_unit = gl[gi].per;
@@ -1063,6 +1067,8 @@ static int* batch_param(rt_batch* b, int32_t param) {
     case RT_PARAM_GRID_OCCLUDED: return &b->prm.grid[1];
     case RT_PARAM_RAY_CLUSTERS_CLOSEST: return &b->prm.ray[0];
     case RT_PARAM_RAY_CLUSTERS_OCCLUDED: return &b->prm.ray[1];
+    case RT_PARAM_RAY_GRID_CLOSEST: return &b->prm.rgrid[0];
+    case RT_PARAM_RAY_GRID_OCCLUDED: return &b->prm.rgrid[1];
     default: return nullptr;
   }
 }
@@ -1078,7 +1084,8 @@ int rt_batch_set_param(rt_batch* b, int32_t param, int32_t value) {
   int* p = batch_param(b, param);
   if (!p) return RT_ERR_INVALID;
   if (param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED ||
-      param == RT_PARAM_RAY_CLUSTERS_CLOSEST || param == RT_PARAM_RAY_CLUSTERS_OCCLUDED) {
+      param == RT_PARAM_RAY_CLUSTERS_CLOSEST || param == RT_PARAM_RAY_CLUSTERS_OCCLUDED ||
+      param == RT_PARAM_RAY_GRID_CLOSEST || param == RT_PARAM_RAY_GRID_OCCLUDED) {
     if (value > (1 << 30)) return RT_ERR_INVALID;
   } else if (value > (1 << 16)) {
     return RT_ERR_INVALID;
@@ -1098,14 +1105,57 @@ int rt_batch_get_param(rt_batch* b, int32_t param, int32_t* value) {
   int* p = batch_param(b, param);
   if (!p) return RT_ERR_INVALID;
   const int mi = (param == RT_PARAM_CLUSTERS_OCCLUDED || param == RT_PARAM_GRID_OCCLUDED ||
-                  param == RT_PARAM_RAY_CLUSTERS_OCCLUDED)
+                  param == RT_PARAM_RAY_CLUSTERS_OCCLUDED || param == RT_PARAM_RAY_GRID_OCCLUDED)
                      ? 1
                      : 0;
-  const bool grid = param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED;
+  const bool grid = param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED ||
+                    param == RT_PARAM_RAY_GRID_CLOSEST || param == RT_PARAM_RAY_GRID_OCCLUDED;
   const bool ray = param == RT_PARAM_RAY_CLUSTERS_CLOSEST || param == RT_PARAM_RAY_CLUSTERS_OCCLUDED;
   *value = *p > 0 ? *p : (grid ? 0 : ray ? kDefaultRay[mi] : kDefaultPer[mi]);
   return RT_OK;
 }
+
+}  // extern "C"
+
+// Times the batch over `cands` launch-shape candidates (each applies itself to b->prm for
+// both modes) and keeps, per mode, the fields `keep` copies from the fastest candidate.
+template <class Apply, class Keep>
+static int autotune_stage(rt_batch* b, hipStream_t hs, int reps, int ncand, Apply apply,
+                          Keep keep) {
+  float best[2] = {INFINITY, INFINITY};
+  RhParams out = b->prm;
+  const RhParams base = b->prm;
+  int st = RT_OK;
+  for (int c = 0; c < ncand && !st; ++c) {
+    b->prm = base;
+    apply(c, b->prm);
+    st = batch_run_locked(b, hs);  // warm-up (allocates)
+    b->timer.used = 0;
+    for (int r = 0; r < reps && !st; ++r) st = batch_run_locked(b, hs);
+    if (st) break;
+    for (int m = 0; m < 2 && !st; ++m) {
+      float ms = 0.f, ms2 = 0.f;
+      int32_t n = 0, n2 = 0;
+      if (m == 0) {
+        st = b->timer.total(0, &ms, &n);
+        if (!st) st = b->timer.total(1, &ms2, &n2);
+        ms += ms2;
+        n += n2;
+      } else {
+        st = b->timer.total(2, &ms, &n);
+      }
+      if (!st && n > 0 && ms < best[m]) {
+        best[m] = ms;
+        keep(m, b->prm, out);
+      }
+    }
+    b->timer.used = 0;
+  }
+  b->prm = out;
+  return st;
+}
+
+extern "C" {
 
 int rt_batch_autotune(rt_batch* b, void* stream, int32_t reps) {
   if (!b || reps <= 0) return RT_ERR_INVALID;
@@ -1113,45 +1163,38 @@ int rt_batch_autotune(rt_batch* b, void* stream, int32_t reps) {
   RT_TRY(hipSetDevice(b->ctx->device));
   hipStream_t hs = (hipStream_t)stream;
   if (b->jobs.empty()) return RT_OK;
-  static const int kPer[] = {2, 4, 8, 16, 32};
-  static const int kGrid[] = {0, 1024, 2048};  // 0: one work-group per resident slot
   const bool was_on = b->timer.on;
   const bool was_graph = b->use_graph;
   b->use_graph = false;
   b->timer.on = true;
-  float best[2] = {INFINITY, INFINITY};
-  RhParams best_prm = b->prm;
-  int st = RT_OK;
-  for (int gi = 0; gi < 3 && !st; ++gi)
-    for (int pi = 0; pi < 5 && !st; ++pi) {
-      b->prm.per[0] = b->prm.per[1] = kPer[pi];
-      b->prm.grid[0] = b->prm.grid[1] = kGrid[gi];
-      st = batch_run_locked(b, hs);  // warm-up (allocates)
-      b->timer.used = 0;
-      for (int r = 0; r < reps && !st; ++r) st = batch_run_locked(b, hs);
-      if (st) break;
-      for (int m = 0; m < 2; ++m) {
-        float ms = 0.f;
-        int32_t n = 0, n2 = 0;
-        float ms2 = 0.f;
-        if (m == 0) {
-          st = b->timer.total(0, &ms, &n);
-          if (!st) st = b->timer.total(1, &ms2, &n2);
-          ms += ms2;
-          n += n2;
-        } else {
-          st = b->timer.total(2, &ms, &n);
-        }
-        if (st) break;
-        if (n > 0 && ms < best[m]) {
-          best[m] = ms;
-          best_prm.per[m] = kPer[pi];
-          best_prm.grid[m] = kGrid[gi];
-        }
-      }
-      b->timer.used = 0;
-    }
-  b->prm = best_prm;
+  // stage 1: which blocks take the per-ray path (1: all, 2^30: none) and the per-ray grid
+  // (one wavefront per work-group: 8 or 16 per CU, or every resident slot)
+  static const int kRay[] = {1, 64, 256, 1 << 30};
+  static const int kRGrid[] = {0, 2048, 4096};
+  int st = autotune_stage(
+      b, hs, reps, 4 * 3,
+      [&](int c, RhParams& p) {
+        p.ray[0] = p.ray[1] = kRay[c / 3];
+        p.rgrid[0] = p.rgrid[1] = kRGrid[c % 3];
+      },
+      [](int m, const RhParams& p, RhParams& o) {
+        o.ray[m] = p.ray[m];
+        o.rgrid[m] = p.rgrid[m];
+      });
+  // stage 2: passing clusters per work unit x persistent grid of the unit kernel
+  static const int kPer[] = {2, 4, 8, 16, 32};
+  static const int kGrid[] = {0, 1024, 2048};  // 0: one work-group per resident slot
+  if (!st)
+    st = autotune_stage(
+        b, hs, reps, 5 * 3,
+        [&](int c, RhParams& p) {
+          p.per[0] = p.per[1] = kPer[c % 5];
+          p.grid[0] = p.grid[1] = kGrid[c / 5];
+        },
+        [](int m, const RhParams& p, RhParams& o) {
+          o.per[m] = p.per[m];
+          o.grid[m] = p.grid[m];
+        });
   b->timer.on = was_on;
   b->timer.used = 0;
   b->use_graph = was_graph;

@@ -119,6 +119,7 @@ struct RhGroup {
   uint32_t max_chunks;  // upper bound of the group's per-ray chunks
   // blocks whose root passes at least this many clusters are traced ray by ray
   uint32_t ray_clusters;
+  int ray_grid;  // work-groups (one wavefront each) of the per-ray kernel; 0 = resident slots
 };
 
 // Search BVH of the per-ray path over triangles in sorted order (see rt_search_bvh.cpp).

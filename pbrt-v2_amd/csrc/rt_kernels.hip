@@ -631,9 +631,9 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
 // passing cluster ids in the job's clist[block * nclusters + ...] and appends work units
 // {job, block, first, count} of at most `per_unit` passing clusters.
 //
-// A block whose root passes at least `ray_clusters` clusters is an incoherent ray group: its
-// candidate set holds most (leaf, triangle) pairs, so it is queued for the per-ray path
-// instead, as 64-ray chunks {job, block << 4 | chunk}.
+// A block whose root passes at least `ray_clusters` clusters, or more than half of the scene's,
+// is an incoherent ray group: its candidate set holds most (leaf, triangle) pairs, so it is
+// queued for the per-ray path instead, as 64-ray chunks {job, block << 4 | chunk}.
 template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
                                                          int nitems, uint4* __restrict__ units,
@@ -673,7 +673,9 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   }
   __syncthreads();
   const int cnt = sCnt;
-  if ((uint32_t)cnt >= ray_clusters) {
+  // incoherent: at least `ray_clusters` clusters pass, or more than half of the scene's
+  // (ray_clusters above 2^29 turns the per-ray path off)
+  if ((uint32_t)cnt >= ray_clusters || (ray_clusters <= (1u << 29) && 2u * (uint32_t)cnt > nclusters)) {
     const int nch = (g.rb + 63) / 64;
     if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
     __syncthreads();
@@ -702,6 +704,7 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
   __shared__ uint4 sUnit;  // the unit taken next; x = kNoUnit when the queue is empty
   const int tid = threadIdx.x;
   const uint32_t nunits = __builtin_amdgcn_readfirstlane(ctr[0]);
+  if (nunits == 0) return;  // every block went to the per-ray path (or had no work)
   if (tid == 0) {
     const uint32_t u = atomicAdd(&ctr[1], 1u);
     sUnit = u < nunits ? units[u] : make_uint4(kNoUnit, 0u, 0u, 0u);
@@ -1053,6 +1056,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
   __shared__ uint32_t sUnit;
   const int tid = threadIdx.x;
   const uint32_t nunits = __builtin_amdgcn_readfirstlane(ctr[0]);
+  if (nunits == 0) return;
   for (;;) {
     if (tid == 0) sUnit = atomicAdd(&ctr[1], 1u);
     __syncthreads();
@@ -1306,6 +1310,7 @@ __global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__
   __shared__ uint32_t sStk[kSearchDepth][64];
   const int lane = threadIdx.x;
   const uint32_t nchunks = __builtin_amdgcn_readfirstlane(ctr[4]);
+  if (nchunks == 0) return;  // no incoherent block: leave without touching the queue
   for (;;) {
     uint32_t e = 0;
     if (lane == 0) e = atomicAdd(&ctr[5], 1u);
@@ -1896,7 +1901,8 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
     if (e != hipSuccess) return e;
   }
   auto launch_rays = [&](auto kernel) {
-    const dim3 grid(resident_grid(kernel, G.max_chunks, 64));
+    const dim3 grid(G.ray_grid > 0 ? std::max(1u, std::min((unsigned)G.ray_grid, G.max_chunks))
+                                   : resident_grid(kernel, G.max_chunks, 64));
     hipLaunchKernelGGL(kernel, grid, dim3(64), 0, s, G.items, G.rq, G.ctr);
   };
   if (G.bvh)

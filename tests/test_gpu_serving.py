@@ -83,3 +83,79 @@ def test_budget_admission(rt, scenes, gpu_ctx):
     print(f"[serve] budget run: {ticks} ticks", flush=True)
     for s in scs:
         s.close()
+
+
+def test_bad_request_fails_alone(rt, scenes, gpu_ctx):
+    """A request its scene cannot trace (16x16-ray leaves: 4,096-ray blocks) is refused at
+    submit and touches no other request; a naive scene is refused at server creation."""
+    good = rt.Scene(gpu_ctx, scenes.random_triangles(1024, seed=1), desc="rayhierarchy")
+    big = rt.Scene(gpu_ctx, scenes.random_triangles(1024, seed=1),
+                   desc=rt.default_desc("rayhierarchy", chunk_x=16, chunk_y=16))
+    naive = rt.Scene(gpu_ctx, scenes.random_triangles(64, seed=2), desc="naive")
+    with pytest.raises(rt.RtError):
+        rt.Server(gpu_ctx, [good, naive])
+    srv = rt.Server(gpu_ctx, [good, big], max_active=8)
+    try:
+        t = srv.submit(0, 64, 64, 1, outputs=True)
+        with pytest.raises(rt.RtError):
+            srv.submit(1, 64, 64, 1, outputs=True)
+        res = srv.wait(t, timeout_s=60.0)
+    finally:
+        srv.close()
+    rays = scenes.pinhole_rays(64, 64)
+    ids = good.trace_closest(rt.DeviceRays.from_numpy(rays)).id.cpu().numpy()
+    assert np.array_equal(res["id"], ids)
+    for s in (good, big, naive):
+        s.close()
+
+
+def test_raybvh_request_matches_plain_trace(rt, scenes, gpu_ctx):
+    """ray-bvh scenes are served like rayhierarchy ones: a request equals the plain traces."""
+    v9 = scenes.room_scene(nclusters=40)
+    sc = rt.Scene(gpu_ctx, v9, desc="ray-bvh")
+    srv = rt.Server(gpu_ctx, [sc], max_active=8)
+    try:
+        t = srv.submit(0, 96, 64, 1, origin=(0.0, 0.0, -9.0), fov_deg=50.0,
+                       light=(1.0, 2.5, -8.0), outputs=True)
+        res = srv.wait(t, timeout_s=60.0)
+    finally:
+        srv.close()
+    rays = scenes.pinhole_rays(96, 64, origin=(0.0, 0.0, -9.0), fov_deg=50.0)
+    d = rt.DeviceRays.from_numpy(rays)
+    h = sc.trace_closest(d, rt.RT_CLOSEST_PRIMARY)
+    assert np.array_equal(res["id"], h.id.cpu().numpy())
+    sh = rt.make_shadow_rays(d, h, (1.0, 2.5, -8.0))
+    assert np.array_equal(res["occ"], sc.trace_occluded(sh).cpu().numpy())
+    sc.close()
+
+
+def test_pipelined_ticks_many_requests(rt, scenes, gpu_ctx):
+    """Requests arriving over many ticks (two ticks in flight at once, blocks reused from the
+    pool) each equal the same request traced alone."""
+    scs = _scenes(rt, scenes, gpu_ctx)
+    g = np.random.default_rng(21)
+    reqs = [dict(scene=int(g.integers(0, 3)), width=int(g.choice([16, 48, 80])),
+                 height=int(g.choice([16, 32, 64])), spp=int(g.integers(1, 3)),
+                 origin=(0.0, 0.0, -12.0), fov_deg=50.0, light=(2.0, 2.0, -2.0), seed=k)
+            for k in range(40)]
+    srv = rt.Server(gpu_ctx, scs, max_active=6, mem_budget=64 << 20)
+    try:
+        alone = [srv.wait(srv.submit(**q, outputs=True), timeout_s=60.0) for q in reqs[:8]]
+        tickets = []
+        for k, q in enumerate(reqs):
+            tickets.append(srv.submit(**q, outputs=True))
+            if k % 5 == 4:
+                srv.wait(tickets[k - 2], timeout_s=60.0)
+                tickets[k - 2] = None
+        res = [srv.wait(t, timeout_s=120.0) if t is not None else None for t in tickets]
+        ticks, most = srv.counters()
+    finally:
+        srv.close()
+    print(f"[serve] {ticks} ticks, largest tick {most} jobs", flush=True)
+    for k in range(8):
+        if res[k] is None:
+            continue
+        assert np.array_equal(alone[k]["id"], res[k]["id"]), f"request {k} ids"
+        assert np.array_equal(alone[k]["occ"], res[k]["occ"]), f"request {k} occlusion"
+    for s in scs:
+        s.close()

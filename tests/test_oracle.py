@@ -856,3 +856,48 @@ def test_flags_do_not_change_results(orc, scenes):
         np.testing.assert_array_equal(a["t"], b["t"])
         assert a["stats"].cull_tests == b["stats"].cull_tests
         assert a["stats"].leaf_pairs == b["stats"].leaf_pairs
+
+
+# ---------------------------------------------------------------- the per-ray path, restated
+@pytest.mark.parametrize("accel", ["rayhierarchy", "ray-bvh"])
+@pytest.mark.parametrize("node", ["box_dir", "ia"])
+def test_search_trace_equals_reference_semantics(orc, scenes, accel, node):
+    # The per-ray path walks a padded search BVH and applies the reference's cull chain only to
+    # triangles that pass the strict leaf test.  Its answer must be the reference's: equal to
+    # the depth-first traversal of the reference structure (without the repair pass, which the
+    # library applies to both paths alike), for coherent and incoherent rays, every mode.
+    v9 = scenes.cluster_scene(3000, seed=60)
+    cam = scenes.pinhole_rays(32, 32, origin=(0.0, 0.0, -12.0), fov_deg=50.0)
+    rnd = scenes.random_rays(32 * 32, seed=61, lo=-5.0, hi=5.0, maxt=np.inf, width=32)
+    rnd["valid"] = (np.random.default_rng(62).random(32 * 32) < 0.8).astype(np.uint8)
+    seg = scenes.random_rays(32 * 32, seed=63, lo=-5.0, hi=5.0, maxt=10.0, width=32)
+    p = orc.params(accel, node=node, repair_run=0)
+    handoff = orc.raybvh_handoff(v9) if accel == "ray-bvh" else None
+    for mode, rays in ((0, cam), (1, rnd), (2, seg)):
+        got = orc.search_trace(v9, rays, p, mode, handoff=handoff, nthreads=4)
+        if accel == "ray-bvh":
+            ref = orc.trace_raybvh(v9, *handoff, rays, p, mode)
+        else:
+            ref = orc.trace(v9, rays, p, mode)
+        if mode == 2:
+            np.testing.assert_array_equal(got["occ"], ref["occ"])
+            assert 10 < ref["occ"].sum() < 1000
+        else:
+            np.testing.assert_array_equal(got["id"], ref["id"])
+            np.testing.assert_array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32))
+            assert (ref["id"] >= 0).sum() > 5
+        st = got["stats"]
+        # the search visits a small part of the tree per ray
+        assert st.visits < 0.1 * st.rays * st.nodes
+
+
+def test_search_trace_mask_and_counts(orc, scenes):
+    v9 = scenes.cluster_scene(2000, seed=64)
+    rays = scenes.random_rays(16 * 16, seed=65, lo=-5.0, hi=5.0, maxt=10.0, width=16)
+    p = orc.params("rayhierarchy")
+    full = orc.search_trace(v9, rays, p, orc.OCCLUDED)
+    mask = (np.arange(256) % 2).astype(np.uint8)
+    half = orc.search_trace(v9, rays, p, orc.OCCLUDED, mask=mask)
+    np.testing.assert_array_equal(half["occ"], full["occ"] * mask)
+    assert half["stats"].rays == 128
+    assert 0 < half["stats"].visits < full["stats"].visits
