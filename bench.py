@@ -1,16 +1,22 @@
-"""Benchmark: Mrays/s of the plain per-scene rayhierarchy accelerator on one MI355X.
+"""Benchmark: batched vs plain per-scene Mrays/s of the MI355X ray-space accelerators, with
+p50/p99 request latency of the serving form.
 
-Workload (default "torus_rayhierarchy_512"): the torus scene (24,778 triangles: UV torus
-R=1 r=0.4 128x96 quads, ground plane, 200 small random triangles), a 512x512 pinhole camera
-with one primary ray per pixel, 4x4-ray leaves, height 3, box_dir nodes, and one shadow ray
-per hit toward a point light (directlighting).  One step = primary closest-hit trace (with
-the fused hierarchy build and repair pass) + shadow-ray generation + shadow occlusion trace,
-all on the device with inputs already resident in HBM.  Mrays/s counts primary rays plus
-valid shadow rays.
+Default workload "uneven_batch_64": 64 independent scenes of uneven size (triangle counts
+log-uniform in [1e3, 3e5], seeded Gaussian triangle clusters in [-5,5]^3), each with a camera
+ray image and a set of random-direction shadow segments (ray counts log-uniform in [1.6e4,
+1e6], half each), rayhierarchy accelerator at the reference defaults (4x4-ray leaves, height 3,
+box_dir nodes, repairRun 1).  One step traces every scene's closest hits and occlusion:
+  - batched: one batch of 128 jobs, launch shape picked by the autotuner, replayed as a HIP
+    graph (the line's `value`);
+  - plain: the 64 per-scene call pairs in turn, rays resident in HBM (`plain`), and the same
+    with every call's rays copied host-to-device first (`plain_h2d`).
+All three must give the same bits; sampled ray blocks are checked against the oracle, with
+near ties, ulp-marginal rays and differences to the exact pbrt predicate counted.
 
-Also reported: the dominant traversal kernel's HIP-event time against the HBM roofline
-(algorithmic bytes of the reference's candidate set), and pbrt's CPU BVH (BVHAccel,
-maxnodeprims 4, inclusive Triangle test) on the same rays on the host cores.
+Also reported: the roofline of the dominant traversal kernel (algorithmic bytes from the
+oracle's restatement of its work, HIP-event time of the kernel), pbrt's CPU BVH on a bounded
+sample of the same rays, the plain rayhierarchy torus workload, and the serving form at
+Poisson request rates.
 """
 import argparse
 import importlib.util
@@ -27,6 +33,7 @@ ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (with packed FP32 instructions)
 FP32_UNPACKED_TFLOPS = 78.6   # without them; this build compiles packed FP32 off
+METRIC = "Mrays/s batched vs plain per-scene launch, same hits; p50/p99 request latency"
 
 
 def _load(name, path, search=None):
@@ -40,25 +47,109 @@ def _load(name, path, search=None):
     return mod
 
 
-def profiled_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters (FETCH_SIZE,
-    WRITE_SIZE, calibrated against a plain copy) and recorded in profiles/*.md as a line
-    `traffic-per-launch <workload> <kernel> <bytes> ...`; None when not profiled."""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def profile_facts(workload, kernel):
+    """Facts about `kernel` on `workload` recorded in profiles/*.md by the rocprofv3 runs, as
+    lines `<fact> <workload> <kernel> <value>`:
+      traffic-per-launch  HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by what the
+                          same counters read for a plain copy of known size)
+      valu-busy           fraction of the launch the VALU was busy (SQ counters)
+    Returns {fact: value} (empty when not profiled)."""
+    out = {}
     for f in sorted((ROOT / "profiles").glob("*.md")):
         for line in f.read_text().splitlines():
             parts = line.split()
-            if len(parts) >= 4 and parts[0] == "traffic-per-launch" and parts[1] == workload:
-                pairs = dict(zip(parts[2::2], parts[3::2]))
-                if kernel in pairs:
-                    return int(float(pairs[kernel]))
-    return None
+            if len(parts) >= 4 and parts[1] == workload and parts[2] == kernel:
+                try:
+                    out[parts[0]] = float(parts[3])
+                except ValueError:
+                    pass
+    return out
 
 
-def bench_uneven(args, rt, scenes, torch):
-    """The uneven batch (64 scenes of log-uniform size, camera + random shadow rays): the
-    plain per-scene form (every scene's closest and occlusion call in turn on one stream)
-    against the batched form (one batch of 128 jobs, graph-replayed), same hits required."""
-    print("[bench] uneven batch: generating 64 scenes", file=sys.stderr, flush=True)
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+
+
+def timed(fn, steps, warmup, stream, label):
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    log(f"timing {steps} {label} steps")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    stream.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bound_of(facts, hbm_meas, hbm_alg, valu_frac):
+    """The roof that limits the kernel.  From the profiled counters when they are recorded:
+    HBM when the measured DRAM traffic uses a larger share of its peak than the VALU is busy,
+    else VALU.  Without them, the same comparison of the algorithmic model's shares."""
+    busy = facts.get("valu-busy")
+    if busy is not None and hbm_meas is not None:
+        return "hbm" if hbm_meas >= busy else "valu"
+    return "hbm" if hbm_alg >= valu_frac else "valu"
+
+
+def per_ray_roofline(name, t_ms, st, mode, workload):
+    """Roofline of a per-ray kernel launch from the oracle's restatement of its work: per
+    traced ray 32 B (o, d, tmin, tmax) + 1 B valid + the result (16 B id/t/u/v, 1 B occlusion),
+    64 B per search-BVH node loaded, 52 B per strict leaf test (v1, e1, e2 and the triangle
+    id), 64 B per chain cull (one hierarchy node) and 36 B per chain (the triangle's
+    vertices).  Arithmetic: 24 FLOP per node (two slab tests), 45 per leaf test, 60 per cull."""
+    out_b = 1 if mode == 2 else 16
+    b = (st.rays * (33 + out_b) + st.visits * 64 + st.tests * 52 + st.culls * 64
+         + st.chains * 36)
+    achieved = b / (t_ms * 1e-3) / 1e9
+    flops = 24 * st.visits + 45 * st.tests + 60 * st.culls
+    tflops = flops / (t_ms * 1e-3) / 1e12
+    facts = profile_facts(workload, name)
+    traffic = facts.get("traffic-per-launch")
+    hbm_meas = None if traffic is None else traffic / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    return {"bound": bound_of(facts, hbm_meas, achieved / HBM_PEAK_GBS,
+                              tflops / FP32_UNPACKED_TFLOPS), "kernel": name,
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if traffic is None else int(traffic),
+            "bytes_per_launch": int(b), "ms_per_launch": round(t_ms, 4),
+            "counts": {"rays": int(st.rays), "node_loads": int(st.visits),
+                       "leaf_tests": int(st.tests), "chain_culls": int(st.culls),
+                       "chains": int(st.chains)},
+            "valu": {"flops_per_launch": int(flops), "achieved": round(tflops, 3),
+                     "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
+                     "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                     "peak_unpacked": FP32_UNPACKED_TFLOPS,
+                     "busy_measured": facts.get("valu-busy")}}
+
+
+def _window(r, x0, y0, w, h):
+    out = {}
+    W = r["width"]
+    for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax", "valid"):
+        a = r.get(k)
+        out[k] = None if a is None else a.reshape(-1, W)[y0:y0 + h, x0:x0 + w].reshape(-1).copy()
+    out["width"], out["height"] = w, h
+    return out
+
+
+def bench_uneven(args, rt, scenes, torch, orc):
+    """The uneven batch: batched vs plain per-scene launches, same hits required."""
+    log("uneven batch: generating 64 scenes")
     jobs = scenes.uneven_batch(64, seed0=100)
     ctx = rt.Context(0)
     stream = torch.cuda.Stream()
@@ -67,6 +158,7 @@ def bench_uneven(args, rt, scenes, torch):
         scs.append(rt.Scene(ctx, j["v9"], desc="rayhierarchy"))
         prims.append(rt.DeviceRays.from_numpy(j["primary"]))
         shads.append(rt.DeviceRays.from_numpy(j["shadow"]))
+    S = len(jobs)
     n_prim = sum(p.count for p in prims)
     n_shad = sum(int(j["shadow"]["valid"].sum()) for j in jobs)
     rays_per_step = n_prim + n_shad
@@ -74,95 +166,207 @@ def bench_uneven(args, rt, scenes, torch):
     occ_p = [torch.empty(s.count, dtype=torch.uint8, device="cuda") for s in shads]
 
     def plain_step():
-        for k in range(len(jobs)):
+        for k in range(S):
             scs[k].trace_closest(prims[k], rt.RT_CLOSEST_PRIMARY, hits=hits_p[k], stream=stream)
             scs[k].trace_occluded(shads[k], out=occ_p[k], stream=stream)
 
+    # the plain form as the reference's Intersect / IntersectP calls take it: rays come from
+    # host memory with every call (pinned staging copies of the same rays)
+    fields = ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax", "valid")
+
+    def pinned(d):
+        return {k: torch.from_numpy(np.ascontiguousarray(d[k])).pin_memory() for k in fields
+                if d.get(k) is not None}
+    h_prim = [pinned(j["primary"]) for j in jobs]
+    h_shad = [pinned(j["shadow"]) for j in jobs]
+
+    def plain_h2d_step():
+        with torch.cuda.stream(stream):
+            for k in range(S):
+                for f, t in h_prim[k].items():
+                    getattr(prims[k], f).copy_(t, non_blocking=True)
+                scs[k].trace_closest(prims[k], rt.RT_CLOSEST_PRIMARY, hits=hits_p[k],
+                                     stream=stream)
+                for f, t in h_shad[k].items():
+                    getattr(shads[k], f).copy_(t, non_blocking=True)
+                scs[k].trace_occluded(shads[k], out=occ_p[k], stream=stream)
+
     b = rt.Batch(ctx)
-    hits_b = [b.add_closest(scs[k], prims[k]) for k in range(len(jobs))]
-    occ_b = [b.add_occluded(scs[k], shads[k]) for k in range(len(jobs))]
-    if args.autotune:
-        print("[bench] autotuning the batch", file=sys.stderr, flush=True)
-        b.autotune(reps=1, stream=stream)
+    hits_b = [b.add_closest(scs[k], prims[k]) for k in range(S)]
+    occ_b = [b.add_occluded(scs[k], shads[k]) for k in range(S)]
+    if not args.no_autotune:
+        log("autotuning the batch")
+        b.autotune(reps=2, stream=stream)
     b.set_param(rt.RT_PARAM_GRAPH, 1)
 
     def batch_step():
         b.run(stream=stream)
 
-    def timed(fn, label):
-        for _ in range(args.warmup):
-            fn()
-        stream.synchronize()
-        print(f"[bench] timing {args.steps} {label} steps", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            fn()
-        stream.synchronize()
-        return (time.perf_counter() - t0) / args.steps
-
-    t_plain = timed(plain_step, "plain")
-    t_batch = timed(batch_step, "batched")
-    # per-mode unit-kernel time of one eager batch run
-    b.set_param(rt.RT_PARAM_GRAPH, 0)
-    b.set_timing(True)
-    b.run(stream=stream)
-    stream.synchronize()
-    k_closest = b.kernel_time(rt.RT_CLOSEST_PRIMARY)[0]
-    k_occl = b.kernel_time(2)[0]
-    split = {"closest_unit": round(b.kernel_time(rt.RT_CLOSEST_PRIMARY + rt.RT_TIME_UNIT)[0], 3),
-             "closest_per_ray": round(b.kernel_time(rt.RT_CLOSEST_PRIMARY + rt.RT_TIME_PER_RAY)[0], 3),
-             "occlusion_unit": round(b.kernel_time(2 + rt.RT_TIME_UNIT)[0], 3),
-             "occlusion_per_ray": round(b.kernel_time(2 + rt.RT_TIME_PER_RAY)[0], 3)}
-    sweep = {}
-    for thr in args.ray_sweep:
-        b.set_param(rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, thr)
-        b.set_param(rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED, thr)
-        b.run(stream=stream)
-        b.run(stream=stream)
-        stream.synchronize()
-        b.set_timing(True)
-        b.run(stream=stream)
-        stream.synchronize()
-        sweep[thr] = {k: round(b.kernel_time(m)[0], 3) for k, m in (
-            ("closest", 0), ("closest_per_ray", 0 + rt.RT_TIME_PER_RAY),
-            ("occlusion", 2), ("occlusion_per_ray", 2 + rt.RT_TIME_PER_RAY))}
-        b.set_timing(False)
-    b.set_param(rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, 0)
-    b.set_param(rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED, 0)
-    b.set_timing(False)
-    mism_id = sum(int((hits_b[k].id != hits_p[k].id).sum().item()) for k in range(len(jobs)))
-    mism_t = sum(int((hits_b[k].t.view(torch.int32) != hits_p[k].t.view(torch.int32)).sum().item())
-                 for k in range(len(jobs)))
-    mism_occ = sum(int((occ_b[k] != occ_p[k]).sum().item()) for k in range(len(jobs)))
-    v_plain = rays_per_step / t_plain / 1e6
-    v_batch = rays_per_step / t_batch / 1e6
+    t_plain = timed(plain_step, args.steps, args.warmup, stream, "plain")
+    t_h2d = timed(plain_h2d_step, args.steps, args.warmup, stream, "plain (rays from host)")
+    t_batch = timed(batch_step, args.steps, args.warmup, stream, "batched")
     tuned = {name: b.get_param(p) for name, p in (
         ("clusters_closest", rt.RT_PARAM_CLUSTERS_CLOSEST),
         ("clusters_occluded", rt.RT_PARAM_CLUSTERS_OCCLUDED),
-        ("grid_closest", rt.RT_PARAM_GRID_CLOSEST), ("grid_occluded", rt.RT_PARAM_GRID_OCCLUDED))}
+        ("grid_closest", rt.RT_PARAM_GRID_CLOSEST), ("grid_occluded", rt.RT_PARAM_GRID_OCCLUDED),
+        ("ray_clusters_closest", rt.RT_PARAM_RAY_CLUSTERS_CLOSEST),
+        ("ray_clusters_occluded", rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED),
+        ("ray_grid_closest", rt.RT_PARAM_RAY_GRID_CLOSEST),
+        ("ray_grid_occluded", rt.RT_PARAM_RAY_GRID_OCCLUDED))}
+
+    # per-kernel time of one eager run (unit kernel and per-ray kernel of each mode)
+    def eager_split():
+        b.set_param(rt.RT_PARAM_GRAPH, 0)
+        b.set_timing(True)
+        b.run(stream=stream)
+        stream.synchronize()
+        sp = {k: b.kernel_time(m)[0] for k, m in (
+            ("closest_unit", rt.RT_CLOSEST_PRIMARY + rt.RT_TIME_UNIT),
+            ("closest_per_ray", rt.RT_CLOSEST_PRIMARY + rt.RT_TIME_PER_RAY),
+            ("occlusion_unit", 2 + rt.RT_TIME_UNIT), ("occlusion_per_ray", 2 + rt.RT_TIME_PER_RAY),
+            ("closest", rt.RT_CLOSEST_PRIMARY), ("occlusion", 2))}
+        b.set_timing(False)
+        return sp
+    split = eager_split()
+
+    # parity: batched vs plain, bit for bit
+    def neq(a, c):
+        return int((a != c).sum().item())
+    mism_id = sum(neq(hits_b[k].id, hits_p[k].id) for k in range(S))
+    mism_t = sum(neq(hits_b[k].t.view(torch.int32), hits_p[k].t.view(torch.int32))
+                 for k in range(S))
+    mism_occ = sum(neq(occ_b[k], occ_p[k]) for k in range(S))
+
+    # parity against the oracle on sampled ray blocks (a 16x16 block is a whole hierarchy
+    # subtree, so the oracle's trace of the block's window is the reference's answer for it)
+    log("oracle on sampled blocks")
+    g = np.random.default_rng(7)
+    p = orc.params("rayhierarchy")
+    smp = {"blocks": 0, "rays": 0, "id_mismatches": 0, "occlusion_mismatches": 0,
+           "near_ties": 0, "ulp_marginal": 0, "oref_vs_oexact_id": 0,
+           "oref_vs_oexact_occlusion": 0}
+    for k in range(0, S, max(1, S // args.oracle_scenes)):
+        v9 = jobs[k]["v9"]
+        ex = orc.CpuBvh(v9, max_prims=4)
+        for kind in ("primary", "shadow"):
+            r = jobs[k][kind]
+            bx = int(g.integers(0, r["width"] // 16)) * 16
+            by = int(g.integers(0, r["height"] // 16)) * 16
+            win = _window(r, bx, by, 16, 16)
+            sel = (np.arange(by, by + 16)[:, None] * r["width"] + np.arange(bx, bx + 16)).ravel()
+            smp["blocks"] += 1
+            smp["rays"] += 256
+            if kind == "primary":
+                ref = orc.trace(v9, win, p, orc.CLOSEST_PRIMARY, flags=True)
+                gid = hits_b[k].id.cpu().numpy()[sel]
+                smp["id_mismatches"] += int((gid != ref["id"]).sum())
+                smp["near_ties"] += int((ref["flags"] & orc.NEAR_TIE != 0).sum())
+                smp["oref_vs_oexact_id"] += int((ex.closest(win)["id"] != ref["id"]).sum())
+            else:
+                ref = orc.trace(v9, win, p, orc.OCCLUDED, flags=True)
+                gocc = occ_b[k].cpu().numpy()[sel]
+                smp["occlusion_mismatches"] += int((gocc != ref["occ"]).sum())
+                smp["oref_vs_oexact_occlusion"] += int((ex.occluded(win) != ref["occ"]).sum())
+            smp["ulp_marginal"] += int((ref["flags"] & orc.ULP_MARGINAL != 0).sum())
+        ex.close()
+
+    # roofline of the heaviest per-ray kernel: its work is every ray it traces, so it is timed
+    # with every block of its mode sent to it, and the oracle's restatement counts the work
+    heavy = "occlusion" if split["occlusion_per_ray"] >= split["closest_per_ray"] else "closest"
+    hm = 2 if heavy == "occlusion" else rt.RT_CLOSEST_PRIMARY
+    pr = rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED if hm == 2 else rt.RT_PARAM_RAY_CLUSTERS_CLOSEST
+    old = b.get_param(pr)
+    b.set_param(pr, 1)
+    b.set_param(rt.RT_PARAM_GRAPH, 0)
+    b.run(stream=stream)
+    times = []
+    for _ in range(3):
+        b.set_timing(True)
+        b.run(stream=stream)
+        stream.synchronize()
+        times.append(b.kernel_time(hm + rt.RT_TIME_PER_RAY)[0])
+        b.set_timing(False)
+    t_kernel = statistics.median(times)
+    b.set_param(pr, old)
+    log("oracle counts of the per-ray kernel's work")
+    tot = None
+    for k in range(S):
+        r = jobs[k]["shadow" if hm == 2 else "primary"]
+        st = orc.search_trace(jobs[k]["v9"], r, orc.params("rayhierarchy"), hm,
+                              nthreads=host_threads())["stats"]
+        if tot is None:
+            tot = st
+        else:
+            for f in ("rays", "visits", "tests", "culls", "chains"):
+                setattr(tot, f, getattr(tot, f) + getattr(st, f))
+    kname = f"ray_path_kernel<{hm},0,false>"
+    roofline = per_ray_roofline(kname, t_kernel, tot, hm, "uneven_batch_64")
+    roofline["measured_as"] = (f"the {heavy} jobs of the batch with every block sent to the "
+                               f"per-ray kernel, median of 3 eager runs")
+
+    # CPU baseline: pbrt's BVHAccel (SAH, maxnodeprims 4, inclusive Triangle test) on the
+    # camera and shadow rays of every second scene, all host threads, median of 3; one core on
+    # every eighth scene
+    log("CPU baseline")
+    cpu = None
+    if not args.no_cpu:
+        thr = host_threads()
+        sample = list(range(0, S, 2))
+        bvhs = [orc.CpuBvh(jobs[k]["v9"], max_prims=4) for k in sample]
+        nr = sum(jobs[k]["primary"]["width"] * jobs[k]["primary"]["height"]
+                 + int(jobs[k]["shadow"]["valid"].sum()) for k in sample)
+        ts = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            for bv, k in zip(bvhs, sample):
+                bv.closest(jobs[k]["primary"], nthreads=thr)
+                bv.occluded(jobs[k]["shadow"], nthreads=thr)
+            ts.append(time.perf_counter() - c0)
+        one = list(range(0, len(sample), 4))
+        nr1 = sum(jobs[sample[i]]["primary"]["width"] * jobs[sample[i]]["primary"]["height"]
+                  + int(jobs[sample[i]]["shadow"]["valid"].sum()) for i in one)
+        c0 = time.perf_counter()
+        for i in one:
+            bvhs[i].closest(jobs[sample[i]]["primary"], nthreads=1)
+            bvhs[i].occluded(jobs[sample[i]]["shadow"], nthreads=1)
+        t1 = time.perf_counter() - c0
+        for bv in bvhs:
+            bv.close()
+        cpu = {"value": round(nr / statistics.median(ts) / 1e6, 3), "unit": "Mrays/s",
+               "cores": thr, "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"{len(sample)} of the 64 scenes (every second), their {nr} camera + "
+                         f"shadow rays, median of 3 runs; BVH build excluded",
+               "value_1core": round(nr1 / t1 / 1e6, 3),
+               "sample_1core": f"{len(one)} of those scenes, {nr1} rays, one run"}
+
     result = {
-        "metric": "Mrays/s batched vs plain per-scene launch, same hits; p50/p99 request latency",
-        "value": round(v_batch, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "uneven_batch_64", "form": "batched (128 jobs, HIP graph)",
-                   "scenes": len(jobs), "triangles": int(sum(j["v9"].shape[0] for j in jobs)),
+        "metric": METRIC, "value": round(rays_per_step / t_batch / 1e6, 3), "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "uneven_batch_64",
+                   "form": f"batched ({2 * S} jobs, HIP graph, autotuned launch shape)",
+                   "scenes": S, "triangles": int(sum(j["v9"].shape[0] for j in jobs)),
                    "primary_rays": n_prim, "shadow_rays": n_shad, "accelerator": "rayhierarchy",
                    "leaf": "4x4", "height": 3, "node": "box_dir", "launch_shape": tuned},
-        "plain": {"value": round(v_plain, 3), "ms_per_step": round(t_plain * 1e3, 3)},
-        "batched_over_plain": round(v_batch / v_plain, 3),
-        "kernels_ms": {"closest_traversal": round(k_closest, 3),
-                       "occlusion_traversal": round(k_occl, 3), "split": split,
-                       "ray_threshold_sweep": sweep},
-        "parity": {"id_mismatches": mism_id, "t_mismatches": mism_t,
-                   "occlusion_mismatches": mism_occ},
-        "roofline": None, "cpu_baseline": None,
+        "plain": {"value": round(rays_per_step / t_plain / 1e6, 3),
+                  "ms_per_step": round(t_plain * 1e3, 3), "rays": "resident in HBM"},
+        "plain_h2d": {"value": round(rays_per_step / t_h2d / 1e6, 3),
+                      "ms_per_step": round(t_h2d * 1e3, 3),
+                      "rays": "copied host-to-device (pinned) with every call"},
+        "batched_over_plain": round(t_plain / t_batch, 3),
+        "batched_over_plain_h2d": round(t_h2d / t_batch, 3),
+        "kernels_ms": {k: round(v, 3) for k, v in split.items()},
+        "parity": {"batched_vs_plain": {"id_mismatches": mism_id, "t_mismatches": mism_t,
+                                        "occlusion_mismatches": mism_occ},
+                   "oracle_sample": smp},
+        "roofline": roofline, "cpu_baseline": cpu,
     }
     b.close()
     for sc in scs:
         sc.close()
     ctx.close()
-    print(json.dumps(result))
+    return result
 
 
 def bench_frame(args, rt, scenes, torch, wl):
@@ -311,109 +515,19 @@ def bench_frame(args, rt, scenes, torch, wl):
     print(json.dumps(result))
 
 
-def bench_serving(rt, scenes, torch, rates, nreq, seed0=200):
-    """Serving form: 16 resident cluster scenes (triangle counts log-uniform in [1e3, 1e6]),
-    Poisson arrivals at each rate, resolutions uniform in {128, 256, 512, 1024} squared,
-    1-4 spp, primary + shadow rays.  Returns p50 / p99 latency and aggregate Mrays/s per
-    rate."""
-    ctx = rt.Context(0)
-    print("[bench] serving: building 16 resident scenes", file=sys.stderr, flush=True)
-    scs = []
-    for i in range(16):
-        g = np.random.Generator(np.random.PCG64(seed0 + i))
-        ntri = int(round(np.exp(g.uniform(np.log(1e3), np.log(1e6)))))
-        scs.append(rt.Scene(ctx, scenes.cluster_scene(ntri, seed=seed0 + i), desc="rayhierarchy"))
-    srv = rt.Server(ctx, scs, max_active=256)
-    points = []
-    try:
-        for rate in rates:
-            g = np.random.Generator(np.random.PCG64(seed0 + int(rate)))
-            gaps = g.exponential(1.0 / rate, size=nreq)
-            reqs = [dict(scene=int(g.integers(0, 16)), width=int(s), height=int(s),
-                         spp=int(g.integers(1, 5)), origin=(0.0, 0.0, -12.0), fov_deg=50.0,
-                         light=(0.0, 8.0, -8.0), seed=k)
-                    for k, s in enumerate(g.choice([128, 256, 512, 1024], size=nreq))]
-            print(f"[bench] serving: {nreq} requests at {rate:g} req/s", file=sys.stderr,
-                  flush=True)
-            t_start = time.perf_counter()
-            due = t_start
-            tickets = []
-            for q, gap in zip(reqs, gaps):
-                due += gap
-                while True:
-                    left = due - time.perf_counter()
-                    if left <= 0:
-                        break
-                    time.sleep(min(left, 0.002))
-                tickets.append(srv.submit(**q, outputs=True))
-            res = [srv.wait(t, timeout_s=300.0) for t in tickets]
-            t_end = time.perf_counter()
-            lat = np.array([r["latency_s"] for r in res]) * 1e3
-            rays = sum(int(r["primary_rays"]) + int((r["id"] >= 0).sum()) for r in res)
-            first = min(r["submit_s"] for r in res)
-            last = max(r["done_s"] for r in res)
-            points.append({"rate_req_s": rate, "requests": nreq,
-                           "p50_ms": round(float(np.percentile(lat, 50)), 2),
-                           "p99_ms": round(float(np.percentile(lat, 99)), 2),
-                           "mrays_s": round(rays / (last - first) / 1e6, 1),
-                           "wall_s": round(t_end - t_start, 2)})
-        ticks, most = srv.counters()
-    finally:
-        srv.close()
-        for sc in scs:
-            sc.close()
-        ctx.close()
-    return {"workload": "serving_16_scenes", "points": points, "ticks": ticks,
-            "largest_tick_jobs": most}
 
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="torus_rayhierarchy_512")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--serve-rates", default="10,20,50,100,200,500",
-                    help="request rates (req/s) of the serving leg; empty to skip it")
-    ap.add_argument("--serve-requests", type=int, default=100)
-    ap.add_argument("--autotune", action="store_true",
-                    help="uneven_batch_64: autotune the batch's launch shape before timing")
-    ap.add_argument("--ray-closest", type=int, default=0,
-                    help="per-ray threshold (passing clusters) of closest-hit traces; 0 = built-in")
-    ap.add_argument("--ray-occluded", type=int, default=0,
-                    help="per-ray threshold of occlusion traces; 0 = built-in")
-    ap.add_argument("--ray-sweep", type=lambda v: [int(x) for x in v.split(",") if x],
-                    default=[], help="uneven_batch_64: per-ray thresholds to time")
-    ap.add_argument("--copy-mb", type=int, default=0,
-                    help="after the timed steps, run one plain device copy of this many MB "
-                         "(calibrates profiler byte counters against a kernel of known traffic)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "sharding is described in DESIGN.md and not launched here.", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-
-    rt = _load("pbrt_v2_amd", ROOT / "pbrt-v2_amd" / "__init__.py", ROOT / "pbrt-v2_amd")
-    scenes = _load("pbrt_v2_amd.scenes", ROOT / "pbrt-v2_amd" / "scenes.py")
-    torch.cuda.set_device(0)
-    if args.workload == "uneven_batch_64":
-        bench_uneven(args, rt, scenes, torch)
-        return
-    wl = scenes.WORKLOADS[args.workload]
-    if "tiles" in wl:
-        bench_frame(args, rt, scenes, torch, wl)
-        return
-    print(f"[bench] workload {args.workload}: building scene", file=sys.stderr, flush=True)
+def bench_plain(args, rt, scenes, torch, orc, name):
+    """One plain per-scene workload (a scene of WORKLOADS): the primary trace, shadow rays and
+    occlusion trace (and path bounces to its depth) on the device; the dominant traversal
+    kernel against the HBM roofline by the reference candidate set's algorithmic bytes."""
+    wl = scenes.WORKLOADS[name]
+    log(f"workload {name}: building scene")
     v9 = scenes.make_scene(wl["scene"])
     rays_np = scenes.pinhole_rays(wl["W"], wl["H"], origin=wl.get("origin", (0.0, 0.0, -3.0)),
                                   fov_deg=wl.get("fov", 60.0), spp=wl.get("spp", 1),
                                   seed=wl.get("seed"))
     light = wl["light"]
     accel = wl["accel"]
-
     ctx = rt.Context(0)
     sc = rt.Scene(ctx, v9, desc=accel)
     if args.ray_closest:
@@ -427,19 +541,17 @@ def main():
                                  light)
     occ = torch.empty(n, dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
-
-    # path bounces (cosine-weighted, device-generated), each with its own shadow rays
     depth = wl.get("depth", 0)
     bounces, bhits, bshadows, boccs = [], [], [], []
     src, src_hits = prim, hits
     for dpt in range(1, depth + 1):
-        b = rt.make_bounce_rays(sc, src, src_hits, seed=17, depth=dpt)
-        bh = sc.trace_closest(b, rt.RT_CLOSEST_SECONDARY)
-        bounces.append(b)
+        bo = rt.make_bounce_rays(sc, src, src_hits, seed=17, depth=dpt)
+        bh = sc.trace_closest(bo, rt.RT_CLOSEST_SECONDARY)
+        bounces.append(bo)
         bhits.append(bh)
-        bshadows.append(rt.make_shadow_rays(b, bh, light))
+        bshadows.append(rt.make_shadow_rays(bo, bh, light))
         boccs.append(torch.empty(n, dtype=torch.uint8, device="cuda"))
-        src, src_hits = b, bh
+        src, src_hits = bo, bh
 
     def step():
         sc.trace_closest(prim, rt.RT_CLOSEST_PRIMARY, hits=hits, stream=stream)
@@ -458,11 +570,10 @@ def main():
         step()
     torch.cuda.synchronize()
     n_shadow = int(shadow.valid.sum().item())
-    n_bounce = sum(int(b.valid.sum().item()) for b in bounces)
-    n_bshadow = sum(int(b.valid.sum().item()) for b in bshadows)
+    n_bounce = sum(int(x.valid.sum().item()) for x in bounces)
+    n_bshadow = sum(int(x.valid.sum().item()) for x in bshadows)
     rays_per_step = n + n_shadow + n_bounce + n_bshadow
-
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    log(f"timing {args.steps} steps")
     sc.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -470,14 +581,224 @@ def main():
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    closest_ms, nc = sc.kernel_time(rt.RT_CLOSEST_PRIMARY)
-    second_ms, ns = sc.kernel_time(rt.RT_CLOSEST_SECONDARY)
-    occl_ms, no = sc.kernel_time(2)
+    km = {}
+    for key, m in (("closest", rt.RT_CLOSEST_PRIMARY), ("secondary", rt.RT_CLOSEST_SECONDARY),
+                   ("occlusion", 2)):
+        ms, cnt = sc.kernel_time(m)
+        if cnt:
+            km[key] = ms / cnt
+            if accel != "naive":
+                km[key + "_unit"] = sc.kernel_time(m + rt.RT_TIME_UNIT)[0] / cnt
+                km[key + "_per_ray"] = sc.kernel_time(m + rt.RT_TIME_PER_RAY)[0] / cnt
     sc.set_timing(False)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = rays_per_step * args.steps / (t1 - t0) / 1e6
     gpu_ids = hits.id.cpu().numpy()
     gpu_occ = occ.cpu().numpy()
+    result = {
+        "metric": METRIC, "value": round(rays_per_step * args.steps / (t1 - t0) / 1e6, 3),
+        "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round((t1 - t0) * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": name, "form": "plain per-scene", "accelerator": accel,
+                   "triangles": int(len(v9)),
+                   "image": f"{wl['W']}x{wl['H']}x{wl.get('spp', 1)}spp",
+                   "primary_rays": n, "shadow_rays": n_shadow + n_bshadow,
+                   "bounce_rays": n_bounce, "path_depth": depth, "leaf": "4x4", "height": 3,
+                   "node": "box_dir"},
+        "kernels_ms": {k: round(v, 4) for k, v in km.items()},
+    }
+    roofline = cpu = None
+    if depth == 0 and not args.no_cpu:
+        log("oracle counts + CPU baseline")
+        p = orc.params(accel)
+        if accel == "ray-bvh":
+            nodes, roots, order = sc.handoff
+            trace = lambda r, m: orc.trace_raybvh(v9, nodes, roots, order, r, p, m, flags=True)
+        else:
+            trace = lambda r, m: orc.trace(v9, r, p, m, flags=True)
+        ref = trace(rays_np, orc.CLOSEST_PRIMARY)
+        sh_np = scenes.shadow_rays_np(rays_np, ref["id"], ref["t"], light)
+        oref = trace(sh_np, orc.OCCLUDED)
+        result["parity"] = {"id_mismatches": int((gpu_ids != ref["id"]).sum()),
+                            "occlusion_mismatches": int((gpu_occ != oref["occ"]).sum()),
+                            "near_ties": int((ref["flags"] & orc.NEAR_TIE != 0).sum()),
+                            "ulp_marginal": int((ref["flags"] & orc.ULP_MARGINAL != 0).sum()
+                                                + (oref["flags"] & orc.ULP_MARGINAL != 0).sum())}
+        if accel != "naive":
+            # the block path's work: rays 32 B + hit 16 B (closest) or 1 B (occlusion), 36 B per
+            # (leaf, triangle) pair reaching the leaf test, 24 B per (hierarchy node, triangle)
+            # cull evaluation (+ 32 B per BVH node for ray-bvh) of the reference candidate set
+            cs, os_ = ref["stats"], oref["stats"]
+            per_cull = 56 if accel == "ray-bvh" else 24
+            bc = n * 48 + cs.leaf_pairs * 36 + cs.cull_tests * per_cull
+            bo_ = n_shadow * 33 + os_.leaf_pairs * 36 + os_.cull_tests * per_cull
+            tc, to = km.get("closest_unit", 0.0), km.get("occlusion_unit", 0.0)
+            name_k, b, t, st = (("occlusion_traversal", bo_, to, os_) if to >= tc
+                                else ("closest_traversal", bc, tc, cs))
+            if t > 0:
+                achieved = b / (t * 1e-3) / 1e9
+                ops = 45 * int(st.ray_tests)
+                tfl = ops / (t * 1e-3) / 1e12
+                facts = profile_facts(name, name_k)
+                traffic = facts.get("traffic-per-launch")
+                hbm_meas = None if traffic is None else traffic / (t * 1e-3) / 1e9 / HBM_PEAK_GBS
+                roofline = {"bound": bound_of(facts, hbm_meas, achieved / HBM_PEAK_GBS,
+                                              tfl / FP32_UNPACKED_TFLOPS),
+                            "kernel": name_k + " (unit kernel)", "achieved": round(achieved, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(achieved / HBM_PEAK_GBS, 4),
+                            "traffic": None if traffic is None else int(traffic),
+                            "bytes_per_launch": int(b),
+                            "counts": {"cull_tests": int(st.cull_tests),
+                                       "leaf_pairs": int(st.leaf_pairs)},
+                            "valu": {"flops_per_launch": ops, "achieved": round(tfl, 3),
+                                     "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
+                                     "frac": round(tfl / FP32_PEAK_TFLOPS, 4),
+                                     "peak_unpacked": FP32_UNPACKED_TFLOPS,
+                                     "busy_measured": facts.get("valu-busy")}}
+        thr = host_threads()
+        bvh = orc.CpuBvh(v9, max_prims=4)
+        ts = []
+        for _ in range(5):
+            c0 = time.perf_counter()
+            bvh.closest(rays_np, nthreads=thr)
+            bvh.occluded(sh_np, nthreads=thr)
+            ts.append(time.perf_counter() - c0)
+        bvh.close()
+        nrays = n + int(sh_np["valid"].sum())
+        cpu = {"value": round(nrays / statistics.median(ts) / 1e6, 3), "unit": "Mrays/s",
+               "cores": thr, "kind": "port", "cpu_model": cpu_model(),
+               "sample": "full workload (primary + shadow rays), median of 5 runs"}
+    result["roofline"] = roofline
+    result["cpu_baseline"] = cpu
+    sc.close()
+    ctx.close()
+    return result
+
+
+def bench_serving(args, rt, scenes, torch, rates, seed0=200):
+    """Serving form: 16 resident cluster scenes (triangle counts log-uniform in [1e3, 1e6]),
+    Poisson arrivals at each rate for about `serve_seconds` (at least 200, at most 2,000
+    requests), resolutions uniform in {128, 256, 512, 1024} squared, 1-4 spp, primary + shadow
+    rays.  p50 / p99 latency and aggregate Mrays/s per rate; sampled requests are traced again
+    alone and must give the same bits as in the mix."""
+    ctx = rt.Context(0)
+    log("serving: building 16 resident scenes")
+    scs = []
+    for i in range(16):
+        g = np.random.Generator(np.random.PCG64(seed0 + i))
+        ntri = int(round(np.exp(g.uniform(np.log(1e3), np.log(1e6)))))
+        scs.append(rt.Scene(ctx, scenes.cluster_scene(ntri, seed=seed0 + i), desc="rayhierarchy"))
+    srv = rt.Server(ctx, scs, max_active=256)
+    points = []
+    samples = []  # (request, result in the mix)
+    try:
+        for rate in rates:
+            g = np.random.Generator(np.random.PCG64(seed0 + int(rate)))
+            nreq = int(min(2000, max(200, rate * args.serve_seconds)))
+            gaps = g.exponential(1.0 / rate, size=nreq)
+            reqs = [dict(scene=int(g.integers(0, 16)), width=int(s), height=int(s),
+                         spp=int(g.integers(1, 5)), origin=(0.0, 0.0, -12.0), fov_deg=50.0,
+                         light=(0.0, 8.0, -8.0), seed=k)
+                    for k, s in enumerate(g.choice([128, 256, 512, 1024], size=nreq))]
+            log(f"serving: {nreq} requests at {rate:g} req/s")
+            t_start = time.perf_counter()
+            due = t_start
+            tickets = []
+            for k, (q, gap) in enumerate(zip(reqs, gaps)):
+                due += gap
+                while True:
+                    left = due - time.perf_counter()
+                    if left <= 0:
+                        break
+                    time.sleep(min(left, 0.002))
+                keep = k % max(1, nreq // 8) == 0 and q["width"] <= 512
+                tickets.append((srv.submit(**q, outputs=keep), keep, q))
+            res = []
+            for tk, keep, q in tickets:
+                r = srv.wait(tk, timeout_s=600.0)
+                res.append(r)
+                if keep:
+                    samples.append((q, r))
+            t_end = time.perf_counter()
+            lat = np.array([r["latency_s"] for r in res]) * 1e3
+            rays = sum(int(r["primary_rays"]) + int(r["shadow_rays"]) for r in res)
+            first = min(r["submit_s"] for r in res)
+            last = max(r["done_s"] for r in res)
+            points.append({"rate_req_s": rate, "requests": nreq,
+                           "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                           "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                           "mrays_s": round(rays / (last - first) / 1e6, 1),
+                           "wall_s": round(t_end - t_start, 2)})
+        ticks, most = srv.counters()
+        # the sampled requests again, each alone: identical bits required
+        log(f"serving: {len(samples)} sampled requests alone")
+        mism = 0
+        for q, r in samples:
+            a = srv.wait(srv.submit(**q, outputs=True), timeout_s=600.0)
+            mism += int((a["id"] != r["id"]).sum()) + int((a["occ"] != r["occ"]).sum())
+            mism += int((a["t"].view(np.uint32) != r["t"].view(np.uint32)).sum())
+    finally:
+        srv.close()
+        for sc in scs:
+            sc.close()
+        ctx.close()
+    return {"workload": "serving_16_scenes", "points": points, "ticks": ticks,
+            "largest_tick_jobs": most,
+            "alone_vs_mix": {"requests": len(samples), "mismatches": mism},
+            "rays_counted": "camera rays + valid shadow rays"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="uneven_batch_64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="uneven_batch_64: keep the built-in launch shape")
+    ap.add_argument("--oracle-scenes", type=int, default=8,
+                    help="uneven_batch_64: scenes whose sampled blocks are checked by the oracle")
+    ap.add_argument("--extra", default="torus_rayhierarchy_512",
+                    help="plain workloads reported beside the default line (comma list; "
+                         "empty for none)")
+    ap.add_argument("--serve-rates", default="10,20,50,100,200,500",
+                    help="request rates (req/s) of the serving leg; empty to skip it")
+    ap.add_argument("--serve-seconds", type=float, default=6.0,
+                    help="arrival time per rate (at least 200, at most 2,000 requests)")
+    ap.add_argument("--ray-closest", type=int, default=0,
+                    help="plain workloads: per-ray threshold of closest-hit traces (0: built-in)")
+    ap.add_argument("--ray-occluded", type=int, default=0,
+                    help="plain workloads: per-ray threshold of occlusion traces (0: built-in)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the run, one plain device copy of this many MB (calibrates "
+                         "profiler byte counters against a kernel of known traffic)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding is described in DESIGN.md and not launched here.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    rt = _load("pbrt_v2_amd", ROOT / "pbrt-v2_amd" / "__init__.py", ROOT / "pbrt-v2_amd")
+    scenes = _load("pbrt_v2_amd.scenes", ROOT / "pbrt-v2_amd" / "scenes.py")
+    orc = _load("rt_oracle_py", ROOT / "oracle" / "oracle.py")
+    torch.cuda.set_device(0)
+    if args.workload == "uneven_batch_64":
+        result = bench_uneven(args, rt, scenes, torch, orc)
+        extras = [x for x in args.extra.split(",") if x]
+        for x in extras:
+            sub = bench_plain(args, rt, scenes, torch, orc, x)
+            result[x] = {k: sub[k] for k in ("value", "unit", "ms_per_step", "config",
+                                             "kernels_ms", "parity", "roofline", "cpu_baseline")
+                         if k in sub}
+    else:
+        wl = scenes.WORKLOADS[args.workload]
+        if "tiles" in wl:
+            bench_frame(args, rt, scenes, torch, wl)
+            return
+        result = bench_plain(args, rt, scenes, torch, orc, args.workload)
     if args.copy_mb > 0:
         src = torch.empty(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -486,108 +807,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-
-    result = {
-        "metric": "Mrays/s batched vs plain per-scene launch, same hits; p50/p99 request latency",
-        "value": round(value, 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "form": "plain per-scene", "accelerator": accel,
-                   "triangles": int(len(v9)),
-                   "image": f"{wl['W']}x{wl['H']}x{wl.get('spp', 1)}spp",
-                   "primary_rays": n, "shadow_rays": n_shadow + n_bshadow,
-                   "bounce_rays": n_bounce, "path_depth": depth, "leaf": "4x4", "height": 3,
-                   "node": "box_dir"},
-        "kernels_ms": {"closest_traversal": round(closest_ms / max(nc, 1), 4),
-                       "occlusion_traversal": round(occl_ms / max(no, 1), 4)},
-    }
-    if ns:
-        result["kernels_ms"]["secondary_traversal"] = round(second_ms / ns, 4)
-
-    roofline = None
-    cpu = None
-    if not args.no_cpu and depth == 0:
-        orc = _load("rt_oracle_py", ROOT / "oracle" / "oracle.py")
-        print("[bench] oracle counts + CPU baseline", file=sys.stderr, flush=True)
-        p = orc.params(accel)
-        if accel == "ray-bvh":
-            nodes, roots, order = sc.handoff
-            trace = lambda r, m: orc.trace_raybvh(v9, nodes, roots, order, r, p, m)
-        else:
-            trace = lambda r, m: orc.trace(v9, r, p, m)
-        ref = trace(rays_np, orc.CLOSEST_PRIMARY)
-        sh_np = scenes.shadow_rays_np(rays_np, ref["id"], ref["t"], light)
-        oref = trace(sh_np, orc.OCCLUDED)
-        result["parity"] = {"id_mismatches": int((gpu_ids != ref["id"]).sum()),
-                            "occlusion_mismatches": int((gpu_occ != oref["occ"]).sum())}
-        # Algorithmic bytes per launch (reference candidate set at the reference defaults):
-        # rays 32 B + hit 16 B (closest) or 1 B (occlusion), 36 B per (leaf, triangle) pair,
-        # 24 B per (hierarchy node, triangle) cull evaluation.
-        cs, os_ = ref["stats"], oref["stats"]
-        # a ray-bvh cull evaluation reads one hierarchy node (24 B) and one GPUNode (32 B)
-        per_cull = 56 if accel == "ray-bvh" else 24
-        bytes_closest = n * 48 + cs.leaf_pairs * 36 + cs.cull_tests * per_cull
-        bytes_occl = n_shadow * 33 + os_.leaf_pairs * 36 + os_.cull_tests * per_cull
-        per_c = closest_ms / max(nc, 1)
-        per_o = occl_ms / max(no, 1)
-        if per_o >= per_c:
-            name, b, t = "occlusion_traversal", bytes_occl, per_o
-        else:
-            name, b, t = "closest_traversal", bytes_closest, per_c
-        achieved = b / (t * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": profiled_traffic(args.workload, name), "bytes_per_launch": int(b),
-                    "counts": {"cull_tests": int((os_ if name[0] == "o" else cs).cull_tests),
-                               "leaf_pairs": int((os_ if name[0] == "o" else cs).leaf_pairs)}}
-        # Arithmetic side: ~45 FLOP per ray-triangle leaf test over the oracle's leaf tests
-        # (valid rays of every reached (leaf, triangle) pair; occlusion stops testing a ray
-        # once the oracle finds it occluded).  The cull evaluations are not counted.
-        ops = 45 * int((os_ if name[0] == "o" else cs).ray_tests)
-        tflops = ops / (t * 1e-3) / 1e12
-        roofline["valu"] = {"flops_per_launch": ops, "achieved": round(tflops, 3),
-                            "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
-                            "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
-                            "peak_unpacked": FP32_UNPACKED_TFLOPS}
-        # CPU baseline: pbrt's BVHAccel on the same primary + shadow rays, all host threads
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        bvh = orc.CpuBvh(v9, max_prims=4)
-        times = []
-        for _ in range(5):
-            c0 = time.perf_counter()
-            bvh.closest(rays_np, nthreads=threads)
-            bvh.occluded(sh_np, nthreads=threads)
-            times.append(time.perf_counter() - c0)
-        # one core, for the per-core rate
-        times1 = []
-        for _ in range(3):
-            c0 = time.perf_counter()
-            bvh.closest(rays_np, nthreads=1)
-            bvh.occluded(sh_np, nthreads=1)
-            times1.append(time.perf_counter() - c0)
-        bvh.close()
-        med = statistics.median(times)
-        nrays = n + int(sh_np["valid"].sum())
-        cpu = {"value": round(nrays / med / 1e6, 3), "unit": "Mrays/s",
-               "cores": threads, "kind": "port",
-               "sample": f"full workload (primary + shadow rays), median of 5 runs, "
-                         f"pbrt BVHAccel SAH maxnodeprims 4 restated in oracle/",
-               "value_1core": round(nrays / statistics.median(times1) / 1e6, 3)}
-    result["roofline"] = roofline
-    result["cpu_baseline"] = cpu
     if args.serve_rates:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
-        result["serving"] = bench_serving(rt, scenes, torch, rates, args.serve_requests)
-    sc.close()
-    ctx.close()
+        result["serving"] = bench_serving(args, rt, scenes, torch, rates)
     print(json.dumps(result))
 
 

@@ -166,11 +166,13 @@ int rt_make_bounce_rays(rt_scene *scene, const rt_rays *in, const int32_t *id, c
  *   unchanged (batches only; needs a non-null stream).
  *   RT_PARAM_RAY_CLUSTERS_CLOSEST / RT_PARAM_RAY_CLUSTERS_OCCLUDED: a block (top-level
  *   hierarchy node) whose root passes at least this many triangle clusters (ray-bvh: clusters
- *   of frontier roots), or more than half of the scene's clusters, is traced ray by ray --
+ *   of frontier roots), or whose rays' directions differ by more than 1.0 in some component
+ *   (an incoherent group, e.g. random shadow segments or diffuse bounces), is traced ray by ray --
  *   each ray walks a conservative BVH of the triangles and the reference's exact cull chain
  *   is evaluated for the triangles that pass its strict leaf test -- in place of enumerating
- *   the block's (leaf, triangle) candidate pairs (built-in: 256).  1 sends every block with
- *   work that way; a value above 2^29 (at most 2^30) never does.
+ *   the block's (leaf, triangle) candidate pairs (built-in: 256 for closest hits, 2^29 for
+ *   occlusion).  1 sends every block with work that way; 2^29 keeps only the direction test;
+ *   a value above 2^29 (at most 2^30) sends no block.
  *   RT_PARAM_RAY_GRID_CLOSEST / RT_PARAM_RAY_GRID_OCCLUDED: work-groups (one wavefront each)
  *   of the per-ray kernel (batches only); 0 = one per resident slot. */
 enum {
@@ -255,6 +257,7 @@ typedef struct rt_request_stats {
   double submit_s, admit_s, done_s; /* steady-clock seconds */
   uint64_t primary_rays;             /* camera rays; one shadow ray slot per camera ray */
   int32_t status;                   /* rt_status of the request */
+  uint64_t shadow_rays;              /* valid shadow rays traced (camera rays that hit) */
 } rt_request_stats;
 /* scenes: rayhierarchy or ray-bvh scenes (with their BVH) that stay resident for the
  * server's lifetime; any other scene gives RT_ERR_UNSUPPORTED.

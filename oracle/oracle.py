@@ -288,7 +288,7 @@ def exact_closest(v9, rays: dict):
 class OrcSearchStats(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_uint64), ("visits", ctypes.c_uint64),
                 ("tests", ctypes.c_uint64), ("culls", ctypes.c_uint64),
-                ("nodes", ctypes.c_uint64)]
+                ("nodes", ctypes.c_uint64), ("chains", ctypes.c_uint64)]
 
 
 def search_trace(v9, rays: dict, p: OrcParams, mode: int, handoff=None, mask=None, nthreads=0):

@@ -1273,7 +1273,7 @@ bool s_chain(const SearchCtx& c, int x, int y, uint32_t tri, uint32_t leaf, uint
 }
 
 struct SCount {
-  uint64_t rays = 0, visits = 0, tests = 0, culls = 0;
+  uint64_t rays = 0, visits = 0, tests = 0, culls = 0, chains = 0;
 };
 
 void s_trace_ray(const SearchCtx& c, const orc_rays* r, size_t i, Out& o, SCount& cnt) {
@@ -1328,11 +1328,13 @@ void s_trace_ray(const SearchCtx& c, const orc_rays* r, size_t i, Out& o, SCount
         if (!mt_strict(org, d, T.tri[slot], t, b1, b2)) continue;
         if (c.mode == ORC_OCCLUDED) {
           if (!(t > tmin && t < tmax)) continue;
+          ++cnt.chains;
           occluded = s_chain(c, x, y, slot, leaf, cnt.culls);
           continue;
         }
         if (!(t > tmin)) continue;
         if (!(t < best_t || (t == best_t && (int32_t)id < best_id))) continue;
+        ++cnt.chains;
         if (!s_chain(c, x, y, slot, leaf, cnt.culls)) continue;
         best_t = t;
         best_id = (int32_t)id;
@@ -1567,13 +1569,14 @@ int orc_search_trace(const float* v9, uint32_t ntri, const orc_gpu_node* nodes, 
       s_trace_ray(c, r, i, o, per[i]);
     });
   if (st) {
-    *st = orc_search_stats{0, 0, 0, 0, 0};
+    *st = orc_search_stats{0, 0, 0, 0, 0, 0};
     st->nodes = T.nodes.size();
     for (const SCount& q : per) {
       st->rays += q.rays;
       st->visits += q.visits;
       st->tests += q.tests;
       st->culls += q.culls;
+      st->chains += q.chains;
     }
   }
   return 0;

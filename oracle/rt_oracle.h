@@ -102,6 +102,7 @@ typedef struct orc_search_stats {
   uint64_t tests;  /* strict leaf tests */
   uint64_t culls;  /* chain cull evaluations (hierarchy node x triangle or BVH node) */
   uint64_t nodes;  /* interior nodes of the search BVH */
+  uint64_t chains; /* cull chains evaluated (strict-test hits that could change the ray) */
 } orc_search_stats;
 /* Per-ray traversal of the search BVH (Morton radix splits, leaves of <= 4, boxes grown by
  * 2^-11 max(1, max |coordinate|)) with the strict leaf test and the reference's exact cull

@@ -87,7 +87,7 @@ class RtRequest(ctypes.Structure):
 class RtRequestStats(ctypes.Structure):
     _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
                 ("done_s", ctypes.c_double), ("primary_rays", ctypes.c_uint64),
-                ("status", ctypes.c_int32)]
+                ("status", ctypes.c_int32), ("shadow_rays", ctypes.c_uint64)]
 
 
 class RtRays(ctypes.Structure):
@@ -491,7 +491,8 @@ class Server:
               "rt_server_wait")
         out = self._out.pop(ticket, None)
         stats = dict(submit_s=st.submit_s, admit_s=st.admit_s, done_s=st.done_s,
-                     latency_s=st.done_s - st.submit_s, primary_rays=st.primary_rays)
+                     latency_s=st.done_s - st.submit_s, primary_rays=st.primary_rays,
+                     shadow_rays=st.shadow_rays)
         if out is not None:
             stats.update(id=out[0], t=out[1], occ=out[2])
         return stats

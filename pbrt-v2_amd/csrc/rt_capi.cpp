@@ -354,8 +354,10 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int kDefaultPer[2] = {8, 4};  // passing clusters per unit: closest, occluded
 // Blocks whose root passes at least this many clusters (64 triangles or 64 frontier roots
-// each), or more than half of the scene's, are traced ray by ray: closest, occluded.
-constexpr int kDefaultRay[2] = {256, 256};
+// each), or whose ray directions spread widely, are traced ray by ray: closest, occluded.
+// 2^29 keeps only the direction test: shadow rays toward a light pass many clusters near
+// depth discontinuities, and the block path still handles those blocks faster.
+constexpr int kDefaultRay[2] = {256, 1 << 29};
 constexpr int kDefaultRootClustersPerUnit = 8;  // passing root clusters per ray-bvh unit
 
 // Lays out the jobs as launch groups (one per mode and node type, in job order) in the
@@ -1169,10 +1171,10 @@ int rt_batch_autotune(rt_batch* b, void* stream, int32_t reps) {
   b->timer.on = true;
   // stage 1: which blocks take the per-ray path (1: all, 2^30: none) and the per-ray grid
   // (one wavefront per work-group: 8 or 16 per CU, or every resident slot)
-  static const int kRay[] = {1, 64, 256, 1 << 30};
+  static const int kRay[] = {1, 64, 256, 1 << 29, 1 << 30};
   static const int kRGrid[] = {0, 2048, 4096};
   int st = autotune_stage(
-      b, hs, reps, 4 * 3,
+      b, hs, reps, 5 * 3,
       [&](int c, RhParams& p) {
         p.ray[0] = p.ray[1] = kRay[c / 3];
         p.rgrid[0] = p.rgrid[1] = kRGrid[c % 3];

@@ -127,7 +127,10 @@ struct RhGroup {
 // (ref0, ref1 as bits, 0, 0).  A ref with bit 31 clear is an interior node index; with bit
 // 31 set it is a leaf of (bits 27..30) + 1 triangles starting at sorted slot (bits 0..26).
 // Boxes are grown outward by the scene's padding.  The depth is at most kSearchDepth.
-constexpr int kSearchDepth = 30;
+#ifndef RT_SEARCH_DEPTH
+#define RT_SEARCH_DEPTH 30
+#endif
+constexpr int kSearchDepth = RT_SEARCH_DEPTH;
 constexpr int kSearchLeaf = 4;
 // Builds the search BVH of `ntri` triangles (v9 in the order the leaves index) whose 30-bit
 // Morton codes `code` are sorted ascending.  Returns the node array (4 float4 per node).
@@ -156,9 +159,10 @@ hipError_t launch_bounce_rays(const RaysDev& in, const float* v9, const int32_t*
                               const float* t, uint32_t seed, int depth, float* ox, float* oy,
                               float* oz, float* dx, float* dy, float* dz, float* tmin,
                               float* tmax, uint8_t* valid, hipStream_t s);
+// count (optional): incremented by the number of valid shadow rays written.
 hipError_t launch_shadow_rays(const RaysDev& pr, const int32_t* id, const float* t, float lx,
                               float ly, float lz, float* ox, float* oy, float* oz, float* dx,
                               float* dy, float* dz, float* tmin, float* tmax, uint8_t* valid,
-                              hipStream_t s);
+                              hipStream_t s, uint32_t* count = nullptr);
 
 }  // namespace rti

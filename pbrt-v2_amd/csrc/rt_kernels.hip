@@ -631,9 +631,11 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
 // passing cluster ids in the job's clist[block * nclusters + ...] and appends work units
 // {job, block, first, count} of at most `per_unit` passing clusters.
 //
-// A block whose root passes at least `ray_clusters` clusters, or more than half of the scene's,
-// is an incoherent ray group: its candidate set holds most (leaf, triangle) pairs, so it is
-// queued for the per-ray path instead, as 64-ray chunks {job, block << 4 | chunk}.
+// A block whose root passes at least `ray_clusters` clusters, or whose rays' directions
+// differ by more than kWideDir in some component, is an incoherent ray group: its candidate
+// set holds most (leaf, triangle) pairs, so it is queued for the per-ray path instead, as
+// 64-ray chunks {job, block << 4 | chunk}.
+constexpr float kWideDir = 1.0f;
 template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
                                                          int nitems, uint4* __restrict__ units,
@@ -673,9 +675,12 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   }
   __syncthreads();
   const int cnt = sCnt;
-  // incoherent: at least `ray_clusters` clusters pass, or more than half of the scene's
-  // (ray_clusters above 2^29 turns the per-ray path off)
-  if ((uint32_t)cnt >= ray_clusters || (ray_clusters <= (1u << 29) && 2u * (uint32_t)cnt > nclusters)) {
+  // incoherent: at least `ray_clusters` clusters pass, or the rays' directions spread over
+  // more than kWideDir in some axis (ray_clusters above 2^29 turns the per-ray path off)
+  const float spread = fmaxf(fmaxf(root.dmax.x - root.dmin.x, root.dmax.y - root.dmin.y),
+                             root.dmax.z - root.dmin.z);
+  if ((uint32_t)cnt >= ray_clusters ||
+      (ray_clusters <= (1u << 29) && cnt > 0 && spread > kWideDir)) {
     const int nch = (g.rb + 63) / 64;
     if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
     __syncthreads();
@@ -1303,8 +1308,11 @@ __device__ __forceinline__ bool ray_chain_bvh(const RhItem& it, int x, int y, ui
   return true;
 }
 
+#ifndef RT_RAY_WAVES
+#define RT_RAY_WAVES 1
+#endif
 template <int MODE, int NODE, bool BVH>
-__global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__ items,
+__global__ __launch_bounds__(64, RT_RAY_WAVES) void ray_path_kernel(const RhItem* __restrict__ items,
                                                       const uint2* __restrict__ rq,
                                                       uint32_t* __restrict__ ctr) {
   __shared__ uint32_t sStk[kSearchDepth][64];
@@ -1328,10 +1336,14 @@ __global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__
     bool have = slot_xy(g, K, J, (int)(wy & 15u) * 64 + lane, x, y, leaf) && it.ntri > 0;
     const size_t idx = (size_t)y * g.W + x;
     if (have && MODE != 0 && r.valid != nullptr && glb(r.valid)[idx] == 0) have = false;
-    if (!have) continue;
-    const V3 o = {glb(r.ox)[idx], glb(r.oy)[idx], glb(r.oz)[idx]};
-    const V3 d = {glb(r.dx)[idx], glb(r.dy)[idx], glb(r.dz)[idx]};
-    const float tmin = glb(r.tmin)[idx], tmax = glb(r.tmax)[idx];
+    V3 o = {0.f, 0.f, 0.f}, d = {0.f, 0.f, 1.f};
+    float tmin = 0.f, tmax = 0.f;
+    if (have) {
+      o = {glb(r.ox)[idx], glb(r.oy)[idx], glb(r.oz)[idx]};
+      d = {glb(r.dx)[idx], glb(r.dy)[idx], glb(r.dz)[idx]};
+      tmin = glb(r.tmin)[idx];
+      tmax = glb(r.tmax)[idx];
+    }
     // inverse direction for the slab test; zero components become tiny ones, so the slab of
     // that axis is (almost) the whole line when the origin is inside it and empty otherwise
     auto safe = [](float v) { return fabsf(v) < 0x1p-80f ? (v < 0.f ? -0x1p-80f : 0x1p-80f) : v; };
@@ -1345,68 +1357,93 @@ __global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__
     uint32_t rep = kNoKey32;
     bool occluded = false;
     int sp = 0;
-    uint32_t cur = 0;  // interior node 0 (the root)
-    for (;;) {
-      if (!(cur & kSearchLeafBit)) {
-        const float4 n0 = ld4(snode, 4 * (size_t)cur), n1 = ld4(snode, 4 * (size_t)cur + 1),
-                     n2 = ld4(snode, 4 * (size_t)cur + 2), n3 = ld4(snode, 4 * (size_t)cur + 3);
-        float thi = MODE == 2 ? tmax : (best == kMissKey ? __int_as_float(0x7f800000)
-                                                         : fkey_inv((uint32_t)(best >> 32)));
-        thi = thi + fabsf(thi) * kSlack;
-        float tn0, tn1;
-        const bool h0 = slab(o, inv, V3{n0.x, n0.y, n0.z}, V3{n0.w, n1.x, n1.y}, tlo, thi, tn0);
-        const bool h1 = slab(o, inv, V3{n1.z, n1.w, n2.x}, V3{n2.y, n2.z, n2.w}, tlo, thi, tn1);
-        const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
-        if (h0 && h1) {
-          const bool first1 = tn1 < tn0;
-          sStk[sp++][lane] = first1 ? r0 : r1;
-          cur = first1 ? r1 : r0;
-          continue;
+    // Traversal in rounds (Aila and Laine's while-while): a lane walks interior nodes until it
+    // holds a leaf, postpones that leaf and walks on while other lanes of the wavefront still
+    // look for one; then every lane tests its postponed leaf.  Node and leaf work of different
+    // lanes thus run in separate phases instead of splitting the wavefront.
+    constexpr uint32_t kDone = 0xffffffffu;  // not a node reference (leaves hold <= 4 triangles)
+    uint32_t cur = have ? 0u : kDone;  // interior node 0 (the root)
+    uint32_t pl = kDone;               // postponed leaf
+    auto pop = [&]() { return sp > 0 ? sStk[--sp][lane] : kDone; };
+    auto test_leaf = [&](uint32_t ref) {
+      const uint32_t first = ref & ((1u << 27) - 1u), cnt = ((ref >> 27) & 15u) + 1u;
+      for (uint32_t f = 0; f < cnt && !occluded; ++f) {
+        const uint32_t slot = first + f;
+        const uint32_t id = perm[slot];
+        uint32_t bl = 0;
+        if constexpr (BVH) {
+          bl = glb(it.tleaf)[id];
+          if (bl == 0xffffffffu) continue;
+          if (MODE == 1 && (bl & 0x80000000u)) continue;  // first primitive only
+          bl &= 0x7fffffffu;
         }
-        if (h0 || h1) {
-          cur = h0 ? r0 : r1;
-          continue;
-        }
-      } else {
-        const uint32_t first = cur & ((1u << 27) - 1u), cnt = ((cur >> 27) & 15u) + 1u;
-        for (uint32_t f = 0; f < cnt && !occluded; ++f) {
-          const uint32_t slot = first + f;
-          const uint32_t id = perm[slot];
-          uint32_t bl = 0;
-          if constexpr (BVH) {
-            bl = glb(it.tleaf)[id];
-            if (bl == 0xffffffffu) continue;
-            if (MODE == 1 && (bl & 0x80000000u)) continue;  // first primitive only
-            bl &= 0x7fffffffu;
-          }
-          float t, b1, b2;
-          if (!mt_strict(o, d, load_mt(tmts, slot), t, b1, b2)) continue;
-          if constexpr (MODE == 2) {
-            if (!(t > tmin && t < tmax)) continue;  // intersectAllLeaves.cl:177-179
-          } else {
-            const bool main = t > tmin && t < __int_as_float(0x7f800000);
-            const bool rp = track_rep && repair_region(b1, b2) && t >= tmin;
-            const uint64_t key = hit_key(t, (int32_t)id);
-            const bool um = main && key < best, ur = rp && fkey(t) < rep;
-            if (!um && !ur) continue;
-            bool ok;
-            if constexpr (BVH) ok = ray_chain_bvh<NODE>(it, x, y, bl);
-            else ok = ray_chain_rh<MODE, NODE>(it, x, y, slot);
-            if (!ok) continue;
-            if (um) best = key;
-            if (ur) rep = fkey(t);
-            continue;
-          }
+        float t, b1, b2;
+        if (!mt_strict(o, d, load_mt(tmts, slot), t, b1, b2)) continue;
+        if constexpr (MODE == 2) {
+          if (!(t > tmin && t < tmax)) continue;  // intersectAllLeaves.cl:177-179
           bool ok;
           if constexpr (BVH) ok = ray_chain_bvh<NODE>(it, x, y, bl);
           else ok = ray_chain_rh<MODE, NODE>(it, x, y, slot);
           occluded = ok;
+        } else {
+          const bool main = t > tmin && t < __int_as_float(0x7f800000);
+          const bool rp = track_rep && repair_region(b1, b2) && t >= tmin;
+          const uint64_t key = hit_key(t, (int32_t)id);
+          const bool um = main && key < best, ur = rp && fkey(t) < rep;
+          if (!um && !ur) continue;
+          bool ok;
+          if constexpr (BVH) ok = ray_chain_bvh<NODE>(it, x, y, bl);
+          else ok = ray_chain_rh<MODE, NODE>(it, x, y, slot);
+          if (!ok) continue;
+          if (um) best = key;
+          if (ur) rep = fkey(t);
         }
-        if (occluded) break;
       }
-      if (sp == 0) break;
-      cur = sStk[--sp][lane];
+    };
+    for (;;) {
+      // node phase
+      for (;;) {
+        if (cur != kDone && !(cur & kSearchLeafBit)) {
+          const float4 n0 = ld4(snode, 4 * (size_t)cur), n1 = ld4(snode, 4 * (size_t)cur + 1),
+                       n2 = ld4(snode, 4 * (size_t)cur + 2),
+                       n3 = ld4(snode, 4 * (size_t)cur + 3);
+          float thi = MODE == 2 ? tmax : (best == kMissKey ? __int_as_float(0x7f800000)
+                                                           : fkey_inv((uint32_t)(best >> 32)));
+          thi = thi + fabsf(thi) * kSlack;
+          float tn0, tn1;
+          const bool h0 = slab(o, inv, V3{n0.x, n0.y, n0.z}, V3{n0.w, n1.x, n1.y}, tlo, thi, tn0);
+          const bool h1 = slab(o, inv, V3{n1.z, n1.w, n2.x}, V3{n2.y, n2.z, n2.w}, tlo, thi, tn1);
+          const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
+          if (h0 && h1) {
+            const bool first1 = tn1 < tn0;
+            sStk[sp++][lane] = first1 ? r0 : r1;
+            cur = first1 ? r1 : r0;
+          } else if (h0 || h1) {
+            cur = h0 ? r0 : r1;
+          } else {
+            cur = pop();
+          }
+          if (pl == kDone && cur != kDone && (cur & kSearchLeafBit)) {
+            pl = cur;
+            cur = pop();
+          }
+        }
+        // walk on while some lane has neither a postponed leaf nor finished
+        if (!__any(pl == kDone && cur != kDone && !(cur & kSearchLeafBit))) break;
+      }
+      // leaf phase
+      if (pl != kDone) {
+        test_leaf(pl);
+        pl = kDone;
+      }
+      if (cur != kDone && (cur & kSearchLeafBit)) {  // a second leaf waits for the next round
+        pl = cur;
+        cur = pop();
+      }
+      if (occluded) cur = pl = kDone;
+      if (!__any(cur != kDone || pl != kDone)) break;
     }
+    if (!have) continue;
     if constexpr (MODE == 2) {
       if (occluded) glb(it.occ)[idx] = 1;
     } else {
@@ -1690,13 +1727,19 @@ __global__ __launch_bounds__(256) void rh_repair_kernel(const RhItem* __restrict
 }
 
 // Shadow rays toward a point light (light.h:79-84, point.cpp:42-49).
+// When `count` is given, the number of valid shadow rays is added to it.
 __global__ __launch_bounds__(256) void shadow_kernel(RaysDev pr, const int32_t* __restrict__ id,
                                                      const float* __restrict__ tt, float lx,
                                                      float ly, float lz, float* ox, float* oy,
                                                      float* oz, float* dx, float* dy, float* dz,
-                                                     float* tmin, float* tmax, uint8_t* valid) {
+                                                     float* tmin, float* tmax, uint8_t* valid,
+                                                     uint32_t* count) {
   const size_t n = (size_t)pr.W * pr.H;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (count) {
+    const uint64_t bal = __ballot(i < n && id[i] >= 0);
+    if (__lane_id() == 0 && bal) atomicAdd(count, (uint32_t)__popcll(bal));
+  }
   if (i >= n) return;
   if (id[i] < 0) {
     ox[i] = oy[i] = oz[i] = 0.f;
@@ -2036,10 +2079,10 @@ hipError_t launch_bounce_rays(const RaysDev& in, const float* v9, const int32_t*
 hipError_t launch_shadow_rays(const RaysDev& pr, const int32_t* id, const float* t, float lx,
                               float ly, float lz, float* ox, float* oy, float* oz, float* dx,
                               float* dy, float* dz, float* tmin, float* tmax, uint8_t* valid,
-                              hipStream_t s) {
+                              hipStream_t s, uint32_t* count) {
   const size_t n = (size_t)pr.W * pr.H;
   hipLaunchKernelGGL(shadow_kernel, grid1(n), dim3(kThreads), 0, s, pr, id, t, lx, ly, lz, ox,
-                     oy, oz, dx, dy, dz, tmin, tmax, valid);
+                     oy, oz, dx, dy, dz, tmin, tmax, valid, count);
   return hipGetLastError();
 }
 

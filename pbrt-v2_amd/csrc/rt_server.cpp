@@ -99,6 +99,7 @@ struct Req {
   int32_t* id = nullptr;
   float *t = nullptr, *u = nullptr, *v = nullptr;
   uint8_t* occ = nullptr;
+  uint32_t* nshadow = nullptr;  // valid shadow rays, counted on the device
 
   rt_rays primary() const {
     rt_rays r{ray[0], ray[1], ray[2], ray[3], ray[4], ray[5], ray[6], ray[7], nullptr,
@@ -145,7 +146,7 @@ int to_status(hipError_t e) {
   return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
 }
 
-size_t block_bytes(size_t n) { return 16 * al(4 * n) + 2 * al(n) + 4 * al(4 * n); }
+size_t block_bytes(size_t n) { return 16 * al(4 * n) + 2 * al(n) + 4 * al(4 * n) + 256; }
 
 int setup_request(rt_server* S, Req* r) {
   const size_t n = r->n;
@@ -168,6 +169,8 @@ int setup_request(rt_server* S, Req* r) {
   r->v = (float*)(p + o);
   o += al(4 * n);
   r->occ = (uint8_t*)(p + o);
+  o += al(n);
+  r->nshadow = (uint32_t*)(p + o);
   // the same double-precision steps as the workloads' camera (degrees -> radians, halve)
   const double tan_half = std::tan((double)r->q.fov_deg * (3.14159265358979323846 / 180.0) / 2.0);
   const float aspect = (float)((double)r->q.width / (double)r->q.height);
@@ -183,13 +186,16 @@ int deliver(rt_server* S, Req* r) {
   const size_t n = r->n;
   hipError_t e = hipSuccess;
   if (r->st.status == RT_OK && r->has_block) {
-    if (r->q.host_id)
+    uint32_t ns = 0;
+    e = hipMemcpyAsync(&ns, r->nshadow, 4, hipMemcpyDeviceToHost, S->copy);
+    if (e == hipSuccess && r->q.host_id)
       e = hipMemcpyAsync(r->q.host_id, r->id, 4 * n, hipMemcpyDeviceToHost, S->copy);
     if (e == hipSuccess && r->q.host_t)
       e = hipMemcpyAsync(r->q.host_t, r->t, 4 * n, hipMemcpyDeviceToHost, S->copy);
     if (e == hipSuccess && r->q.host_occluded)
       e = hipMemcpyAsync(r->q.host_occluded, r->occ, n, hipMemcpyDeviceToHost, S->copy);
     if (e == hipSuccess) e = hipStreamSynchronize(S->copy);
+    r->st.shadow_rays = ns;
   }
   return to_status(e);
 }
@@ -316,9 +322,16 @@ void loop(rt_server* S) {
       if (st) break;
       if (r->phase == 1) {
         const rt_rays rays = r->primary();
-        st = rt_make_shadow_rays(&rays, r->id, r->t, r->q.light, r->sray[0], r->sray[1],
-                                 r->sray[2], r->sray[3], r->sray[4], r->sray[5], r->sray[6],
-                                 r->sray[7], r->svalid, S->stream);
+        const rti::RaysDev pr{rays.ox,   rays.oy,   rays.oz, rays.dx,   rays.dy,
+                              rays.dz,   rays.tmin, rays.tmax, nullptr, rays.width,
+                              rays.height};
+        st = to_status(rti::launch_fill_u8((uint8_t*)r->nshadow, 0, 4, S->stream));
+        if (!st)
+          st = to_status(rti::launch_shadow_rays(pr, r->id, r->t, r->q.light[0], r->q.light[1],
+                                                 r->q.light[2], r->sray[0], r->sray[1],
+                                                 r->sray[2], r->sray[3], r->sray[4], r->sray[5],
+                                                 r->sray[6], r->sray[7], r->svalid, S->stream,
+                                                 r->nshadow));
         r->phase = 2;
       } else if (r->phase == 2) {
         r->phase = 4;

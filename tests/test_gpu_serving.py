@@ -31,6 +31,7 @@ def test_request_matches_plain_trace(rt, scenes, gpu_ctx):
     print(f"[serve] hits {int((ids >= 0).sum())} / {ids.size}", flush=True)
     assert np.array_equal(res["id"], ids)
     assert np.array_equal(res["t"].view(np.uint32), ts.view(np.uint32))
+    assert res["shadow_rays"] == int((ids >= 0).sum())
     sh = rt.make_shadow_rays(d, h, (2.0, 2.0, -2.0))
     occ = scs[0].trace_occluded(sh).cpu().numpy()
     assert np.array_equal(res["occ"], occ)
