@@ -29,7 +29,8 @@ typedef enum rt_status {
   RT_ERR_OOM = 3,         /* device allocation failed */
   RT_ERR_UNSUPPORTED = 4, /* parameter combination this build does not implement */
   RT_ERR_NO_DEVICE = 5,   /* no usable gfx950 device */
-  RT_ERR_STATE = 6        /* call order violated (e.g. ray-bvh trace before rt_scene_set_bvh) */
+  RT_ERR_STATE = 6,       /* call order violated (e.g. ray-bvh trace before rt_scene_set_bvh) */
+  RT_ERR_TOO_LARGE = 7    /* input exceeds the static scratch of rt_trace_closest_noalloc */
 } rt_status;
 
 /* Accelerator names of MakeAccelerator (src/core/api.cpp:576-596). */
@@ -133,6 +134,18 @@ int rt_trace_closest(rt_scene *scene, const rt_rays *rays, int32_t mode, const r
 /* Batched occlusion: IntersectP (rayhierarchy.cpp:1170-1326, rayBVH.cpp:1170-1323,
  * naive.cpp:357-437).  occluded[i] = 1 when ray i is valid and blocked in (tmin, tmax). */
 int rt_trace_occluded(rt_scene *scene, const rt_rays *rays, uint8_t *occluded, void *stream);
+
+/* rt_trace_closest without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and the same results as rt_trace_closest; its
+ * scratch (hierarchy, reduction keys, work queue, job table) lives in a static device array of
+ * 64 MiB per device, set up by rt_ctx_create and held until the process ends.  A call needs
+ * at most about 22 bytes per ray, its hierarchy included (naive: 8 bytes per ray), so up to
+ * about 3 million rays; a larger one returns RT_ERR_TOO_LARGE.  Every call on a device shares that array, which keeps
+ * the last call's contents: calls that overlap in time on one device race (serialize them,
+ * e.g. on one stream). */
+int rt_trace_closest_noalloc(rt_scene *scene, const rt_rays *rays, int32_t mode,
+                             const rt_hits *hits, void *stream);
 
 /* Shadow rays toward a point light from closest hits, as the directlighting integrator builds
  * them (VisibilityTester::SetSegment, src/core/light.h:79-84, PointLight::Sample_L,

@@ -37,7 +37,7 @@ EXPORTS = (
     "rt_batch_add_occluded", "rt_batch_run", "rt_batch_set_param", "rt_batch_get_param",
     "rt_batch_autotune", "rt_batch_set_timing", "rt_batch_kernel_time",
     "rt_server_create", "rt_server_submit", "rt_server_wait", "rt_server_counters",
-    "rt_server_destroy", "rt_make_bounce_rays",
+    "rt_server_destroy", "rt_make_bounce_rays", "rt_trace_closest_noalloc",
 )
 (RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED, RT_PARAM_GRID_CLOSEST,
  RT_PARAM_GRID_OCCLUDED, RT_PARAM_GRAPH, RT_PARAM_RAY_CLUSTERS_CLOSEST,

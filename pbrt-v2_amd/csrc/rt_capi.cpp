@@ -360,6 +360,45 @@ constexpr int kDefaultPer[2] = {8, 4};  // passing clusters per unit: closest, o
 constexpr int kDefaultRay[2] = {256, 1 << 29};
 constexpr int kDefaultRootClustersPerUnit = 8;  // passing root clusters per ray-bvh unit
 
+// The job-table entry of one job: geometry of its ray image and the scene's resident arrays
+// (scratch pointers and prefix sums are filled by the caller).
+int fill_item(const RhJob& jb, RhItem& it) {
+  const bool bvh = jb.s->desc.kind == RT_ACCEL_RAYBVH;
+  std::memset(&it, 0, sizeof(it));
+  if (rh_geometry(jb.r.W, jb.r.H, jb.s->desc.chunk_x, jb.s->desc.chunk_y, jb.s->desc.height,
+                  &it.g))
+    return RT_ERR_UNSUPPORTED;
+  // LDS-staged blocks hold at most 1024 rays; queue entries pack the triangle slot in 26 bits
+  if (it.g.rb > 1024 || jb.s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
+  if (bvh && jb.s->ntri > 0 && !jb.s->has_bvh) return RT_ERR_STATE;
+  it.r = jb.r;
+  it.v9 = jb.s->d_v9;
+  it.uv6 = jb.s->d_uv6;
+  it.v9s = jb.s->d_v9s;
+  it.perm = jb.s->d_perm;
+  it.cbox = jb.s->d_cbox;
+  it.tbox = jb.s->d_tbox;
+  it.tmt = jb.s->d_tmt;
+  it.snode = jb.s->d_snode;
+  it.tmts = jb.s->d_tmts;
+  it.tleaf = jb.s->d_tleaf;
+  it.ntri = jb.s->ntri;
+  it.nclusters = bvh ? (jb.s->ntri ? jb.s->nrclusters : 0) : jb.s->nclusters;
+  if (bvh) {
+    it.cbox = jb.s->d_rcbox;
+    it.nroots = jb.s->nroots;
+    it.bvh = jb.s->d_bvh;
+    it.roots = jb.s->d_roots;
+    it.prim_order = jb.s->d_prim_order;
+    it.bvh_parent = jb.s->d_bvh_parent;
+    it.bvh_depth = jb.s->d_bvh_depth;
+  }
+  it.out = jb.out;
+  it.occ = jb.occ;
+  it.repair_run = (jb.mode == 0 && !bvh) ? jb.s->desc.repair_run : 0;
+  return RT_OK;
+}
+
 // Lays out the jobs as launch groups (one per mode and node type, in job order) in the
 // workspace arena, fills the job tables and uploads them when they changed.  `changed` tells
 // whether the device tables or arena were rewritten.
@@ -416,39 +455,9 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     for (int j : members[gi]) {
       const RhJob& jb = jobs[j];
       RhItem& it = items[j];
-      std::memset(&it, 0, sizeof(it));
-      if (rh_geometry(jb.r.W, jb.r.H, jb.s->desc.chunk_x, jb.s->desc.chunk_y, jb.s->desc.height,
-                      &it.g))
-        return RT_ERR_UNSUPPORTED;
-      // LDS-staged blocks hold at most 1024 rays; queue entries pack the triangle slot in 26 bits
-      if (it.g.rb > 1024 || jb.s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
-      if (bvh && jb.s->ntri > 0 && !jb.s->has_bvh) return RT_ERR_STATE;
+      const int fst = fill_item(jb, it);
+      if (fst) return fst;
       const size_t n = (size_t)jb.r.W * jb.r.H;
-      it.r = jb.r;
-      it.v9 = jb.s->d_v9;
-      it.uv6 = jb.s->d_uv6;
-      it.v9s = jb.s->d_v9s;
-      it.perm = jb.s->d_perm;
-      it.cbox = jb.s->d_cbox;
-      it.tbox = jb.s->d_tbox;
-      it.tmt = jb.s->d_tmt;
-      it.snode = jb.s->d_snode;
-      it.tmts = jb.s->d_tmts;
-      it.tleaf = jb.s->d_tleaf;
-      it.ntri = jb.s->ntri;
-      it.nclusters = bvh ? (jb.s->ntri ? jb.s->nrclusters : 0) : jb.s->nclusters;
-      if (bvh) {
-        it.cbox = jb.s->d_rcbox;
-        it.nroots = jb.s->nroots;
-        it.bvh = jb.s->d_bvh;
-        it.roots = jb.s->d_roots;
-        it.prim_order = jb.s->d_prim_order;
-        it.bvh_parent = jb.s->d_bvh_parent;
-        it.bvh_depth = jb.s->d_bvh_depth;
-      }
-      it.out = jb.out;
-      it.occ = jb.occ;
-      it.repair_run = (mode == 0 && !bvh) ? jb.s->desc.repair_run : 0;
       it.block_base = (uint32_t)blocks;
       it.ray_base = (uint32_t)rays;
       blocks += (uint64_t)it.g.ntop;
@@ -597,6 +606,8 @@ int rt_ctx_create(int32_t device, rt_ctx** out) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return RT_ERR_NO_DEVICE;
   if (device < 0 || device >= n) return RT_ERR_INVALID;
   RT_TRY(hipSetDevice(device));
+  // the allocation-free trace's scratch: its address on this device, looked up once
+  if (noalloc_setup(device)) return RT_ERR_HIP;
   rt_ctx* c = new (std::nothrow) rt_ctx{device};
   if (!c) return RT_ERR_OOM;
   *out = c;
@@ -904,6 +915,78 @@ int rt_trace_closest(rt_scene* s, const rt_rays* rays, int32_t mode, const rt_hi
 int rt_trace_occluded(rt_scene* s, const rt_rays* rays, uint8_t* occluded, void* stream) {
   if (!s || !occluded) return RT_ERR_INVALID;
   return trace_impl(s, rays, 2, nullptr, occluded, stream);
+}
+
+int rt_trace_closest_noalloc(rt_scene* s, const rt_rays* rays, int32_t mode, const rt_hits* hits,
+                             void* stream) {
+  if (!s || !hits || !hits->id || !hits->t || !hits->u || !hits->v) return RT_ERR_INVALID;
+  if (mode != RT_CLOSEST_PRIMARY && mode != RT_CLOSEST_SECONDARY) return RT_ERR_INVALID;
+  int st = check_rays(rays, true);
+  if (st) return st;
+  char* base = (char*)noalloc_scratch(s->ctx->device);
+  if (!base) return RT_ERR_STATE;  // the context's set-up found no scratch on this device
+  RT_TRY(hipSetDevice(s->ctx->device));
+  hipStream_t hs = (hipStream_t)stream;
+  const size_t n = (size_t)rays->width * rays->height;
+  const RaysDev r = to_dev(rays);
+  const HitOut out{hits->id, hits->t, hits->u, hits->v, hits->dpdu, hits->dpdv};
+  const int kind = s->desc.kind;
+  if (kind == RT_ACCEL_NAIVE || s->ntri == 0) {
+    // naive brute force (or an empty scene): the reduction keys are the only scratch
+    if (align256(sizeof(uint64_t) * n) > kNoAllocScratch) return RT_ERR_TOO_LARGE;
+    uint64_t* keys = (uint64_t*)base;
+    RT_TRY(launch_fill_u64(keys, ~0ull, n, hs));
+    if (s->ntri > 0) {
+      const int nblk = (int)((n + 255) / 256);
+      const int tchunks = pick_tchunks(nblk, s->ntri, 64);
+      RT_TRY(launch_naive_trace(r, mode, s->d_v9, s->ntri, tchunks, keys, nullptr, hs));
+    }
+    RT_TRY(launch_finalize(r, s->d_v9, s->d_uv6, keys, out, hs));
+    return RT_OK;
+  }
+  // rayhierarchy / ray-bvh: one job whose blocks all take the per-ray path (the same result as
+  // the block path), so no cluster lists are needed
+  RhItem it;
+  st = fill_item(RhJob{s, r, mode, out, nullptr}, it);
+  if (st) return st;
+  const uint32_t nch = (uint32_t)((it.g.rb + 63) / 64);
+  const bool repair = it.repair_run > 0;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += align256(std::max<size_t>(bytes, 1));
+    return base + at;
+  };
+  RhItem* table = (RhItem*)take(sizeof(RhItem));
+  it.nodes = (float4*)take(sizeof(float4) * 4 * (size_t)it.g.nnodes);
+  it.keys = (uint64_t*)take(sizeof(uint64_t) * n);
+  it.repkeys = repair ? (uint32_t*)take(sizeof(uint32_t) * n) : nullptr;
+  it.clist = nullptr;
+  RhGroup G{};
+  G.units = (uint4*)take(sizeof(uint4));
+  G.rq = (uint2*)take(sizeof(uint2) * (size_t)it.g.ntop * nch);
+  G.ctr = (uint32_t*)take(256);
+  G.replist = repair ? (uint32_t*)take(sizeof(uint32_t) * n) : nullptr;
+  if (off > kNoAllocScratch || n >= (1ull << 31)) return RT_ERR_TOO_LARGE;
+  it.block_base = 0;
+  it.ray_base = 0;
+  G.items = table;
+  G.nitems = 1;
+  G.mode = mode;
+  G.node = s->desc.node;
+  G.bvh = kind == RT_ACCEL_RAYBVH ? 1 : 0;
+  G.big = (it.g.rb > 256 || it.g.L > 3) ? 1 : 0;
+  G.nblocks = (uint32_t)it.g.ntop;
+  G.nrays = (uint32_t)n;
+  G.per_unit = 1;
+  G.grid = 1;
+  G.max_units = 0;
+  G.max_chunks = (uint32_t)it.g.ntop * nch;
+  G.ray_clusters = 0;
+  G.ray_grid = 0;
+  RT_TRY(launch_write_table(it, table, hs));
+  RT_TRY(launch_rh_group(G, hs, nullptr, nullptr, nullptr));
+  return RT_OK;
 }
 
 int rt_make_shadow_rays(const rt_rays* primary, const int32_t* id, const float* t,
@@ -1227,6 +1310,7 @@ const char* rt_error_string(int status) {
     case RT_ERR_UNSUPPORTED: return "unsupported parameter combination";
     case RT_ERR_NO_DEVICE: return "no HIP device";
     case RT_ERR_STATE: return "call order violated";
+    case RT_ERR_TOO_LARGE: return "input exceeds the static scratch of the allocation-free trace";
     default: return "unknown status";
   }
 }

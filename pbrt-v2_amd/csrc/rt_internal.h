@@ -117,7 +117,8 @@ struct RhGroup {
   int grid;           // work-groups of the persistent unit kernel; 0 = one per resident slot
   uint32_t max_units;  // upper bound of the group's work units
   uint32_t max_chunks;  // upper bound of the group's per-ray chunks
-  // blocks whose root passes at least this many clusters are traced ray by ray
+  // blocks whose root passes at least this many clusters are traced ray by ray; 0 sends every
+  // block with a traced ray that way without the cluster scan (job cluster lists unused)
   uint32_t ray_clusters;
   int ray_grid;  // work-groups (one wavefront each) of the per-ray kernel; 0 = resident slots
 };
@@ -136,6 +137,16 @@ constexpr int kSearchLeaf = 4;
 // Morton codes `code` are sorted ascending.  Returns the node array (4 float4 per node).
 void build_search_bvh(const float* v9, const uint32_t* code, uint32_t ntri,
                       std::vector<float4>& nodes);
+
+// Allocation-free trace: a static device array of kNoAllocScratch bytes per device (defined in
+// rt_kernels.hip).  noalloc_setup looks up its address on `device`, which must be current
+// (once per device; 0 on success); noalloc_scratch returns it (null before setup).
+constexpr size_t kNoAllocScratch = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+int noalloc_setup(int device);
+void* noalloc_scratch(int device);
+// Writes a job table into device memory through the kernel arguments (no copy, no allocation).
+hipError_t launch_write_table(const RhItem& item, RhItem* dst, hipStream_t s);
 
 // Launchers (all asynchronous on `s`).
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s);

@@ -634,7 +634,8 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
 // A block whose root passes at least `ray_clusters` clusters, or whose rays' directions
 // differ by more than kWideDir in some component, is an incoherent ray group: its candidate
 // set holds most (leaf, triangle) pairs, so it is queued for the per-ray path instead, as
-// 64-ray chunks {job, block << 4 | chunk}.
+// 64-ray chunks {job, block << 4 | chunk}.  ray_clusters == 0 queues every block with a
+// traced ray that way without scanning the clusters.
 constexpr float kWideDir = 1.0f;
 template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
@@ -659,6 +660,15 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   root.dmax = {a1.x, a1.y, a1.z};
   root.omin = {a2.x, a2.y, a2.z};
   root.omax = {a3.x, a3.y, a3.z};
+  if (ray_clusters == 0) {
+    // every block to the per-ray path, without the cluster scan (no cluster list needed)
+    const int nch = (g.rb + 63) / 64;
+    if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
+    __syncthreads();
+    for (int q = tid; q < nch; q += kThreads)
+      rq[sBase + q] = make_uint2((uint32_t)k, ((uint32_t)b << 4) | (uint32_t)q);
+    return;
+  }
   if (tid == 0) sCnt = 0;
   __syncthreads();
   const uint32_t nclusters = it.nclusters;
@@ -1968,7 +1978,46 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
   return hipGetLastError();
 }
 
+// Copies a job table passed by value (in the kernel arguments) into device memory, so that a
+// launch group can be set up without a host-to-device copy.
+__global__ __launch_bounds__(64) void write_table_kernel(RhItem item, RhItem* __restrict__ dst) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&item);
+  uint32_t* out = reinterpret_cast<uint32_t*>(dst);
+  for (uint32_t i = threadIdx.x; i < sizeof(RhItem) / 4; i += 64) out[i] = src[i];
+}
+
+// Scratch of the allocation-free trace (rt_trace_closest_noalloc): one array per device, held
+// until the process ends.  Its device address is looked up in this file, once per device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
 }  // namespace
+
+// Device address of `scratch` on each device, looked up by noalloc_setup.
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return 1;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) return 1;  // the device must be current
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return 1;
+  g_scratch_ptr[device] = p;
+  return 0;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+hipError_t launch_write_table(const RhItem& item, RhItem* dst, hipStream_t s) {
+  hipLaunchKernelGGL(write_table_kernel, dim3(1), dim3(64), 0, s, item, dst);
+  return hipGetLastError();
+}
 
 int rh_geometry(int W, int H, int cx, int cy, int height, RHGeom* g) {
   if (W <= 0 || H <= 0 || cx <= 0 || cy <= 0 || height <= 0) return 1;
