@@ -354,10 +354,13 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int kDefaultPer[2] = {8, 4};  // passing clusters per unit: closest, occluded
 // Blocks whose root passes at least this many clusters (64 triangles or 64 frontier roots
-// each), or whose ray directions spread widely, are traced ray by ray: closest, occluded.
-// 2^29 keeps only the direction test: shadow rays toward a light pass many clusters near
-// depth discontinuities, and the block path still handles those blocks faster.
-constexpr int kDefaultRay[2] = {256, 1 << 29};
+// each), or whose leaves' ray directions spread widely, are traced ray by ray.  2^29 keeps
+// only the direction test.  Built-in choice per accelerator and mode (closest, occluded),
+// measured on the torus and room workloads: rayhierarchy's block path keeps coherent shadow
+// blocks faster than the per-ray path even when they pass many clusters, ray-bvh's lockstep
+// block path does not (its shadow rays across the room pass most frontier roots).
+constexpr int kDefaultRay[2][2] = {{1 << 29, 1 << 29}, {1 << 29, 64}};
+int default_ray(bool bvh, int mi) { return kDefaultRay[bvh ? 1 : 0][mi]; }
 constexpr int kDefaultRootClustersPerUnit = 8;  // passing root clusters per ray-bvh unit
 
 // The job-table entry of one job: geometry of its ray image and the scene's resident arrays
@@ -443,7 +446,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     GroupLayout& L = gl[gi];
     // work units: passing clusters of triangles (rayhierarchy) or of frontier roots (ray-bvh)
     L.per = prm.per[mi] > 0 ? prm.per[mi] : (bvh ? kDefaultRootClustersPerUnit : kDefaultPer[mi]);
-    L.ray = (uint32_t)(prm.ray[mi] > 0 ? prm.ray[mi] : kDefaultRay[mi]);
+    L.ray = (uint32_t)(prm.ray[mi] > 0 ? prm.ray[mi] : default_ray(bvh, mi));
     L.rgrid = prm.rgrid[mi];
     L.max_units = 0;
     L.max_chunks = 0;
@@ -1196,7 +1199,8 @@ int rt_batch_get_param(rt_batch* b, int32_t param, int32_t* value) {
   const bool grid = param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED ||
                     param == RT_PARAM_RAY_GRID_CLOSEST || param == RT_PARAM_RAY_GRID_OCCLUDED;
   const bool ray = param == RT_PARAM_RAY_CLUSTERS_CLOSEST || param == RT_PARAM_RAY_CLUSTERS_OCCLUDED;
-  *value = *p > 0 ? *p : (grid ? 0 : ray ? kDefaultRay[mi] : kDefaultPer[mi]);
+  // 0: the built-in choice (which for the per-ray thresholds depends on the accelerator)
+  *value = *p > 0 ? *p : ((grid || ray) ? 0 : kDefaultPer[mi]);
   return RT_OK;
 }
 

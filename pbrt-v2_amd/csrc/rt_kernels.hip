@@ -631,12 +631,12 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
 // passing cluster ids in the job's clist[block * nclusters + ...] and appends work units
 // {job, block, first, count} of at most `per_unit` passing clusters.
 //
-// A block whose root passes at least `ray_clusters` clusters, or whose rays' directions
-// differ by more than kWideDir in some component, is an incoherent ray group: its candidate
-// set holds most (leaf, triangle) pairs, so it is queued for the per-ray path instead, as
-// 64-ray chunks {job, block << 4 | chunk}.  ray_clusters == 0 queues every block with a
-// traced ray that way without scanning the clusters.
-constexpr float kWideDir = 1.0f;
+// A block whose root passes at least `ray_clusters` clusters, or whose leaves' ray directions
+// spread by more than kIncoherent on average, is an incoherent ray group: its candidate set
+// holds most (leaf, triangle) pairs, so it is queued for the per-ray path instead, as 64-ray
+// chunks {job, block << 4 | chunk}.  ray_clusters == 0 queues every block with a traced ray
+// that way without scanning the clusters.
+constexpr float kIncoherent = 0.25f;
 template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
                                                          int nitems, uint4* __restrict__ units,
@@ -645,6 +645,8 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
                                                          int per_unit, uint32_t ray_clusters) {
   __shared__ int sCnt;
   __shared__ uint32_t sBase;
+  __shared__ float sSpread;
+  __shared__ int sLeaves;
   const int tid = threadIdx.x;
   const int k = find_block_item(items, nitems, blockIdx.x);
   const RhItem& it = items[k];
@@ -669,8 +671,26 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
       rq[sBase + q] = make_uint2((uint32_t)k, ((uint32_t)b << 4) | (uint32_t)q);
     return;
   }
-  if (tid == 0) sCnt = 0;
+  if (tid == 0) {
+    sCnt = 0;
+    sSpread = 0.f;
+    sLeaves = 0;
+  }
   __syncthreads();
+  // mean direction spread of the block's leaves (largest component range of each leaf's ray
+  // directions): near 0 for camera rays and shadow rays toward a light from nearby points,
+  // large for random segments, diffuse bounces and shadow rays from scattered points
+  if (tid < g.nlx * g.nlx) {
+    const int gk = K * g.nlx + tid % g.nlx, gj = J * g.nlx + tid / g.nlx;
+    if (gk < g.gw[0] && gj < g.gh[0]) {
+      const float4* q = it.nodes + 4 * ((size_t)g.off[0] + (size_t)gj * g.gw[0] + gk);
+      const float4 lo = q[0], hi = q[1];
+      if (lo.w != 0.f) {
+        atomicAdd(&sSpread, fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z));
+        atomicAdd(&sLeaves, 1);
+      }
+    }
+  }
   const uint32_t nclusters = it.nclusters;
   const float4* __restrict__ cbox = it.cbox;
   uint32_t* out = it.clist + (size_t)b * nclusters;
@@ -685,12 +705,10 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   }
   __syncthreads();
   const int cnt = sCnt;
-  // incoherent: at least `ray_clusters` clusters pass, or the rays' directions spread over
-  // more than kWideDir in some axis (ray_clusters above 2^29 turns the per-ray path off)
-  const float spread = fmaxf(fmaxf(root.dmax.x - root.dmin.x, root.dmax.y - root.dmin.y),
-                             root.dmax.z - root.dmin.z);
-  if ((uint32_t)cnt >= ray_clusters ||
-      (ray_clusters <= (1u << 29) && cnt > 0 && spread > kWideDir)) {
+  // incoherent: at least `ray_clusters` clusters pass, or the block's leaves spread their ray
+  // directions by more than kIncoherent on average (ray_clusters above 2^29: never)
+  const bool wide = sLeaves > 0 && sSpread > kIncoherent * (float)sLeaves;
+  if ((uint32_t)cnt >= ray_clusters || (ray_clusters <= (1u << 29) && cnt > 0 && wide)) {
     const int nch = (g.rb + 63) / 64;
     if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
     __syncthreads();
@@ -1318,65 +1336,135 @@ __device__ __forceinline__ bool ray_chain_bvh(const RhItem& it, int x, int y, ui
   return true;
 }
 
-#ifndef RT_RAY_WAVES
-#define RT_RAY_WAVES 1
-#endif
+// Per-ray kernel: one wavefront per work-group, every lane one ray at a time.  The queue holds
+// 64-ray chunks; a wavefront takes a chunk and deals its rays to lanes without one, and when a
+// lane's ray is done it takes the next ray of the chunk (or of the next chunk) at the next
+// round, so lanes of long traversals do not leave the others idle.  A round walks interior
+// nodes until every lane holds a leaf or is done (Aila and Laine's while-while), then tests
+// the postponed leaves.
 template <int MODE, int NODE, bool BVH>
-__global__ __launch_bounds__(64, RT_RAY_WAVES) void ray_path_kernel(const RhItem* __restrict__ items,
+__global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__ items,
                                                       const uint2* __restrict__ rq,
                                                       uint32_t* __restrict__ ctr) {
   __shared__ uint32_t sStk[kSearchDepth][64];
+  constexpr uint32_t kDone = 0xffffffffu;  // not a node reference (leaves hold <= 4 triangles)
   const int lane = threadIdx.x;
+  const uint64_t lanes_below = (1ull << lane) - 1ull;
   const uint32_t nchunks = __builtin_amdgcn_readfirstlane(ctr[4]);
   if (nchunks == 0) return;  // no incoherent block: leave without touching the queue
+  // wavefront-uniform: the chunk being dealt and its next slot
+  uint32_t cjob = 0, cblk = 0, cpart = 0, cslot = 64;
+  bool more = true;
+  // lane state
+  bool act = false;
+  uint32_t jk = 0;
+  int x = 0, y = 0;
+  size_t idx = 0;
+  V3 o = {0.f, 0.f, 0.f}, d = {0.f, 0.f, 1.f}, inv = {1.f, 1.f, 1.f};
+  float tmin = 0.f, tmax = 0.f, tlo = 0.f;
+  uint64_t best = kMissKey;
+  uint32_t rep = kNoKey32;
+  bool occluded = false, track_rep = false;
+  int sp = 0;
+  uint32_t cur = kDone, pl = kDone;
+  auto pop = [&]() { return sp > 0 ? sStk[--sp][lane] : kDone; };
   for (;;) {
-    uint32_t e = 0;
-    if (lane == 0) e = atomicAdd(&ctr[5], 1u);
-    e = __builtin_amdgcn_readfirstlane(e);
-    if (e >= nchunks) break;
-    const uint2 w = rq[e];
-    const RhItem& it = items[__builtin_amdgcn_readfirstlane(w.x)];
-    const RHGeom& g = it.g;
-    const RaysDev& r = it.r;
-    const uint32_t wy = __builtin_amdgcn_readfirstlane(w.y);
-    const int top = g.L - 1;
-    const int b = (int)(wy >> 4);
-    const int K = b % g.gw[top], J = b / g.gw[top];
-    int x = 0, y = 0, leaf = 0;
-    bool have = slot_xy(g, K, J, (int)(wy & 15u) * 64 + lane, x, y, leaf) && it.ntri > 0;
-    const size_t idx = (size_t)y * g.W + x;
-    if (have && MODE != 0 && r.valid != nullptr && glb(r.valid)[idx] == 0) have = false;
-    V3 o = {0.f, 0.f, 0.f}, d = {0.f, 0.f, 1.f};
-    float tmin = 0.f, tmax = 0.f;
-    if (have) {
-      o = {glb(r.ox)[idx], glb(r.oy)[idx], glb(r.oz)[idx]};
-      d = {glb(r.dx)[idx], glb(r.dy)[idx], glb(r.dz)[idx]};
-      tmin = glb(r.tmin)[idx];
-      tmax = glb(r.tmax)[idx];
+    // deal rays to the lanes without one
+    for (;;) {
+      const uint64_t need = __ballot(!act);
+      if (need == 0 || !more) break;
+      if (cslot >= 64) {
+        uint32_t e = 0;
+        if (lane == 0) e = atomicAdd(&ctr[5], 1u);
+        e = __builtin_amdgcn_readfirstlane(e);
+        if (e >= nchunks) {
+          more = false;
+          break;
+        }
+        const uint2 w = rq[e];
+        cjob = __builtin_amdgcn_readfirstlane(w.x);
+        cblk = __builtin_amdgcn_readfirstlane(w.y) >> 4;
+        cpart = __builtin_amdgcn_readfirstlane(w.y) & 15u;
+        cslot = 0;
+      }
+      const uint32_t avail = 64u - cslot;
+      const uint32_t rank = (uint32_t)__popcll(need & lanes_below);
+      if (!act && rank < avail) {
+        const RhItem& it = items[cjob];
+        const RHGeom& g = it.g;
+        const int top = g.L - 1;
+        const int K = (int)cblk % g.gw[top], J = (int)cblk / g.gw[top];
+        int leaf = 0;
+        bool have = slot_xy(g, K, J, (int)(cpart * 64u + cslot + rank), x, y, leaf) && it.ntri > 0;
+        if (have) {
+          idx = (size_t)y * g.W + x;
+          const RaysDev& r = it.r;
+          if (MODE != 0 && r.valid != nullptr && glb(r.valid)[idx] == 0) have = false;
+        }
+        if (have) {
+          const RaysDev& r = it.r;
+          jk = cjob;
+          o = {glb(r.ox)[idx], glb(r.oy)[idx], glb(r.oz)[idx]};
+          d = {glb(r.dx)[idx], glb(r.dy)[idx], glb(r.dz)[idx]};
+          tmin = glb(r.tmin)[idx];
+          tmax = glb(r.tmax)[idx];
+          // inverse direction for the slab test; zero components become tiny ones, so the
+          // slab of that axis is (almost) the whole line when the origin is inside it and
+          // empty otherwise
+          auto safe = [](float v) {
+            return fabsf(v) < 0x1p-80f ? (v < 0.f ? -0x1p-80f : 0x1p-80f) : v;
+          };
+          inv = {1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z)};
+          tlo = tmin - fabsf(tmin) * kSlack;
+          best = kMissKey;
+          rep = kNoKey32;
+          occluded = false;
+          track_rep = MODE == 0 && !BVH && it.repkeys != nullptr;
+          sp = 0;
+          cur = 0u;  // interior node 0 (the root)
+          pl = kDone;
+          act = true;
+        }
+      }
+      cslot += min(avail, (uint32_t)__popcll(need));
     }
-    // inverse direction for the slab test; zero components become tiny ones, so the slab of
-    // that axis is (almost) the whole line when the origin is inside it and empty otherwise
-    auto safe = [](float v) { return fabsf(v) < 0x1p-80f ? (v < 0.f ? -0x1p-80f : 0x1p-80f) : v; };
-    const V3 inv = {1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z)};
-    const float tlo = tmin - fabsf(tmin) * kSlack;
+    if (!__any(act)) break;
+    const RhItem& it = items[jk];
     const auto snode = glb(it.snode);
     const auto tmts = glb(it.tmts);
     const auto perm = glb(it.perm);
-    const bool track_rep = MODE == 0 && !BVH && it.repkeys != nullptr;
-    uint64_t best = kMissKey;
-    uint32_t rep = kNoKey32;
-    bool occluded = false;
-    int sp = 0;
-    // Traversal in rounds (Aila and Laine's while-while): a lane walks interior nodes until it
-    // holds a leaf, postpones that leaf and walks on while other lanes of the wavefront still
-    // look for one; then every lane tests its postponed leaf.  Node and leaf work of different
-    // lanes thus run in separate phases instead of splitting the wavefront.
-    constexpr uint32_t kDone = 0xffffffffu;  // not a node reference (leaves hold <= 4 triangles)
-    uint32_t cur = have ? 0u : kDone;  // interior node 0 (the root)
-    uint32_t pl = kDone;               // postponed leaf
-    auto pop = [&]() { return sp > 0 ? sStk[--sp][lane] : kDone; };
-    auto test_leaf = [&](uint32_t ref) {
-      const uint32_t first = ref & ((1u << 27) - 1u), cnt = ((ref >> 27) & 15u) + 1u;
+    // node phase
+    for (;;) {
+      if (act && cur != kDone && !(cur & kSearchLeafBit)) {
+        const float4 n0 = ld4(snode, 4 * (size_t)cur), n1 = ld4(snode, 4 * (size_t)cur + 1),
+                     n2 = ld4(snode, 4 * (size_t)cur + 2), n3 = ld4(snode, 4 * (size_t)cur + 3);
+        float thi = MODE == 2 ? tmax : (best == kMissKey ? __int_as_float(0x7f800000)
+                                                         : fkey_inv((uint32_t)(best >> 32)));
+        thi = thi + fabsf(thi) * kSlack;
+        float tn0, tn1;
+        const bool h0 = slab(o, inv, V3{n0.x, n0.y, n0.z}, V3{n0.w, n1.x, n1.y}, tlo, thi, tn0);
+        const bool h1 = slab(o, inv, V3{n1.z, n1.w, n2.x}, V3{n2.y, n2.z, n2.w}, tlo, thi, tn1);
+        const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
+        if (h0 && h1) {
+          const bool first1 = tn1 < tn0;
+          sStk[sp++][lane] = first1 ? r0 : r1;
+          cur = first1 ? r1 : r0;
+        } else if (h0 || h1) {
+          cur = h0 ? r0 : r1;
+        } else {
+          cur = pop();
+        }
+        if (pl == kDone && cur != kDone && (cur & kSearchLeafBit)) {
+          pl = cur;
+          cur = pop();
+        }
+      }
+      // walk on while some lane has neither a postponed leaf nor finished
+      if (!__any(act && pl == kDone && cur != kDone && !(cur & kSearchLeafBit))) break;
+    }
+    // leaf phase
+    if (act && pl != kDone) {
+      const uint32_t first = pl & ((1u << 27) - 1u), cnt = ((pl >> 27) & 15u) + 1u;
       for (uint32_t f = 0; f < cnt && !occluded; ++f) {
         const uint32_t slot = first + f;
         const uint32_t id = perm[slot];
@@ -1409,56 +1497,21 @@ __global__ __launch_bounds__(64, RT_RAY_WAVES) void ray_path_kernel(const RhItem
           if (ur) rep = fkey(t);
         }
       }
-    };
-    for (;;) {
-      // node phase
-      for (;;) {
-        if (cur != kDone && !(cur & kSearchLeafBit)) {
-          const float4 n0 = ld4(snode, 4 * (size_t)cur), n1 = ld4(snode, 4 * (size_t)cur + 1),
-                       n2 = ld4(snode, 4 * (size_t)cur + 2),
-                       n3 = ld4(snode, 4 * (size_t)cur + 3);
-          float thi = MODE == 2 ? tmax : (best == kMissKey ? __int_as_float(0x7f800000)
-                                                           : fkey_inv((uint32_t)(best >> 32)));
-          thi = thi + fabsf(thi) * kSlack;
-          float tn0, tn1;
-          const bool h0 = slab(o, inv, V3{n0.x, n0.y, n0.z}, V3{n0.w, n1.x, n1.y}, tlo, thi, tn0);
-          const bool h1 = slab(o, inv, V3{n1.z, n1.w, n2.x}, V3{n2.y, n2.z, n2.w}, tlo, thi, tn1);
-          const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
-          if (h0 && h1) {
-            const bool first1 = tn1 < tn0;
-            sStk[sp++][lane] = first1 ? r0 : r1;
-            cur = first1 ? r1 : r0;
-          } else if (h0 || h1) {
-            cur = h0 ? r0 : r1;
-          } else {
-            cur = pop();
-          }
-          if (pl == kDone && cur != kDone && (cur & kSearchLeafBit)) {
-            pl = cur;
-            cur = pop();
-          }
-        }
-        // walk on while some lane has neither a postponed leaf nor finished
-        if (!__any(pl == kDone && cur != kDone && !(cur & kSearchLeafBit))) break;
-      }
-      // leaf phase
-      if (pl != kDone) {
-        test_leaf(pl);
-        pl = kDone;
-      }
-      if (cur != kDone && (cur & kSearchLeafBit)) {  // a second leaf waits for the next round
-        pl = cur;
-        cur = pop();
-      }
-      if (occluded) cur = pl = kDone;
-      if (!__any(cur != kDone || pl != kDone)) break;
+      pl = kDone;
     }
-    if (!have) continue;
-    if constexpr (MODE == 2) {
-      if (occluded) glb(it.occ)[idx] = 1;
-    } else {
-      if (best != kMissKey) glb(it.keys)[idx] = best;
-      if (track_rep && rep != kNoKey32) glb(it.repkeys)[idx] = rep;
+    if (act && cur != kDone && (cur & kSearchLeafBit)) {  // a second leaf waits for the next round
+      pl = cur;
+      cur = pop();
+    }
+    if (act && occluded) cur = pl = kDone;
+    if (act && cur == kDone && pl == kDone) {  // the ray is done: its result, then a new ray
+      if constexpr (MODE == 2) {
+        if (occluded) glb(it.occ)[idx] = 1;
+      } else {
+        if (best != kMissKey) glb(it.keys)[idx] = best;
+        if (track_rep && rep != kNoKey32) glb(it.repkeys)[idx] = rep;
+      }
+      act = false;
     }
   }
 }
