@@ -179,14 +179,15 @@ int rt_make_bounce_rays(rt_scene *scene, const rt_rays *in, const int32_t *id, c
  *   unchanged (batches only; needs a non-null stream).
  *   RT_PARAM_RAY_CLUSTERS_CLOSEST / RT_PARAM_RAY_CLUSTERS_OCCLUDED: a block (top-level
  *   hierarchy node) whose root passes at least this many triangle clusters (ray-bvh: clusters
- *   of frontier roots), or whose leaves' ray directions spread by more than 0.25 on average
+ *   of frontier roots) of a scene holding at least four times as many, or whose leaves' ray
+ *   directions spread by more than 0.25 on average
  *   (largest component range per leaf: an incoherent group such as random shadow segments,
  *   diffuse bounces or shadow rays from scattered points), is traced ray by ray --
  *   each ray walks a conservative BVH of the triangles and the reference's exact cull chain
  *   is evaluated for the triangles that pass its strict leaf test -- in place of enumerating
  *   the block's (leaf, triangle) candidate pairs.  1 sends every block with work that way;
  *   2^29 keeps only the direction test; a value above 2^29 (at most 2^30) sends no block.
- *   Built-in (0): 2^29, except 64 for ray-bvh occlusion.
+ *   Built-in (0): 2^29 for closest hits, 256 (rayhierarchy) or 64 (ray-bvh) for occlusion.
  *   RT_PARAM_RAY_GRID_CLOSEST / RT_PARAM_RAY_GRID_OCCLUDED: work-groups (one wavefront each)
  *   of the per-ray kernel (batches only); 0 = one per resident slot. */
 enum {

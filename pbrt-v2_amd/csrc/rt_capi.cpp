@@ -354,12 +354,12 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int kDefaultPer[2] = {8, 4};  // passing clusters per unit: closest, occluded
 // Blocks whose root passes at least this many clusters (64 triangles or 64 frontier roots
-// each), or whose leaves' ray directions spread widely, are traced ray by ray.  2^29 keeps
-// only the direction test.  Built-in choice per accelerator and mode (closest, occluded),
-// measured on the torus and room workloads: rayhierarchy's block path keeps coherent shadow
-// blocks faster than the per-ray path even when they pass many clusters, ray-bvh's lockstep
-// block path does not (its shadow rays across the room pass most frontier roots).
-constexpr int kDefaultRay[2][2] = {{1 << 29, 1 << 29}, {1 << 29, 64}};
+// each) in a scene of four times as many, or whose leaves' ray directions spread widely, are
+// traced ray by ray.  2^29 keeps only the direction test.  Built-in choice per accelerator and
+// mode (closest, occluded), measured on the torus, room, uneven-batch and serving workloads:
+// coherent camera blocks are fastest on the block path; shadow blocks that pass many clusters
+// of a large scene are faster ray by ray, more so behind ray-bvh's lockstep block path.
+constexpr int kDefaultRay[2][2] = {{1 << 29, 256}, {1 << 29, 64}};
 int default_ray(bool bvh, int mi) { return kDefaultRay[bvh ? 1 : 0][mi]; }
 constexpr int kDefaultRootClustersPerUnit = 8;  // passing root clusters per ray-bvh unit
 

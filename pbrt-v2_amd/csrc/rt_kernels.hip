@@ -631,8 +631,9 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
 // passing cluster ids in the job's clist[block * nclusters + ...] and appends work units
 // {job, block, first, count} of at most `per_unit` passing clusters.
 //
-// A block whose root passes at least `ray_clusters` clusters, or whose leaves' ray directions
-// spread by more than kIncoherent on average, is an incoherent ray group: its candidate set
+// A block whose root passes at least `ray_clusters` clusters of a scene of at least four times
+// as many, or whose leaves' ray directions spread by more than kIncoherent on average, is an
+// incoherent ray group: its candidate set
 // holds most (leaf, triangle) pairs, so it is queued for the per-ray path instead, as 64-ray
 // chunks {job, block << 4 | chunk}.  ray_clusters == 0 queues every block with a traced ray
 // that way without scanning the clusters.
@@ -705,10 +706,14 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   }
   __syncthreads();
   const int cnt = sCnt;
-  // incoherent: at least `ray_clusters` clusters pass, or the block's leaves spread their ray
-  // directions by more than kIncoherent on average (ray_clusters above 2^29: never)
+  // incoherent: at least `ray_clusters` clusters pass in a scene of at least 4 * ray_clusters
+  // clusters (in smaller scenes the block path's work is bounded by the scene), or the block's
+  // leaves spread their ray directions by more than kIncoherent on average (ray_clusters above
+  // 2^29: never; 1: every block with a passing cluster)
   const bool wide = sLeaves > 0 && sSpread > kIncoherent * (float)sLeaves;
-  if ((uint32_t)cnt >= ray_clusters || (ray_clusters <= (1u << 29) && cnt > 0 && wide)) {
+  const bool heavy = (uint32_t)cnt >= ray_clusters &&
+                     (ray_clusters == 1u || (uint64_t)nclusters >= 4ull * ray_clusters);
+  if (heavy || (ray_clusters <= (1u << 29) && cnt > 0 && wide)) {
     const int nch = (g.rb + 63) / 64;
     if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
     __syncthreads();
