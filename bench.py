@@ -134,7 +134,8 @@ def per_ray_roofline(name, t_ms, st, mode, workload):
                      "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
                      "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
                      "peak_unpacked": FP32_UNPACKED_TFLOPS,
-                     "busy_measured": facts.get("valu-busy")}}
+                     "busy_measured": facts.get("valu-busy"),
+                     "lane_utilisation_measured": facts.get("lane-utilisation")}}
 
 
 def _window(r, x0, y0, w, h):
