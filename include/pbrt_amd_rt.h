@@ -35,8 +35,10 @@ typedef enum rt_status {
 
 /* Accelerator names of MakeAccelerator (src/core/api.cpp:576-596). */
 enum { RT_ACCEL_NAIVE = 0, RT_ACCEL_RAYHIERARCHY = 1, RT_ACCEL_RAYBVH = 2 };
-/* Ray-hierarchy node types (rayhierarchy.cpp:70-108): "box_dir" (6-D boxes, default), "ia". */
-enum { RT_NODE_BOX_DIR = 0, RT_NODE_IA = 1 };
+/* Ray-hierarchy node types (rayhierarchy.cpp:68-123, rayBVH.cpp:62-100): "box_dir" (6-D boxes,
+ * default), "ia" (interval arithmetic), "box_uv" (origin box + intervals of d.y/d.x and d.z,
+ * culled by intersectsNode5DB.cl as C parses it). */
+enum { RT_NODE_BOX_DIR = 0, RT_NODE_IA = 1, RT_NODE_BOX_UV = 2 };
 /* Closest-hit modes: primary camera rays (Intersect without bounce, rayhierarchy.cpp:816) and
  * secondary rays with validity (Intersect(..., bounce), rayhierarchy.cpp:602). */
 enum { RT_CLOSEST_PRIMARY = 0, RT_CLOSEST_SECONDARY = 1 };

@@ -85,9 +85,12 @@ class _RayKeep:
         self.n = int(r["width"]) * int(r["height"])
 
 
+NODE_KINDS = {"box_dir": 0, "ia": 1, "box_uv": 2}
+
+
 def params(kind="rayhierarchy", node="box_dir", cx=4, cy=4, height=3, repair_run=None):
     k = {"naive": 0, "rayhierarchy": 1, "ray-bvh": 2}[kind] if isinstance(kind, str) else kind
-    nd = {"box_dir": 0, "ia": 1}[node] if isinstance(node, str) else node
+    nd = NODE_KINDS[node] if isinstance(node, str) else node
     if repair_run is None:
         repair_run = 1 if k == 1 else 0
     return OrcParams(k, nd, cx, cy, height, repair_run)
@@ -187,12 +190,15 @@ def rh_well_formed(W, H, cx=4, cy=4, height=3) -> bool:
     return bool(lib().orc_rh_reference_well_formed(W, H, cx, cy, height))
 
 
-def rh_build(rays: dict, cx=4, cy=4, height=3, use_valid=False):
+def rh_build(rays: dict, cx=4, cy=4, height=3, use_valid=False, node="box_dir"):
+    """Hierarchy nodes, 16 floats each: {dmin, valid, dmax, 0, omin, 0, omax, 0}; for node
+    "box_uv" dmin/dmax hold (slope d.y/d.x, d.z, 0)."""
     levels = rh_levels(int(rays["width"]), int(rays["height"]), cx, cy, height)
     total = sum(w * h for w, h in levels)
     nodes = np.zeros((total, 16), np.float32)
     keep = _RayKeep(rays)
-    got = lib().orc_rh_build(ctypes.byref(keep.s), cx, cy, height, 1 if use_valid else 0,
+    nd = NODE_KINDS[node] if isinstance(node, str) else node
+    got = lib().orc_rh_build(ctypes.byref(keep.s), cx, cy, height, 1 if use_valid else 0, nd,
                              _fp(nodes.ctypes.data))
     assert got == total
     return levels, nodes
@@ -213,7 +219,8 @@ def mt(kind: str, o, d, v9):
 def cull(kind: str, node16, v9=None, bmin=None, bmax=None) -> bool:
     if kind == "tri":
         return bool(lib().orc_cull_tri(_vec(node16), _vec(v9)))
-    fn = lib().orc_cull_node6 if kind == "node6" else lib().orc_cull_ia
+    fn = {"node6": lib().orc_cull_node6, "ia": lib().orc_cull_ia,
+          "5db": lib().orc_cull_5db}[kind]
     return bool(fn(_vec(node16), _vec(bmin), _vec(bmax)))
 
 

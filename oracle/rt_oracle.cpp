@@ -313,6 +313,56 @@ inline bool cull_ia(const Node& n, V3 bmin, V3 bmax) {
   return sx < sy + kEps;
 }
 
+// intersectsNode5DB.cl:1-83 (node "box_uv"; the node's dmin/dmax hold (slope, dz) intervals:
+// dmin = (min d.y/d.x, min d.z, 0), dmax = (max d.y/d.x, max d.z, 0)).  As C parses it,
+// line 79 tests range & (0x1100 == 0x1100) || range & (0x0011 == 0x0011), i.e. range & 1:
+// some corner direction has y not below zero (:18-73 set bit 0 for y >= 0 or NaN).  On line
+// 81 the first conjunct is (max(..) < min(..)) + EPS, an int 0 or 1 plus EPS, never zero, so
+// the slope intervals never decide.  What remains: the node passes when a normalised corner
+// direction of the origin-grown box has y >= 0, else when the corners' z range meets the
+// node's dz range within EPS.  Corners in the kernel's order (:17, 23, 31, 39, 47, 55, 63, 71).
+inline bool cull_5db(const Node& n, V3 bmin, V3 bmax) {
+  V3 h = {(n.omax.x - n.omin.x) / 2, (n.omax.y - n.omin.y) / 2, (n.omax.z - n.omin.z) / 2};
+  bmin = sub(bmin, h);
+  bmax = add(bmax, h);
+  V3 oc = add(h, n.omin);
+  const V3 corners[8] = {{bmin.x, bmin.y, bmin.z}, {bmax.x, bmax.y, bmax.z},
+                         {bmin.x, bmax.y, bmin.z}, {bmin.x, bmax.y, bmax.z},
+                         {bmin.x, bmin.y, bmax.z}, {bmax.x, bmin.y, bmin.z},
+                         {bmax.x, bmax.y, bmin.z}, {bmax.x, bmin.y, bmax.z}};
+  bool up = false, ymarg = false;
+  float zlo = 0.f, zhi = 0.f;
+  for (int i = 0; i < 8; ++i) {
+    const V3 r = normalize(sub(corners[i], oc));
+    if (!(r.y < 0.f)) up = true;
+    ymarg |= near_zero(r.y, 1.f);
+    if (i == 0) {
+      zlo = zhi = r.z;
+    } else {
+      zlo = fmn(zlo, r.z);
+      zhi = fmx(zhi, r.z);
+    }
+  }
+  if (g_track) mark(ymarg);
+  if (up) return true;
+  const float f0 = fmx(zlo, n.dmin.y), f1 = fmn(zhi, n.dmax.y);
+  if (g_track) mark(near_thr(f0, f1 + kEps));
+  return f0 < f1 + kEps;
+}
+
+// The cull of a hierarchy node against an axis-aligned box (a triangle's bounds or a BVH
+// node): intersectsNode of the node type's header.
+inline bool cull_box(int node_kind, const Node& n, V3 bmin, V3 bmax) {
+  if (node_kind == 1) return cull_ia(n, bmin, bmax);
+  if (node_kind == 2) return cull_5db(n, bmin, bmax);
+  return cull_node6(n, bmin, bmax);
+}
+// The 5DB BVH kernels skip a node with no primitives and primitive offset 0
+// (intersection5DB_BVH.cl:18,56 and the P / 2 variants); the 6DB ones have no such test.
+inline bool bvh_dead(int node_kind, const orc_gpu_node& b) {
+  return node_kind == 2 && b.nPrimitives == 0 && b.primOffset == 0;
+}
+
 inline Node node_from16(const float* p) {
   Node n;
   n.dmin = {p[0], p[1], p[2]};
@@ -362,10 +412,40 @@ inline bool ray_valid(const orc_rays* r, size_t i) { return r->valid == nullptr 
 inline V3 ray_o(const orc_rays* r, size_t i) { return {r->ox[i], r->oy[i], r->oz[i]}; }
 inline V3 ray_d(const orc_rays* r, size_t i) { return {r->dx[i], r->dy[i], r->dz[i]}; }
 
-// rayhconstruct6DB.cl:24-38 / rayhconstructP6DB.cl:71-94 and the 2x2 merges of
-// levelConstruct6DB.cl / levelConstructP6DB.cl: every node is the component-wise min/max of
-// d and o over the (valid) rays below it.
-RH rh_build(const orc_rays* r, int cx, int cy, int height, bool use_valid) {
+// Direction part of a leaf for node "box_uv": the slope d.y/d.x and d.z.
+//   rayhconstruct5DB.cl:21-40 (primary rays): slope 0 when d.x == 0, the slope and d.z
+//     intervals are the min/max over the tile;
+//   rayhconstructP5DB.cl:21-67 (validity-aware): a valid ray with d.x == 0 sets the slope
+//     interval to [-1, 1], dropping what the tile's earlier rays gave (the kernel assigns, it
+//     does not merge), later rays widen it again.  The tile is read row by row, left to right;
+//     the reads the kernel clamps to the image edge repeat an edge ray right after itself,
+//     which changes nothing.
+inline void uv_leaf_ray(V3 d, bool first, bool validity_aware, Node& n) {
+  const bool flat = d.x == 0.f;
+  const float s = flat ? 0.f : d.y / d.x;
+  if (validity_aware && flat) {
+    n.dmin.x = -1.f;
+    n.dmax.x = 1.f;
+  } else if (first) {
+    n.dmin.x = n.dmax.x = s;
+  } else {
+    n.dmin.x = fmn(n.dmin.x, s);
+    n.dmax.x = fmx(n.dmax.x, s);
+  }
+  if (first) {
+    n.dmin.y = n.dmax.y = d.z;
+  } else {
+    n.dmin.y = fmn(n.dmin.y, d.z);
+    n.dmax.y = fmx(n.dmax.y, d.z);
+  }
+  n.dmin.z = n.dmax.z = 0.f;
+}
+
+// rayhconstruct6DB.cl:24-38 / rayhconstructP6DB.cl:71-94 (5DB: uv_leaf_ray) and the 2x2
+// merges of levelConstruct6DB.cl / levelConstructP6DB.cl / levelConstruct5DB.cl:32-59 /
+// levelConstructP5DB.cl:25-94: every node is the component-wise min/max of the direction part
+// and of o over the (valid) rays below it.
+RH rh_build(const orc_rays* r, int cx, int cy, int height, bool use_valid, int node_kind) {
   RH h;
   h.W = r->W;
   h.H = r->H;
@@ -389,7 +469,12 @@ RH rh_build(const orc_rays* r, int cx, int cy, int height, bool use_valid) {
           size_t i = (size_t)y * r->W + x;
           if (use_valid && !ray_valid(r, i)) continue;
           V3 d = ray_d(r, i), o = ray_o(r, i);
-          if (!n.valid) {
+          if (node_kind == 2) {
+            uv_leaf_ray(d, !n.valid, use_valid, n);
+            n.omin = n.valid ? vmin(n.omin, o) : o;
+            n.omax = n.valid ? vmax(n.omax, o) : o;
+            n.valid = true;
+          } else if (!n.valid) {
             n.dmin = n.dmax = d;
             n.omin = n.omax = o;
             n.valid = true;
@@ -529,9 +614,10 @@ inline void leaf_test(LeafCtx& c, int k, int j, const Tri& T, int32_t tid, bool 
 
 inline bool rh_cull(const Node& n, const Tri& T, int mode, int node_kind, orc_stats* st) {
   st->cull_tests++;
-  if (node_kind == 1) return cull_ia(n, T.bmin, T.bmax);
-  if (mode == ORC_CLOSEST_PRIMARY) return cull_tri(n, T);
-  return cull_node6(n, T.bmin, T.bmax);
+  // 6DB primary rays use intersectsTri; every other cull is intersectsNode on the triangle's
+  // bounds (the 5DB kernels, yetAnotherIntersection5DB.cl:78 included, use it in every mode)
+  if (node_kind == 0 && mode == ORC_CLOSEST_PRIMARY) return cull_tri(n, T);
+  return cull_box(node_kind, n, T.bmin, T.bmax);
 }
 
 // DFS of intersection6DB.cl:51-119 (and the IA / secondary / shadow variants) for one triangle.
@@ -612,7 +698,7 @@ void trace_naive(const float* v9, uint32_t ntri, const orc_rays* r, int mode, Ou
 void trace_rh(const float* v9, uint32_t ntri, const orc_rays* r, const orc_params* p, int mode,
               Out& o, orc_stats* st, Margins* mg) {
   bool use_valid = mode != ORC_CLOSEST_PRIMARY;
-  RH h = rh_build(r, p->cx, p->cy, p->height, use_valid);
+  RH h = rh_build(r, p->cx, p->cy, p->height, use_valid, p->node);
   LeafCtx c{r, &h, mode, o, st, mg};
   int top = h.L - 1;
   for (uint32_t tid = 0; tid < ntri; ++tid) {
@@ -678,13 +764,14 @@ struct BvhCtx {
 // One step of the lockstep product traversal: the cull of (hierarchy node (l, k, j), BVH node
 // ni).  Returns false when the pair stops the walk; `shadow` / `chain_marg` as in rh_visit.
 inline bool bvh_step(BvhCtx& c, int l, int k, int j, uint32_t ni, bool& shadow, bool& chain_marg) {
+  const orc_gpu_node& b = c.nodes[ni];
+  if (bvh_dead(c.node_kind, b)) return false;
   const Node& n = c.h->at(l, k, j);
   if (c.use_valid && !n.valid) return false;
-  const orc_gpu_node& b = c.nodes[ni];
   V3 bmin = {b.ax, b.ay, b.az}, bmax = {b.bx, b.by, b.bz};
   if (!shadow) c.st->cull_tests++;
   g_marg = false;
-  bool pass = c.node_kind == 1 ? cull_ia(n, bmin, bmax) : cull_node6(n, bmin, bmax);
+  bool pass = cull_box(c.node_kind, n, bmin, bmax);
   const bool m = g_track && g_marg;
   if (!pass) {
     if (!m) return false;
@@ -1246,9 +1333,9 @@ bool s_chain(const SearchCtx& c, int x, int y, uint32_t tri, uint32_t leaf, uint
       const Node& n = rnode(l);
       ++culls;
       if (!n.valid) return false;
-      bool pass = c.node_kind == 1 ? cull_ia(n, T.bmin, T.bmax)
-                                   : (c.mode == ORC_CLOSEST_PRIMARY ? cull_tri(n, T)
-                                                                    : cull_node6(n, T.bmin, T.bmax));
+      bool pass = c.node_kind == 0 && c.mode == ORC_CLOSEST_PRIMARY
+                      ? cull_tri(n, T)
+                      : cull_box(c.node_kind, n, T.bmin, T.bmax);
       if (!pass) return false;
     }
     return true;
@@ -1266,8 +1353,9 @@ bool s_chain(const SearchCtx& c, int x, int y, uint32_t tri, uint32_t leaf, uint
     ++culls;
     if (!n.valid) return false;
     const orc_gpu_node& b = c.nodes[node];
+    if (bvh_dead(c.node_kind, b)) return false;
     const V3 bmin = {b.ax, b.ay, b.az}, bmax = {b.bx, b.by, b.bz};
-    if (!(c.node_kind == 1 ? cull_ia(n, bmin, bmax) : cull_node6(n, bmin, bmax))) return false;
+    if (!cull_box(c.node_kind, n, bmin, bmax)) return false;
   }
   return true;
 }
@@ -1320,7 +1408,8 @@ void s_trace_ray(const SearchCtx& c, const orc_rays* r, size_t i, Out& o, SCount
         if (c.kind == 2) {
           leaf = c.tleaf[id];
           if (leaf == 0xffffffffu) continue;
-          if (c.mode == ORC_CLOSEST_SECONDARY && (leaf & 0x80000000u)) continue;
+          if (c.mode == ORC_CLOSEST_SECONDARY && c.node_kind != 2 && (leaf & 0x80000000u))
+            continue;
           leaf &= 0x7fffffffu;
         }
         ++cnt.tests;
@@ -1386,6 +1475,9 @@ int orc_cull_tri(const float node16[16], const float v9[9]) {
 int orc_cull_node6(const float node16[16], const float bmin[3], const float bmax[3]) {
   return cull_node6(node_from16(node16), {bmin[0], bmin[1], bmin[2]}, {bmax[0], bmax[1], bmax[2]});
 }
+int orc_cull_5db(const float node16[16], const float bmin[3], const float bmax[3]) {
+  return cull_5db(node_from16(node16), {bmin[0], bmin[1], bmin[2]}, {bmax[0], bmax[1], bmax[2]});
+}
 int orc_cull_ia(const float node16[16], const float bmin[3], const float bmax[3]) {
   return cull_ia(node_from16(node16), {bmin[0], bmin[1], bmin[2]}, {bmax[0], bmax[1], bmax[2]});
 }
@@ -1411,8 +1503,8 @@ int orc_rh_reference_well_formed(int32_t W, int32_t H, int32_t cx, int32_t cy, i
 }
 
 int64_t orc_rh_build(const orc_rays* r, int32_t cx, int32_t cy, int32_t height, int32_t use_valid,
-                     float* nodes) {
-  RH h = rh_build(r, cx, cy, height, use_valid != 0);
+                     int32_t node, float* nodes) {
+  RH h = rh_build(r, cx, cy, height, use_valid != 0, node);
   for (size_t i = 0; i < h.nodes.size(); ++i) node_to16(h.nodes[i], nodes + 16 * i);
   return (int64_t)h.nodes.size();
 }
@@ -1480,7 +1572,7 @@ int orc_trace_raybvh(const float* v9, uint32_t ntri, const orc_gpu_node* nodes, 
   if (flags) mg.init((size_t)r->W * r->H);
   g_track = flags != nullptr;
   bool use_valid = mode != ORC_CLOSEST_PRIMARY;
-  RH h = rh_build(r, p->cx, p->cy, p->height, use_valid);
+  RH h = rh_build(r, p->cx, p->cy, p->height, use_valid, p->node);
   LeafCtx lc{r, &h, mode, o, st, flags ? &mg : nullptr};
   BvhCtx c{&h, nodes, prim_order, v9, mode, p->node, use_valid, &lc, st};
   int top = h.L - 1;
@@ -1497,7 +1589,9 @@ int orc_trace_raybvh(const float* v9, uint32_t ntri, const orc_gpu_node* nodes, 
   std::vector<Pair> pairs;
   auto emit = [&](int k, int j, uint32_t ni, bool shadow, bool cm) {
     const orc_gpu_node& b = nodes[ni];
-    uint32_t cnt = mode == ORC_CLOSEST_SECONDARY ? 1u : b.nPrimitives;
+    // the 6DB kernel tests the first primitive only (intersection6DB_BVH2.cl:88), the 5DB
+    // one every primitive (intersection5DB_BVH2.cl:84)
+    uint32_t cnt = mode == ORC_CLOSEST_SECONDARY && p->node != 2 ? 1u : b.nPrimitives;
     for (uint32_t f = 0; f < cnt; ++f) {
       const uint32_t orig = prim_order[b.primOffset + f];
       if (mode == ORC_OCCLUDED) leaf_test(lc, k, j, load_tri(v9, orig), (int32_t)orig, shadow, cm);
@@ -1527,7 +1621,7 @@ int orc_search_trace(const float* v9, uint32_t ntri, const orc_gpu_node* nodes, 
   init_out(r, mode, o);
   const SearchTree T = search_tree(v9, ntri);
   const bool use_valid = mode != ORC_CLOSEST_PRIMARY;
-  const RH h = rh_build(r, p->cx, p->cy, p->height, use_valid);
+  const RH h = rh_build(r, p->cx, p->cy, p->height, use_valid, p->node);
   SearchCtx c;
   c.T = &T;
   c.h = &h;
@@ -1547,6 +1641,7 @@ int orc_search_trace(const float* v9, uint32_t ntri, const orc_gpu_node* nodes, 
         const uint32_t n = stk.back();
         stk.pop_back();
         const orc_gpu_node& b = nodes[n];
+        if (bvh_dead(p->node, b)) continue;
         if (b.nPrimitives) {
           for (uint32_t k = 0; k < b.nPrimitives; ++k)
             c.tleaf[prim_order[b.primOffset + k]] = n | (k > 0 ? 0x80000000u : 0u);

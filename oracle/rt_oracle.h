@@ -33,7 +33,7 @@ typedef struct orc_rays {
 /* Accelerator parameters (reference defaults: chunk 4x4, height 3, box_dir, repairRun 1). */
 typedef struct orc_params {
   int32_t kind;       /* 0 naive, 1 rayhierarchy, 2 ray-bvh */
-  int32_t node;       /* 0 box_dir (6DB), 1 ia */
+  int32_t node;       /* 0 box_dir (6DB), 1 ia, 2 box_uv (5DB) */
   int32_t cx, cy;     /* leaf tile size in rays */
   int32_t height;     /* hierarchy height */
   int32_t repair_run; /* rayhierarchy primary repair passes */
@@ -70,15 +70,18 @@ int orc_mt_pbrt(const float o[3], const float d[3], const float v9[9], float out
 int orc_cull_tri(const float node16[16], const float v9[9]);
 int orc_cull_node6(const float node16[16], const float bmin[3], const float bmax[3]);
 int orc_cull_ia(const float node16[16], const float bmin[3], const float bmax[3]);
+/* 5DB: node16 = {min slope, min dz, 0, valid, max slope, max dz, 0, 0, omin, 0, omax, 0} */
+int orc_cull_5db(const float node16[16], const float bmin[3], const float bmax[3]);
 
 /* ---- ray hierarchy ---- */
 /* number of levels and per-level grid dims (dims[2*l], dims[2*l+1]; l=0 is the leaf level) */
 int orc_rh_levels(int32_t W, int32_t H, int32_t cx, int32_t cy, int32_t height, int32_t *dims);
 /* 1 when the reference's image indexing is well formed for this shape */
 int orc_rh_reference_well_formed(int32_t W, int32_t H, int32_t cx, int32_t cy, int32_t height);
-/* builds every level (leaf first) into nodes (16 floats each); returns the node count */
+/* builds every level (leaf first) into nodes (16 floats each); returns the node count.
+ * node 2 (box_uv) stores the slope and dz intervals in place of the direction box. */
 int64_t orc_rh_build(const orc_rays *r, int32_t cx, int32_t cy, int32_t height, int32_t use_valid,
-                     float *nodes);
+                     int32_t node, float *nodes);
 
 /* ---- traces ---- */
 /* closest modes write id/t/b1/b2 (miss: id -1, t INF); occluded writes occ (0/1).

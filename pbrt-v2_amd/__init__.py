@@ -21,7 +21,8 @@ LIB_PATH = PKG_DIR / "librt_amd.so"
 
 RT_OK = 0
 RT_ACCEL_NAIVE, RT_ACCEL_RAYHIERARCHY, RT_ACCEL_RAYBVH = 0, 1, 2
-RT_NODE_BOX_DIR, RT_NODE_IA = 0, 1
+RT_NODE_BOX_DIR, RT_NODE_IA, RT_NODE_BOX_UV = 0, 1, 2
+NODE_NAMES = {"box_dir": RT_NODE_BOX_DIR, "ia": RT_NODE_IA, "box_uv": RT_NODE_BOX_UV}
 RT_CLOSEST_PRIMARY, RT_CLOSEST_SECONDARY = 0, 1
 
 ACCEL_NAMES = {"naive": RT_ACCEL_NAIVE, "rayhierarchy": RT_ACCEL_RAYHIERARCHY,
@@ -174,7 +175,7 @@ def default_desc(kind, **overrides) -> AccelDesc:
     check(lib().rt_accel_desc_default(kind, ctypes.byref(d)), "rt_accel_desc_default")
     for k, v in overrides.items():
         if k == "node" and isinstance(v, str):
-            v = {"box_dir": RT_NODE_BOX_DIR, "ia": RT_NODE_IA}[v]
+            v = NODE_NAMES[v]
         setattr(d, k, int(v))
     return d
 

@@ -242,7 +242,8 @@ int pick_tchunks(int nblocks, uint32_t nwork, uint32_t per_tile) {
 int desc_ok(const rt_accel_desc* d) {
   if (d->kind < RT_ACCEL_NAIVE || d->kind > RT_ACCEL_RAYBVH) return RT_ERR_INVALID;
   if (d->kind == RT_ACCEL_NAIVE) return RT_OK;
-  if (d->node != RT_NODE_BOX_DIR && d->node != RT_NODE_IA) return RT_ERR_UNSUPPORTED;
+  if (d->node != RT_NODE_BOX_DIR && d->node != RT_NODE_IA && d->node != RT_NODE_BOX_UV)
+    return RT_ERR_INVALID;
   if (d->chunk_x <= 0 || d->chunk_y <= 0 || d->height <= 0) return RT_ERR_INVALID;
   if (d->repair_run < 0) return RT_ERR_INVALID;
   if (d->kind == RT_ACCEL_RAYBVH) {
@@ -410,11 +411,11 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
   groups.clear();
   *changed = false;
   // group membership in job order
-  std::vector<std::pair<int, int>> keys;  // (mode, node + 2 * is_raybvh)
+  std::vector<std::pair<int, int>> keys;  // (mode, node + 4 * is_raybvh), node 0..2
   std::vector<std::vector<int>> members;
   for (size_t j = 0; j < jobs.size(); ++j) {
     const int bvh = jobs[j].s->desc.kind == RT_ACCEL_RAYBVH ? 1 : 0;
-    const std::pair<int, int> k{jobs[j].mode, jobs[j].s->desc.node + 2 * bvh};
+    const std::pair<int, int> k{jobs[j].mode, jobs[j].s->desc.node + 4 * bvh};
     size_t gi = 0;
     while (gi < keys.size() && keys[gi] != k) ++gi;
     if (gi == keys.size()) {
@@ -441,7 +442,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
   std::vector<GroupLayout> gl(keys.size());
   for (size_t gi = 0; gi < keys.size(); ++gi) {
     const int mode = keys[gi].first;
-    const bool bvh = keys[gi].second >= 2;
+    const bool bvh = (keys[gi].second >> 2) != 0;
     const int mi = mode == 2 ? 1 : 0;
     GroupLayout& L = gl[gi];
     // work units: passing clusters of triangles (rayhierarchy) or of frontier roots (ray-bvh)
@@ -528,8 +529,8 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     for (int j : members[gi]) table.push_back(items[j]);
     G.nitems = (int)members[gi].size();
     G.mode = keys[gi].first;
-    G.node = keys[gi].second & 1;
-    G.bvh = keys[gi].second >= 2 ? 1 : 0;
+    G.node = keys[gi].second & 3;
+    G.bvh = keys[gi].second >> 2;
     G.big = gl[gi].big ? 1 : 0;
     G.nblocks = gl[gi].nblocks;
     G.nrays = gl[gi].nrays;
@@ -742,6 +743,9 @@ int rt_scene_set_bvh(rt_scene* s, const rt_gpu_node* nodes, uint32_t nnodes, con
       const uint32_t n = stack.back();
       stack.pop_back();
       if (nodes[n].n_primitives) continue;
+      // the 5DB kernels skip a node with no primitives and offset 0 with its subtree
+      // (intersection5DB_BVH.cl:18,56); its children are not reached through it
+      if (s->desc.node == RT_NODE_BOX_UV && nodes[n].prim_offset == 0) continue;
       if (depth[n] >= 12) return RT_ERR_UNSUPPORTED;
       for (uint32_t c = nodes[n].prim_offset; c <= nodes[n].prim_offset + 1; ++c) {
         if (seen[c]) return RT_ERR_INVALID;
@@ -753,8 +757,9 @@ int rt_scene_set_bvh(rt_scene* s, const rt_gpu_node* nodes, uint32_t nnodes, con
     }
   }
   // Reference BVH leaf of every triangle below a frontier root, for the per-ray path: the leaf
-  // node index, bit 31 set when the triangle is not the leaf's first primitive (secondary
-  // rays test only the first, intersection6DB_BVH2.cl); ~0u for a triangle in no such leaf.
+  // node index, bit 31 set when the triangle is not the leaf's first primitive (6DB / IA
+  // secondary rays test only the first, intersection6DB_BVH2.cl:88); ~0u for a triangle in no
+  // reachable leaf.
   std::vector<uint32_t> tleaf(s->ntri, 0xffffffffu);
   for (uint32_t n = 0; n < nnodes; ++n) {
     if (!seen[n] || !nodes[n].n_primitives) continue;

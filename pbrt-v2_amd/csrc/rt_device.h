@@ -216,6 +216,38 @@ __device__ __forceinline__ bool cull_ia(const Node& n, V3 bmin, V3 bmax) {
   return sx < sy + kEps;
 }
 
+// intersectsNode5DB.cl:1-83, node "box_uv", whose dmin/dmax hold the slope and dz intervals of
+// its rays: dmin = (min d.y/d.x, min d.z, 0), dmax = (max d.y/d.x, max d.z, 0).  As C parses
+// the kernel, line 79 tests range & 1 (0x1100 == 0x1100 is 1): some corner direction has y not
+// below zero; on line 81 the first conjunct is a comparison (0 or 1) plus EPS, never zero, so
+// the slope intervals never decide.  The node passes when a normalised corner direction of the
+// origin-grown box has y >= 0, else when the corners' z range meets [dz min, dz max] within
+// EPS.  Corners in the kernel's order.
+__device__ __forceinline__ bool cull_5db(const Node& n, V3 bmin, V3 bmax) {
+  V3 h = {(n.omax.x - n.omin.x) / 2, (n.omax.y - n.omin.y) / 2, (n.omax.z - n.omin.z) / 2};
+  bmin = sub(bmin, h);
+  bmax = add(bmax, h);
+  V3 oc = add(h, n.omin);
+  V3 r = normalize(sub(V3{bmin.x, bmin.y, bmin.z}, oc));
+  bool up = !(r.y < 0.f);
+  float zlo = r.z, zhi = r.z;
+#define RT_UV_CORNER(X, Y, Z)           \
+  r = normalize(sub(V3{X, Y, Z}, oc)); \
+  up |= !(r.y < 0.f);                   \
+  zlo = fmn(zlo, r.z);                  \
+  zhi = fmx(zhi, r.z);
+  RT_UV_CORNER(bmax.x, bmax.y, bmax.z)
+  RT_UV_CORNER(bmin.x, bmax.y, bmin.z)
+  RT_UV_CORNER(bmin.x, bmax.y, bmax.z)
+  RT_UV_CORNER(bmin.x, bmin.y, bmax.z)
+  RT_UV_CORNER(bmax.x, bmin.y, bmin.z)
+  RT_UV_CORNER(bmax.x, bmax.y, bmin.z)
+  RT_UV_CORNER(bmax.x, bmin.y, bmax.z)
+#undef RT_UV_CORNER
+  if (up) return true;
+  return fmx(zlo, n.dmin.y) < fmn(zhi, n.dmax.y) + kEps;
+}
+
 // Exact conservative prefilters.  The normalised direction of every corner (or vertex) has
 // its squared length between the sums of the per-axis minimum and maximum squares, and the
 // exact test's scale factor (correctly rounded sqrt and division, both monotone) lies between
@@ -292,6 +324,31 @@ __device__ __forceinline__ bool cluster_node6(const Node& n, V3 cmin, V3 cmax) {
   V3 mn, mx;
   interval_squares(lo, hi, mn, mx);
   return outer_box_logic(lo, hi, mn, mx, n);
+}
+
+// Bound of cull_5db for every box inside [cmin, cmax] (one triangle's bounds, a cluster of
+// triangles or of BVH boxes): never false when cull_5db is true for such a box.  A grown
+// corner's components lie between the endpoint differences lo / hi (FP32 subtraction is
+// monotone).  While hi.y < 0 every corner's y is negative; its normalised y stays negative
+// unless the product underflows, which the test against 2^-100 excludes.  The normalised z then
+// lies between outer_axis' bounds, and the z test is monotone in them.
+__device__ __forceinline__ bool bound_5db(const Node& n, V3 cmin, V3 cmax) {
+  const V3 h = {(n.omax.x - n.omin.x) / 2, (n.omax.y - n.omin.y) / 2, (n.omax.z - n.omin.z) / 2};
+  const V3 b0 = sub(cmin, h), b1 = add(cmax, h);
+  const V3 oc = add(h, n.omin);
+  const V3 lo = sub(b0, oc), hi = sub(b1, oc);
+  if (!(hi.y < 0.f)) return true;
+  V3 mn, mx;
+  interval_squares(lo, hi, mn, mx);
+  const float l2min = (mn.x + mn.y) + mn.z;
+  const float l2max = (mx.x + mx.y) + mx.z;
+  if (!(l2min >= 0x1p-100f) || !(l2max <= 0x1p+100f)) return true;
+  const float inv_hi = __builtin_amdgcn_rsqf(l2min) * (1.0f + 0x1p-16f);
+  const float inv_lo = __builtin_amdgcn_rsqf(l2max) * (1.0f - 0x1p-16f);
+  if (!(hi.y * inv_lo < -0x1p-100f)) return true;
+  float zlo, zhi;
+  outer_axis(lo.z, hi.z, inv_lo, inv_hi, zlo, zhi);
+  return fmx(zlo, n.dmin.y) < fmn(zhi, n.dmax.y) + kEps;
 }
 
 // Order-preserving 32-bit key of a float (+0 and -0 share the key of +0), and the 64-bit

@@ -123,12 +123,63 @@ __device__ __forceinline__ void box_combine(float* a, bool& ha, const float* b, 
   for (int k = 9; k < 12; ++k) a[k] = fmx(a[k], b[k]);
 }
 
+// Direction part of ray i: 6DB / IA keep d.  Node "box_uv" keeps (slope, d.z, 0) with the
+// slope d.y/d.x, 0 when d.x == 0 (rayhconstruct5DB.cl:23,32-33); the validity-aware build
+// (rayhconstructP5DB.cl:25-29,43-47,52-57) gives a ray with d.x == 0 the slope interval
+// [-1, 1] and marks it as a restart: the kernel assigns that interval, dropping what the
+// leaf's earlier rays gave, and later rays widen it again.
+template <bool USE_VALID, bool UV>
+__device__ __forceinline__ void ray_box(const RaysDev& r, size_t i, float* v, bool& restart) {
+  const float dx = r.dx[i], dy = r.dy[i], dz = r.dz[i];
+  restart = false;
+  if constexpr (UV) {
+    const bool flat = dx == 0.f;
+    v[0] = v[3] = flat ? 0.f : dy / dx;
+    if (USE_VALID && flat) {
+      v[0] = -1.f;
+      v[3] = 1.f;
+      restart = true;
+    }
+    v[1] = v[4] = dz;
+    v[2] = v[5] = 0.f;
+  } else {
+    v[0] = v[3] = dx;
+    v[1] = v[4] = dy;
+    v[2] = v[5] = dz;
+  }
+  v[6] = v[9] = r.ox[i];
+  v[7] = v[10] = r.oy[i];
+  v[8] = v[11] = r.oz[i];
+}
+
+// Merge of the rays of one leaf in the reference's read order, a before b: the component-wise
+// min/max, except that a restart in b (validity-aware 5DB) replaces a's slope interval.
+__device__ __forceinline__ void leaf_merge(float* a, bool& ha, bool& ra, const float* b, bool hb,
+                                           bool rb) {
+  if (!hb) return;
+  if (!ha) {
+    for (int k = 0; k < 12; ++k) a[k] = b[k];
+    ha = true;
+    ra = rb;
+    return;
+  }
+  box_combine(a, ha, b, true);
+  if (rb) {
+    a[0] = b[0];
+    a[3] = b[3];
+  }
+  ra = ra || rb;
+}
+
 // ---------------------------------------------------------------------------------------
 // Fused hierarchy build: one work-group builds the leaves and every level of one block
-// (rayhconstruct6DB.cl / rayhconstructP6DB.cl + levelConstruct6DB.cl / levelConstructP6DB.cl
-// folded into one launch instead of 1 + (height-1) launches with host waits,
-// rayhierarchy.cpp:375-549).
-template <bool USE_VALID>
+// (rayhconstruct6DB.cl / rayhconstructP6DB.cl / rayhconstruct(P)5DB.cl + levelConstruct6DB.cl /
+// levelConstructP6DB.cl / levelConstruct(P)5DB.cl folded into one launch instead of
+// 1 + (height-1) launches with host waits, rayhierarchy.cpp:375-549).  A leaf's rays are
+// merged in the reference's read order (row by row, left to right): lane pairs at distance
+// 1, 2, 4, ... combine the lower lane's part first, so every lane ends with the in-order
+// merge.
+template <bool USE_VALID, bool UV>
 __device__ void build_block(const RHGeom& g, const RaysDev& r, float4* __restrict__ nodes, int b,
                             float (*sBox)[12], int* sVal) {
   const int tid = threadIdx.x;
@@ -143,27 +194,28 @@ __device__ void build_block(const RHGeom& g, const RaysDev& r, float4* __restric
       int rl = base + tid;
       int x = 0, y = 0, leaf = 0;
       float v[12];
-      bool have = false;
+      bool have = false, restart = false;
       bool in_block = rl < g.rb;
       if (slot_xy(g, K, J, rl, x, y, leaf)) {
         size_t i = (size_t)y * g.W + x;
         have = !USE_VALID || r.valid == nullptr || r.valid[i] != 0;
-        if (have) {
-          v[0] = v[3] = r.dx[i];
-          v[1] = v[4] = r.dy[i];
-          v[2] = v[5] = r.dz[i];
-          v[6] = v[9] = r.ox[i];
-          v[7] = v[10] = r.oy[i];
-          v[8] = v[11] = r.oz[i];
-        }
+        if (have) ray_box<USE_VALID, UV>(r, i, v, restart);
       }
       if (!have)
         for (int k = 0; k < 12; ++k) v[k] = 0.f;
-      for (int m = g.g >> 1; m >= 1; m >>= 1) {
+      for (int m = 1; m < g.g; m <<= 1) {
         float o[12];
         for (int k = 0; k < 12; ++k) o[k] = __shfl_xor(v[k], m);
         bool ho = __shfl_xor((int)have, m) != 0;
-        box_combine(v, have, o, ho);
+        bool ro = __shfl_xor((int)restart, m) != 0;
+        if (tid & m) {  // the partner's rays come first: ours are merged after them
+          leaf_merge(o, ho, ro, v, have, restart);
+          for (int k = 0; k < 12; ++k) v[k] = o[k];
+          have = ho;
+          restart = ro;
+        } else {
+          leaf_merge(v, have, restart, o, ho, ro);
+        }
       }
       if (in_block && (rl % g.g) == 0) {
         int li = rl / g.g;
@@ -174,16 +226,17 @@ __device__ void build_block(const RHGeom& g, const RaysDev& r, float4* __restric
   } else {
     for (int li = tid; li < nleaf; li += kThreads) {
       float v[12];
-      bool have = false;
+      bool have = false, restart = false;
       int lk = li % g.nlx, lj = li / g.nlx;
       int x0 = K * g.bw + lk * g.cx, y0 = J * g.bh + lj * g.cy;
       for (int y = y0; y < min(g.H, y0 + g.cy); ++y)
         for (int x = x0; x < min(g.W, x0 + g.cx); ++x) {
           size_t i = (size_t)y * g.W + x;
           if (USE_VALID && r.valid != nullptr && r.valid[i] == 0) continue;
-          float b[12] = {r.dx[i], r.dy[i], r.dz[i], r.dx[i], r.dy[i], r.dz[i],
-                         r.ox[i], r.oy[i], r.oz[i], r.ox[i], r.oy[i], r.oz[i]};
-          box_combine(v, have, b, true);
+          float b[12];
+          bool rb;
+          ray_box<USE_VALID, UV>(r, i, b, rb);
+          leaf_merge(v, have, restart, b, true, rb);
         }
       if (!have)
         for (int k = 0; k < 12; ++k) v[k] = 0.f;
@@ -243,14 +296,15 @@ __device__ __forceinline__ int find_ray_item(const RhItem* __restrict__ items, i
   return lo;
 }
 
-template <bool USE_VALID>
+template <bool USE_VALID, bool UV>
 __global__ __launch_bounds__(256) void rh_group_build_kernel(const RhItem* __restrict__ items,
                                                              int nitems) {
   __shared__ float sBox[kMaxSub][12];
   __shared__ int sVal[kMaxSub];
   const int k = find_block_item(items, nitems, blockIdx.x);
   const RhItem& it = items[k];
-  build_block<USE_VALID>(it.g, it.r, it.nodes, (int)(blockIdx.x - it.block_base), sBox, sVal);
+  build_block<USE_VALID, UV>(it.g, it.r, it.nodes, (int)(blockIdx.x - it.block_base), sBox,
+                             sVal);
 }
 
 // Clears the reduction state of every job: keys, repair keys, occlusion flags, counters.
@@ -338,6 +392,7 @@ constexpr int kExpand = 256;     // parents expanded per round (4 children each,
 template <int MODE, int NODE, bool BVH = false>
 __device__ __forceinline__ bool cluster_bound(const Node& n, V3 cmin, V3 cmax) {
   if (NODE == 1) return cull_ia(n, cmin, cmax);
+  if (NODE == 2) return bound_5db(n, cmin, cmax);
   if (MODE == 0 && !BVH) return cluster_tri(n, cmin, cmax);
   return cluster_node6(n, cmin, cmax);
 }
@@ -347,6 +402,7 @@ __device__ __forceinline__ V3 tri_bmax(const Tri& T) { return vmax(vmax(T.v1, T.
 template <int MODE, int NODE>
 __device__ __forceinline__ bool cull_bound(const Node& n, const Tri& T) {
   if (NODE == 1) return cull_ia(n, tri_bmin(T), tri_bmax(T));
+  if (NODE == 2) return bound_5db(n, tri_bmin(T), tri_bmax(T));
   if (MODE == 0) return prefilter_tri(n, T);
   return prefilter_node6(n, tri_bmin(T), tri_bmax(T));
 }
@@ -358,12 +414,16 @@ __device__ __forceinline__ bool cull_bound_slot(const Node& n, PF v9s, P4 tbox, 
   const float4 lo = ld4(tbox, 2 * (size_t)slot), hi = ld4(tbox, 2 * (size_t)slot + 1);
   const V3 bmin = {lo.x, lo.y, lo.z}, bmax = {hi.x, hi.y, hi.z};
   if (NODE == 1) return cull_ia(n, bmin, bmax);
+  if (NODE == 2) return bound_5db(n, bmin, bmax);
   return prefilter_node6(n, bmin, bmax);
 }
-// The reference's cull.
+// The reference's cull: intersectsTri for 6DB primary rays, else intersectsNode of the node
+// type on the triangle's bounds (the 5DB kernels use it in every mode, the repair pass's
+// yetAnotherIntersection5DB.cl:78 included).
 template <int MODE, int NODE>
 __device__ __forceinline__ bool cull_exact(const Node& n, const Tri& T) {
   if (NODE == 1) return cull_ia(n, tri_bmin(T), tri_bmax(T));
+  if (NODE == 2) return cull_5db(n, tri_bmin(T), tri_bmax(T));
   if (MODE == 0) return cull_tri(n, T);
   return cull_node6(n, tri_bmin(T), tri_bmax(T));
 }
@@ -687,7 +747,10 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
       const float4* q = it.nodes + 4 * ((size_t)g.off[0] + (size_t)gj * g.gw[0] + gk);
       const float4 lo = q[0], hi = q[1];
       if (lo.w != 0.f) {
-        atomicAdd(&sSpread, fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z));
+        // box_uv leaves hold slope and d.z intervals: the d.z range is the spread
+        const float sp = NODE == 2 ? hi.y - lo.y
+                                   : fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+        atomicAdd(&sSpread, sp);
         atomicAdd(&sLeaves, 1);
       }
     }
@@ -907,12 +970,20 @@ __device__ __forceinline__ void bvh_box(P bvh, uint32_t n, V3& bmin, V3& bmax, u
 template <int NODE>
 __device__ __forceinline__ bool bvh_bound(const Node& n, V3 bmin, V3 bmax) {
   if (NODE == 1) return cull_ia(n, bmin, bmax);
+  if (NODE == 2) return bound_5db(n, bmin, bmax);
   return prefilter_node6(n, bmin, bmax);
 }
 template <int NODE>
 __device__ __forceinline__ bool bvh_cull(const Node& n, V3 bmin, V3 bmax) {
   if (NODE == 1) return cull_ia(n, bmin, bmax);
+  if (NODE == 2) return cull_5db(n, bmin, bmax);
   return cull_node6(n, bmin, bmax);
+}
+// The 5DB BVH kernels skip a node with no primitives and primitive offset 0 with its subtree
+// (intersection5DB_BVH.cl:18,56 and variants); the 6DB ones do not test it.
+template <int NODE>
+__device__ __forceinline__ bool bvh_live(uint32_t off, uint32_t np) {
+  return NODE != 2 || np != 0 || off != 0;
 }
 
 // Sub-level of subtree node index i (0 root, then 4, 16, 64 nodes).
@@ -940,14 +1011,16 @@ __device__ __forceinline__ bool bvh_chain(const RHGeom& g, const SH& sh, const R
     V3 bmin, bmax;
     uint32_t off, np;
     bvh_box(glb(it.bvh), node, bmin, bmax, off, np);
+    if (!bvh_live<NODE>(off, np)) return false;
     if (!sh.sV[idx] || !bvh_cull<NODE>(sh.sN[idx], bmin, bmax)) return false;
   }
   return true;
 }
 
 // First primitive position and number of primitives that pair e tests: its BVH leaf's, or
-// only the first one for secondary rays (intersection6DB_BVH2.cl).
-template <int MODE, class SH>
+// only the first one for 6DB / IA secondary rays (intersection6DB_BVH2.cl:88; the 5DB kernel
+// tests them all, intersection5DB_BVH2.cl:84).
+template <int MODE, int NODE, class SH>
 __device__ __forceinline__ void pair_prims(const SH& sh, const RhItem& it, int e, uint32_t& off,
                                            uint32_t& cnt) {
   if constexpr (MODE == 2) {
@@ -956,7 +1029,7 @@ __device__ __forceinline__ void pair_prims(const SH& sh, const RhItem& it, int e
   } else {
     const float4 b = ld4(glb(it.bvh), 2 * (size_t)(sh.sPn[e] >> 6) + 1);
     off = __float_as_uint(b.z);
-    cnt = MODE == 1 ? 1u : __float_as_uint(b.w);
+    cnt = MODE == 1 && NODE != 2 ? 1u : __float_as_uint(b.w);
   }
 }
 
@@ -979,7 +1052,7 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
     const int slot = (int)(sh.sPn[e] & 63u) * G + r;
     if (!sh.sRv[slot]) continue;  // cleared once the ray is found occluded
     uint32_t off, cnt;
-    pair_prims<MODE>(sh, it, e, off, cnt);
+    pair_prims<MODE, NODE>(sh, it, e, off, cnt);
     for (uint32_t f = 0; f < cnt; ++f) {
       float t, b1, b2;
       if (leaf_hit<TM>(sh, slot, load_mt(glb(it.tmt), glb(it.prim_order)[off + f]), t, b1,
@@ -1053,7 +1126,7 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
       }
       if (!sh.sRv[slot]) continue;
       uint32_t off, cnt;
-      pair_prims<MODE>(sh, it, e, off, cnt);
+      pair_prims<MODE, NODE>(sh, it, e, off, cnt);
       for (uint32_t f = 0; f < cnt; ++f) {
         const uint32_t tri = glb(it.prim_order)[off + f];
         float t, b1, b2;
@@ -1154,7 +1227,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
           V3 bmin, bmax;
           uint32_t off, np;
           bvh_box(glb(it.bvh), node, bmin, bmax, off, np);
-          if (bvh_bound<NODE>(sh.sN[0], bmin, bmax)) {
+          if (bvh_live<NODE>(off, np) && bvh_bound<NODE>(sh.sN[0], bmin, bmax)) {
             if (top == 0 && np > 0) {
               bvh_emit<MODE>(sh, node, off, np, 0);
             } else {
@@ -1221,7 +1294,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
                 V3 cmin, cmax;
                 uint32_t coff, cnp;
                 bvh_box(glb(it.bvh), bc, cmin, cmax, coff, cnp);
-                if (bvh_bound<NODE>(sh.sN[Rn], cmin, cmax)) {
+                if (bvh_live<NODE>(coff, cnp) && bvh_bound<NODE>(sh.sN[Rn], cmin, cmax)) {
                   if (sub_level(Rn) == top && cnp > 0)
                     bvh_emit<MODE>(sh, bc, coff, cnp, Rn - pow4_off(top));
                   else {
@@ -1336,7 +1409,7 @@ __device__ __forceinline__ bool ray_chain_bvh(const RhItem& it, int x, int y, ui
     V3 bmin, bmax;
     uint32_t off, np;
     bvh_box(glb(it.bvh), node, bmin, bmax, off, np);
-    if (!bvh_cull<NODE>(n, bmin, bmax)) return false;
+    if (!bvh_live<NODE>(off, np) || !bvh_cull<NODE>(n, bmin, bmax)) return false;
   }
   return true;
 }
@@ -1477,7 +1550,7 @@ __global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__
         if constexpr (BVH) {
           bl = glb(it.tleaf)[id];
           if (bl == 0xffffffffu) continue;
-          if (MODE == 1 && (bl & 0x80000000u)) continue;  // first primitive only
+          if (MODE == 1 && NODE != 2 && (bl & 0x80000000u)) continue;  // first primitive only
           bl &= 0x7fffffffu;
         }
         float t, b1, b2;
@@ -1755,9 +1828,10 @@ __global__ __launch_bounds__(256) void rh_repair_kernel(const RhItem* __restrict
         if (ti < ntri) {
           const Tri T = load_tri(v9, ti);
           bool pass = true;
-          if (NODE == 1) {
+          if (NODE != 0) {
             const V3 bmin = vmin(vmin(T.v1, T.v2), T.v3), bmax = vmax(vmax(T.v1, T.v2), T.v3);
-            for (int l = g.L - 1; l >= 0 && pass; --l) pass = cull_ia(sChain[l], bmin, bmax);
+            for (int l = g.L - 1; l >= 0 && pass; --l)
+              pass = NODE == 1 ? cull_ia(sChain[l], bmin, bmax) : cull_5db(sChain[l], bmin, bmax);
           } else {
             for (int l = g.L - 1; l >= 0 && pass; --l) pass = cull_tri(sChain[l], T);
           }
@@ -1978,8 +2052,8 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
                              hipEvent_t t1) {
   hipLaunchKernelGGL((rh_init_kernel<MODE>), grid1(std::max<uint32_t>(G.nrays, 4)),
                      dim3(kThreads), 0, s, G.items, G.nitems, G.nrays, G.ctr);
-  hipLaunchKernelGGL((rh_group_build_kernel<MODE != 0>), dim3(G.nblocks), dim3(kThreads), 0, s,
-                     G.items, G.nitems);
+  hipLaunchKernelGGL((rh_group_build_kernel<MODE != 0, NODE == 2>), dim3(G.nblocks),
+                     dim3(kThreads), 0, s, G.items, G.nitems);
   if (G.bvh)
     hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE, true>), dim3(G.nblocks), dim3(kThreads), 0,
                        s, G.items, G.nitems, G.units, G.rq, G.ctr, G.per_unit, G.ray_clusters);
@@ -2134,6 +2208,11 @@ hipError_t launch_rh_group(const RhGroup& G, hipStream_t s, hipEvent_t t0, hipEv
     if (G.mode == 0) return RT_CALL(0, 1);
     if (G.mode == 1) return RT_CALL(1, 1);
     return RT_CALL(2, 1);
+  }
+  if (G.node == 2) {
+    if (G.mode == 0) return RT_CALL(0, 2);
+    if (G.mode == 1) return RT_CALL(1, 2);
+    return RT_CALL(2, 2);
   }
   if (G.mode == 0) return RT_CALL(0, 0);
   if (G.mode == 1) return RT_CALL(1, 0);

@@ -78,7 +78,8 @@ rt_accel_desc make_desc(int32_t kind) {
 int32_t node_kind(const std::string& node) {
   if (node == "box_dir") return RT_NODE_BOX_DIR;
   if (node == "ia") return RT_NODE_IA;
-  Severe("node type \"" + node + "\" is not supported (box_dir, ia)");
+  if (node == "box_uv") return RT_NODE_BOX_UV;
+  Severe("Unknown accelarator node type " + node);
 }
 
 }  // namespace

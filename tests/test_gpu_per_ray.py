@@ -64,7 +64,7 @@ def _oracle(orc, sc, accel, node, rays, mode, **kw):
 
 
 @pytest.mark.parametrize("accel", ["rayhierarchy", "ray-bvh"])
-@pytest.mark.parametrize("node", ["box_dir", "ia"])
+@pytest.mark.parametrize("node", ["box_dir", "ia", "box_uv"])
 def test_per_ray_matches_block_and_oracle(rt, scenes, orc, gpu_ctx, accel, node):
     v9 = scenes.cluster_scene(6000, seed=11)
     sc = rt.Scene(gpu_ctx, v9, desc=rt.default_desc(accel, node=node))
@@ -125,16 +125,18 @@ def test_per_ray_big_blocks_and_repair(rt, scenes, orc, gpu_ctx):
     sc.close()
 
 
-def test_per_ray_raybvh_first_primitive_rule(rt, scenes, orc, gpu_ctx):
-    """ray-bvh with maxBVHPrim 4: secondary rays test only the first primitive of each BVH leaf
-    (intersection6DB_BVH2.cl), primary and shadow rays every primitive."""
+@pytest.mark.parametrize("node", ["box_dir", "box_uv"])
+def test_per_ray_raybvh_first_primitive_rule(rt, scenes, orc, gpu_ctx, node):
+    """ray-bvh with maxBVHPrim 4: 6DB secondary rays test only the first primitive of each BVH
+    leaf (intersection6DB_BVH2.cl:88), 5DB ones every primitive (intersection5DB_BVH2.cl:84);
+    primary and shadow rays every primitive."""
     v9 = scenes.cluster_scene(3000, seed=17)
-    desc = rt.default_desc("ray-bvh", max_bvh_prim=4)
+    desc = rt.default_desc("ray-bvh", max_bvh_prim=4, node=node)
     sc = rt.Scene(gpu_ctx, v9, desc=desc)
     rnd = scenes.random_rays(48 * 48, seed=18, lo=-5.0, hi=5.0, maxt=np.inf, width=48)
     for mode in (0, 1):
         out = _paths(rt, sc, rt.DeviceRays.from_numpy(rnd), mode)
-        _check_paths(out, mode, f"maxBVHPrim 4 mode {mode}")
-        ref = _oracle(orc, sc, "ray-bvh", "box_dir", rnd, mode)
+        _check_paths(out, mode, f"maxBVHPrim 4 {node} mode {mode}")
+        ref = _oracle(orc, sc, "ray-bvh", node, rnd, mode)
         assert np.array_equal(out["ray"]["id"], ref["id"]), f"mode {mode} vs oracle"
     sc.close()

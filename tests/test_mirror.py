@@ -94,7 +94,7 @@ def _closest_rays(r):
     ("kdtree-gpu", "", "unknown"),
     ("naive", "method=RayTri", "RayTri"),
     ("naive", "onGPU=false", "onGPU"),
-    ("rayhierarchy", "node=box_uv", "box_uv"),
+    ("rayhierarchy", "node=box_xyz", "Unknown accelarator node type box_xyz"),
     ("ray-bvh", "splitmethod=middle", "middle"),
 ])
 def test_factory_rejects(scenes, accel, params, msg):
@@ -159,7 +159,7 @@ def test_scene_dispatch(scenes, accel):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("accel,node", [("rayhierarchy", "box_dir"), ("rayhierarchy", "ia"),
-                                        ("ray-bvh", "box_dir")])
+                                        ("ray-bvh", "box_dir"), ("ray-bvh", "box_uv")])
 def test_layer_primary_and_shadow(scenes, orc, accel, node):
     v9 = scenes.torus_scene() if accel == "rayhierarchy" else scenes.room_scene(nclusters=60)
     ntri = v9.shape[0]

@@ -858,9 +858,233 @@ def test_flags_do_not_change_results(orc, scenes):
         assert a["stats"].leaf_pairs == b["stats"].leaf_pairs
 
 
+# ---------------------------------------------------------------- box_uv (5DB)
+# Node "box_uv" stores the slope d.y/d.x and d.z intervals of its rays.  As C parses
+# intersectsNode5DB.cl:79-81, a node passes when some corner direction of the origin-grown box
+# has y >= 0, else when the corners' z range meets [dz min, dz max] within EPS = 0.002; the
+# slope intervals never decide.  A point box seen from a point origin has one direction:
+# (0, -3, 4) normalises to (0, -0.6, 0.8).
+def uv16(slope, dz, omin=(0, 0, 0), omax=(0, 0, 0)):
+    return node16((slope[0], dz[0], 0), (slope[1], dz[1], 0), omin, omax)
+
+
+def test_cull_5db_z_interval_and_eps(orc):
+    p = (0.0, -3.0, 4.0)
+    cull = lambda dz, slope=(-1, 1): orc.cull("5db", uv16(slope, dz), bmin=p, bmax=p)
+    assert cull((0.70, 0.90))
+    assert cull((0.70, 0.7985))      # 0.8 < 0.7985 + EPS
+    assert not cull((0.70, 0.797))   # 0.8 >= 0.797 + EPS
+    assert cull((0.8005, 0.90))      # 0.8005 < 0.8 + EPS
+    assert not cull((0.803, 0.90))   # EPS widens the upper ends only
+    # the slope intervals never decide: slope 0 of this direction lies outside [100, 200]
+    assert cull((0.70, 0.90), slope=(100, 200))
+    assert not cull((-0.9, -0.7), slope=(-1e9, 1e9))
+
+
+def test_cull_5db_y_sign_rule(orc):
+    far = ((0, 0, 0), (5, 6))  # slope, dz: no corner direction has z near 5
+    for p in ((0.0, 0.0, 5.0), (0.0, 1e-3, -5.0), (3.0, 2.0, 1.0)):
+        assert orc.cull("5db", uv16(*far), bmin=p, bmax=p), p   # y >= 0: passes
+    for p in ((0.0, -1e-3, 5.0), (3.0, -2.0, 1.0)):
+        assert not orc.cull("5db", uv16(*far), bmin=p, bmax=p), p
+    # one corner of a box with y >= 0 is enough
+    assert orc.cull("5db", uv16(*far), bmin=(1, -2, 1), bmax=(2, 0, 2))
+    assert not orc.cull("5db", uv16(*far), bmin=(1, -2, 1), bmax=(2, -1e-3, 2))
+
+
+def test_cull_5db_grows_by_origin_extent(orc):
+    # origin box [-1, 1]^3 (centre 0, half extent 1): the triangle box is grown by 1
+    nd = lambda dz: uv16((0, 0), dz, omin=(-1, -1, -1), omax=(1, 1, 1))
+    assert orc.cull("5db", nd((5, 6)), bmin=(0, -0.5, 4), bmax=(0, -0.5, 4))    # grown y reaches 0.5
+    assert not orc.cull("5db", nd((5, 6)), bmin=(0, -2.5, 4), bmax=(0, -2.5, 4))
+    assert orc.cull("5db", nd((-1, 1)), bmin=(0, -2.5, 4), bmax=(0, -2.5, 4))
+
+
+def _row_rays(d, W, H, valid=None):
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    n = W * H
+    r = _rays_of(np.zeros((n, 3)), d, W, H)
+    if valid is not None:
+        r["valid"] = np.asarray(valid, np.uint8)
+    return r
+
+
+def test_rh_build_5db_leaf(orc):
+    # one 4x4 tile holding a 4x1 image: slopes 2, (d.x = 0), 0.5, -1; d.z 0.5, 0.3, 0.1, 0.7
+    d = [(1, 2, 0.5), (0, 1, 0.3), (4, 2, 0.1), (2, -2, 0.7)]
+    rays = _row_rays(d, 4, 1)
+    _, nodes = orc.rh_build(rays, node="box_uv")
+    # primary rays (rayhconstruct5DB.cl): d.x == 0 gives slope 0
+    assert list(nodes[0][[0, 4, 1, 5, 2, 6]]) == [-1.0, 2.0, 0.1, 0.7, 0.0, 0.0]
+    _, nodes = orc.rh_build(rays, use_valid=True, node="box_uv")
+    # validity-aware (rayhconstructP5DB.cl): the d.x == 0 ray sets [-1, 1], dropping the slope 2
+    # read before it; the later slopes 0.5 and -1 lie inside
+    assert list(nodes[0][[0, 4, 1, 5]]) == [-1.0, 1.0, 0.1, 0.7]
+    assert nodes[0][3] == 1
+    # a flat ray first: later slopes widen [-1, 1]
+    _, nodes = orc.rh_build(_row_rays([(0, 1, 0.3), (1, 3, 0.2)], 2, 1), use_valid=True,
+                            node="box_uv")
+    assert list(nodes[0][[0, 4]]) == [-1.0, 3.0]
+    # an invalid flat ray is skipped: plain union of the valid slopes 2 and 0.5
+    _, nodes = orc.rh_build(_row_rays(d[:3], 3, 1, valid=[1, 0, 1]), use_valid=True,
+                            node="box_uv")
+    assert list(nodes[0][[0, 4, 1, 5]]) == [0.5, 2.0, 0.1, 0.5]
+    # no valid ray: the leaf is invalid
+    _, nodes = orc.rh_build(_row_rays(d[:2], 2, 1, valid=[0, 0]), use_valid=True, node="box_uv")
+    assert nodes[0][3] == 0
+
+
+def test_rh_build_5db_row_major_order(orc):
+    # 4x2 image in one tile: rows are read top to bottom, each left to right
+    base = np.tile(np.float32([2, 1, 0.5]), (8, 1))      # slope 0.5
+    a = base.copy()
+    a[3] = (0, 1, 0.5)   # flat at row 0, column 3
+    a[4] = (1, 3, 0.5)   # slope 3 at row 1, column 0: read after the flat ray
+    _, nodes = orc.rh_build(_row_rays(a, 4, 2), use_valid=True, node="box_uv")
+    assert list(nodes[0][[0, 4]]) == [-1.0, 3.0]
+    b = base.copy()
+    b[3] = (1, 3, 0.5)   # slope 3 first
+    b[4] = (0, 1, 0.5)   # then the flat ray
+    _, nodes = orc.rh_build(_row_rays(b, 4, 2), use_valid=True, node="box_uv")
+    assert list(nodes[0][[0, 4]]) == [-1.0, 1.0]
+    # primary rays merge: both orders give [0, 3]
+    for r in (a, b):
+        _, nodes = orc.rh_build(_row_rays(r, 4, 2), node="box_uv")
+        assert list(nodes[0][[0, 4]]) == [0.0, 3.0]
+
+
+def test_rh_build_5db_levels_merge(orc, scenes):
+    # levelConstructP5DB.cl merges children by min/max only (no reset above the leaves)
+    rays = scenes.random_rays(16 * 16, seed=70, width=16)
+    rays["dx"][::7] = 0.0
+    rays["valid"] = (np.random.default_rng(71).random(256) < 0.7).astype(np.uint8)
+    levels, nodes = orc.rh_build(rays, use_valid=True, node="box_uv")
+    (gw0, gh0), (gw1, gh1) = levels[0], levels[1]
+    for j in range(gh1):
+        for k in range(gw1):
+            kids = [nodes[(2 * j + a) * gw0 + 2 * k + b] for a in (0, 1) for b in (0, 1)]
+            kids = [c for c in kids if c[3] == 1]
+            nd = nodes[gw0 * gh0 + j * gw1 + k]
+            assert nd[3] == (1 if kids else 0)
+            if kids:
+                kk = np.array(kids)
+                np.testing.assert_array_equal(nd[[0, 1, 8, 9, 10]], kk[:, [0, 1, 8, 9, 10]].min(0))
+                np.testing.assert_array_equal(nd[[4, 5, 12, 13, 14]], kk[:, [4, 5, 12, 13, 14]].max(0))
+
+
+def test_rh_5db_compares_raw_dz(orc):
+    # The node keeps the rays' raw d.z while the cull normalises corner directions: a ray
+    # toward a triangle below its origin hits it with a unit direction and misses it with the
+    # same direction scaled by 5 (d.z 4 against corner z near 0.8).  Above the origin the y
+    # rule passes every node and both hit.
+    below = np.array([[-0.5, -3.2, 4, 0.5, -3.2, 4, 0, -2.8, 4]], np.float32)
+    above = below * np.float32([1, -1, 1] * 3)
+    p = orc.params("rayhierarchy", node="box_uv")
+    for v9, dy in ((below, -1), (above, 1)):
+        for scale in (1.0, 5.0):
+            rays = _rays_of([(0, 0, 0)], [(0, dy * 0.6 * scale, 0.8 * scale)], 1, 1)
+            got = orc.trace(v9, rays, p, orc.CLOSEST_PRIMARY)
+            exp = -1 if (dy < 0 and scale == 5.0) else 0
+            assert got["id"][0] == exp, (dy, scale)
+            if exp == 0:
+                assert abs(got["t"][0] * scale - 5.0) < 1e-5
+
+
+def _above_scene(scenes, n, seed):
+    """Triangles with every vertex y >= -1.4, above the camera at y = -1.5 and the origins of
+    _below_rays."""
+    return scenes.random_triangles(n, seed=seed, extent=1.0, size=0.4)
+
+
+def _below_rays(scenes, seed, maxt):
+    r = scenes.random_rays(256, seed=seed, lo=-1.5, hi=1.5, maxt=maxt, width=16)
+    r["oy"] = np.random.default_rng(seed).uniform(-3.0, -1.5, 256).astype(np.float32)
+    r["dy"] = np.abs(r["dy"])  # upward, toward the triangles
+    r["valid"] = (np.arange(256) % 5 != 0).astype(np.uint8)
+    return r
+
+
+def test_rh_5db_all_pass_equals_brute_force(orc, scenes):
+    # Every triangle lies above every ray origin, so some corner of each grown box is above
+    # each node's origin centre and every 5DB cull passes: the hierarchy must return the
+    # exhaustive strict nearest hit (primary, secondary) and occlusion.
+    v9 = _above_scene(scenes, 128, 72)
+    p = orc.params("rayhierarchy", node="box_uv")
+    cam = scenes.pinhole_rays(16, 16, origin=(0.0, -1.5, -3.0), fov_deg=90.0)
+    got = orc.trace(v9, cam, p, orc.CLOSEST_PRIMARY)
+    ids, ts = _strict_brute(v9, cam)
+    np.testing.assert_array_equal(got["id"], ids)
+    np.testing.assert_array_equal(got["t"], ts)
+    assert (ids >= 0).sum() > 20
+    sec = _below_rays(scenes, 73, np.inf)
+    got = orc.trace(v9, sec, p, orc.CLOSEST_SECONDARY)
+    ids, _ = _strict_brute(v9, sec, valid_only=True)
+    np.testing.assert_array_equal(got["id"], ids)
+    assert (ids >= 0).sum() > 5
+    sh = _below_rays(scenes, 74, 6.0)
+    got = orc.trace(v9, sh, p, orc.OCCLUDED)
+    exp = _strict_occ_brute(v9, sh)
+    np.testing.assert_array_equal(got["occ"], exp)
+    assert 5 < exp.sum() < 200
+
+
+def test_raybvh_5db_all_pass_every_primitive(orc, scenes):
+    # ray-bvh with box_uv: with every cull passing, primary AND secondary rays see every
+    # primitive of a BVH leaf (intersection5DB_BVH2.cl:84 loops over all of them, unlike
+    # intersection6DB_BVH2.cl:88), and occlusion is exhaustive.
+    v9 = _paired_scene(40, seed=75)
+    v9[:, 1::3] += np.float32(2.5)   # every vertex above y = 0.4 (a dyadic shift keeps the pairs)
+    nodes, roots, order = orc.raybvh_handoff(v9, max_prims=4, bvh_height=3)
+    assert (nodes[nodes["n_primitives"] > 0]["n_primitives"] >= 2).sum() >= 10
+    p = orc.params("ray-bvh", node="box_uv")
+    # rays from below aimed at the triangles' centroids
+    g = np.random.default_rng(76)
+    tgt = v9.reshape(-1, 3, 3).mean(1)[g.integers(0, len(v9), 256)]
+    o = np.stack([g.uniform(-1.5, 1.5, 256), g.uniform(-3.0, -1.5, 256),
+                  g.uniform(-1.5, 1.5, 256)], 1).astype(np.float32)
+    d = (tgt - o) / np.linalg.norm(tgt - o, axis=1, keepdims=True)
+    sec = _rays_of(o, d, 16, 16)
+    sec["valid"] = (np.arange(256) % 5 != 0).astype(np.uint8)
+    got = orc.trace_raybvh(v9, nodes, roots, order, sec, p, orc.CLOSEST_SECONDARY)
+    ids, _ = _strict_brute(v9, sec, valid_only=True)
+    np.testing.assert_array_equal(got["id"], ids)
+    assert (ids >= 0).sum() > 100
+    p6 = orc.params("ray-bvh", node="box_dir")
+    got6 = orc.trace_raybvh(v9, nodes, roots, order, sec, p6, orc.CLOSEST_SECONDARY)
+    assert (got6["id"] != ids).sum() > 0, "box_dir's first-primitive rule must differ here"
+    cam = scenes.pinhole_rays(16, 16, origin=(0.0, -2.0, -3.0), fov_deg=90.0)
+    got = orc.trace_raybvh(v9, nodes, roots, order, cam, p, orc.CLOSEST_PRIMARY)
+    ids, ts = _strict_brute(v9, cam)
+    np.testing.assert_array_equal(got["id"], ids)
+    sh = _below_rays(scenes, 77, 8.0)
+    got = orc.trace_raybvh(v9, nodes, roots, order, sh, p, orc.OCCLUDED)
+    np.testing.assert_array_equal(got["occ"], _strict_occ_brute(v9, sh))
+
+
+def test_raybvh_5db_skips_empty_nodes(orc):
+    # intersection5DB_BVH.cl:18,56: a node with no primitives and primitive offset 0 is
+    # skipped with its subtree.  Nodes 0 and 1 are leaves of one triangle each; node 2 is an
+    # interior node over them (children 0 and 1), i.e. prim_offset 0 with no primitives.
+    v9 = np.array([[-1, 1, 4, 1, 1, 4, 0, 3, 4], [-1, 1, 6, 1, 1, 6, 0, 3, 6]], np.float32)
+    nodes = np.zeros(3, orc.GPU_NODE_DTYPE)
+    for i, z in ((0, 4), (1, 6)):
+        nodes[i] = (-1, 1, z, 1, 3, z, i, 1)
+    nodes[2] = (-1, 1, 4, 1, 3, 6, 0, 0)
+    order = np.arange(2, dtype=np.uint32)
+    rays = _rays_of([(0, 0, 0)] * 4, [(0, 0.4, 1)] * 4, 2, 2)
+    p = orc.params("ray-bvh", node="box_uv")
+    got = orc.trace_raybvh(v9, nodes, np.array([2], np.uint32), order, rays, p, orc.CLOSEST_PRIMARY)
+    assert np.all(got["id"] == -1)
+    got = orc.trace_raybvh(v9, nodes, np.array([0, 1], np.uint32), order, rays, p,
+                           orc.CLOSEST_PRIMARY)
+    assert np.all(got["id"] == 0)
+    occ = orc.trace_raybvh(v9, nodes, np.array([2], np.uint32), order, rays, p, orc.OCCLUDED)
+    assert np.all(occ["occ"] == 0)
+
+
 # ---------------------------------------------------------------- the per-ray path, restated
 @pytest.mark.parametrize("accel", ["rayhierarchy", "ray-bvh"])
-@pytest.mark.parametrize("node", ["box_dir", "ia"])
+@pytest.mark.parametrize("node", ["box_dir", "ia", "box_uv"])
 def test_search_trace_equals_reference_semantics(orc, scenes, accel, node):
     # The per-ray path walks a padded search BVH and applies the reference's cull chain only to
     # triangles that pass the strict leaf test.  Its answer must be the reference's: equal to
