@@ -112,21 +112,30 @@ int rt_scene_set_bvh(rt_scene *scene, const rt_gpu_node *nodes, uint32_t nnodes,
                      const uint32_t *roots, uint32_t nroots, const uint32_t *prim_order);
 int rt_scene_destroy(rt_scene *scene);
 
+/* BVHAccel split methods ("splitmethod", bvh.cpp:214-221): "sah" (default), "middle",
+ * "equal". */
+enum { RT_SPLIT_SAH = 0, RT_SPLIT_MIDDLE = 1, RT_SPLIT_EQUAL = 2 };
+
 /* Host-side CPU BVH of ray-bvh, as RayBVH builds it before its first trace
  * (rayBVH.cpp:144-146,204-276; BVHAccel bvh.cpp:209-434; GPUbufferWide bvh.cpp:154-206):
- * global BVH order, parts of at most 60,000 triangles, per-part SAH BVH (12 buckets,
+ * global BVH order, parts of at most 60,000 triangles, per-part BVH (split_method RT_SPLIT_*,
  * max_prims <= 4 per leaf) flattened breadth first keeping the lowest bvh_height levels.
  * Returns malloc'ed HOST arrays for rt_scene_set_bvh; release them with
  * rt_raybvh_free_handoff. */
 int rt_raybvh_build_handoff(const float *v9, uint32_t ntri, uint32_t max_prims,
-                            uint32_t bvh_height, rt_gpu_node **nodes, uint32_t *nnodes,
-                            uint32_t **roots, uint32_t *nroots, uint32_t **prim_order);
+                            int32_t split_method, uint32_t bvh_height, rt_gpu_node **nodes,
+                            uint32_t *nnodes, uint32_t **roots, uint32_t *nroots,
+                            uint32_t **prim_order);
 int rt_raybvh_free_handoff(rt_gpu_node *nodes, uint32_t *roots, uint32_t *prim_order);
-/* Primitive order of pbrt's BVHAccel over the triangles (SAH, 12 buckets, leaves of at most
- * min(255, max_prims) primitives; bvh.cpp:209-434): order[i] = input index of the triangle at
- * BVH position i.  RayHieararchy's "bool sort" parameter swaps its primitives into this order
- * before flattening them (rayhierarchy.cpp:152-155, parameter at :1332).  HOST arrays. */
-int rt_bvh_order(const float *v9, uint32_t ntri, uint32_t max_prims, uint32_t *order);
+/* Primitive order of pbrt's BVHAccel over the triangles (split_method RT_SPLIT_*, leaves of at
+ * most min(255, max_prims) primitives; bvh.cpp:209-434): order[i] = input index of the
+ * triangle at BVH position i.  RayHieararchy's "bool sort" parameter swaps its primitives into
+ * this order before flattening them (rayhierarchy.cpp:152-155, parameters at :1332,1337-1338).
+ * Where the reference's SAH build lists a leaf's primitives twice (bvh.cpp:417-428, more than 4
+ * primitives per leaf) the first occurrences are kept: the later copies never win a tie.
+ * HOST arrays. */
+int rt_bvh_order(const float *v9, uint32_t ntri, uint32_t max_prims, int32_t split_method,
+                 uint32_t *order);
 
 /* Batched closest hit: RayHieararchy::Intersect (rayhierarchy.cpp:602-813,816-1140),
  * RayBVH::Intersect (rayBVH.cpp:615-836,839-1138), NaiveAccel::Intersect (naive.cpp:204-338).

@@ -57,6 +57,7 @@ def lib():
         _LIB.orc_rh_build.restype = ctypes.c_int64
         _LIB.orc_bvh_build.restype = ctypes.c_int64
         _LIB.orc_raybvh_handoff.restype = ctypes.c_int64
+        _LIB.orc_bvh_order.restype = ctypes.c_int64
         _LIB.orc_cpu_bvh_create.restype = ctypes.c_void_p
         _LIB.orc_cpu_bvh_destroy.argtypes = [ctypes.c_void_p]
         _LIB.orc_cpu_bvh_closest.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
@@ -164,7 +165,7 @@ def trace_raybvh(v9, nodes, roots, prim_order, rays: dict, p: OrcParams, mode: i
     return out
 
 
-def shading(v9, uv6, rays: dict, hit_id):
+def shading(v9, uv6, rays: dict, hit_id, naive=False):
     v9 = _f32(v9).reshape(-1)
     ntri = v9.size // 9
     keep = _RayKeep(rays)
@@ -175,7 +176,8 @@ def shading(v9, uv6, rays: dict, hit_id):
     uvk = None if uv6 is None else _f32(uv6).reshape(-1)
     uvp = None if uvk is None else _fp(uvk.ctypes.data)
     lib().orc_shading(_fp(v9.ctypes.data), uvp, ctypes.c_uint32(ntri), ctypes.byref(keep.s),
-                      _fp(hit_id.ctypes.data), _fp(u.ctypes.data), _fp(v.ctypes.data),
+                      _fp(hit_id.ctypes.data), 1 if naive else 0, _fp(u.ctypes.data),
+                      _fp(v.ctypes.data),
                       _fp(dpdu.ctypes.data), _fp(dpdv.ctypes.data))
     return dict(u=u, v=v, dpdu=dpdu, dpdv=dpdv)
 
@@ -224,19 +226,37 @@ def cull(kind: str, node16, v9=None, bmin=None, bmax=None) -> bool:
     return bool(fn(_vec(node16), _vec(bmin), _vec(bmax)))
 
 
-def bvh_build(v9, max_prims=4, gpu=True, bvh_height=3):
+SPLIT = {"sah": 0, "middle": 1, "equal": 2}
+
+
+def bvh_build(v9, max_prims=4, gpu=True, bvh_height=3, split="sah"):
     v9 = _f32(v9).reshape(-1)
     ntri = v9.size // 9
     order = np.zeros(max(ntri, 1), np.uint32)
     nodes = np.zeros(max(2 * ntri, 1), GPU_NODE_DTYPE)
     top = ctypes.c_uint32()
     nn = lib().orc_bvh_build(_fp(v9.ctypes.data), ctypes.c_uint32(ntri), ctypes.c_uint32(max_prims),
-                             ctypes.c_int32(1 if gpu else 0), ctypes.c_uint32(bvh_height),
-                             _fp(order.ctypes.data), _fp(nodes.ctypes.data), ctypes.byref(top))
+                             ctypes.c_int32(SPLIT[split]), ctypes.c_int32(1 if gpu else 0),
+                             ctypes.c_uint32(bvh_height), _fp(order.ctypes.data),
+                             _fp(nodes.ctypes.data), ctypes.byref(top))
+    assert nn >= 0
     return nodes[:nn].copy(), int(top.value), order[:ntri].copy()
 
 
-def raybvh_handoff(v9, max_prims=1, bvh_height=3):
+def bvh_order(v9, max_prims=1, split="sah"):
+    """BVHAccel's orderedPrims, repeats included (see orc_bvh_order)."""
+    v9 = _f32(v9).reshape(-1)
+    ntri = v9.size // 9
+    cap = 16 * max(ntri, 1)
+    order = np.zeros(cap, np.uint32)
+    n = lib().orc_bvh_order(_fp(v9.ctypes.data), ctypes.c_uint32(ntri), ctypes.c_uint32(max_prims),
+                            ctypes.c_int32(SPLIT[split]), _fp(order.ctypes.data),
+                            ctypes.c_uint64(cap))
+    assert n >= 0
+    return order[:n].copy()
+
+
+def raybvh_handoff(v9, max_prims=1, bvh_height=3, split="sah"):
     v9 = _f32(v9).reshape(-1)
     ntri = v9.size // 9
     nodes = np.zeros(max(2 * ntri + 8, 1), GPU_NODE_DTYPE)
@@ -244,7 +264,8 @@ def raybvh_handoff(v9, max_prims=1, bvh_height=3):
     order = np.zeros(max(ntri, 1), np.uint32)
     nr = ctypes.c_uint32()
     nn = lib().orc_raybvh_handoff(_fp(v9.ctypes.data), ctypes.c_uint32(ntri),
-                                  ctypes.c_uint32(max_prims), ctypes.c_uint32(bvh_height),
+                                  ctypes.c_uint32(max_prims), ctypes.c_int32(SPLIT[split]),
+                                  ctypes.c_uint32(bvh_height),
                                   _fp(nodes.ctypes.data), _fp(roots.ctypes.data), ctypes.byref(nr),
                                   _fp(order.ctypes.data))
     return nodes[:nn].copy(), roots[:nr.value].copy(), order[:ntri].copy()

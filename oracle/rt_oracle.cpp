@@ -389,21 +389,26 @@ struct RH {
 };
 
 int rh_levels(int W, int H, int cx, int cy, int height, int* dims) {
-  // Leaf grid ceil(W/cx) x ceil(H/cy) (rayhierarchy.cpp:377); each further level halves the
-  // grid, and the build stops after a level whose width or height is odd (:438-442).
-  // Parents of odd-sized child levels cover the existing children (ceil); the reference
-  // only builds such levels from odd leaf grids, where its image indexing breaks.
+  // Leaf grid ceil(W/cx) x ceil(H/cy) (rayhierarchy.cpp:377).  The reference halves the grid
+  // with floor at every level and stops after a level whose floor width or height is odd
+  // (:412-442); the level count follows that rule exactly.  Where it halves an odd dimension
+  // its parents leave the last child column (row) uncovered and those rays are never tested;
+  // here the parents cover every child (ceil), a superset of the reference's coverage with the
+  // same chain length for the rays it does cover.
   int gw = (W + cx - 1) / cx, gh = (H + cy - 1) / cy;
+  int fw = gw, fh = gh;
   dims[0] = gw;
   dims[1] = gh;
   int L = 1;
   for (int i = 1; i < height; ++i) {
     gw = (gw + 1) / 2;
     gh = (gh + 1) / 2;
+    fw /= 2;
+    fh /= 2;
     dims[2 * L] = gw;
     dims[2 * L + 1] = gh;
     ++L;
-    if ((gw & 1) || (gh & 1)) break;
+    if ((fw & 1) || (fh & 1)) break;
   }
   return L;
 }
@@ -854,8 +859,9 @@ struct BuildNode {
 
 struct BvhBuilder {
   uint32_t maxPrims;
+  int split = 0;  // 0 sah, 1 middle, 2 equal (bvh.cpp:214-221)
   std::vector<BuildNode> nodes;
-  std::vector<uint32_t> ordered;
+  std::vector<uint32_t> ordered;  // orderedPrims, with the repeats of the SAH fall-through
 
   int build(std::vector<PrimInfo>& bd, uint32_t start, uint32_t end) {
     int me = (int)nodes.size();
@@ -882,12 +888,25 @@ struct BvhBuilder {
       make_leaf();
       return me;
     }
-    if (nPrims <= 4) {
+    auto equal_counts = [&]() {
       mid = (start + end) / 2;
       std::nth_element(&bd[start], &bd[mid], &bd[end - 1] + 1,
                        [dim](const PrimInfo& a, const PrimInfo& b) {
                          return vget(a.centroid, dim) < vget(b.centroid, dim);
                        });
+    };
+    if (split == 1) {
+      // SPLIT_MIDDLE, bvh.cpp:333-340: CompareToMid is centroid < pmid.  An empty side (the
+      // midpoint rounded onto an end of the range) makes the reference recurse without end;
+      // equal counts stand in for that case.
+      const float pmid = .5f * (cb.get_lo(dim) + cb.get_hi(dim));
+      PrimInfo* pm = std::partition(&bd[start], &bd[end - 1] + 1, [&](const PrimInfo& p) {
+        return vget(p.centroid, dim) < pmid;
+      });
+      mid = (uint32_t)(pm - &bd[0]);
+      if (mid == start || mid == end) equal_counts();
+    } else if (split == 2 || nPrims <= 4) {
+      equal_counts();  // SPLIT_EQUAL_COUNTS (bvh.cpp:342-347), small SAH nodes (:351-356)
     } else {
       const int nB = 12;
       int cnt[nB] = {0};
@@ -914,20 +933,26 @@ struct BvhBuilder {
         cost[i] = .125f + (c0 * b0.SurfaceArea() + c1 * b1.SurfaceArea()) / bbox.SurfaceArea();
       }
       float minCost = cost[0];
-      int split = 0;
+      int best = 0;
       for (int i = 1; i < nB - 1; ++i)
         if (cost[i] < minCost) {
           minCost = cost[i];
-          split = i;
+          best = i;
         }
-      // maxPrims <= 4 always splits here (nPrims > 4), the only branch reachable at the
-      // reference defaults (maxnodeprims 4, maxBVHPrim 1).
-      PrimInfo* pm = std::partition(&bd[start], &bd[end - 1] + 1, [&](const PrimInfo& p) {
-        int b = (int)(nB * ((vget(p.centroid, dim) - lo) / (hi - lo)));
-        if (b == nB) b = nB - 1;
-        return b <= split;
-      });
-      mid = (uint32_t)(pm - &bd[0]);
+      if (nPrims > maxPrims || minCost < nPrims) {
+        PrimInfo* pm = std::partition(&bd[start], &bd[end - 1] + 1, [&](const PrimInfo& p) {
+          int b = (int)(nB * ((vget(p.centroid, dim) - lo) / (hi - lo)));
+          if (b == nB) b = nB - 1;
+          return b <= best;
+        });
+        mid = (uint32_t)(pm - &bd[0]);
+      } else {
+        // bvh.cpp:417-428: the primitives are recorded as a leaf, but without a return the
+        // code goes on to InitInterior with mid = (start + end) / 2 and builds both halves,
+        // which records them again (reachable only with maxPrims > 4)
+        make_leaf();
+        mid = (start + end) / 2;
+      }
     }
     int c0 = build(bd, start, mid);
     int c1 = build(bd, mid, end);
@@ -1678,7 +1703,8 @@ int orc_search_trace(const float* v9, uint32_t ntri, const orc_gpu_node* nodes, 
 }
 
 void orc_shading(const float* v9, const float* uv6, uint32_t ntri, const orc_rays* r,
-                 const int32_t* id, float* u, float* v, float* dpdu, float* dpdv) {
+                 const int32_t* id, int32_t naive_rule, float* u, float* v, float* dpdu,
+                 float* dpdv) {
   (void)ntri;
   size_t n = (size_t)r->W * r->H;
   static const float kDefUV[6] = {0.f, 0.f, 1.f, 0.f, 1.f, 1.f};
@@ -1703,7 +1729,8 @@ void orc_shading(const float* v9, const float* uv6, uint32_t ntri, const orc_ray
     V3 dp1 = sub(T.v1, T.v3), dp2 = sub(T.v2, T.v3);
     float det = du1 * dv2 - dv1 * du2;
     V3 pu, pv;
-    if (det == 0.f) {
+    // naive.cl:60-61 (inline copy in the naive kernels): fallback for |det| < EPS
+    if (naive_rule ? (det > -kEps && det < kEps) : det == 0.f) {
       V3 tn = normalize(cross(T.e2, T.e1));
       if (std::fabs(tn.x) > std::fabs(tn.y)) {
         float il = 1.0f / std::sqrt(tn.x * tn.x + tn.z * tn.z);
@@ -1728,11 +1755,24 @@ void orc_shading(const float* v9, const float* uv6, uint32_t ntri, const orc_ray
   }
 }
 
-int64_t orc_bvh_build(const float* v9, uint32_t ntri, uint32_t max_prims, int32_t gpu,
-                      uint32_t bvh_height, uint32_t* order_out, orc_gpu_node* nodes_out,
-                      uint32_t* top_out) {
+int64_t orc_bvh_order(const float* v9, uint32_t ntri, uint32_t max_prims, int32_t split,
+                      uint32_t* order_out, uint64_t cap) {
   BvhBuilder b;
   b.maxPrims = std::min(255u, max_prims);
+  b.split = split;
+  b.run(v9, ntri);
+  if (b.ordered.size() > cap) return -1;
+  for (size_t i = 0; i < b.ordered.size(); ++i) order_out[i] = b.ordered[i];
+  return (int64_t)b.ordered.size();
+}
+
+int64_t orc_bvh_build(const float* v9, uint32_t ntri, uint32_t max_prims, int32_t split,
+                      int32_t gpu, uint32_t bvh_height, uint32_t* order_out,
+                      orc_gpu_node* nodes_out, uint32_t* top_out) {
+  if (max_prims > 4) return -1;  // the node layout of the fall-through is not restated here
+  BvhBuilder b;
+  b.maxPrims = std::min(255u, max_prims);
+  b.split = split;
   b.run(v9, ntri);
   for (uint32_t i = 0; i < ntri; ++i) order_out[i] = b.ordered[i];
   if (!gpu) return (int64_t)b.nodes.size();
@@ -1743,12 +1783,14 @@ int64_t orc_bvh_build(const float* v9, uint32_t ntri, uint32_t max_prims, int32_
   return b.gpu_wide(bvh_height, nodes_out, top_out);
 }
 
-int64_t orc_raybvh_handoff(const float* v9, uint32_t ntri, uint32_t max_prims, uint32_t bvh_height,
-                           orc_gpu_node* nodes_out, uint32_t* roots_out, uint32_t* nroots_out,
-                           uint32_t* prim_order_out) {
+int64_t orc_raybvh_handoff(const float* v9, uint32_t ntri, uint32_t max_prims, int32_t split,
+                           uint32_t bvh_height, orc_gpu_node* nodes_out, uint32_t* roots_out,
+                           uint32_t* nroots_out, uint32_t* prim_order_out) {
+  if (max_prims > 4) return -1;
   // global BVH order (rayBVH.cpp:144-146)
   BvhBuilder g;
   g.maxPrims = std::min(255u, max_prims);
+  g.split = split;
   g.run(v9, ntri);
   std::vector<uint32_t> gorder = g.ordered;
   // 60k-triangle parts (rayBVH.cpp:288-297), per-part BVH (rayBVH.cpp:204-231)
@@ -1766,6 +1808,7 @@ int64_t orc_raybvh_handoff(const float* v9, uint32_t ntri, uint32_t max_prims, u
       std::memcpy(&sub9[(size_t)(i - lo) * 9], v9 + 9 * (size_t)gorder[i], 9 * sizeof(float));
     BvhBuilder b;
     b.maxPrims = std::min(255u, max_prims);
+    b.split = split;
     b.run(sub9.data(), hi - lo);
     std::vector<orc_gpu_node> tmp(2 * (size_t)(hi - lo) + 1);
     uint32_t top = 0;

@@ -119,9 +119,11 @@ int orc_search_trace(const float *v9, uint32_t ntri, const orc_gpu_node *nodes, 
                      int32_t nthreads, int32_t *id, float *t, float *b1, float *b2, uint8_t *occ,
                      orc_search_stats *st);
 
-/* shading record of computeDpTuTv.cl:8-91 for hits (id>=0): uv (2/ray), dpdu, dpdv (3/ray) */
+/* shading record of computeDpTuTv.cl:8-91 for hits (id>=0): uv (2/ray), dpdu, dpdv (3/ray);
+ * naive_rule: the naive kernels' fallback for |det| < EPS (naive.cl:60-61) */
 void orc_shading(const float *v9, const float *uv6, uint32_t ntri, const orc_rays *r,
-                 const int32_t *id, float *u, float *v, float *dpdu, float *dpdv);
+                 const int32_t *id, int32_t naive_rule, float *u, float *v, float *dpdu,
+                 float *dpdv);
 
 /* ---- reference BVH build (bvh.cpp:209-455) ---- */
 /* Builds the SAH BVH (12 buckets) over triangle bounds and returns the BVH primitive order
@@ -129,12 +131,18 @@ void orc_shading(const float *v9, const float *uv6, uint32_t ntri, const orc_ray
  * (GPUbufferWide, bvh.cpp:154-206): nodes_out (capacity 2*n), returns node count, top-level
  * count in *top_out.  gpu=0 produces the depth-first LinearBVHNode layout in an internal
  * handle used by orc_cpu_bvh_* (returned via *handle_out). */
-int64_t orc_bvh_build(const float *v9, uint32_t ntri, uint32_t max_prims, int32_t gpu,
-                      uint32_t bvh_height, uint32_t *order_out, orc_gpu_node *nodes_out,
-                      uint32_t *top_out);
+int64_t orc_bvh_build(const float *v9, uint32_t ntri, uint32_t max_prims, int32_t split,
+                      int32_t gpu, uint32_t bvh_height, uint32_t *order_out,
+                      orc_gpu_node *nodes_out, uint32_t *top_out);
+/* orderedPrims of BVHAccel (split 0 sah, 1 middle, 2 equal; max_prims up to 255), with the
+ * repeats that the SAH leaf branch without a return adds (bvh.cpp:417-428).  Returns the
+ * length, or -1 when it exceeds cap. */
+int64_t orc_bvh_order(const float *v9, uint32_t ntri, uint32_t max_prims, int32_t split,
+                      uint32_t *order_out, uint64_t cap);
 /* ray-bvh handoff exactly as RayBVH::Preprocess (rayBVH.cpp:144-146,204-276): global BVH
  * order, 60k-triangle parts, per-part BVH + GPUbufferWide.  Returns node count. */
-int64_t orc_raybvh_handoff(const float *v9, uint32_t ntri, uint32_t max_prims, uint32_t bvh_height,
+int64_t orc_raybvh_handoff(const float *v9, uint32_t ntri, uint32_t max_prims, int32_t split,
+                           uint32_t bvh_height,
                            orc_gpu_node *nodes_out, uint32_t *roots_out, uint32_t *nroots_out,
                            uint32_t *prim_order_out);
 

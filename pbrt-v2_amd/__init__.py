@@ -130,11 +130,12 @@ def lib() -> ctypes.CDLL:
                                        P(ctypes.c_int32)]
     L.rt_error_string.argtypes = [ctypes.c_int]
     L.rt_error_string.restype = ctypes.c_char_p
-    L.rt_raybvh_build_handoff.argtypes = [_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                                          P(ctypes.c_void_p), P(ctypes.c_uint32), P(ctypes.c_void_p),
-                                          P(ctypes.c_uint32), P(ctypes.c_void_p)]
+    L.rt_raybvh_build_handoff.argtypes = [_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
+                                          ctypes.c_uint32, P(ctypes.c_void_p), P(ctypes.c_uint32),
+                                          P(ctypes.c_void_p), P(ctypes.c_uint32),
+                                          P(ctypes.c_void_p)]
     L.rt_raybvh_free_handoff.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-    L.rt_bvh_order.argtypes = [_fp, ctypes.c_uint32, ctypes.c_uint32, _fp]
+    L.rt_bvh_order.argtypes = [_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, _fp]
     L.rt_batch_create.argtypes = [ctypes.c_void_p, P(ctypes.c_void_p)]
     L.rt_batch_destroy.argtypes = [ctypes.c_void_p]
     L.rt_batch_clear.argtypes = [ctypes.c_void_p]
@@ -180,15 +181,19 @@ def default_desc(kind, **overrides) -> AccelDesc:
     return d
 
 
-def raybvh_handoff(v9: np.ndarray, max_prims: int = 1, bvh_height: int = 3):
-    """CPU-built ray-bvh handoff (global BVH order, 60k-triangle parts, per-part SAH BVH and
-    breadth-first GPUNode layout).  Returns (nodes, roots, prim_order) numpy arrays."""
+SPLIT_METHODS = {"sah": 0, "middle": 1, "equal": 2}
+
+
+def raybvh_handoff(v9: np.ndarray, max_prims: int = 1, bvh_height: int = 3, split="sah"):
+    """CPU-built ray-bvh handoff (global BVH order, 60k-triangle parts, per-part BVH with the
+    named split method, breadth-first GPUNode layout).  Returns (nodes, roots, prim_order)."""
     L = lib()
     v9 = np.ascontiguousarray(v9, dtype=np.float32).reshape(-1)
     ntri = v9.size // 9
     nodes_p, roots_p, order_p = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
     nn, nr = ctypes.c_uint32(), ctypes.c_uint32()
-    check(L.rt_raybvh_build_handoff(v9.ctypes.data, ntri, max_prims, bvh_height,
+    check(L.rt_raybvh_build_handoff(v9.ctypes.data, ntri, max_prims, SPLIT_METHODS[split],
+                                    bvh_height,
                                     ctypes.byref(nodes_p), ctypes.byref(nn), ctypes.byref(roots_p),
                                     ctypes.byref(nr), ctypes.byref(order_p)),
           "rt_raybvh_build_handoff")
@@ -204,13 +209,14 @@ def raybvh_handoff(v9: np.ndarray, max_prims: int = 1, bvh_height: int = 3):
     return nodes, roots, order
 
 
-def bvh_order(v9: np.ndarray, max_prims: int = 1) -> np.ndarray:
+def bvh_order(v9: np.ndarray, max_prims: int = 1, split="sah") -> np.ndarray:
     """BVHAccel primitive order over the triangles (position -> input index), the order
     RayHieararchy's "sort" parameter swaps its triangles into."""
     v9 = np.ascontiguousarray(v9, dtype=np.float32).reshape(-1)
     ntri = v9.size // 9
     order = np.zeros(max(ntri, 1), np.uint32)
-    check(lib().rt_bvh_order(v9.ctypes.data, ntri, max_prims, order.ctypes.data), "rt_bvh_order")
+    check(lib().rt_bvh_order(v9.ctypes.data, ntri, max_prims, SPLIT_METHODS[split],
+                             order.ctypes.data), "rt_bvh_order")
     return order[:ntri].copy()
 
 

@@ -10,7 +10,9 @@
 // The parts are concatenated: node indices and primitive positions are rebased per part, and
 // prim_order maps every primitive position back to the original triangle index (the remap of
 // rayBVH.cpp:795-799).  Sorting uses std::partition / std::nth_element as bvh.cpp does, so
-// the node layout follows the same libstdc++ algorithms.
+// the node layout follows the same libstdc++ algorithms.  The split methods are BVHAccel's
+// (bvh.cpp:214-221,332-427): "sah" (12 buckets), "middle" (centroid midpoint) and "equal"
+// (equal counts).
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
@@ -67,7 +69,8 @@ struct BNode {
 
 class Builder {
  public:
-  explicit Builder(uint32_t max_prims) : max_prims_(std::min(255u, max_prims)) {}
+  Builder(uint32_t max_prims, int split_method)
+      : max_prims_(std::min(255u, max_prims)), method_(split_method) {}
 
   // Builds over `ntri` triangles (9 floats each); fills order (position -> input index).
   void build(const float* v9, uint32_t ntri) {
@@ -168,9 +171,21 @@ class Builder {
     const float lo = cb.lo.v[ax], hi = cb.hi.v[ax];
     if (hi == lo) return leaf();
     uint32_t mid = (s + e) / 2;
-    if (n <= 4) {
+    auto equal_counts = [&]() {
+      mid = (s + e) / 2;
       std::nth_element(&it[s], &it[mid], &it[e - 1] + 1,
                        [ax](const Item& a, const Item& b) { return a.c.v[ax] < b.c.v[ax]; });
+    };
+    if (method_ == RT_SPLIT_MIDDLE) {
+      // bvh.cpp:333-340.  When the midpoint rounds onto an end of the centroid range every
+      // primitive lands on one side and the reference recurses without end; equal counts here.
+      const float pmid = .5f * (lo + hi);
+      Item* m = std::partition(&it[s], &it[e - 1] + 1,
+                               [&](const Item& p) { return p.c.v[ax] < pmid; });
+      mid = (uint32_t)(m - &it[0]);
+      if (mid == s || mid == e) equal_counts();
+    } else if (method_ == RT_SPLIT_EQUAL || n <= 4) {
+      equal_counts();  // bvh.cpp:342-347, and SAH's small nodes (:351-356)
     } else {
       int cnt[12] = {0};
       Box bb[12];
@@ -199,7 +214,15 @@ class Builder {
           best_split = k;
         }
       }
-      // With at most 4 primitives per leaf a node of more than 4 primitives always splits.
+      if (n <= max_prims_ && !(best < (float)n)) {
+        // bvh.cpp:417-428: the reference records these primitives as a leaf but has no return
+        // there, so the node is rebuilt as an interior node over [s, (s+e)/2) and
+        // [(s+e)/2, e), which appends the same primitives again.  Every later copy has a
+        // larger index than the first one and never wins a tie against it: the first
+        // occurrences, in this range's current order, are the order that counts.  Only
+        // reachable with more than 4 primitives per leaf (the sort order, not the handoff).
+        return leaf();
+      }
       Item* m = std::partition(&it[s], &it[e - 1] + 1, [&](const Item& p) {
         return bucket(p.c.v[ax], lo, hi) <= best_split;
       });
@@ -217,6 +240,7 @@ class Builder {
   }
 
   uint32_t max_prims_;
+  int method_;
   std::vector<BNode> nodes_;
   std::vector<uint32_t> order_;
 };
@@ -224,18 +248,20 @@ class Builder {
 }  // namespace
 
 extern "C" int rt_raybvh_build_handoff(const float* v9, uint32_t ntri, uint32_t max_prims,
-                                       uint32_t bvh_height, rt_gpu_node** nodes_out,
-                                       uint32_t* nnodes_out, uint32_t** roots_out,
-                                       uint32_t* nroots_out, uint32_t** prim_order_out) {
+                                       int32_t split_method, uint32_t bvh_height,
+                                       rt_gpu_node** nodes_out, uint32_t* nnodes_out,
+                                       uint32_t** roots_out, uint32_t* nroots_out,
+                                       uint32_t** prim_order_out) {
   if (!nodes_out || !nnodes_out || !roots_out || !nroots_out || !prim_order_out)
     return RT_ERR_INVALID;
   if (ntri && !v9) return RT_ERR_INVALID;
+  if (split_method < RT_SPLIT_SAH || split_method > RT_SPLIT_EQUAL) return RT_ERR_INVALID;
   if (max_prims == 0 || max_prims > 4) return RT_ERR_UNSUPPORTED;
   *nodes_out = nullptr;
   *roots_out = *prim_order_out = nullptr;
   *nnodes_out = *nroots_out = 0;
   // 1. global BVH order
-  Builder global(max_prims);
+  Builder global(max_prims, split_method);
   global.build(v9, ntri);
   const std::vector<uint32_t> gorder = global.order();
   // 2. parts of at most 60,000 triangles
@@ -254,7 +280,7 @@ extern "C" int rt_raybvh_build_handoff(const float* v9, uint32_t ntri, uint32_t 
     for (uint32_t i = lo; i < hi; ++i)
       std::memcpy(&sub[(size_t)(i - lo) * 9], v9 + 9 * (size_t)gorder[i], 9 * sizeof(float));
     // 3. per-part BVH and breadth-first layout
-    Builder b(max_prims);
+    Builder b(max_prims, split_method);
     b.build(sub.data(), hi - lo);
     uint32_t top = 0;
     const uint32_t nn = b.wide(bvh_height, tmp, &top);
@@ -294,10 +320,12 @@ extern "C" int rt_raybvh_free_handoff(rt_gpu_node* nodes, uint32_t* roots, uint3
   return RT_OK;
 }
 
-extern "C" int rt_bvh_order(const float* v9, uint32_t ntri, uint32_t max_prims, uint32_t* order) {
+extern "C" int rt_bvh_order(const float* v9, uint32_t ntri, uint32_t max_prims,
+                            int32_t split_method, uint32_t* order) {
   if (ntri && (!v9 || !order)) return RT_ERR_INVALID;
   if (max_prims == 0) return RT_ERR_INVALID;
-  Builder b(max_prims);  // leaves of at most min(255, max_prims) primitives, as BVHAccel
+  if (split_method < RT_SPLIT_SAH || split_method > RT_SPLIT_EQUAL) return RT_ERR_INVALID;
+  Builder b(max_prims, split_method);  // leaves of at most min(255, max_prims), as BVHAccel
   b.build(v9, ntri);
   const std::vector<uint32_t>& o = b.order();
   std::copy(o.begin(), o.end(), order);

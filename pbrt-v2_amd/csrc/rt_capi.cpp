@@ -909,7 +909,8 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
       return launch_naive_trace(r, mode, s->d_v9, ntri, tchunks, s->d_keys, occ, hs);
     }));
   }
-  if (closest) RT_TRY(launch_finalize(r, s->d_v9, s->d_uv6, s->d_keys, out, hs));
+  if (closest)
+    RT_TRY(launch_finalize(r, s->d_v9, s->d_uv6, s->d_keys, out, kind == RT_ACCEL_NAIVE, hs));
   return RT_OK;
 }
 
@@ -949,7 +950,7 @@ int rt_trace_closest_noalloc(rt_scene* s, const rt_rays* rays, int32_t mode, con
       const int tchunks = pick_tchunks(nblk, s->ntri, 64);
       RT_TRY(launch_naive_trace(r, mode, s->d_v9, s->ntri, tchunks, keys, nullptr, hs));
     }
-    RT_TRY(launch_finalize(r, s->d_v9, s->d_uv6, keys, out, hs));
+    RT_TRY(launch_finalize(r, s->d_v9, s->d_uv6, keys, out, kind == RT_ACCEL_NAIVE, hs));
     return RT_OK;
   }
   // rayhierarchy / ray-bvh: one job whose blocks all take the per-ray path (the same result as

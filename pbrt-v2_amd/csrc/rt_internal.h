@@ -161,7 +161,7 @@ hipError_t launch_naive_trace(const RaysDev& r, int mode, const float* v9, uint3
                               int tchunks, uint64_t* keys, uint8_t* occ, hipStream_t s);
 // Hit records of the naive path from the reduction keys.
 hipError_t launch_finalize(const RaysDev& r, const float* v9, const float* uv6,
-                           const uint64_t* keys, HitOut out, hipStream_t s);
+                           const uint64_t* keys, HitOut out, bool naive_rule, hipStream_t s);
 hipError_t launch_camera_rays(int W, int H, int spp, const float origin[3], float tan_half,
                               float aspect, uint32_t seed, float* ox, float* oy, float* oz,
                               float* dx, float* dy, float* dz, float* tmin, float* tmax,

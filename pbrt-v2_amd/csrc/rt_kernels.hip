@@ -1676,9 +1676,12 @@ __global__ __launch_bounds__(256) void naive_kernel(RaysDev r, const float* __re
 
 // ---------------------------------------------------------------------------------------
 // Hit record: id, t, (u, v) and optionally dpdu/dpdv of computeDpTuTv.cl:8-91.
+// Hit record of ray i (computeDpTuTv.cl:8-91).  naive_rule: the naive kernels' inline copy of
+// that code takes the coordinate-system fallback for |det| < EPS (naive.cl:60-61,153-154)
+// where computeDpTuTv.cl:57 takes it for det == 0 only.
 __device__ void write_hit(const RaysDev& r, const float* __restrict__ v9,
                           const float* __restrict__ uv6, size_t i, int32_t id, float t,
-                          HitOut out) {
+                          HitOut out, bool naive_rule = false) {
   if (id < 0) {
     out.id[i] = -1;
     out.t[i] = __int_as_float(0x7f800000);
@@ -1709,7 +1712,7 @@ __device__ void write_hit(const RaysDev& r, const float* __restrict__ v9,
     const V3 dp1 = sub(T.v1, T.v3), dp2 = sub(T.v2, T.v3);
     const float det = du1 * dv2 - dv1 * du2;
     V3 pu, pv;
-    if (det == 0.f) {
+    if (naive_rule ? (det > -kEps && det < kEps) : det == 0.f) {
       const V3 tn = normalize(cross(T.e2, T.e1));
       if (fabsf(tn.x) > fabsf(tn.y)) {
         const float il = 1.0f / sqrtf(tn.x * tn.x + tn.z * tn.z);
@@ -1738,24 +1741,26 @@ __device__ void write_hit(const RaysDev& r, const float* __restrict__ v9,
 // Hit record of one ray from its reduction key; returns t (INFINITY on a miss).
 __device__ __forceinline__ float finalize_ray(const RaysDev& r, const float* __restrict__ v9,
                                               const float* __restrict__ uv6, uint64_t key,
-                                              size_t i, const HitOut& out) {
+                                              size_t i, const HitOut& out,
+                                              bool naive_rule = false) {
   if (key == kMissKey) {
     write_hit(r, v9, uv6, i, -1, 0.f, out);
     return __int_as_float(0x7f800000);
   }
   const float t = fkey_inv((uint32_t)(key >> 32));
-  write_hit(r, v9, uv6, i, (int32_t)(uint32_t)key, t, out);
+  write_hit(r, v9, uv6, i, (int32_t)(uint32_t)key, t, out, naive_rule);
   return t;
 }
 
+// Hit records of the naive accelerator (and of an empty scene's misses).
 __global__ __launch_bounds__(256) void finalize_kernel(RaysDev r, const float* __restrict__ v9,
                                                        const float* __restrict__ uv6,
                                                        const uint64_t* __restrict__ keys,
-                                                       HitOut out) {
+                                                       HitOut out, int naive_rule) {
   const size_t n = (size_t)r.W * r.H;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  finalize_ray(r, v9, uv6, keys[i], i, out);
+  finalize_ray(r, v9, uv6, keys[i], i, out, naive_rule != 0);
 }
 
 // Hit records of every job of a closest-hit group.  Rays whose nearest repair-region
@@ -2161,15 +2166,21 @@ int rh_geometry(int W, int H, int cx, int cy, int height, RHGeom* g) {
   g->gw[0] = gw;
   g->gh[0] = gh;
   int L = 1;
-  // Stop after a level whose width or height is odd (rayhierarchy.cpp:438-442).
+  // The level count is the reference's: it halves with floor and stops after a level whose
+  // floor width or height is odd (rayhierarchy.cpp:412-442).  The grids themselves halve with
+  // ceil, so a parent also covers the last child column (row) that the reference's floor
+  // leaves out where it halves an odd dimension.
+  int fw = gw, fh = gh;
   for (int i = 1; i < height; ++i) {
     if (L >= kMaxLevels) return 2;
     gw = (gw + 1) / 2;
     gh = (gh + 1) / 2;
+    fw /= 2;
+    fh /= 2;
     g->gw[L] = gw;
     g->gh[L] = gh;
     ++L;
-    if ((gw & 1) || (gh & 1)) break;
+    if ((fw & 1) || (fh & 1)) break;
   }
   g->L = L;
   int off = 0;
@@ -2235,9 +2246,10 @@ hipError_t launch_naive_trace(const RaysDev& r, int mode, const float* v9, uint3
 }
 
 hipError_t launch_finalize(const RaysDev& r, const float* v9, const float* uv6,
-                           const uint64_t* keys, HitOut out, hipStream_t s) {
+                           const uint64_t* keys, HitOut out, bool naive_rule, hipStream_t s) {
   const size_t n = (size_t)r.W * r.H;
-  hipLaunchKernelGGL(finalize_kernel, grid1(n), dim3(kThreads), 0, s, r, v9, uv6, keys, out);
+  hipLaunchKernelGGL(finalize_kernel, grid1(n), dim3(kThreads), 0, s, r, v9, uv6, keys, out,
+                     naive_rule ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -168,8 +168,8 @@ struct Aggregate::Staging {
 };
 
 Aggregate::Aggregate(const std::vector<TriangleMesh>& meshes, const rt_accel_desc& desc,
-                     uint32_t sort_max_prims)
-    : desc_(desc), st_(new Staging) {
+                     uint32_t sort_max_prims, int32_t split_method)
+    : desc_(desc), split_(split_method), st_(new Staging) {
   std::vector<float> v9, uv6;
   flatten(meshes, &v9, &uv6, &tri_prim_, &tri_shape_);
   ntri_ = (uint32_t)tri_prim_.size();
@@ -179,7 +179,8 @@ Aggregate::Aggregate(const std::vector<TriangleMesh>& meshes, const rt_accel_des
     // flattened (rayhierarchy.cpp:152-155), so ids, and the smallest-id rule among hits at
     // equal t, follow that order.
     tri_orig_.resize(ntri_);
-    check(rt_bvh_order(v9.data(), ntri_, sort_max_prims, tri_orig_.data()), "rt_bvh_order");
+    check(rt_bvh_order(v9.data(), ntri_, sort_max_prims, split_, tri_orig_.data()),
+          "rt_bvh_order");
     std::vector<float> v9s(v9.size()), uv6s(uv6.size());
     std::vector<uint32_t> prim(ntri_);
     std::vector<int32_t> shape(ntri_);
@@ -204,7 +205,7 @@ Aggregate::Aggregate(const std::vector<TriangleMesh>& meshes, const rt_accel_des
     rt_gpu_node* nodes = nullptr;
     uint32_t nnodes = 0, nroots = 0;
     uint32_t *roots = nullptr, *order = nullptr;
-    check(rt_raybvh_build_handoff(v9.data(), ntri_, (uint32_t)desc_.max_bvh_prim,
+    check(rt_raybvh_build_handoff(v9.data(), ntri_, (uint32_t)desc_.max_bvh_prim, split_,
                                   (uint32_t)desc_.bvh_height, &nodes, &nnodes, &roots, &nroots,
                                   &order),
           "rt_raybvh_build_handoff");
@@ -378,11 +379,20 @@ void NaiveAccel::IntersectP(const Ray* r, char* occluded, size_t count, const bo
 
 // --- RayHieararchy (rayhierarchy.cpp:35-1348) --------------------------------------------------
 
+// BVHAccel's split method names (bvh.cpp:214-221): an unknown name warns and uses "sah".
+static int32_t split_kind(const std::string& sm) {
+  if (sm == "sah") return RT_SPLIT_SAH;
+  if (sm == "middle") return RT_SPLIT_MIDDLE;
+  if (sm == "equal") return RT_SPLIT_EQUAL;
+  std::fprintf(stderr, "Warning: BVH split method \"%s\" unknown.  Using \"sah\".\n",
+               sm.c_str());
+  return RT_SPLIT_SAH;
+}
+
 static rt_accel_desc rh_desc(int32_t kind, bool onGPU, int chunkX, int chunkY, int height,
-                             const std::string& node, const std::string& sm, int maxBVHPrim,
-                             int BVHheight, unsigned repairRun) {
+                             const std::string& node, int maxBVHPrim, int BVHheight,
+                             unsigned repairRun) {
   if (!onGPU) Severe("ray-space accelerators need the GPU (onGPU=false is not provided)");
-  if (sm != "sah") Severe("splitmethod \"" + sm + "\" is not supported; only \"sah\" is");
   rt_accel_desc d = make_desc(kind);
   d.node = node_kind(node);
   d.chunk_x = chunkX;
@@ -397,9 +407,9 @@ static rt_accel_desc rh_desc(int32_t kind, bool onGPU, int chunkX, int chunkY, i
 RayHieararchy::RayHieararchy(const std::vector<TriangleMesh>& meshes, bool onGPU, int chunkX,
                              int chunkY, int height, const std::string& node, bool sortVert,
                              const std::string& sm, int maxBVHPrim, unsigned repairRun)
-    : Aggregate(meshes, rh_desc(RT_ACCEL_RAYHIERARCHY, onGPU, chunkX, chunkY, height, node, sm,
+    : Aggregate(meshes, rh_desc(RT_ACCEL_RAYHIERARCHY, onGPU, chunkX, chunkY, height, node,
                                 maxBVHPrim, 0, repairRun),
-                sortVert ? (uint32_t)std::max(maxBVHPrim, 1) : 0u) {}
+                sortVert ? (uint32_t)std::max(maxBVHPrim, 1) : 0u, split_kind(sm)) {}
 void RayHieararchy::Intersect(const RayDifferential* r, Intersection* in, bool* hit,
                               unsigned count) {
   Closest(r, in, hit, count, false);
@@ -417,8 +427,9 @@ void RayHieararchy::IntersectP(const Ray* r, char* occluded, size_t count, const
 RayBVH::RayBVH(const std::vector<TriangleMesh>& meshes, bool onGPU, int chunkX, int chunkY,
                int height, const std::string& node, const std::string& sm, int maxBVHPrim,
                int BVHheight)
-    : Aggregate(meshes, rh_desc(RT_ACCEL_RAYBVH, onGPU, chunkX, chunkY, height, node, sm,
-                                maxBVHPrim, BVHheight, 0)) {}
+    : Aggregate(meshes, rh_desc(RT_ACCEL_RAYBVH, onGPU, chunkX, chunkY, height, node,
+                                maxBVHPrim, BVHheight, 0),
+                0u, split_kind(sm)) {}
 void RayBVH::Intersect(const RayDifferential* r, Intersection* in, bool* hit, unsigned count) {
   Closest(r, in, hit, count, false);
 }

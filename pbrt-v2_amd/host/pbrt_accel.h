@@ -143,8 +143,9 @@ class Aggregate {
  protected:
   // sort_max_prims > 0 reorders the triangles into BVHAccel order with leaves of that many
   // primitives (RayHieararchy's "sort"); Intersection::triangle stays the input index.
+  // split_method (RT_SPLIT_*) is BVHAccel's for that order and for ray-bvh's CPU BVH.
   Aggregate(const std::vector<TriangleMesh>& meshes, const rt_accel_desc& desc,
-            uint32_t sort_max_prims = 0);
+            uint32_t sort_max_prims = 0, int32_t split_method = RT_SPLIT_SAH);
   // Ray image for `count` rays: the preprocessed task image (xResolution * spp) x yResolution,
   // grown by whole rows when the batch is larger; one row for naive or without Preprocess.
   void ImageShape(unsigned count, int* w, int* h) const;
@@ -157,6 +158,7 @@ class Aggregate {
   std::vector<uint32_t> tri_prim_;   // triangle -> primitiveId
   std::vector<int32_t> tri_shape_;   // triangle -> shapeId
   std::vector<uint32_t> tri_orig_;   // with "sort": device triangle -> input index
+  int32_t split_ = RT_SPLIT_SAH;     // BVHAccel split method (sort order, ray-bvh BVH)
   int xres_ = 0, yres_ = 0;
   unsigned spp_ = 1;
 

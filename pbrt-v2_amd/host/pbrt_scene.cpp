@@ -97,8 +97,11 @@ V3f normalized(V3f v) {
   const float inv = 1.f / length(v);  // Vector::operator/ multiplies by the reciprocal
   return {v.x * inv, v.y * inv, v.z * inv};
 }
+// Cross (geometry.h:481-488): the products and differences in double, rounded to float once.
 V3f cross3(V3f a, V3f b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+  const double ax = a.x, ay = a.y, az = a.z, bx = b.x, by = b.y, bz = b.z;
+  return {(float)((ay * bz) - (az * by)), (float)((az * bx) - (ax * bz)),
+          (float)((ax * by) - (ay * bx))};
 }
 
 Xform translate_x(float dx, float dy, float dz) {

@@ -46,13 +46,21 @@ def test_invalid_arguments_rejected_without_device(rt):
     assert L.rt_trace_occluded(None, None, None, None) == 1
 
 
-def test_handoff_matches_oracle_builder(rt, orc, scenes):
-    # two independent restatements of BVHAccel + GPUbufferWide + parts agree exactly
+@pytest.mark.parametrize("split", ["sah", "middle", "equal"])
+def test_handoff_matches_oracle_builder(rt, orc, scenes, split):
+    # two independent restatements of BVHAccel + GPUbufferWide + parts agree exactly, for each
+    # split method (bvh.cpp:214-221,332-427) and leaf size
     for v9 in (scenes.random_triangles(2500, seed=3), scenes.cluster_scene(61000, seed=4)):
-        a = rt.raybvh_handoff(v9, 1, 3)
-        b = orc.raybvh_handoff(v9, 1, 3)
-        for x, y in zip(a, b):
-            assert np.array_equal(x, y)
+        for mp in ((1, 4) if split == "sah" else (1, 2)):
+            a = rt.raybvh_handoff(v9, mp, 3, split=split)
+            b = orc.raybvh_handoff(v9, mp, 3, split=split)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+    if split != "sah":
+        # the split methods build different trees
+        v9 = scenes.random_triangles(2500, seed=3)
+        assert not np.array_equal(rt.raybvh_handoff(v9, 1, 3, split=split)[2],
+                                  rt.raybvh_handoff(v9, 1, 3)[2])
 
 
 def test_bvh_order_matches_oracle_builder(rt, orc, scenes):
@@ -60,9 +68,41 @@ def test_bvh_order_matches_oracle_builder(rt, orc, scenes):
     # BVHAccel restatement, at leaf sizes 1 and 4 and over a scene with coincident centroids
     dup = np.repeat(scenes.random_triangles(200, seed=8), 3, axis=0)
     for v9 in (scenes.random_triangles(3000, seed=5), scenes.cluster_scene(20000, seed=6), dup):
-        for mp in (1, 4):
-            a = rt.bvh_order(v9, mp)
-            b = orc.bvh_build(v9, max_prims=mp, gpu=False)[2]
-            assert np.array_equal(np.sort(a), np.arange(len(v9)))
-            assert np.array_equal(a, b)
-    assert rt.lib().rt_bvh_order(None, 3, 1, None) == 1
+        for split in ("sah", "middle", "equal"):
+            for mp in (1, 4):
+                a = rt.bvh_order(v9, mp, split=split)
+                b = orc.bvh_build(v9, max_prims=mp, gpu=False, split=split)[2]
+                assert np.array_equal(np.sort(a), np.arange(len(v9)))
+                assert np.array_equal(a, b), (split, mp)
+    assert rt.lib().rt_bvh_order(None, 3, 1, 0, None) == 1
+    assert rt.lib().rt_bvh_order(None, 0, 1, 3, None) == 1  # unknown split method
+
+
+def _first_occurrences(order):
+    _, first = np.unique(order, return_index=True)
+    return order[np.sort(first)]
+
+
+def test_bvh_order_sah_leaf_fallthrough(rt, orc, scenes):
+    # With more than 4 primitives per leaf the reference's SAH branch records a leaf and then
+    # falls through to InitInterior (no return, bvh.cpp:417-428): orderedPrims lists those
+    # primitives again.  The oracle keeps the repeats; rt_bvh_order returns the first
+    # occurrences, the order that decides ties (a later copy never wins against the first).
+    # groups of 7 copies of one triangle, shifted by 1e-4 along x (distinct centroids, boxes
+    # of nearly the group's size): every split of a group costs about 7.125 > 7, so a group
+    # takes the leaf branch once 7 <= max_prims; groups lie 10 apart and split cleanly
+    base = np.array([0, 0, 0, 1, 1, 0, 1, 0, 1], np.float32)
+    g = np.random.default_rng(9)
+    tris = []
+    for c in g.integers(-20, 20, size=(60, 3)) * 10.0:
+        for i in g.permutation(7):
+            t = base.reshape(3, 3) + c + np.array([i * 1e-4, 0, 0])
+            tris.append(t.reshape(-1))
+    v9 = np.array(tris, np.float32)
+    for mp in (8, 16, 255):
+        full = orc.bvh_order(v9, max_prims=mp)
+        assert len(full) > len(v9), "the fall-through must occur here"
+        a = rt.bvh_order(v9, mp)
+        assert np.array_equal(a, _first_occurrences(full)), mp
+    # at 4 and below there is no repeat
+    assert len(orc.bvh_order(v9, max_prims=4)) == len(v9)

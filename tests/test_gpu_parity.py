@@ -181,17 +181,25 @@ def test_raybvh_room_window(rt, scenes, orc, gpu_ctx, node):
 
 
 # ---------------------------------------------------------------- shading record, shadow rays
-def test_shading_record_with_uvs(rt, scenes, orc, gpu_ctx):
+@pytest.mark.parametrize("accel", ["rayhierarchy", "naive"])
+def test_shading_record_with_uvs(rt, scenes, orc, gpu_ctx, accel):
+    """dpdu / dpdv / u / v of the hits.  Zero UV determinants take the coordinate-system
+    fallback; determinants of +-0.001 take it only in the naive kernels (|det| < EPS,
+    naive.cl:60-61), the others divide by them (computeDpTuTv.cl:57)."""
     v9 = scenes.random_triangles(500, seed=41)
     uv6 = np.random.default_rng(3).random((500, 6)).astype(np.float32)
     uv6[::7] = 0.0  # zero UV determinant -> coordinate-system fallback
+    uv6[3::7] = [0, 0, 1, 0, 1, 0.001]    # determinant 0.001
+    uv6[5::7] = [0, 0, 1, 0, 1, -0.001]   # determinant -0.001
     rays = scenes.pinhole_rays(64, 64)
-    sc = rt.Scene(gpu_ctx, v9, uv6=uv6, desc="rayhierarchy")
+    sc = rt.Scene(gpu_ctx, v9, uv6=uv6, desc=accel)
     hd = sc.trace_closest(_dev(rt, rays), rt.RT_CLOSEST_PRIMARY, shading=True)
     h = _to_np(hd)
-    ref = orc.trace(v9, rays, orc.params("rayhierarchy"), orc.CLOSEST_PRIMARY)
+    ref = orc.trace(v9, rays, orc.params(accel), orc.CLOSEST_PRIMARY)
     assert np.array_equal(h["id"], ref["id"])
-    s = orc.shading(v9, uv6, rays, ref["id"])
+    s = orc.shading(v9, uv6, rays, ref["id"], naive=accel == "naive")
+    small = np.isin(ref["id"], np.r_[np.arange(3, 500, 7), np.arange(5, 500, 7)])
+    assert small.sum() > 20, "hits on the small-determinant triangles"
     hit = ref["id"] >= 0
     np.testing.assert_allclose(h["u"][hit], s["u"][hit], rtol=RTOL, atol=1e-6)
     np.testing.assert_allclose(h["v"][hit], s["v"][hit], rtol=RTOL, atol=1e-6)

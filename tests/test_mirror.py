@@ -63,8 +63,9 @@ def run(accel, params, v9, rays, mode, hit=None, uv6=None, pre=None, calls=1):
                 occ=occ.view("S1").astype(str))
 
 
-def expect_closest(out, ref_id, ref_t, rays, orc, v9, uv6, label):
-    """Checks the unpacked Intersections against oracle ids / t and the oracle's shading."""
+def expect_closest(out, ref_id, ref_t, rays, orc, v9, uv6, label, naive=False):
+    """Checks the unpacked Intersections against oracle ids / t and the oracle's shading
+    (naive: the naive kernels' |det| < EPS fallback)."""
     hit = ref_id >= 0
     print(f"[{label}] hits={int(hit.sum())}", flush=True)
     assert np.array_equal(out["hit"], hit), f"{label}: hit flags differ"
@@ -72,7 +73,7 @@ def expect_closest(out, ref_id, ref_t, rays, orc, v9, uv6, label):
     assert np.array_equal(out["prim"], np.where(hit, np.where(ref_id < SPLIT, 7, 8), -1))
     np.testing.assert_allclose(out["maxt"][hit], ref_t[hit], rtol=RTOL, atol=0)
     assert np.array_equal(out["maxt"][~hit], rays["tmax"][~hit]), "maxt of a miss changed"
-    s = orc.shading(v9, uv6, rays, np.where(hit, ref_id, -1).astype(np.int32))
+    s = orc.shading(v9, uv6, rays, np.where(hit, ref_id, -1).astype(np.int32), naive=naive)
     np.testing.assert_allclose(out["u"][hit], s["u"][hit], rtol=RTOL, atol=1e-6)
     np.testing.assert_allclose(out["v"][hit], s["v"][hit], rtol=RTOL, atol=1e-6)
     np.testing.assert_allclose(out["dpdu"][hit], s["dpdu"].reshape(-1, 3)[hit], rtol=1e-4,
@@ -95,7 +96,6 @@ def _closest_rays(r):
     ("naive", "method=RayTri", "RayTri"),
     ("naive", "onGPU=false", "onGPU"),
     ("rayhierarchy", "node=box_xyz", "Unknown accelarator node type box_xyz"),
-    ("ray-bvh", "splitmethod=middle", "middle"),
 ])
 def test_factory_rejects(scenes, accel, params, msg):
     v9 = scenes.random_triangles(8, seed=1)
@@ -158,9 +158,13 @@ def test_scene_dispatch(scenes, accel):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("accel,node", [("rayhierarchy", "box_dir"), ("rayhierarchy", "ia"),
-                                        ("ray-bvh", "box_dir"), ("ray-bvh", "box_uv")])
-def test_layer_primary_and_shadow(scenes, orc, accel, node):
+@pytest.mark.parametrize("accel,node,split", [("rayhierarchy", "box_dir", "sah"),
+                                              ("rayhierarchy", "ia", "sah"),
+                                              ("ray-bvh", "box_dir", "sah"),
+                                              ("ray-bvh", "box_uv", "sah"),
+                                              ("ray-bvh", "box_dir", "middle"),
+                                              ("ray-bvh", "box_dir", "equal")])
+def test_layer_primary_and_shadow(scenes, orc, accel, node, split):
     v9 = scenes.torus_scene() if accel == "rayhierarchy" else scenes.room_scene(nclusters=60)
     ntri = v9.shape[0]
     uv6 = np.random.default_rng(5).random((ntri, 6)).astype(np.float32)
@@ -172,16 +176,17 @@ def test_layer_primary_and_shadow(scenes, orc, accel, node):
         rays = scenes.subimage(full, 64, 64, 96, 96)
     w = 128 if accel == "rayhierarchy" else 96
     rays["width"], rays["height"] = w, w
-    params = f"node={node}"
+    params = f"node={node};splitmethod={split}"
     out = run(accel, params, v9, rays, 0, uv6=uv6, pre=(w, w, 1), calls=2)
     p = orc.params(accel, node=node)
     cr = _closest_rays(rays)
     if accel == "ray-bvh":
-        nodes, roots, order = orc.raybvh_handoff(v9, max_prims=1, bvh_height=3)
+        nodes, roots, order = orc.raybvh_handoff(v9, max_prims=1, bvh_height=3, split=split)
         ref = orc.trace_raybvh(v9, nodes, roots, order, cr, p, orc.CLOSEST_PRIMARY)
     else:
         ref = orc.trace(v9, cr, p, orc.CLOSEST_PRIMARY)
-    expect_closest(out, ref["id"], ref["t"], rays, orc, v9, uv6, f"{accel}/{node} primary")
+    expect_closest(out, ref["id"], ref["t"], rays, orc, v9, uv6,
+                   f"{accel}/{node}/{split} primary")
 
     sh = scenes.shadow_rays_np(rays, ref["id"], ref["t"], (1.0, 2.5, -8.0))
     sh["width"], sh["height"] = w, w
@@ -197,25 +202,27 @@ def test_layer_primary_and_shadow(scenes, orc, accel, node):
 
 
 @pytest.mark.gpu
-def test_layer_sort_param(scenes, orc, rt):
+@pytest.mark.parametrize("split", ["sah", "middle", "equal"])
+def test_layer_sort_param(scenes, orc, rt, split):
     """RayHieararchy "bool sort": the triangles are traced in BVHAccel order
-    (rayhierarchy.cpp:152-155), so among hits at equal t the smallest SORTED id wins; the
-    Intersection still names the triangle by its input index.  The scene holds coincident
-    duplicates (input ids i and n + i) so that the tie rule is exercised."""
+    (rayhierarchy.cpp:152-155) built with the "splitmethod" parameter, so among hits at equal t
+    the smallest SORTED id wins; the Intersection still names the triangle by its input index.
+    The scene holds coincident duplicates (input ids i and n + i) so that the tie rule is
+    exercised."""
     base = scenes.torus_scene()
     v9 = np.concatenate([base, base[:3000]], axis=0)
-    order = rt.bvh_order(v9, 1)
+    order = rt.bvh_order(v9, 1, split=split)
     full = scenes.pinhole_rays(512, 512)
     rays = scenes.subimage(full, 192, 256, 128, 128)
     rays["width"], rays["height"] = 128, 128
-    out = run("rayhierarchy", "sort=true", v9, rays, 0, pre=(128, 128, 1))
+    out = run("rayhierarchy", f"sort=true;splitmethod={split}", v9, rays, 0, pre=(128, 128, 1))
     p = orc.params("rayhierarchy")
     ref = orc.trace(v9[order], _closest_rays(rays), p, orc.CLOSEST_PRIMARY)
     ids = np.where(ref["id"] >= 0, order[np.maximum(ref["id"], 0)].astype(np.int32), -1)
     plain = orc.trace(v9, _closest_rays(rays), p, orc.CLOSEST_PRIMARY)["id"]
     print(f"[sort] rays whose winner differs from the unsorted order: "
           f"{int(np.count_nonzero(ids != plain))}", flush=True)
-    expect_closest(out, ids, ref["t"], rays, orc, v9, None, "rayhierarchy sort")
+    expect_closest(out, ids, ref["t"], rays, orc, v9, None, f"rayhierarchy sort {split}")
 
 
 @pytest.mark.gpu
@@ -257,7 +264,7 @@ def test_layer_naive_quirks(scenes, orc):
     n0 = int((ref["id"] == 0).sum())
     assert n0 > 50, "the test scene must put triangle 0 in view"
     ids = np.where(ref["id"] == 0, -1, ref["id"])
-    expect_closest(out, ids, ref["t"], rays, orc, v9, None, "naive primary")
+    expect_closest(out, ids, ref["t"], rays, orc, v9, None, "naive primary", naive=True)
 
     n = 4096
     br = scenes.random_rays(n, seed=14, lo=-1.5, hi=1.5, maxt=np.inf, width=64)
@@ -268,7 +275,7 @@ def test_layer_naive_quirks(scenes, orc):
     r2["valid"] = (sel & ~(br["dx"] < -5)).astype(np.uint8)
     refb = orc.trace(v9, r2, orc.params("naive"), orc.CLOSEST_SECONDARY)
     ids = np.where(refb["id"] == 0, -1, refb["id"])
-    expect_closest(ob, ids, refb["t"], br, orc, v9, None, "naive bounce")
+    expect_closest(ob, ids, refb["t"], br, orc, v9, None, "naive bounce", naive=True)
     assert not np.any(ob["hit"][::9])
 
 
@@ -325,14 +332,133 @@ def test_breadth_first_directlighting_image(scenes, orc):
     np.testing.assert_allclose(img, ref, rtol=1e-5, atol=1e-7)
 
 
+def _mix32(h):
+    h = h.astype(np.uint32)
+    h ^= h >> np.uint32(16)
+    h = (h * np.uint32(0x7feb352d)).astype(np.uint32)
+    h ^= h >> np.uint32(15)
+    h = (h * np.uint32(0x846ca68b)).astype(np.uint32)
+    h ^= h >> np.uint32(16)
+    return h
+
+
+def _libm(name, argtype):
+    fn = getattr(ctypes.CDLL("libm.so.6"), name)
+    fn.restype = fn.argtypes = None
+    fn.restype = argtype
+    fn.argtypes = [argtype]
+    return fn
+
+
+def _oracle_path_image(orc, v9, accel, params, xres, yres, spp, max_depth, light=(2.0, 2.0, -2.0),
+                       intensity=10.0, albedo=0.5, seed=1):
+    """The breadth-first path loop restated over the oracle's traces (one task: the whole image
+    in sampler order): camera rays through (x + (k + 0.5) / spp, y + 0.5), a shadow ray per hit
+    toward the point light (mint 1e-3 t, maxt the distance), Lambertian direct light, and
+    cosine-weighted bounces from the mix32 hash of (seed, ray, depth, task), traced as bounce
+    rays of the hits.  FP32 in the renderer's operation order; cosf / sinf / tan from libm,
+    as the renderer calls them."""
+    F = np.float32
+    cosf, sinf, tan = _libm("cosf", ctypes.c_float), _libm("sinf", ctypes.c_float), \
+        _libm("tan", ctypes.c_double)
+    tri = v9.reshape(-1, 3, 3)
+    e1, e2 = (tri[:, 1] - tri[:, 0]).astype(F), (tri[:, 2] - tri[:, 0]).astype(F)
+    nrm = np.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1], e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
+                    e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], 1).astype(F)
+    ln = np.sqrt((nrm[:, 0] * nrm[:, 0] + nrm[:, 1] * nrm[:, 1]) + nrm[:, 2] * nrm[:, 2])
+    nrm = np.where(ln[:, None] == 0, nrm, nrm * (F(1) / np.where(ln == 0, F(1), ln))[:, None]).astype(F)
+    tan_half = F(tan(60.0 * (3.14159265358979323846 / 180.0) / 2.0))
+    aspect = F(xres / yres)
+    ys, xs, ks = np.meshgrid(np.arange(yres), np.arange(xres), np.arange(spp), indexing="ij")
+    fx = xs.astype(F) + ((ks.astype(F) + F(0.5)) / F(spp))
+    fy = ys.astype(F) + F(0.5)
+    px = (((fx / F(xres)) * F(2) - F(1)) * tan_half * aspect).reshape(-1)
+    py = ((F(1) - (fy / F(yres)) * F(2)) * tan_half).reshape(-1)
+    pz = np.ones_like(px)
+    inv = F(1) / np.sqrt((px * px + py * py) + pz * pz)
+    n = px.size
+    o = np.tile(np.array([0.0, 0.0, -3.0], F), (n, 1))
+    d = np.stack([px * inv, py * inv, pz * inv], 1).astype(F)
+    W, H = xres * spp, yres
+    kind = {"naive": 0, "rayhierarchy": 1, "ray-bvh": 2}[accel]
+    p = orc.params(accel, **params)
+    handoff = orc.raybvh_handoff(v9, max_prims=1, bvh_height=3) if kind == 2 else None
+
+    def trace(o, d, mint, maxt, valid, mode):
+        r = dict(ox=o[:, 0].copy(), oy=o[:, 1].copy(), oz=o[:, 2].copy(), dx=d[:, 0].copy(),
+                 dy=d[:, 1].copy(), dz=d[:, 2].copy(), tmin=mint.astype(F), tmax=maxt.astype(F),
+                 valid=None if valid is None else valid.astype(np.uint8), width=W, height=H)
+        if kind == 2:
+            return orc.trace_raybvh(v9, *handoff, r, p, mode)
+        return orc.trace(v9, r, p, mode)
+
+    def dot(a, b):
+        return (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
+
+    h = trace(o, d, np.zeros(n, F), np.full(n, np.inf, F), None, orc.CLOSEST_PRIMARY)
+    hit = h["id"] >= (1 if kind == 0 else 0)   # the naive layer drops hits on triangle 0
+    tid, t = np.maximum(h["id"], 0), h["t"]
+    L, beta = np.zeros(n, F), np.ones(n, F)
+    Lc = F(albedo) * F(0.31830988618379067154)
+    lt = np.array(light, F)
+    for depth in range(max_depth + 1):
+        pt = (o + d * np.where(hit, t, F(0))[:, None]).astype(F)
+        eps = (F(1e-3) * np.where(hit, t, F(0))).astype(F)
+        v = (lt[None, :] - pt).astype(F)
+        dist = np.sqrt(dot(v, v))
+        sd = (v * (F(1) / np.where(hit, dist, F(1)))[:, None]).astype(F)
+        occ = trace(pt, sd, eps, dist, hit, orc.OCCLUDED)["occ"]
+        nn = nrm[tid]
+        nn = np.where((dot(nn, d) > 0)[:, None], -nn, nn).astype(F)
+        c = np.maximum(F(0), dot(nn, sd))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            add = (((beta * Lc) * F(intensity)) * c) / (dist * dist)
+        L = np.where(hit & (occ == 0), L + add, L).astype(F)
+        if depth >= max_depth or not hit.any():
+            break
+        j = np.arange(n, dtype=np.uint64)
+        inner = _mix32(np.full(n, (depth + 0 * 977) & 0xffffffff, np.uint32))
+        hh = _mix32(np.uint32(seed) ^ _mix32(((j * 0x9e3779b9) & 0xffffffff).astype(np.uint32) ^ inner))
+        u1 = (hh >> np.uint32(8)).astype(F) * F(1.0 / 16777216.0)
+        u2 = (_mix32(hh) >> np.uint32(8)).astype(F) * F(1.0 / 16777216.0)
+        r = np.sqrt(u1)
+        phi = F(6.28318530717958647692) * u2
+        lx = r * np.array([cosf(float(x)) for x in phi], F)
+        ly = r * np.array([sinf(float(x)) for x in phi], F)
+        lz = np.sqrt(np.maximum(F(0), F(1) - u1))
+        sgn = np.where(nn[:, 2] >= 0, F(1), F(-1))
+        a = F(-1) / (sgn + nn[:, 2])
+        b = nn[:, 0] * nn[:, 1] * a
+        t1 = np.stack([F(1) + sgn * nn[:, 0] * nn[:, 0] * a, sgn * b, -sgn * nn[:, 0]], 1)
+        t2 = np.stack([b, sgn + nn[:, 1] * nn[:, 1] * a, -nn[:, 1]], 1)
+        wo = (t1 * lx[:, None] + t2 * ly[:, None]) + nn * lz[:, None]
+        o = np.where(hit[:, None], pt, o).astype(F)
+        d = np.where(hit[:, None], wo, d).astype(F)
+        beta = np.where(hit, beta * F(albedo), beta).astype(F)
+        h = trace(o, d, eps, np.full(n, np.inf, F), hit, orc.CLOSEST_SECONDARY)
+        hit = hit & (h["id"] >= (1 if kind == 0 else 0))
+        tid, t = np.maximum(h["id"], 0), h["t"]
+    img = (L / F(spp)).reshape(yres, xres, spp)
+    out = np.zeros((yres, xres), F)
+    for k in range(spp):
+        out += img[:, :, k]
+    return out
+
+
 @pytest.mark.gpu
-def test_breadth_first_path_runs_all_accelerators(scenes):
-    """Path tracing with two bounces issues Intersect(..., bounce) and IntersectP batches;
-    the three accelerators agree pixel for pixel on a small scene where their candidate sets
-    cover every hit (ia culls are conservative; naive tests every pair)."""
+@pytest.mark.parametrize("accel", ["naive", "rayhierarchy", "ray-bvh"])
+def test_breadth_first_path_matches_oracle(scenes, orc, accel):
+    """Path tracing with two bounces issues Intersect(..., bounce) and IntersectP batches; the
+    rendered image equals the same loop run over the oracle's traces (naive: its own
+    predicates and the layer's drop of triangle 0; rayhierarchy / ray-bvh with box_dir)."""
     v9 = scenes.random_triangles(600, seed=3)
-    imgs = {}
-    for accel, params in (("rayhierarchy", "node=ia"), ("ray-bvh", "node=ia")):
-        imgs[accel], st = _render(accel, v9, 96, 64, spp=2, max_depth=2, params=params)
-        assert st[5] > 0, "no bounce rays were traced"
-    np.testing.assert_allclose(imgs["rayhierarchy"], imgs["ray-bvh"], rtol=1e-5, atol=1e-7)
+    params = {} if accel == "naive" else {"node": "box_dir"}
+    img, st = _render(accel, v9, 96, 64, spp=2, max_depth=2,
+                      params=";".join(f"{k}={v}" for k, v in params.items()))
+    assert st[5] > 0, "no bounce rays were traced"
+    ref = _oracle_path_image(orc, v9, accel, params, 96, 64, 2, 2)
+    assert np.count_nonzero(ref) > 500
+    bad = int(np.count_nonzero(img != ref))
+    print(f"[path {accel}] pixels {img.size}, lit {np.count_nonzero(ref)}, differing {bad}",
+          flush=True)
+    np.testing.assert_allclose(img, ref, rtol=1e-5, atol=1e-7)

@@ -196,8 +196,12 @@ def test_rh_levels_rule(orc):
     assert orc.rh_levels(24, 24) == [(6, 6), (3, 3)]
     assert orc.rh_levels(1024, 256) == [(256, 64), (128, 32), (64, 16)]
     assert orc.rh_levels(16, 16, height=1) == [(4, 4)]
-    # ceil semantics for partial tiles
+    # ceil grids for partial tiles
     assert orc.rh_levels(10, 6) == [(3, 2), (2, 1)]
+    # the level count halves with floor as the reference does (11 -> 5 is odd: two levels),
+    # the grids with ceil (6 parents also cover the 11th leaf column)
+    assert orc.rh_levels(33, 17, cx=3, cy=5) == [(11, 4), (6, 2)]
+    assert orc.rh_levels(52, 32) == [(13, 8), (7, 4), (4, 2)]  # 13 -> 6 -> 3
     assert orc.rh_well_formed(512, 512) and orc.rh_well_formed(1024, 256)
     assert not orc.rh_well_formed(10, 6)
     assert not orc.rh_well_formed(12, 16)  # 3 leaves wide: reference breaks above the leaves
@@ -382,6 +386,20 @@ def test_shading_closed_form(orc, scenes):
     du, dv = s2["dpdu"], s2["dpdv"]
     assert abs(np.dot(du, dv)) < 1e-6 and abs(np.linalg.norm(du) - 1) < 1e-6
     assert abs(abs(dv[0] * 0 + dv[1] * 0) + 0) < 1e-6 and abs(np.linalg.norm(dv) - 1) < 1e-6
+    # UVs (0,0), (1,0), (1,d) have determinant d exactly (du1 = -1, du2 = 0, dv1 = dv2 = -d).
+    # The naive kernels fall back for |det| < EPS = 0.002 (naive.cl:60-61), computeDpTuTv only
+    # for det == 0: with d = +-0.001 the two rules differ, at d = 0.002 (not below EPS) and
+    # d = 0.004 they agree.
+    for d, differ in ((0.001, True), (-0.001, True), (0.002, False), (0.004, False)):
+        uvd = np.array([0, 0, 1, 0, 1, d], np.float32)
+        a = orc.shading(v9, uvd, rays, np.array([0], np.int32))
+        b = orc.shading(v9, uvd, rays, np.array([0], np.int32), naive=True)
+        assert (not np.array_equal(a["dpdv"], b["dpdv"])) == differ, d
+        if differ:
+            np.testing.assert_array_equal(b["dpdu"], s2["dpdu"])   # the fallback pair
+            np.testing.assert_array_equal(b["dpdv"], s2["dpdv"])
+            assert np.linalg.norm(a["dpdv"]) > 100                 # -dp2 / det
+        np.testing.assert_array_equal(a["u"], b["u"])              # u, v do not depend on it
 
 
 # ---------------------------------------------------------------- BVH build / handoff

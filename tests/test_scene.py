@@ -156,6 +156,43 @@ def test_scene_geometry_and_options(tmp_path):
     assert any("ObjectInstance" in m for m in msgs)
 
 
+def _f32_normalize(v):
+    # Normalize (geometry.h:511): v * (1 / sqrtf((x*x + y*y) + z*z)) in float
+    l2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]
+    inv = F(1) / np.sqrt(F(l2))
+    return np.array([v[0] * inv, v[1] * inv, v[2] * inv], F)
+
+
+def _cross(a, b, dtype):
+    a, b = a.astype(dtype), b.astype(dtype)
+    return np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                     a[0] * b[1] - a[1] * b[0]]).astype(F)
+
+
+def test_scene_lookat_cross_in_double(tmp_path):
+    # LookAt (transform.cpp:228-254) builds left = Normalize(Cross(Normalize(up), dir)) and
+    # newUp = Cross(dir, left); pbrt's Cross computes in double and rounds once
+    # (geometry.h:481-488).  For this LookAt an FP32 cross product gives a different frame.
+    eye, look, up = (np.array(x, F) for x in ((1.4, -2.3, -4.6), (-4.8, 3.1, 4.1), (0.2, 0.5, 0.1)))
+    text = f"""LookAt {' '.join(map(str, [1.4, -2.3, -4.6, -4.8, 3.1, 4.1, 0.2, 0.5, 0.1]))}
+Camera "perspective"
+WorldBegin
+Shape "trianglemesh" "point P" [0 0 0 1 0 0 0 1 0] "integer indices" [0 1 2]
+WorldEnd
+"""
+    _, info, _ = parse(write(tmp_path, "l.pbrt", text))
+    c2w = floats(info["camera_to_world"]).reshape(3, 4)
+
+    def frame(dtype):
+        d = _f32_normalize((look - eye).astype(F))
+        left = _f32_normalize(_cross(_f32_normalize(up), d, dtype))
+        return np.stack([left, _cross(d, left, dtype), d], 1)
+
+    assert np.array_equal(c2w[:, :3], frame(np.float64))
+    assert not np.array_equal(frame(np.float64), frame(np.float32))
+    assert np.array_equal(c2w[:, 3], eye)
+
+
 def test_scene_defaults_and_light_frames(tmp_path):
     # no options: pbrt's defaults; the light follows CoordSysTransform "camera"
     text = """LookAt 1 2 3  1 2 4  0 1 0
