@@ -408,7 +408,9 @@ int rh_levels(int W, int H, int cx, int cy, int height, int* dims) {
     dims[2 * L] = gw;
     dims[2 * L + 1] = gh;
     ++L;
-    if ((fw & 1) || (fh & 1)) break;
+    // a zero floor dimension leaves the reference no top-level node (it traces nothing):
+    // stop there too
+    if ((fw & 1) || (fh & 1) || fw == 0 || fh == 0) break;
   }
   return L;
 }

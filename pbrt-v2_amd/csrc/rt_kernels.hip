@@ -2180,7 +2180,9 @@ int rh_geometry(int W, int H, int cx, int cy, int height, RHGeom* g) {
     g->gw[L] = gw;
     g->gh[L] = gh;
     ++L;
-    if ((fw & 1) || (fh & 1)) break;
+    // a zero floor dimension leaves the reference no top-level node (it traces nothing):
+    // stop there too
+    if ((fw & 1) || (fh & 1) || fw == 0 || fh == 0) break;
   }
   g->L = L;
   int off = 0;

@@ -34,6 +34,9 @@ def load_oracle():
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
+    # torch first: its HIP runtime must be the process's before the product libraries (and
+    # the C++ test driver, which links them) load the system one, whatever the test order
+    import torch  # noqa: F401
 
 
 @pytest.fixture(scope="session")

@@ -202,6 +202,8 @@ def test_rh_levels_rule(orc):
     # the grids with ceil (6 parents also cover the 11th leaf column)
     assert orc.rh_levels(33, 17, cx=3, cy=5) == [(11, 4), (6, 2)]
     assert orc.rh_levels(52, 32) == [(13, 8), (7, 4), (4, 2)]  # 13 -> 6 -> 3
+    # one leaf: the reference's floor gives 0 parents (nothing traced); stop above the leaves
+    assert orc.rh_levels(16, 16, cx=16, cy=16) == [(1, 1), (1, 1)]
     assert orc.rh_well_formed(512, 512) and orc.rh_well_formed(1024, 256)
     assert not orc.rh_well_formed(10, 6)
     assert not orc.rh_well_formed(12, 16)  # 3 leaves wide: reference breaks above the leaves
