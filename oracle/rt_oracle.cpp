@@ -1179,7 +1179,7 @@ void parallel_for(size_t n, int nthreads, F&& fn) {
 // a CPU restatement of that traversal (not of the reference): its results must equal the
 // reference-semantics traces above, and its per-ray counts (interior nodes loaded, leaf
 // tests, chain culls) are the work model of the per-ray kernel.
-constexpr int kSDepth = 30;
+constexpr int kSDepth = 26;
 constexpr int kSLeaf = 4;
 
 uint32_t spread10(uint32_t x) {

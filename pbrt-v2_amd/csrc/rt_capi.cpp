@@ -424,6 +424,9 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     }
     members[gi].push_back((int)j);
   }
+  // the per-ray kernel's deferred-chain queue names a job in 16 bits
+  for (const auto& m : members)
+    if (m.size() > 65535u) return RT_ERR_UNSUPPORTED;
   // geometry and sizes
   std::vector<RhItem> items(jobs.size());
   std::vector<size_t> off_nodes(jobs.size()), off_clist(jobs.size()), off_keys(jobs.size()),

@@ -129,7 +129,7 @@ struct RhGroup {
 // 31 set it is a leaf of (bits 27..30) + 1 triangles starting at sorted slot (bits 0..26).
 // Boxes are grown outward by the scene's padding.  The depth is at most kSearchDepth.
 #ifndef RT_SEARCH_DEPTH
-#define RT_SEARCH_DEPTH 30
+#define RT_SEARCH_DEPTH 26
 #endif
 constexpr int kSearchDepth = RT_SEARCH_DEPTH;
 constexpr int kSearchLeaf = 4;

@@ -1420,11 +1420,35 @@ __device__ __forceinline__ bool ray_chain_bvh(const RhItem& it, int x, int y, ui
 // round, so lanes of long traversals do not leave the others idle.  A round walks interior
 // nodes until every lane holds a leaf or is done (Aila and Laine's while-while), then tests
 // the postponed leaves.
+//
+// Occlusion defers the cull chains: a candidate (strict hit inside the ray's interval) goes to
+// a wavefront queue in LDS as {job, ray, triangle or BVH leaf, lane and ray serial}, and the
+// queue is evaluated 64 candidates at a time, one per lane, once it holds kChainFlush.  A
+// passing chain sets the ray's occlusion flag itself and ends the traversal of its lane if the
+// lane still holds that ray.  The ray is occluded when any of its candidates passes its chain,
+// whatever the order, so the result is the same; the lanes no longer run a chain one or two at
+// a time, and a lane does not wait for its candidates before it takes the next ray.  The queue
+// is also evaluated at least every kChainAge rounds (a lane takes at most one ray per round,
+// so the 10-bit serial cannot wrap while an entry waits) and at the end.
+// A round's walk stops once at most 1/2^kWalkShift of the lanes holding a ray still search for
+// a leaf (the rest advance in the next round): fewer idle lanes wait for the longest walk.
+#ifndef RT_WALK_SHIFT
+#define RT_WALK_SHIFT 3
+#endif
+constexpr uint32_t kChainFlush = 32;
+constexpr uint32_t kChainQ = kChainFlush + 64;  // below kChainFlush, plus one per lane a round
+constexpr uint32_t kChainAge = 512;
+// Five wavefronts per SIMD (at most 96 VGPRs; the LDS of 20 work-groups per CU): the walk is
+// latency-bound and gains more from resident wavefronts than it loses to a few spilled
+// registers of the chain evaluation.
 template <int MODE, int NODE, bool BVH>
-__global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__ items,
-                                                      const uint2* __restrict__ rq,
-                                                      uint32_t* __restrict__ ctr) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void ray_path_kernel(
+    const RhItem* __restrict__ items, const uint2* __restrict__ rq, uint32_t* __restrict__ ctr) {
   __shared__ uint32_t sStk[kSearchDepth][64];
+  constexpr uint32_t kQ = MODE == 2 ? kChainQ : 1u;
+  __shared__ uint32_t sQIdx[kQ], sQRef[kQ], sQMeta[kQ];
+  __shared__ unsigned long long sQHit;
+  __shared__ uint32_t sQN;
   constexpr uint32_t kDone = 0xffffffffu;  // not a node reference (leaves hold <= 4 triangles)
   const int lane = threadIdx.x;
   const uint64_t lanes_below = (1ull << lane) - 1ull;
@@ -1446,6 +1470,48 @@ __global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__
   int sp = 0;
   uint32_t cur = kDone, pl = kDone;
   auto pop = [&]() { return sp > 0 ? sStk[--sp][lane] : kDone; };
+  // occlusion: deferred chain candidates (count sQN), the serial of this lane's ray, and the
+  // rounds since the queue was last evaluated (wavefront-uniform)
+  uint32_t serial = 0, age = 0;
+  if constexpr (MODE == 2) {
+    if (lane == 0) {
+      sQN = 0;
+      sQHit = 0ull;
+    }
+    __syncthreads();
+  }
+  // Evaluates the cq queued candidates, 64 per pass (one per lane).  Wavefront-uniform, after
+  // a barrier that made the entries visible.
+  auto run_chains = [&](uint32_t cq) {
+    for (uint32_t base = 0; base < cq; base += 64) {
+      const uint32_t e = base + (uint32_t)lane;
+      const bool have = e < cq;
+      const uint32_t meta = have ? sQMeta[e] : 0u;
+      const uint32_t owner = (meta >> 10) & 63u;
+      const uint32_t owner_serial = (uint32_t)__shfl((int)serial, (int)owner);
+      if (have) {
+        const RhItem& oit = items[meta >> 16];
+        const uint32_t ri = sQIdx[e];
+        const int ox = (int)(ri % (uint32_t)oit.g.W), oy = (int)(ri / (uint32_t)oit.g.W);
+        bool ok;
+        if constexpr (BVH) ok = ray_chain_bvh<NODE>(oit, ox, oy, sQRef[e]);
+        else ok = ray_chain_rh<MODE, NODE>(oit, ox, oy, sQRef[e]);
+        if (ok) {
+          glb(oit.occ)[ri] = 1;
+          if ((meta & 1023u) == owner_serial) atomicOr(&sQHit, 1ull << owner);
+        }
+      }
+    }
+    __syncthreads();
+    if (act && ((sQHit >> lane) & 1ull)) occluded = true;
+    __syncthreads();
+    if (lane == 0) {
+      sQN = 0;
+      sQHit = 0ull;
+    }
+    __syncthreads();
+    age = 0;
+  };
   for (;;) {
     // deal rays to the lanes without one
     for (;;) {
@@ -1502,11 +1568,18 @@ __global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__
           cur = 0u;  // interior node 0 (the root)
           pl = kDone;
           act = true;
+          serial = (serial + 1u) & 1023u;
         }
       }
       cslot += min(avail, (uint32_t)__popcll(need));
     }
-    if (!__any(act)) break;
+    if (!__any(act)) {
+      if constexpr (MODE == 2) {  // the candidates still queued
+        const uint32_t cq = __builtin_amdgcn_readfirstlane(sQN);
+        if (cq > 0u) run_chains(cq);
+      }
+      break;
+    }
     const RhItem& it = items[jk];
     const auto snode = glb(it.snode);
     const auto tmts = glb(it.tmts);
@@ -1537,13 +1610,48 @@ __global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__
           cur = pop();
         }
       }
-      // walk on while some lane has neither a postponed leaf nor finished
-      if (!__any(act && pl == kDone && cur != kDone && !(cur & kSearchLeafBit))) break;
+      // walk on while more than a share of the lanes holding a ray have neither a postponed
+      // leaf nor finished; the others go on from their node in the next round (every round
+      // walks at least one step, so each lane advances)
+      const uint32_t searching =
+          (uint32_t)__popcll(__ballot(act && pl == kDone && cur != kDone && !(cur & kSearchLeafBit)));
+      if (searching <= ((uint32_t)__popcll(__ballot(act)) >> RT_WALK_SHIFT)) break;
     }
     // leaf phase
-    if (act && pl != kDone) {
+    if constexpr (MODE == 2) {
+      // wavefront-uniform rounds over the leaf's (at most kSearchLeaf) triangles, so the queue
+      // can be evaluated between them
+      const bool haveleaf = act && pl != kDone;
+      const uint32_t first = pl & ((1u << 27) - 1u);
+      const uint32_t cnt = haveleaf ? ((pl >> 27) & 15u) + 1u : 0u;
+      for (uint32_t f = 0; f < (uint32_t)kSearchLeaf; ++f) {
+        if (!__any(f < cnt)) break;
+        if (f < cnt) {
+          const uint32_t slot = first + f;
+          uint32_t ref = slot;
+          bool ok = true;
+          if constexpr (BVH) {
+            const uint32_t bl = glb(it.tleaf)[perm[slot]];
+            ok = bl != 0xffffffffu;
+            ref = bl & 0x7fffffffu;
+          }
+          float t, b1, b2;
+          // intersectAllLeaves.cl:177-179; the chain waits in the queue
+          if (ok && mt_strict(o, d, load_mt(tmts, slot), t, b1, b2) && t > tmin && t < tmax) {
+            const uint32_t e = atomicAdd(&sQN, 1u);  // below kChainQ, see kChainQ
+            sQIdx[e] = (uint32_t)idx;
+            sQRef[e] = ref;
+            sQMeta[e] = (jk << 16) | ((uint32_t)lane << 10) | serial;
+          }
+        }
+        __syncthreads();
+        const uint32_t cq = __builtin_amdgcn_readfirstlane(sQN);
+        if (cq >= kChainFlush) run_chains(cq);
+      }
+      if (haveleaf) pl = kDone;
+    } else if (act && pl != kDone) {
       const uint32_t first = pl & ((1u << 27) - 1u), cnt = ((pl >> 27) & 15u) + 1u;
-      for (uint32_t f = 0; f < cnt && !occluded; ++f) {
+      for (uint32_t f = 0; f < cnt; ++f) {
         const uint32_t slot = first + f;
         const uint32_t id = perm[slot];
         uint32_t bl = 0;
@@ -1554,14 +1662,9 @@ __global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__
           bl &= 0x7fffffffu;
         }
         float t, b1, b2;
-        if (!mt_strict(o, d, load_mt(tmts, slot), t, b1, b2)) continue;
-        if constexpr (MODE == 2) {
-          if (!(t > tmin && t < tmax)) continue;  // intersectAllLeaves.cl:177-179
-          bool ok;
-          if constexpr (BVH) ok = ray_chain_bvh<NODE>(it, x, y, bl);
-          else ok = ray_chain_rh<MODE, NODE>(it, x, y, slot);
-          occluded = ok;
-        } else {
+        const bool hit = mt_strict(o, d, load_mt(tmts, slot), t, b1, b2);
+        {
+          if (!hit) continue;
           const bool main = t > tmin && t < __int_as_float(0x7f800000);
           const bool rp = track_rep && repair_region(b1, b2) && t >= tmin;
           const uint64_t key = hit_key(t, (int32_t)id);
@@ -1580,6 +1683,13 @@ __global__ __launch_bounds__(64) void ray_path_kernel(const RhItem* __restrict__
     if (act && cur != kDone && (cur & kSearchLeafBit)) {  // a second leaf waits for the next round
       pl = cur;
       cur = pop();
+    }
+    if constexpr (MODE == 2) {
+      if (++age >= kChainAge) {
+        const uint32_t cq = __builtin_amdgcn_readfirstlane(sQN);
+        if (cq > 0u) run_chains(cq);
+        age = 0;
+      }
     }
     if (act && occluded) cur = pl = kDone;
     if (act && cur == kDone && pl == kDone) {  // the ray is done: its result, then a new ray
