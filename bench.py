@@ -633,8 +633,9 @@ def bench_plain(args, rt, scenes, torch, orc, name):
             bc = n * 48 + cs.leaf_pairs * 36 + cs.cull_tests * per_cull
             bo_ = n_shadow * 33 + os_.leaf_pairs * 36 + os_.cull_tests * per_cull
             tc, to = km.get("closest_unit", 0.0), km.get("occlusion_unit", 0.0)
-            name_k, b, t, st = (("occlusion_traversal", bo_, to, os_) if to >= tc
-                                else ("closest_traversal", bc, tc, cs))
+            # the unit kernels of this single-job launch: 256-ray blocks, 3 levels
+            name_k, b, t, st = (("rh_unit_kernel<2,0,256,3>", bo_, to, os_) if to >= tc
+                                else ("rh_unit_kernel<0,0,256,3>", bc, tc, cs))
             if t > 0:
                 achieved = b / (t * 1e-3) / 1e9
                 ops = 45 * int(st.ray_tests)
@@ -644,7 +645,7 @@ def bench_plain(args, rt, scenes, torch, orc, name):
                 hbm_meas = None if traffic is None else traffic / (t * 1e-3) / 1e9 / HBM_PEAK_GBS
                 roofline = {"bound": bound_of(facts, hbm_meas, achieved / HBM_PEAK_GBS,
                                               tfl / FP32_UNPACKED_TFLOPS),
-                            "kernel": name_k + " (unit kernel)", "achieved": round(achieved, 1),
+                            "kernel": name_k, "achieved": round(achieved, 1),
                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(achieved / HBM_PEAK_GBS, 4),
                             "traffic": None if traffic is None else int(traffic),
