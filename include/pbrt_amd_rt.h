@@ -200,7 +200,10 @@ int rt_make_bounce_rays(rt_scene *scene, const rt_rays *in, const int32_t *id, c
  *   2^29 keeps only the direction test; a value above 2^29 (at most 2^30) sends no block.
  *   Built-in (0): 2^29 for closest hits, 256 (rayhierarchy) or 64 (ray-bvh) for occlusion.
  *   RT_PARAM_RAY_GRID_CLOSEST / RT_PARAM_RAY_GRID_OCCLUDED: work-groups (one wavefront each)
- *   of the per-ray kernel (batches only); 0 = one per resident slot. */
+ *   of the per-ray kernel (batches only); 0 = one per resident slot.
+ *   RT_PARAM_WALK_CLOSEST / RT_PARAM_WALK_OCCLUDED: v in 1..7: a round of the per-ray kernel
+ *   walks the search BVH until at most 1/2^v of its lanes still look for a leaf (7: until
+ *   every lane holds one); 0 = the built-in 3. */
 enum {
   RT_PARAM_CLUSTERS_CLOSEST = 1,
   RT_PARAM_CLUSTERS_OCCLUDED = 2,
@@ -210,7 +213,9 @@ enum {
   RT_PARAM_RAY_CLUSTERS_CLOSEST = 6,
   RT_PARAM_RAY_CLUSTERS_OCCLUDED = 7,
   RT_PARAM_RAY_GRID_CLOSEST = 8,
-  RT_PARAM_RAY_GRID_OCCLUDED = 9
+  RT_PARAM_RAY_GRID_OCCLUDED = 9,
+  RT_PARAM_WALK_CLOSEST = 10,
+  RT_PARAM_WALK_OCCLUDED = 11
 };
 int rt_scene_set_param(rt_scene *scene, int32_t param, int32_t value);
 

@@ -43,7 +43,7 @@ EXPORTS = (
 (RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED, RT_PARAM_GRID_CLOSEST,
  RT_PARAM_GRID_OCCLUDED, RT_PARAM_GRAPH, RT_PARAM_RAY_CLUSTERS_CLOSEST,
  RT_PARAM_RAY_CLUSTERS_OCCLUDED, RT_PARAM_RAY_GRID_CLOSEST,
- RT_PARAM_RAY_GRID_OCCLUDED) = 1, 2, 3, 4, 5, 6, 7, 8, 9
+ RT_PARAM_RAY_GRID_OCCLUDED, RT_PARAM_WALK_CLOSEST, RT_PARAM_WALK_OCCLUDED) = range(1, 12)
 # a per-ray threshold above any cluster count: every block enumerates its candidate pairs
 RAY_PATH_OFF = 1 << 30
 # kernel_time modes: + RT_TIME_PER_RAY = only the per-ray kernel, + RT_TIME_UNIT = only the

@@ -119,6 +119,7 @@ struct RhParams {
   int grid[2] = {0, 0};
   int ray[2] = {0, 0};  // passing clusters from which a block takes the per-ray path
   int rgrid[2] = {0, 0};  // work-groups of the per-ray kernel
+  int walk[2] = {0, 0};   // per-ray walk rounds (RT_PARAM_WALK_*)
 };
 
 struct rt_scene {
@@ -140,6 +141,7 @@ struct rt_scene {
   uint32_t nclusters = 0;
   int clusters_closest = 0, clusters_occluded = 0;  // 0 = built-in choice
   int ray_closest = 0, ray_occluded = 0;            // per-ray path thresholds; 0 = built-in
+  int walk_closest = 0, walk_occluded = 0;          // per-ray walk rounds; 0 = built-in
   // ray-bvh handoff
   float4* d_bvh = nullptr;
   uint32_t* d_roots = nullptr;
@@ -437,7 +439,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
   struct GroupLayout {
     size_t units, rq, ctr, replist, max_units, max_chunks;
     uint32_t ray;
-    int rgrid;
+    int rgrid, walk;
     uint32_t nblocks, nrays;
     bool big, repair;
     int per, grid;
@@ -452,6 +454,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     L.per = prm.per[mi] > 0 ? prm.per[mi] : (bvh ? kDefaultRootClustersPerUnit : kDefaultPer[mi]);
     L.ray = (uint32_t)(prm.ray[mi] > 0 ? prm.ray[mi] : default_ray(bvh, mi));
     L.rgrid = prm.rgrid[mi];
+    L.walk = prm.walk[mi] > 0 ? prm.walk[mi] : kDefaultWalk;
     L.max_units = 0;
     L.max_chunks = 0;
     L.nblocks = 0;
@@ -542,6 +545,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     G.max_chunks = (uint32_t)std::min<size_t>(gl[gi].max_chunks, 0xffffffffu);
     G.ray_clusters = gl[gi].ray;
     G.ray_grid = gl[gi].rgrid;
+    G.walk = gl[gi].walk;
     G.ctr = (uint32_t*)(A + gl[gi].ctr);
     G.replist = gl[gi].repair ? (uint32_t*)(A + gl[gi].replist) : nullptr;
     G.per_unit = gl[gi].per;
@@ -886,6 +890,8 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
     prm.per[1] = kind == RT_ACCEL_RAYBVH ? s->roots_per_unit : s->clusters_occluded;
     prm.ray[0] = s->ray_closest;
     prm.ray[1] = s->ray_occluded;
+    prm.walk[0] = s->walk_closest;
+    prm.walk[1] = s->walk_occluded;
     st = rh_prepare(s->ws[mode], jobs, prm, hs, groups, &changed);
     if (st) return st;
     Timer::Slot* t = s->timer.next(mode, true);
@@ -996,6 +1002,7 @@ int rt_trace_closest_noalloc(rt_scene* s, const rt_rays* rays, int32_t mode, con
   G.max_chunks = (uint32_t)it.g.ntop * nch;
   G.ray_clusters = 0;
   G.ray_grid = 0;
+  G.walk = kDefaultWalk;
   RT_TRY(launch_write_table(it, table, hs));
   RT_TRY(launch_rh_group(G, hs, nullptr, nullptr, nullptr));
   return RT_OK;
@@ -1042,6 +1049,8 @@ int rt_scene_set_param(rt_scene* s, int32_t param, int32_t value) {
   else if (param == RT_PARAM_CLUSTERS_OCCLUDED) s->clusters_occluded = value;
   else if (param == RT_PARAM_RAY_CLUSTERS_CLOSEST) s->ray_closest = value;
   else if (param == RT_PARAM_RAY_CLUSTERS_OCCLUDED) s->ray_occluded = value;
+  else if (param == RT_PARAM_WALK_CLOSEST && value <= 7) s->walk_closest = value;
+  else if (param == RT_PARAM_WALK_OCCLUDED && value <= 7) s->walk_occluded = value;
   else return RT_ERR_INVALID;
   return RT_OK;
 }
@@ -1166,6 +1175,8 @@ static int* batch_param(rt_batch* b, int32_t param) {
     case RT_PARAM_RAY_CLUSTERS_OCCLUDED: return &b->prm.ray[1];
     case RT_PARAM_RAY_GRID_CLOSEST: return &b->prm.rgrid[0];
     case RT_PARAM_RAY_GRID_OCCLUDED: return &b->prm.rgrid[1];
+    case RT_PARAM_WALK_CLOSEST: return &b->prm.walk[0];
+    case RT_PARAM_WALK_OCCLUDED: return &b->prm.walk[1];
     default: return nullptr;
   }
 }
@@ -1184,6 +1195,8 @@ int rt_batch_set_param(rt_batch* b, int32_t param, int32_t value) {
       param == RT_PARAM_RAY_CLUSTERS_CLOSEST || param == RT_PARAM_RAY_CLUSTERS_OCCLUDED ||
       param == RT_PARAM_RAY_GRID_CLOSEST || param == RT_PARAM_RAY_GRID_OCCLUDED) {
     if (value > (1 << 30)) return RT_ERR_INVALID;
+  } else if (param == RT_PARAM_WALK_CLOSEST || param == RT_PARAM_WALK_OCCLUDED) {
+    if (value > 7) return RT_ERR_INVALID;
   } else if (value > (1 << 16)) {
     return RT_ERR_INVALID;
   }
@@ -1202,9 +1215,14 @@ int rt_batch_get_param(rt_batch* b, int32_t param, int32_t* value) {
   int* p = batch_param(b, param);
   if (!p) return RT_ERR_INVALID;
   const int mi = (param == RT_PARAM_CLUSTERS_OCCLUDED || param == RT_PARAM_GRID_OCCLUDED ||
-                  param == RT_PARAM_RAY_CLUSTERS_OCCLUDED || param == RT_PARAM_RAY_GRID_OCCLUDED)
+                  param == RT_PARAM_RAY_CLUSTERS_OCCLUDED || param == RT_PARAM_RAY_GRID_OCCLUDED ||
+                  param == RT_PARAM_WALK_OCCLUDED)
                      ? 1
                      : 0;
+  if (param == RT_PARAM_WALK_CLOSEST || param == RT_PARAM_WALK_OCCLUDED) {
+    *value = *p > 0 ? *p : kDefaultWalk;
+    return RT_OK;
+  }
   const bool grid = param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED ||
                     param == RT_PARAM_RAY_GRID_CLOSEST || param == RT_PARAM_RAY_GRID_OCCLUDED;
   const bool ray = param == RT_PARAM_RAY_CLUSTERS_CLOSEST || param == RT_PARAM_RAY_CLUSTERS_OCCLUDED;
@@ -1279,7 +1297,13 @@ int rt_batch_autotune(rt_batch* b, void* stream, int32_t reps) {
         o.ray[m] = p.ray[m];
         o.rgrid[m] = p.rgrid[m];
       });
-  // stage 2: passing clusters per work unit x persistent grid of the unit kernel
+  // stage 2: how far a round of the per-ray kernel walks before its leaf tests
+  static const int kWalk[] = {1, 2, 3, 7};
+  if (!st)
+    st = autotune_stage(
+        b, hs, reps, 4, [&](int c, RhParams& p) { p.walk[0] = p.walk[1] = kWalk[c]; },
+        [](int m, const RhParams& p, RhParams& o) { o.walk[m] = p.walk[m]; });
+  // stage 3: passing clusters per work unit x persistent grid of the unit kernel
   static const int kPer[] = {2, 4, 8, 16, 32};
   static const int kGrid[] = {0, 1024, 2048};  // 0: one work-group per resident slot
   if (!st)

@@ -121,7 +121,9 @@ struct RhGroup {
   // block with a traced ray that way without the cluster scan (job cluster lists unused)
   uint32_t ray_clusters;
   int ray_grid;  // work-groups (one wavefront each) of the per-ray kernel; 0 = resident slots
+  int walk;      // per-ray walk rounds stop at <= 1/2^walk searching lanes (1..7)
 };
+constexpr int kDefaultWalk = 3;
 
 // Search BVH of the per-ray path over triangles in sorted order (see rt_search_bvh.cpp).
 // Node = 4 float4: child 0 box (lo.xyz, hi.x), (hi.yz, child 1 lo.xy), (lo.z, hi.xyz),

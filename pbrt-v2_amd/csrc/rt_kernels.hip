@@ -1430,11 +1430,6 @@ __device__ __forceinline__ bool ray_chain_bvh(const RhItem& it, int x, int y, ui
 // a time, and a lane does not wait for its candidates before it takes the next ray.  The queue
 // is also evaluated at least every kChainAge rounds (a lane takes at most one ray per round,
 // so the 10-bit serial cannot wrap while an entry waits) and at the end.
-// A round's walk stops once at most 1/2^kWalkShift of the lanes holding a ray still search for
-// a leaf (the rest advance in the next round): fewer idle lanes wait for the longest walk.
-#ifndef RT_WALK_SHIFT
-#define RT_WALK_SHIFT 3
-#endif
 constexpr uint32_t kChainFlush = 32;
 constexpr uint32_t kChainQ = kChainFlush + 64;  // below kChainFlush, plus one per lane a round
 constexpr uint32_t kChainAge = 512;
@@ -1443,7 +1438,8 @@ constexpr uint32_t kChainAge = 512;
 // registers of the chain evaluation.
 template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void ray_path_kernel(
-    const RhItem* __restrict__ items, const uint2* __restrict__ rq, uint32_t* __restrict__ ctr) {
+    const RhItem* __restrict__ items, const uint2* __restrict__ rq, uint32_t* __restrict__ ctr,
+    uint32_t walk) {
   __shared__ uint32_t sStk[kSearchDepth][64];
   constexpr uint32_t kQ = MODE == 2 ? kChainQ : 1u;
   __shared__ uint32_t sQIdx[kQ], sQRef[kQ], sQMeta[kQ];
@@ -1610,12 +1606,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
           cur = pop();
         }
       }
-      // walk on while more than a share of the lanes holding a ray have neither a postponed
-      // leaf nor finished; the others go on from their node in the next round (every round
-      // walks at least one step, so each lane advances)
+      // walk on while more than 1/2^walk of the lanes holding a ray have neither a postponed
+      // leaf nor finished (fewer lanes idle behind the longest walk); the others go on from
+      // their node in the next round (every round walks at least one step, so each lane
+      // advances)
       const uint32_t searching =
           (uint32_t)__popcll(__ballot(act && pl == kDone && cur != kDone && !(cur & kSearchLeafBit)));
-      if (searching <= ((uint32_t)__popcll(__ballot(act)) >> RT_WALK_SHIFT)) break;
+      if (searching <= ((uint32_t)__popcll(__ballot(act)) >> walk)) break;
     }
     // leaf phase
     if constexpr (MODE == 2) {
@@ -2203,7 +2200,8 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
   auto launch_rays = [&](auto kernel) {
     const dim3 grid(G.ray_grid > 0 ? std::max(1u, std::min((unsigned)G.ray_grid, G.max_chunks))
                                    : resident_grid(kernel, G.max_chunks, 64));
-    hipLaunchKernelGGL(kernel, grid, dim3(64), 0, s, G.items, G.rq, G.ctr);
+    hipLaunchKernelGGL(kernel, grid, dim3(64), 0, s, G.items, G.rq, G.ctr,
+                       (uint32_t)std::min(std::max(G.walk, 1), 7));
   };
   if (G.bvh)
     launch_rays(ray_path_kernel<MODE, NODE, true>);

@@ -139,6 +139,30 @@ def test_batch_launch_shapes_same_results(rt, scenes, gpu_ctx):
                 _same(_np(outs[i]), ref[i], f"grid {grid} per {per} job {i}")
                 assert np.array_equal(occs[i].cpu().numpy(), ref_occ[i]), \
                     f"grid {grid} per {per} job {i} occlusion"
+    # the per-ray path (every block) with each walk length and per-ray grid
+    for p_ in (rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED):
+        b.set_param(p_, 1)
+    for walk in (1, 2, 3, 7):
+        for rgrid in (1, 7, 0):
+            for p_ in (rt.RT_PARAM_WALK_CLOSEST, rt.RT_PARAM_WALK_OCCLUDED):
+                b.set_param(p_, walk)
+            for p_ in (rt.RT_PARAM_RAY_GRID_CLOSEST, rt.RT_PARAM_RAY_GRID_OCCLUDED):
+                b.set_param(p_, rgrid)
+            print(f"[batch] per-ray walk {walk} grid {rgrid}", flush=True)
+            for o in outs:
+                o.id.fill_(-7)
+            for o in occs:
+                o.fill_(7)
+            b.run()
+            for i in range(len(jobs)):
+                _same(_np(outs[i]), ref[i], f"walk {walk} rgrid {rgrid} job {i}")
+                assert np.array_equal(occs[i].cpu().numpy(), ref_occ[i]), \
+                    f"walk {walk} rgrid {rgrid} job {i} occlusion"
+    assert b.get_param(rt.RT_PARAM_WALK_CLOSEST) == 7
+    b.set_param(rt.RT_PARAM_WALK_CLOSEST, 0)
+    assert b.get_param(rt.RT_PARAM_WALK_CLOSEST) == 3
+    with pytest.raises(rt.RtError):
+        b.set_param(rt.RT_PARAM_WALK_OCCLUDED, 8)
     b.close()
     for s_ in sc.values():
         s_.close()
