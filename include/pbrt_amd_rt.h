@@ -98,6 +98,18 @@ int rt_accel_desc_default(int32_t kind, rt_accel_desc *desc);
 /* Opens the HIP device (replaces OpenCL::OpenCL, GPUparallel.cpp:39-124). */
 int rt_ctx_create(int32_t device, rt_ctx **out);
 int rt_ctx_destroy(rt_ctx *ctx);
+/* Conditions the traversal kernels record instead of failing (no counterpart in the
+ * reference, whose kernels have none of these bounded queues).  rt_ctx_check waits for the
+ * context's device, returns the RT_FLAG_* bits recorded since the last check in *flags and
+ * clears them.  0 means every trace since then completed its traversal; a set bit means
+ * some ray-bvh results may be incomplete (the host-side subtree limit of rt_scene_set_bvh
+ * keeps both within their bounds).
+ *   RT_FLAG_BVH_STACK: a block's lockstep traversal stack would have overflowed;
+ *   RT_FLAG_PAIR_CAP: a block produced more (leaf, BVH leaf) pairs in one round than its
+ *   pair queue holds. */
+#define RT_FLAG_BVH_STACK 1u
+#define RT_FLAG_PAIR_CAP 2u
+int rt_ctx_check(rt_ctx *ctx, uint32_t *flags);
 
 /* Uploads the flattened triangles once (replaces the per-call vertex uploads of
  * naive.cpp:266-278, rayhierarchy.cpp:933-948, rayBVH.cpp:705-706).

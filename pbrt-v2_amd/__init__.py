@@ -24,13 +24,14 @@ RT_ACCEL_NAIVE, RT_ACCEL_RAYHIERARCHY, RT_ACCEL_RAYBVH = 0, 1, 2
 RT_NODE_BOX_DIR, RT_NODE_IA, RT_NODE_BOX_UV = 0, 1, 2
 NODE_NAMES = {"box_dir": RT_NODE_BOX_DIR, "ia": RT_NODE_IA, "box_uv": RT_NODE_BOX_UV}
 RT_CLOSEST_PRIMARY, RT_CLOSEST_SECONDARY = 0, 1
+FLAG_BVH_STACK, FLAG_PAIR_CAP = 1, 2  # rt_ctx_check bits (RT_FLAG_*)
 
 ACCEL_NAMES = {"naive": RT_ACCEL_NAIVE, "rayhierarchy": RT_ACCEL_RAYHIERARCHY,
                "ray-bvh": RT_ACCEL_RAYBVH}
 
 # exported C symbols (every function declared in include/pbrt_amd_rt.h)
 EXPORTS = (
-    "rt_accel_desc_default", "rt_ctx_create", "rt_ctx_destroy", "rt_scene_create",
+    "rt_accel_desc_default", "rt_ctx_create", "rt_ctx_destroy", "rt_ctx_check", "rt_scene_create",
     "rt_scene_set_bvh", "rt_scene_destroy", "rt_trace_closest", "rt_trace_occluded",
     "rt_make_shadow_rays", "rt_scene_set_timing", "rt_scene_kernel_time",
     "rt_error_string", "rt_raybvh_build_handoff", "rt_raybvh_free_handoff", "rt_bvh_order", "rt_scene_set_param",
@@ -117,6 +118,7 @@ def lib() -> ctypes.CDLL:
     L.rt_accel_desc_default.argtypes = [ctypes.c_int32, P(AccelDesc)]
     L.rt_ctx_create.argtypes = [ctypes.c_int32, P(ctypes.c_void_p)]
     L.rt_ctx_destroy.argtypes = [ctypes.c_void_p]
+    L.rt_ctx_check.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32)]
     L.rt_scene_create.argtypes = [ctypes.c_void_p, _fp, _fp, ctypes.c_uint32, P(AccelDesc),
                                   P(ctypes.c_void_p)]
     L.rt_scene_set_bvh.argtypes = [ctypes.c_void_p, _fp, ctypes.c_uint32, _fp, ctypes.c_uint32, _fp]
@@ -300,6 +302,14 @@ class Context:
         self.handle = ctypes.c_void_p()
         check(lib().rt_ctx_create(device, ctypes.byref(self.handle)), "rt_ctx_create")
         self.device = device
+
+    def check(self) -> int:
+        """RT_FLAG_* bits the traversal kernels recorded since the last check (then cleared):
+        FLAG_BVH_STACK, a ray-bvh block's traversal stack would have overflowed; FLAG_PAIR_CAP,
+        a block's pair queue was full.  0 means every trace completed its traversal."""
+        f = ctypes.c_uint32(0)
+        check(lib().rt_ctx_check(self.handle, ctypes.byref(f)), "rt_ctx_check")
+        return f.value
 
     def close(self):
         if self.handle:

@@ -389,6 +389,7 @@ int fill_item(const RhJob& jb, RhItem& it) {
   it.tmts = jb.s->d_tmts;
   it.tleaf = jb.s->d_tleaf;
   it.ntri = jb.s->ntri;
+  it.flags = jb.s->ctx->d_flags;
   it.nclusters = bvh ? (jb.s->ntri ? jb.s->nrclusters : 0) : jb.s->nclusters;
   if (bvh) {
     it.cbox = jb.s->d_rcbox;
@@ -619,14 +620,34 @@ int rt_ctx_create(int32_t device, rt_ctx** out) {
   RT_TRY(hipSetDevice(device));
   // the allocation-free trace's scratch: its address on this device, looked up once
   if (noalloc_setup(device)) return RT_ERR_HIP;
-  rt_ctx* c = new (std::nothrow) rt_ctx{device};
+  rt_ctx* c = new (std::nothrow) rt_ctx{device, nullptr};
   if (!c) return RT_ERR_OOM;
+  hipError_t e = hipMalloc((void**)&c->d_flags, 256);
+  if (e == hipSuccess) e = hipMemset(c->d_flags, 0, 256);
+  if (e != hipSuccess) {
+    (void)hipFree(c->d_flags);
+    delete c;
+    return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+  }
   *out = c;
   return RT_OK;
 }
 
 int rt_ctx_destroy(rt_ctx* c) {
+  if (c) {
+    (void)hipSetDevice(c->device);
+    (void)hipFree(c->d_flags);
+  }
   delete c;
+  return RT_OK;
+}
+
+int rt_ctx_check(rt_ctx* c, uint32_t* flags) {
+  if (!c || !flags) return RT_ERR_INVALID;
+  RT_TRY(hipSetDevice(c->device));
+  RT_TRY(hipDeviceSynchronize());
+  RT_TRY(hipMemcpy(flags, c->d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  RT_TRY(hipMemset(c->d_flags, 0, sizeof(uint32_t)));
   return RT_OK;
 }
 

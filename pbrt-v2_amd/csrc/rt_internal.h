@@ -7,6 +7,7 @@
 
 struct rt_ctx {
   int device;
+  uint32_t* d_flags;  // RT_FLAG_* recorded by the traversal kernels (rt_ctx_check)
 };
 struct rt_scene;
 
@@ -93,6 +94,7 @@ struct RhItem {
   uint32_t* repkeys;     // primary with repair: nearest repair-region t key per ray, else null
   uint8_t* occ;          // occlusion flags
   HitOut out;            // closest-hit outputs
+  uint32_t* flags;       // the context's RT_FLAG_* word
   uint32_t block_base, ray_base;
   int32_t repair_run, pad;
 };

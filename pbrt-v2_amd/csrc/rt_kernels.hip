@@ -29,6 +29,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "../../include/pbrt_amd_rt.h"
 #include "rt_device.h"
 #include "rt_internal.h"
 
@@ -1148,7 +1149,8 @@ __device__ __forceinline__ void bvh_leaf_rays(const RHGeom& g, const RhItem& it,
 // Appends the (BVH leaf, hierarchy leaf) pair of a reached leaf x leaf pair; its primitives
 // are tested together (only the first one for secondary rays, intersection6DB_BVH2.cl).
 template <int MODE, class SH>
-__device__ __forceinline__ void bvh_emit(SH& sh, uint32_t b, uint32_t off, uint32_t np, int li) {
+__device__ __forceinline__ void bvh_emit(SH& sh, uint32_t b, uint32_t off, uint32_t np, int li,
+                                         uint32_t* flags) {
   const int e = atomicAdd(&sh.sPq, 1);
   if (e < kPairCap) {  // room for every emission of one round is kept free
     sh.sPn[e] = (b << 6) | (uint32_t)li;
@@ -1156,6 +1158,8 @@ __device__ __forceinline__ void bvh_emit(SH& sh, uint32_t b, uint32_t off, uint3
       sh.sPt[e] = off;
       sh.sPc[e] = np;
     }
+  } else {
+    atomicOr(flags, RT_FLAG_PAIR_CAP);  // reported by rt_ctx_check
   }
 }
 
@@ -1229,7 +1233,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
           bvh_box(glb(it.bvh), node, bmin, bmax, off, np);
           if (bvh_live<NODE>(off, np) && bvh_bound<NODE>(sh.sN[0], bmin, bmax)) {
             if (top == 0 && np > 0) {
-              bvh_emit<MODE>(sh, node, off, np, 0);
+              bvh_emit<MODE>(sh, node, off, np, 0, it.flags);
             } else {
               const int slot = atomicAdd(&sh.sTq, 1);
               sh.sT[slot] = node << 7;  // at most 256 seeds on an empty stack
@@ -1251,7 +1255,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
           int npop = min(kBvhPop, q);
           while (npop > 0 && q - npop + 8 * npop > kBvhQCap) --npop;
           if (npop == 0) {  // cannot happen for subtrees of depth <= 12 (checked on the host)
-            if (tid == 0) atomicOr(&ctr[3], 1u);
+            if (tid == 0) atomicOr(it.flags, RT_FLAG_BVH_STACK);  // reported by rt_ctx_check
             break;
           }
           uint32_t item = 0;
@@ -1296,7 +1300,7 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
                 bvh_box(glb(it.bvh), bc, cmin, cmax, coff, cnp);
                 if (bvh_live<NODE>(coff, cnp) && bvh_bound<NODE>(sh.sN[Rn], cmin, cmax)) {
                   if (sub_level(Rn) == top && cnp > 0)
-                    bvh_emit<MODE>(sh, bc, coff, cnp, Rn - pow4_off(top));
+                    bvh_emit<MODE>(sh, bc, coff, cnp, Rn - pow4_off(top), it.flags);
                   else {
                     push = true;
                     child = (bc << 7) | (uint32_t)Rn;

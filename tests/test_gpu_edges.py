@@ -1,8 +1,8 @@
 """GPU edge cases through the C ABI, each against the oracle: empty scenes, single-ray and
 all-invalid images, exactly coincident triangles (tie -> smallest id), degenerate triangles,
 rays starting on a surface, repairRun 2, ray-bvh leaf sizes 2-4 with the first-primitive
-rule of secondary rays, BVH heights 1 and 6, and a scene of more than 60,000 triangles (two
-BVH parts)."""
+rule of secondary rays, BVH heights 1 and 6, a scene of more than 60,000 triangles (two
+BVH parts), and hand-built subtrees at the 12-level depth limit of the ray-bvh traversal."""
 import numpy as np
 import pytest
 
@@ -142,3 +142,83 @@ def test_raybvh_two_parts(rt, scenes, orc, gpu_ctx):
     assert np.array_equal(ids, _ref(orc, sc, v9, rays, orc.params("ray-bvh"),
                                     orc.CLOSEST_PRIMARY)["id"])
     sc.close()
+
+
+
+def _comb_bvh(rt, v9, order, nleaves):
+    """Two comb-shaped subtrees of nleaves leaves each: interior node 2k has the leaf 2k+1 and
+    the node 2k+2 as children, so the last two leaves sit nleaves-1 levels below their root.
+    One triangle per leaf, BVH position p holding triangle order[p]; each box is the exact
+    union of the triangle bounds below it."""
+    tri = v9.reshape(-1, 9)
+    lo = np.minimum(np.minimum(tri[:, 0:3], tri[:, 3:6]), tri[:, 6:9])
+    hi = np.maximum(np.maximum(tri[:, 0:3], tri[:, 3:6]), tri[:, 6:9])
+    nodes = np.zeros(2 * (2 * nleaves - 1), rt.GPU_NODE_DTYPE)
+    roots = []
+    for c in range(2):
+        nb = c * (2 * nleaves - 1)
+        roots.append(nb)
+        box = {}
+        for k in range(nleaves):  # leaf k at 2k+1, the last one at 2(nleaves-1)
+            idx = nb + (2 * k + 1 if k < nleaves - 1 else 2 * k)
+            pos = c * nleaves + k
+            box[idx] = (lo[order[pos]], hi[order[pos]])
+            nodes[idx] = (*box[idx][0], *box[idx][1], pos, 1)
+        for k in range(nleaves - 2, -1, -1):
+            a, b = box[nb + 2 * k + 1], box[nb + 2 * k + 2]
+            box[nb + 2 * k] = (np.minimum(a[0], b[0]), np.maximum(a[1], b[1]))
+            nodes[nb + 2 * k] = (*box[nb + 2 * k][0], *box[nb + 2 * k][1], nb + 2 * k + 1, 0)
+    return nodes, np.array(roots, np.uint32)
+
+
+def _comb_scene(nleaves, seed):
+    g = np.random.default_rng(seed)
+    n = 2 * nleaves
+    centre = np.concatenate([g.uniform(-0.8, 0.8, (n, 2)), g.uniform(1.0, 3.0, (n, 1))], 1)
+    v9 = (centre[:, None, :] + g.uniform(-0.6, 0.6, (n, 3, 3))).astype(np.float32)
+    return v9.reshape(-1, 9), g.permutation(n).astype(np.uint32)
+
+
+@pytest.mark.parametrize("node", ["box_dir", "ia", "box_uv"])
+def test_raybvh_depth_limit_subtrees(rt, scenes, orc, gpu_ctx, node):
+    """ray-bvh over frontier subtrees 12 levels deep, the deepest rt_scene_set_bvh accepts:
+    the block traversal's bounded stack and pair queue hold, so the per-ray, block and
+    default paths agree with the oracle bit for bit and the context's status word
+    (rt_ctx_check) stays 0.  Subtrees 13 levels deep are rejected."""
+    v9, order = _comb_scene(13, seed=71)
+    nodes, roots = _comb_bvh(rt, v9, order, 13)
+    gpu_ctx.check()  # clear what earlier traces recorded
+    sc = rt.Scene(gpu_ctx, v9, desc=rt.default_desc("ray-bvh", node=node), build_bvh=False)
+    sc.set_bvh(nodes, roots, order)
+    p = orc.params("ray-bvh", node=node)
+    cam = scenes.pinhole_rays(64, 64, origin=(0.0, 0.0, -3.0), fov_deg=24)
+    # incoherent secondary rays: camera directions in random order from jittered origins
+    g = np.random.default_rng(72)
+    perm = g.permutation(64 * 64)
+    rnd = {k: (v[perm].copy() if isinstance(v, np.ndarray) else v) for k, v in cam.items()}
+    for k in ("ox", "oy", "oz"):
+        rnd[k] = (rnd[k] + g.uniform(-0.3, 0.3, 64 * 64)).astype(np.float32)
+    rnd["valid"] = (g.random(64 * 64) < 0.9).astype(np.uint8)
+    for thr in (1, rt.RAY_PATH_OFF, 0):  # per-ray, block, the default choice
+        sc.set_param(rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, thr)
+        sc.set_param(rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED, thr)
+        for mode, rays in ((rt.RT_CLOSEST_PRIMARY, cam), (rt.RT_CLOSEST_SECONDARY, rnd)):
+            ids, ts = _closest(sc, rt, rays, mode)
+            ref = orc.trace_raybvh(v9, nodes, roots, order, rays, p, mode)
+            assert np.array_equal(ids, ref["id"]), (thr, mode)
+            assert np.array_equal(ts.view(np.uint32), ref["t"].view(np.uint32)), (thr, mode)
+            assert (ids >= 0).sum() > 300, (thr, mode)
+        ids, ts = _closest(sc, rt, cam, rt.RT_CLOSEST_PRIMARY)
+        sh = scenes.shadow_rays_np(cam, ids, ts, (0.3, 2.0, -1.0))
+        occ = sc.trace_occluded(_dev(rt, sh)).cpu().numpy()
+        ref = orc.trace_raybvh(v9, nodes, roots, order, sh, p, orc.OCCLUDED)["occ"]
+        assert np.array_equal(occ, ref), thr
+        assert 0 < ref.sum() < (ids >= 0).sum()
+        assert gpu_ctx.check() == 0, thr
+    sc.close()
+    v9d, orderd = _comb_scene(14, seed=73)
+    deep = rt.Scene(gpu_ctx, v9d, desc=rt.default_desc("ray-bvh", node=node), build_bvh=False)
+    with pytest.raises(rt.RtError) as e:
+        deep.set_bvh(*_comb_bvh(rt, v9d, orderd, 14), orderd)
+    assert e.value.status == 4  # RT_ERR_UNSUPPORTED
+    deep.close()
