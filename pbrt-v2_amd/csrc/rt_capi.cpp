@@ -374,8 +374,8 @@ int fill_item(const RhJob& jb, RhItem& it) {
   if (rh_geometry(jb.r.W, jb.r.H, jb.s->desc.chunk_x, jb.s->desc.chunk_y, jb.s->desc.height,
                   &it.g))
     return RT_ERR_UNSUPPORTED;
-  // LDS-staged blocks hold at most 1024 rays; queue entries pack the triangle slot in 26 bits
-  if (it.g.rb > 1024 || jb.s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
+  // queue entries pack the triangle slot in 26 bits
+  if (jb.s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
   if (bvh && jb.s->ntri > 0 && !jb.s->has_bvh) return RT_ERR_STATE;
   it.r = jb.r;
   it.v9 = jb.s->d_v9;
@@ -473,7 +473,10 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
       it.ray_base = (uint32_t)rays;
       blocks += (uint64_t)it.g.ntop;
       rays += n;
-      if (it.g.rb > 256 || it.g.L > 3) L.big = true;
+      // the 1024-ray unit kernels, for blocks beyond 256 rays or 3 levels that the unit
+      // kernels can take (larger blocks take the per-ray path)
+      if (it.g.rb <= kMaxBlockRays && it.g.L <= kUnitLevels && (it.g.rb > 256 || it.g.L > 3))
+        L.big = true;
       if (it.repair_run > 0) L.repair = true;
       L.max_units += (size_t)it.g.ntop * ((it.nclusters + L.per - 1) / L.per);
       L.max_chunks += (size_t)it.g.ntop * (size_t)((it.g.rb + 63) / 64);
@@ -591,7 +594,7 @@ int scene_check_image(const rt_scene* s, int W, int H) {
   RHGeom g;
   if (rh_geometry(W, H, s->desc.chunk_x, s->desc.chunk_y, s->desc.height, &g))
     return RT_ERR_UNSUPPORTED;
-  if (g.rb > 1024 || s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
+  if (s->ntri >= (1u << 26)) return RT_ERR_UNSUPPORTED;
   return RT_OK;
 }
 }  // namespace rti

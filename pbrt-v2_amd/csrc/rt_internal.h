@@ -21,10 +21,15 @@ uint32_t scene_nclusters(const rt_scene* s);
 // batch would return.
 int scene_check_image(const rt_scene* s, int W, int H);
 
-constexpr int kMaxLevels = 4;   // leaves per block <= 64 (uint64 masks)
-constexpr int kMaxSub = 85;     // 1 + 4 + 16 + 64 nodes in a block subtree
+constexpr int kMaxLevels = 12;  // hierarchy levels of a ray image
+constexpr int kUnitLevels = 4;  // levels of the blocks the unit kernels and the LDS build take
+                                // (leaves per block <= 64, uint64 masks); deeper blocks are
+                                // built in global memory and take the per-ray path
+constexpr int kMaxSub = 85;     // 1 + 4 + 16 + 64 nodes in such a block's subtree
 constexpr int kThreads = 256;   // work-group size of every traversal kernel
-constexpr int kMaxRpt = 16;     // rays per thread (block of <= 4096 rays)
+constexpr int kMaxBlockRays = 1024;      // blocks the unit kernels stage in LDS; larger
+                                         // blocks always take the per-ray path
+constexpr int64_t kMaxRaysPerBlock = 1 << 24;  // top-level blocks of at most 2^24 rays
 constexpr int kClusterSize = 64;  // triangles per Morton cluster
 
 // Ray batch in device memory (structure of arrays, row-major ray image).

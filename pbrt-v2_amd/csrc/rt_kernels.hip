@@ -172,6 +172,25 @@ __device__ __forceinline__ void leaf_merge(float* a, bool& ha, bool& ra, const f
   ra = ra || rb;
 }
 
+// Box of the leaf whose first ray is (x0, y0): its rays merged one by one in the reference's
+// read order (row by row, left to right); zeros and false when no ray is traced.
+template <bool USE_VALID, bool UV>
+__device__ bool leaf_box(const RHGeom& g, const RaysDev& r, int x0, int y0, float* v) {
+  bool have = false, restart = false;
+  for (int y = y0; y < min(g.H, y0 + g.cy); ++y)
+    for (int x = x0; x < min(g.W, x0 + g.cx); ++x) {
+      const size_t i = (size_t)y * g.W + x;
+      if (USE_VALID && r.valid != nullptr && r.valid[i] == 0) continue;
+      float b[12];
+      bool rb;
+      ray_box<USE_VALID, UV>(r, i, b, rb);
+      leaf_merge(v, have, restart, b, true, rb);
+    }
+  if (!have)
+    for (int k = 0; k < 12; ++k) v[k] = 0.f;
+  return have;
+}
+
 // ---------------------------------------------------------------------------------------
 // Fused hierarchy build: one work-group builds the leaves and every level of one block
 // (rayhconstruct6DB.cl / rayhconstructP6DB.cl / rayhconstruct(P)5DB.cl + levelConstruct6DB.cl /
@@ -227,20 +246,8 @@ __device__ void build_block(const RHGeom& g, const RaysDev& r, float4* __restric
   } else {
     for (int li = tid; li < nleaf; li += kThreads) {
       float v[12];
-      bool have = false, restart = false;
-      int lk = li % g.nlx, lj = li / g.nlx;
-      int x0 = K * g.bw + lk * g.cx, y0 = J * g.bh + lj * g.cy;
-      for (int y = y0; y < min(g.H, y0 + g.cy); ++y)
-        for (int x = x0; x < min(g.W, x0 + g.cx); ++x) {
-          size_t i = (size_t)y * g.W + x;
-          if (USE_VALID && r.valid != nullptr && r.valid[i] == 0) continue;
-          float b[12];
-          bool rb;
-          ray_box<USE_VALID, UV>(r, i, b, rb);
-          leaf_merge(v, have, restart, b, true, rb);
-        }
-      if (!have)
-        for (int k = 0; k < 12; ++k) v[k] = 0.f;
+      const bool have = leaf_box<USE_VALID, UV>(g, r, K * g.bw + (li % g.nlx) * g.cx,
+                                                J * g.bh + (li / g.nlx) * g.cy, v);
       for (int k = 0; k < 12; ++k) sBox[leaf_off + li][k] = v[k];
       sVal[leaf_off + li] = have ? 1 : 0;
     }
@@ -297,6 +304,49 @@ __device__ __forceinline__ int find_ray_item(const RhItem* __restrict__ items, i
   return lo;
 }
 
+// Build of a block of more than kUnitLevels levels (heights of 5 and more where the image
+// allows them): its subtree does not fit the LDS of build_block, so the leaves and every level
+// go straight to the node array, and each level reads its four children back from it (the
+// work-group's own writes, ordered by the barrier between levels).  Children outside a level's
+// grid are skipped, as build_block skips the invalid ones; the merge order is the same.
+template <bool USE_VALID, bool UV>
+__device__ void build_block_deep(const RHGeom& g, const RaysDev& r, float4* __restrict__ nodes,
+                                 int b) {
+  const int tid = threadIdx.x;
+  const int top = g.L - 1;
+  const int K = b % g.gw[top], J = b / g.gw[top];
+  for (int li = tid; li < g.nlx * g.nlx; li += kThreads) {
+    const int gk = K * g.nlx + li % g.nlx, gj = J * g.nlx + li / g.nlx;
+    if (gk >= g.gw[0] || gj >= g.gh[0]) continue;
+    float v[12];
+    const bool have = leaf_box<USE_VALID, UV>(g, r, gk * g.cx, gj * g.cy, v);
+    node_store(nodes, (size_t)g.off[0] + (size_t)gj * g.gw[0] + gk, v, have);
+  }
+  __syncthreads();
+  for (int l = 1; l < g.L; ++l) {
+    const int nl = g.nlx >> l;
+    for (int i = tid; i < nl * nl; i += kThreads) {
+      const int gk = K * nl + i % nl, gj = J * nl + i / nl;
+      if (gk >= g.gw[l] || gj >= g.gh[l]) continue;
+      float v[12];
+      bool have = false;
+      for (int a = 0; a < 4; ++a) {
+        const int ck = 2 * gk + (a & 1), cj = 2 * gj + (a >> 1);
+        if (ck >= g.gw[l - 1] || cj >= g.gh[l - 1]) continue;
+        const float4* p = nodes + 4 * ((size_t)g.off[l - 1] + (size_t)cj * g.gw[l - 1] + ck);
+        const float4 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
+        const float c[12] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z,
+                             p2.x, p2.y, p2.z, p3.x, p3.y, p3.z};
+        box_combine(v, have, c, p0.w != 0.f);
+      }
+      if (!have)
+        for (int k = 0; k < 12; ++k) v[k] = 0.f;
+      node_store(nodes, (size_t)g.off[l] + (size_t)gj * g.gw[l] + gk, v, have);
+    }
+    __syncthreads();
+  }
+}
+
 template <bool USE_VALID, bool UV>
 __global__ __launch_bounds__(256) void rh_group_build_kernel(const RhItem* __restrict__ items,
                                                              int nitems) {
@@ -304,8 +354,11 @@ __global__ __launch_bounds__(256) void rh_group_build_kernel(const RhItem* __res
   __shared__ int sVal[kMaxSub];
   const int k = find_block_item(items, nitems, blockIdx.x);
   const RhItem& it = items[k];
-  build_block<USE_VALID, UV>(it.g, it.r, it.nodes, (int)(blockIdx.x - it.block_base), sBox,
-                             sVal);
+  if (it.g.L > kUnitLevels)
+    build_block_deep<USE_VALID, UV>(it.g, it.r, it.nodes, (int)(blockIdx.x - it.block_base));
+  else
+    build_block<USE_VALID, UV>(it.g, it.r, it.nodes, (int)(blockIdx.x - it.block_base), sBox,
+                               sVal);
 }
 
 // Clears the reduction state of every job: keys, repair keys, occlusion flags, counters.
@@ -339,7 +392,7 @@ __device__ __forceinline__ void load_subtree(const RHGeom& g, int K, int J, P no
     // selected per lane, not loaded per lane one after another
     int gw = 0, gh = 0, off = 0;
 #pragma unroll
-    for (int q = 0; q < kMaxLevels; ++q) {
+    for (int q = 0; q < kUnitLevels; ++q) {
       if (q == l) {
         gw = g.gw[q];
         gh = g.gh[q];
@@ -480,7 +533,7 @@ struct RhShared {
   PairState<MODE> sPair[kQ0Cap];  // phase C pair state
   uint32_t sQ0[kQ0Cap];
   uint32_t sQ[LMAX > 1 ? LMAX - 1 : 1][kQCap];  // level s >= 1 survivors live in sQ[s - 1]
-  int sCnt[kMaxLevels];
+  int sCnt[kUnitLevels];
   int sSeg[kThreads / 64];  // phase C: flagged pairs of each wavefront's quarter of the list
   int sTraced, sOccluded;
 };
@@ -696,8 +749,10 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
 // as many, or whose leaves' ray directions spread by more than kIncoherent on average, is an
 // incoherent ray group: its candidate set
 // holds most (leaf, triangle) pairs, so it is queued for the per-ray path instead, as 64-ray
-// chunks {job, block << 4 | chunk}.  ray_clusters == 0 queues every block with a traced ray
-// that way without scanning the clusters.
+// chunks {job, block * chunks per block + chunk}.  ray_clusters == 0 queues every block with a
+// traced ray that way without scanning the clusters, and so does a block of more than
+// kMaxBlockRays rays or kUnitLevels levels (the unit kernels stage at most that much; the
+// reference's own scenes use such blocks, e.g. 100x4 leaves at height 3 or 4).
 constexpr float kIncoherent = 0.25f;
 template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
@@ -724,13 +779,13 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   root.dmax = {a1.x, a1.y, a1.z};
   root.omin = {a2.x, a2.y, a2.z};
   root.omax = {a3.x, a3.y, a3.z};
-  if (ray_clusters == 0) {
-    // every block to the per-ray path, without the cluster scan (no cluster list needed)
+  if (ray_clusters == 0 || g.rb > kMaxBlockRays || g.L > kUnitLevels) {
+    // to the per-ray path without the cluster scan (no cluster list needed)
     const int nch = (g.rb + 63) / 64;
     if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
     __syncthreads();
     for (int q = tid; q < nch; q += kThreads)
-      rq[sBase + q] = make_uint2((uint32_t)k, ((uint32_t)b << 4) | (uint32_t)q);
+      rq[sBase + q] = make_uint2((uint32_t)k, (uint32_t)b * (uint32_t)nch + (uint32_t)q);
     return;
   }
   if (tid == 0) {
@@ -782,7 +837,7 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
     if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
     __syncthreads();
     for (int q = tid; q < nch; q += kThreads)
-      rq[sBase + q] = make_uint2((uint32_t)k, ((uint32_t)b << 4) | (uint32_t)q);
+      rq[sBase + q] = make_uint2((uint32_t)k, (uint32_t)b * (uint32_t)nch + (uint32_t)q);
     return;
   }
   const int nu = (cnt + per_unit - 1) / per_unit;
@@ -811,7 +866,7 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
     const uint32_t u = atomicAdd(&ctr[1], 1u);
     sUnit = u < nunits ? units[u] : make_uint4(kNoUnit, 0u, 0u, 0u);
   }
-  if (tid < kMaxLevels) sh.sCnt[tid] = 0;
+  if (tid < kUnitLevels) sh.sCnt[tid] = 0;
   if (tid == 0) {
     sh.sTraced = 0;
     sh.sOccluded = 0;
@@ -914,7 +969,7 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
       sh.sTraced = 0;
       sh.sOccluded = 0;
     }
-    if (tid < kMaxLevels) sh.sCnt[tid] = 0;
+    if (tid < kUnitLevels) sh.sCnt[tid] = 0;
     __syncthreads();
   }
 }
@@ -1527,8 +1582,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
         }
         const uint2 w = rq[e];
         cjob = __builtin_amdgcn_readfirstlane(w.x);
-        cblk = __builtin_amdgcn_readfirstlane(w.y) >> 4;
-        cpart = __builtin_amdgcn_readfirstlane(w.y) & 15u;
+        const uint32_t c = __builtin_amdgcn_readfirstlane(w.y);
+        const uint32_t nch = ((uint32_t)items[cjob].g.rb + 63u) / 64u;
+        cblk = c / nch;
+        cpart = c - cblk * nch;
         cslot = 0;
       }
       const uint32_t avail = 64u - cslot;
@@ -2270,7 +2327,8 @@ hipError_t launch_write_table(const RhItem& item, RhItem* dst, hipStream_t s) {
 
 int rh_geometry(int W, int H, int cx, int cy, int height, RHGeom* g) {
   if (W <= 0 || H <= 0 || cx <= 0 || cy <= 0 || height <= 0) return 1;
-  int gw = (W + cx - 1) / cx, gh = (H + cy - 1) / cy;
+  if (cx > (1 << 24) || cy > (1 << 24)) return 3;
+  int gw = (int)(((int64_t)W + cx - 1) / cx), gh = (int)(((int64_t)H + cy - 1) / cy);
   g->W = W;
   g->H = H;
   g->cx = cx;
@@ -2304,13 +2362,13 @@ int rh_geometry(int W, int H, int cx, int cy, int height, RHGeom* g) {
   }
   g->nnodes = off;
   g->nlx = 1 << (L - 1);
+  if ((int64_t)cx * g->nlx * cy * g->nlx > kMaxRaysPerBlock) return 3;
   g->bw = cx * g->nlx;
   g->bh = cy * g->nlx;
   g->rb = g->bw * g->bh;
   g->g = cx * cy;
   g->nsub = ((1 << (2 * L)) - 1) / 3;
   g->ntop = g->gw[L - 1] * g->gh[L - 1];
-  if (g->rb > kThreads * kMaxRpt) return 3;
   return 0;
 }
 

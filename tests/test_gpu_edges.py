@@ -2,7 +2,8 @@
 all-invalid images, exactly coincident triangles (tie -> smallest id), degenerate triangles,
 rays starting on a surface, repairRun 2, ray-bvh leaf sizes 2-4 with the first-primitive
 rule of secondary rays, BVH heights 1 and 6, a scene of more than 60,000 triangles (two
-BVH parts), and hand-built subtrees at the 12-level depth limit of the ray-bvh traversal."""
+BVH parts), hand-built subtrees at the 12-level depth limit of the ray-bvh traversal, and the
+block shapes of the reference's own scene files (top-level blocks of 1,600-6,400 rays)."""
 import numpy as np
 import pytest
 
@@ -222,3 +223,64 @@ def test_raybvh_depth_limit_subtrees(rt, scenes, orc, gpu_ctx, node):
         deep.set_bvh(*_comb_bvh(rt, v9d, orderd, 14), orderd)
     assert e.value.status == 4  # RT_ERR_UNSUPPORTED
     deep.close()
+
+
+@pytest.mark.parametrize("accel,node,W,H,cx,cy,height", [
+    # prt-teapot.pbrt / sibenik.pbrt: 100x4 leaves, height 4, at 128x128 (blocks of 200x8)
+    ("rayhierarchy", "ia", 128, 128, 100, 4, 4),
+    ("rayhierarchy", "box_uv", 128, 128, 100, 4, 4),
+    # conference.pbrt: 100x4 leaves, height 3 (three levels here: blocks of 400x16)
+    ("rayhierarchy", "box_dir", 320, 64, 100, 4, 3),
+    # the ray-bvh lines of the same files: 32x2 leaves, height 5 (four levels: 256x16)
+    ("ray-bvh", "box_dir", 256, 64, 32, 2, 5),
+    # prt-teapot.pbrt's ray-bvh line: 2x2 leaves, height 5, BVHheight 5 (five levels)
+    ("ray-bvh", "ia", 128, 128, 2, 2, 5),
+    # six levels of 2x2 leaves (blocks of 64x64 rays)
+    ("rayhierarchy", "box_dir", 128, 128, 2, 2, 6),
+])
+def test_reference_scene_block_shapes(rt, scenes, orc, gpu_ctx, accel, node, W, H, cx, cy,
+                                      height):
+    """Top-level blocks of more than 1,024 rays or more than four levels, as the reference's
+    scene files configure them: they are built in global memory where deeper than four levels,
+    take the per-ray path whatever the routing thresholds say, and match the oracle in every
+    mode (primary with repair, secondary with validity, occlusion), alone and batched with a
+    job of small blocks."""
+    v9 = scenes.room_scene(nclusters=40)
+    extra = {"bvh_height": 5} if (accel == "ray-bvh" and cx == 2) else {}
+    desc = rt.default_desc(accel, node=node, chunk_x=cx, chunk_y=cy, height=height, **extra)
+    sc = rt.Scene(gpu_ctx, v9, desc=desc)
+    p = orc.params(accel, node=node, cx=cx, cy=cy, height=height)
+    cam = scenes.pinhole_rays(W, H, origin=(0.0, 0.0, -9.0), fov_deg=50)
+    rnd = scenes.random_rays(W * H, seed=31, lo=-4, hi=4, maxt=np.inf, width=W)
+    rnd["valid"] = (np.arange(W * H) % 7 != 0).astype(np.uint8)
+    refs = {mode: _ref(orc, sc, v9, rays, p, mode)
+            for mode, rays in ((rt.RT_CLOSEST_PRIMARY, cam), (rt.RT_CLOSEST_SECONDARY, rnd))}
+    sh = scenes.shadow_rays_np(cam, refs[0]["id"], refs[0]["t"], (1.0, 2.5, -8.0))
+    ref_occ = _ref(orc, sc, v9, sh, p, orc.OCCLUDED)["occ"]
+    assert ref_occ.sum() > 500
+    for thr in (0, rt.RAY_PATH_OFF):
+        sc.set_param(rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, thr)
+        sc.set_param(rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED, thr)
+        for mode, rays in ((rt.RT_CLOSEST_PRIMARY, cam), (rt.RT_CLOSEST_SECONDARY, rnd)):
+            ids, ts = _closest(sc, rt, rays, mode)
+            assert np.array_equal(ids, refs[mode]["id"]), (thr, mode)
+            assert np.array_equal(ts.view(np.uint32), refs[mode]["t"].view(np.uint32)), (thr, mode)
+        occ = sc.trace_occluded(_dev(rt, sh)).cpu().numpy()
+        assert np.array_equal(occ, ref_occ), thr
+    sc.set_param(rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, 0)
+    sc.set_param(rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED, 0)
+    # one batch: this job and a job of 4x4 leaves (256-ray blocks) in the same launch group
+    small = rt.Scene(gpu_ctx, v9, desc=rt.default_desc(accel, node=node))
+    b = rt.Batch(gpu_ctx)
+    dcam = _dev(rt, cam)
+    h_big = b.add_closest(sc, dcam, rt.RT_CLOSEST_PRIMARY)
+    h_small = b.add_closest(small, dcam, rt.RT_CLOSEST_PRIMARY)
+    b.run()
+    import torch
+    torch.cuda.synchronize()
+    ref_small = _ref(orc, small, v9, cam, orc.params(accel, node=node), orc.CLOSEST_PRIMARY)
+    assert np.array_equal(h_big.id.cpu().numpy(), refs[0]["id"])
+    assert np.array_equal(h_small.id.cpu().numpy(), ref_small["id"])
+    b.close()
+    small.close()
+    sc.close()

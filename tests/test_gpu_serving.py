@@ -87,22 +87,28 @@ def test_budget_admission(rt, scenes, gpu_ctx):
 
 
 def test_bad_request_fails_alone(rt, scenes, gpu_ctx):
-    """A request its scene cannot trace (16x16-ray leaves: 4,096-ray blocks) is refused at
-    submit and touches no other request; a naive scene is refused at server creation."""
+    """A request whose buffers exceed the server's pool is refused at submit (RT_ERR_OOM) and
+    touches no other request; a naive scene is refused at server creation.  A scene of
+    16x16-ray leaves (4,096-ray blocks, per-ray path) is served like any other."""
     good = rt.Scene(gpu_ctx, scenes.random_triangles(1024, seed=1), desc="rayhierarchy")
     big = rt.Scene(gpu_ctx, scenes.random_triangles(1024, seed=1),
                    desc=rt.default_desc("rayhierarchy", chunk_x=16, chunk_y=16))
     naive = rt.Scene(gpu_ctx, scenes.random_triangles(64, seed=2), desc="naive")
     with pytest.raises(rt.RtError):
         rt.Server(gpu_ctx, [good, naive])
-    srv = rt.Server(gpu_ctx, [good, big], max_active=8)
+    srv = rt.Server(gpu_ctx, [good, big], mem_budget=32 << 20, max_active=8)
     try:
         t = srv.submit(0, 64, 64, 1, outputs=True)
-        with pytest.raises(rt.RtError):
-            srv.submit(1, 64, 64, 1, outputs=True)
+        with pytest.raises(rt.RtError) as e:
+            srv.submit(1, 2048, 2048, 4)
+        assert e.value.status == 3  # RT_ERR_OOM
+        t_big = srv.submit(1, 64, 64, 1, outputs=True)
         res = srv.wait(t, timeout_s=60.0)
+        res_big = srv.wait(t_big, timeout_s=60.0)
     finally:
         srv.close()
+    ids_big = big.trace_closest(rt.DeviceRays.from_numpy(scenes.pinhole_rays(64, 64)))
+    assert np.array_equal(res_big["id"], ids_big.id.cpu().numpy())
     rays = scenes.pinhole_rays(64, 64)
     ids = good.trace_closest(rt.DeviceRays.from_numpy(rays)).id.cpu().numpy()
     assert np.array_equal(res["id"], ids)
