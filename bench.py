@@ -214,7 +214,8 @@ def bench_uneven(args, rt, scenes, torch, orc):
         ("ray_clusters_occluded", rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED),
         ("ray_grid_closest", rt.RT_PARAM_RAY_GRID_CLOSEST),
         ("ray_grid_occluded", rt.RT_PARAM_RAY_GRID_OCCLUDED),
-        ("walk_closest", rt.RT_PARAM_WALK_CLOSEST), ("walk_occluded", rt.RT_PARAM_WALK_OCCLUDED))}
+        ("walk_closest", rt.RT_PARAM_WALK_CLOSEST), ("walk_occluded", rt.RT_PARAM_WALK_OCCLUDED),
+        ("tile_closest", rt.RT_PARAM_TILE_CLOSEST), ("tile_occluded", rt.RT_PARAM_TILE_OCCLUDED))}
 
     # per-kernel time of one eager run (unit kernel and per-ray kernel of each mode)
     def eager_split():

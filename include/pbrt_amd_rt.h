@@ -215,7 +215,11 @@ int rt_make_bounce_rays(rt_scene *scene, const rt_rays *in, const int32_t *id, c
  *   of the per-ray kernel (batches only); 0 = one per resident slot.
  *   RT_PARAM_WALK_CLOSEST / RT_PARAM_WALK_OCCLUDED: v in 1..7: a round of the per-ray kernel
  *   walks the search BVH until at most 1/2^v of its lanes still look for a leaf (7: until
- *   every lane holds one); 0 = the built-in 3. */
+ *   every lane holds one); 0 = the built-in 3.
+ *   RT_PARAM_TILE_CLOSEST / RT_PARAM_TILE_OCCLUDED: LDS tile of the block path's unit kernel:
+ *   0 = by block size (256 rays and three levels where every block fits, else 1,024 rays and
+ *   four levels); 1 = the 1,024-ray tile for every block (more LDS per work-group, fewer
+ *   work-groups per CU). */
 enum {
   RT_PARAM_CLUSTERS_CLOSEST = 1,
   RT_PARAM_CLUSTERS_OCCLUDED = 2,
@@ -227,7 +231,9 @@ enum {
   RT_PARAM_RAY_GRID_CLOSEST = 8,
   RT_PARAM_RAY_GRID_OCCLUDED = 9,
   RT_PARAM_WALK_CLOSEST = 10,
-  RT_PARAM_WALK_OCCLUDED = 11
+  RT_PARAM_WALK_OCCLUDED = 11,
+  RT_PARAM_TILE_CLOSEST = 12,
+  RT_PARAM_TILE_OCCLUDED = 13
 };
 int rt_scene_set_param(rt_scene *scene, int32_t param, int32_t value);
 

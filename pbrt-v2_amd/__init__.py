@@ -44,7 +44,8 @@ EXPORTS = (
 (RT_PARAM_CLUSTERS_CLOSEST, RT_PARAM_CLUSTERS_OCCLUDED, RT_PARAM_GRID_CLOSEST,
  RT_PARAM_GRID_OCCLUDED, RT_PARAM_GRAPH, RT_PARAM_RAY_CLUSTERS_CLOSEST,
  RT_PARAM_RAY_CLUSTERS_OCCLUDED, RT_PARAM_RAY_GRID_CLOSEST,
- RT_PARAM_RAY_GRID_OCCLUDED, RT_PARAM_WALK_CLOSEST, RT_PARAM_WALK_OCCLUDED) = range(1, 12)
+ RT_PARAM_RAY_GRID_OCCLUDED, RT_PARAM_WALK_CLOSEST, RT_PARAM_WALK_OCCLUDED,
+ RT_PARAM_TILE_CLOSEST, RT_PARAM_TILE_OCCLUDED) = range(1, 14)
 # a per-ray threshold above any cluster count: every block enumerates its candidate pairs
 RAY_PATH_OFF = 1 << 30
 # kernel_time modes: + RT_TIME_PER_RAY = only the per-ray kernel, + RT_TIME_UNIT = only the

@@ -120,6 +120,7 @@ struct RhParams {
   int ray[2] = {0, 0};  // passing clusters from which a block takes the per-ray path
   int rgrid[2] = {0, 0};  // work-groups of the per-ray kernel
   int walk[2] = {0, 0};   // per-ray walk rounds (RT_PARAM_WALK_*)
+  int tile[2] = {0, 0};   // 1: the 1024-ray LDS tile for every block (RT_PARAM_TILE_*)
 };
 
 struct rt_scene {
@@ -142,6 +143,7 @@ struct rt_scene {
   int clusters_closest = 0, clusters_occluded = 0;  // 0 = built-in choice
   int ray_closest = 0, ray_occluded = 0;            // per-ray path thresholds; 0 = built-in
   int walk_closest = 0, walk_occluded = 0;          // per-ray walk rounds; 0 = built-in
+  int tile_closest = 0, tile_occluded = 0;          // unit-kernel LDS tile; 0 = by block size
   // ray-bvh handoff
   float4* d_bvh = nullptr;
   uint32_t* d_roots = nullptr;
@@ -460,7 +462,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     L.max_chunks = 0;
     L.nblocks = 0;
     L.nrays = 0;
-    L.big = false;
+    L.big = prm.tile[mi] == 1;
     L.repair = false;
     uint64_t blocks = 0, rays = 0;
     for (int j : members[gi]) {
@@ -916,6 +918,8 @@ static int trace_impl(rt_scene* s, const rt_rays* rays, int mode, const rt_hits*
     prm.ray[1] = s->ray_occluded;
     prm.walk[0] = s->walk_closest;
     prm.walk[1] = s->walk_occluded;
+    prm.tile[0] = s->tile_closest;
+    prm.tile[1] = s->tile_occluded;
     st = rh_prepare(s->ws[mode], jobs, prm, hs, groups, &changed);
     if (st) return st;
     Timer::Slot* t = s->timer.next(mode, true);
@@ -1075,6 +1079,8 @@ int rt_scene_set_param(rt_scene* s, int32_t param, int32_t value) {
   else if (param == RT_PARAM_RAY_CLUSTERS_OCCLUDED) s->ray_occluded = value;
   else if (param == RT_PARAM_WALK_CLOSEST && value <= 7) s->walk_closest = value;
   else if (param == RT_PARAM_WALK_OCCLUDED && value <= 7) s->walk_occluded = value;
+  else if (param == RT_PARAM_TILE_CLOSEST && value <= 1) s->tile_closest = value;
+  else if (param == RT_PARAM_TILE_OCCLUDED && value <= 1) s->tile_occluded = value;
   else return RT_ERR_INVALID;
   return RT_OK;
 }
@@ -1201,6 +1207,8 @@ static int* batch_param(rt_batch* b, int32_t param) {
     case RT_PARAM_RAY_GRID_OCCLUDED: return &b->prm.rgrid[1];
     case RT_PARAM_WALK_CLOSEST: return &b->prm.walk[0];
     case RT_PARAM_WALK_OCCLUDED: return &b->prm.walk[1];
+    case RT_PARAM_TILE_CLOSEST: return &b->prm.tile[0];
+    case RT_PARAM_TILE_OCCLUDED: return &b->prm.tile[1];
     default: return nullptr;
   }
 }
@@ -1221,6 +1229,8 @@ int rt_batch_set_param(rt_batch* b, int32_t param, int32_t value) {
     if (value > (1 << 30)) return RT_ERR_INVALID;
   } else if (param == RT_PARAM_WALK_CLOSEST || param == RT_PARAM_WALK_OCCLUDED) {
     if (value > 7) return RT_ERR_INVALID;
+  } else if (param == RT_PARAM_TILE_CLOSEST || param == RT_PARAM_TILE_OCCLUDED) {
+    if (value > 1) return RT_ERR_INVALID;
   } else if (value > (1 << 16)) {
     return RT_ERR_INVALID;
   }
@@ -1245,6 +1255,10 @@ int rt_batch_get_param(rt_batch* b, int32_t param, int32_t* value) {
                      : 0;
   if (param == RT_PARAM_WALK_CLOSEST || param == RT_PARAM_WALK_OCCLUDED) {
     *value = *p > 0 ? *p : kDefaultWalk;
+    return RT_OK;
+  }
+  if (param == RT_PARAM_TILE_CLOSEST || param == RT_PARAM_TILE_OCCLUDED) {
+    *value = *p;
     return RT_OK;
   }
   const bool grid = param == RT_PARAM_GRID_CLOSEST || param == RT_PARAM_GRID_OCCLUDED ||
@@ -1341,6 +1355,11 @@ int rt_batch_autotune(rt_batch* b, void* stream, int32_t reps) {
           o.per[m] = p.per[m];
           o.grid[m] = p.grid[m];
         });
+  // stage 4: the unit kernel's LDS tile (by block size, or 1024 rays for every block)
+  if (!st)
+    st = autotune_stage(
+        b, hs, reps, 2, [&](int c, RhParams& p) { p.tile[0] = p.tile[1] = c; },
+        [](int m, const RhParams& p, RhParams& o) { o.tile[m] = p.tile[m]; });
   b->timer.on = was_on;
   b->timer.used = 0;
   b->use_graph = was_graph;

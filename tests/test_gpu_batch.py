@@ -139,6 +139,21 @@ def test_batch_launch_shapes_same_results(rt, scenes, gpu_ctx):
                 _same(_np(outs[i]), ref[i], f"grid {grid} per {per} job {i}")
                 assert np.array_equal(occs[i].cpu().numpy(), ref_occ[i]), \
                     f"grid {grid} per {per} job {i} occlusion"
+    # the 1024-ray LDS tile for every block of the block path
+    for tile in (1, 0):
+        for p_ in (rt.RT_PARAM_TILE_CLOSEST, rt.RT_PARAM_TILE_OCCLUDED):
+            b.set_param(p_, tile)
+        print(f"[batch] tile {tile}", flush=True)
+        for o in outs:
+            o.id.fill_(-7)
+        for o in occs:
+            o.fill_(7)
+        b.run()
+        for i in range(len(jobs)):
+            _same(_np(outs[i]), ref[i], f"tile {tile} job {i}")
+            assert np.array_equal(occs[i].cpu().numpy(), ref_occ[i]), f"tile {tile} job {i} occ"
+    with pytest.raises(rt.RtError):
+        b.set_param(rt.RT_PARAM_TILE_CLOSEST, 2)
     # the per-ray path (every block) with each walk length and per-ray grid
     for p_ in (rt.RT_PARAM_RAY_CLUSTERS_CLOSEST, rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED):
         b.set_param(p_, 1)

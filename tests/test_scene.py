@@ -461,8 +461,14 @@ def _camera_rays(W, H, fov, c2w, screen):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("accel", ["rayhierarchy", "ray-bvh"])
-def test_scene_file_render_matches_oracle(scenes, orc, tmp_path, accel):
+@pytest.mark.parametrize("accel,accel_params,kw", [
+    ("rayhierarchy", "", {}),
+    ("ray-bvh", "", {}),
+    # the Accelerator line of prt-teapot.pbrt and sibenik.pbrt (200x8-ray blocks here)
+    ("rayhierarchy", '"integer chunkXSize" [100] "integer chunkYSize" [4] "integer height" [4] '
+     '"string node" ["ia"]', {"cx": 100, "cy": 4, "height": 4, "node": "ia"}),
+], ids=["rayhierarchy", "ray-bvh", "rayhierarchy-100x4-ia"])
+def test_scene_file_render_matches_oracle(scenes, orc, tmp_path, accel, accel_params, kw):
     """A torus scene written as a pbrt file: the reader returns its vertices bit for bit, and
     RenderScene (MakeAccelerator from the file's Accelerator, the breadth-first loop) gives
     the direct-lighting image of the oracle's hits and occlusion on the same camera rays."""
@@ -474,7 +480,7 @@ def test_scene_file_render_matches_oracle(scenes, orc, tmp_path, accel):
 Camera "perspective" "float fov" [60]
 Film "image" "integer xresolution" [{W}] "integer yresolution" [{H}]
 Sampler "stratified" "integer xsamples" [1] "integer ysamples" [1]
-Accelerator "{accel}"
+Accelerator "{accel}" {accel_params}
 Renderer "breadth-first"
 SurfaceIntegrator "directlighting"
 WorldBegin
@@ -502,7 +508,7 @@ WorldEnd
     r = _camera_rays(W, H, 60.0, c2w, floats(info["screen"]))
     light = tuple(floats(info["light"]).tolist())
     intensity = F(float(info["intensity"]))
-    p = orc.params(accel)
+    p = orc.params(accel, **kw)
     if accel == "ray-bvh":
         nodes, roots, order = orc.raybvh_handoff(v9, max_prims=1, bvh_height=3)
         trace = lambda q, m: orc.trace_raybvh(v9, nodes, roots, order, q, p, m)
