@@ -1173,23 +1173,15 @@ void parallel_for(size_t n, int nthreads, F&& fn) {
 
 // ---- per-ray path restatement ------------------------------------------------------------
 // The library traces incoherent ray groups ray by ray: each ray walks a search BVH of the
-// triangles (Morton order of centroids, radix splits, leaves of at most 4, boxes grown by
-// 2^-11 * max(1, largest |coordinate|)), runs the strict leaf test on the triangles of the
-// leaves it reaches and evaluates the reference's cull chain for the ones that pass.  This is
-// a CPU restatement of that traversal (not of the reference): its results must equal the
-// reference-semantics traces above, and its per-ray counts (interior nodes loaded, leaf
-// tests, chain culls) are the work model of the per-ray kernel.
+// triangles (binned SAH, leaves of at most 4, boxes grown by 2^-11 * max(1, largest
+// |coordinate|)), runs the strict leaf test on the triangles of the leaves it reaches and
+// evaluates the reference's cull chain for the ones that pass.  This is a CPU restatement of
+// that traversal (not of the reference): its results must equal the reference-semantics
+// traces above, and its per-ray counts (interior nodes loaded, leaf tests, chain culls) are
+// the work model of the per-ray kernel, so the tree is built by the same rules as the
+// library's.
 constexpr int kSDepth = 26;
 constexpr int kSLeaf = 4;
-
-uint32_t spread10(uint32_t x) {
-  x &= 1023u;
-  x = (x | (x << 16)) & 0x030000FFu;
-  x = (x | (x << 8)) & 0x0300F00Fu;
-  x = (x | (x << 4)) & 0x030C30C3u;
-  x = (x | (x << 2)) & 0x09249249u;
-  return x;
-}
 
 struct SBoxC {
   float lo[3], hi[3];
@@ -1266,41 +1258,172 @@ uint32_t s_build(SearchTree& T, const float* v9, const std::vector<uint32_t>& co
   return node;
 }
 
+// Binned SAH build: 16 bins on the axis of largest centroid extent, split cost nL*area(L) +
+// nR*area(R) (the first minimum), stable partitions.  A range of at most kSLeaf triangles is
+// a leaf unless its best split costs less than testing them all (node and triangle cost 1 per
+// unit area); a range whose centroids coincide on that axis, or whose SAH split would take the
+// tree past kSDepth interior levels, splits at the median by (centroid, id).
+struct SPrim {
+  float lo[3], hi[3], c[3];
+};
+constexpr float kSNodeCost = 1.f, kSTriCost = 1.f;
+
+float s_area(const SBoxC& b) {
+  const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+  return 2.f * ((dx * dy + dy * dz) + dz * dx);
+}
+
+void s_grow(SBoxC& b, const SPrim& p) {
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = std::min(b.lo[k], p.lo[k]);
+    b.hi[k] = std::max(b.hi[k], p.hi[k]);
+  }
+}
+
+void s_empty(SBoxC& b) {
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = kInf;
+    b.hi[k] = -kInf;
+  }
+}
+
+uint32_t s_build_sah(SearchTree& T, const std::vector<SPrim>& P, uint32_t a, uint32_t b, int depth,
+                     float pad, SBoxC& box) {
+  const uint32_t n = b - a;
+  uint32_t* idx = T.perm.data();
+  if (n == 1) {
+    s_empty(box);
+    for (uint32_t i = a; i < b; ++i) s_grow(box, P[idx[i]]);
+    return 0x80000000u | ((n - 1) << 27) | a;
+  }
+  float clo[3] = {kInf, kInf, kInf}, chi[3] = {-kInf, -kInf, -kInf};
+  for (uint32_t i = a; i < b; ++i)
+    for (int k = 0; k < 3; ++k) {
+      clo[k] = std::min(clo[k], P[idx[i]].c[k]);
+      chi[k] = std::max(chi[k], P[idx[i]].c[k]);
+    }
+  int ax = 0;
+  for (int k = 1; k < 3; ++k)
+    if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+  const float ext = chi[ax] - clo[ax];
+  uint32_t m = 0;
+  bool median = ext <= 0.f;
+  if (!median) {
+    constexpr int kB = 16;
+    const float scale = (float)kB / ext;
+    auto bin = [&](uint32_t t) {
+      const int k = (int)((P[t].c[ax] - clo[ax]) * scale);
+      return std::min(std::max(k, 0), kB - 1);
+    };
+    SBoxC bb[kB];
+    uint32_t cnt[kB] = {0};
+    for (int k = 0; k < kB; ++k) s_empty(bb[k]);
+    for (uint32_t i = a; i < b; ++i) {
+      const int k = bin(idx[i]);
+      s_grow(bb[k], P[idx[i]]);
+      ++cnt[k];
+    }
+    float rarea[kB];
+    uint32_t rcnt[kB];
+    SBoxC acc;
+    s_empty(acc);
+    uint32_t c = 0;
+    for (int k = kB - 1; k > 0; --k) {
+      c += cnt[k];
+      if (cnt[k]) {
+        for (int q = 0; q < 3; ++q) {
+          acc.lo[q] = std::min(acc.lo[q], bb[k].lo[q]);
+          acc.hi[q] = std::max(acc.hi[q], bb[k].hi[q]);
+        }
+      }
+      rarea[k] = c ? s_area(acc) : 0.f;
+      rcnt[k] = c;
+    }
+    s_empty(acc);
+    c = 0;
+    int best = -1;
+    float best_cost = kInf;
+    for (int k = 0; k < kB - 1; ++k) {
+      c += cnt[k];
+      if (cnt[k]) {
+        for (int q = 0; q < 3; ++q) {
+          acc.lo[q] = std::min(acc.lo[q], bb[k].lo[q]);
+          acc.hi[q] = std::max(acc.hi[q], bb[k].hi[q]);
+        }
+      }
+      if (c == 0 || rcnt[k + 1] == 0) continue;
+      const float cost = (float)c * s_area(acc) + (float)rcnt[k + 1] * rarea[k + 1];
+      if (cost < best_cost) {
+        best_cost = cost;
+        best = k;
+      }
+    }
+    if (n <= (uint32_t)kSLeaf) {
+      SBoxC all;
+      s_empty(all);
+      for (uint32_t i = a; i < b; ++i) s_grow(all, P[idx[i]]);
+      const float A = s_area(all);
+      if (best < 0 || !(kSNodeCost * A + kSTriCost * best_cost < kSTriCost * (float)n * A)) {
+        box = all;
+        return 0x80000000u | ((n - 1) << 27) | a;
+      }
+    }
+    if (best < 0) {
+      median = true;
+    } else {
+      uint32_t* mid = std::stable_partition(idx + a, idx + b,
+                                            [&](uint32_t t) { return bin(t) <= best; });
+      m = (uint32_t)(mid - idx);
+      if (depth + s_levels(m - a) > kSDepth || depth + s_levels(b - m) > kSDepth) median = true;
+    }
+  }
+  if (median && n <= (uint32_t)kSLeaf) {
+    s_empty(box);
+    for (uint32_t i = a; i < b; ++i) s_grow(box, P[idx[i]]);
+    return 0x80000000u | ((n - 1) << 27) | a;
+  }
+  if (median) {
+    std::stable_sort(idx + a, idx + b, [&](uint32_t s0, uint32_t s1) {
+      return P[s0].c[ax] < P[s1].c[ax] || (P[s0].c[ax] == P[s1].c[ax] && s0 < s1);
+    });
+    m = a + n / 2;
+  }
+  const uint32_t node = (uint32_t)T.nodes.size();
+  T.nodes.emplace_back();
+  SBoxC b0, b1;
+  const uint32_t r0 = s_build_sah(T, P, a, m, depth + 1, pad, b0);
+  const uint32_t r1 = s_build_sah(T, P, m, b, depth + 1, pad, b1);
+  SNodeC& nd = T.nodes[node];
+  for (int k = 0; k < 3; ++k) {
+    nd.b[0].lo[k] = b0.lo[k] - pad;
+    nd.b[0].hi[k] = b0.hi[k] + pad;
+    nd.b[1].lo[k] = b1.lo[k] - pad;
+    nd.b[1].hi[k] = b1.hi[k] + pad;
+    box.lo[k] = std::min(b0.lo[k], b1.lo[k]);
+    box.hi[k] = std::max(b0.hi[k], b1.hi[k]);
+  }
+  nd.ref[0] = r0;
+  nd.ref[1] = r1;
+  return node;
+}
+
 SearchTree search_tree(const float* v9, uint32_t ntri) {
   SearchTree T;
-  std::vector<float> c(3 * (size_t)ntri);
-  float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
-  for (uint32_t i = 0; i < ntri; ++i)
-    for (int a = 0; a < 3; ++a) {
-      const float* p = v9 + 9 * (size_t)i;
-      const float v = (p[a] + p[3 + a] + p[6 + a]) * (1.0f / 3.0f);
-      c[3 * (size_t)i + a] = v;
-      lo[a] = std::min(lo[a], v);
-      hi[a] = std::max(hi[a], v);
-    }
-  std::vector<uint32_t> code(ntri);
-  for (uint32_t i = 0; i < ntri; ++i) {
-    uint32_t q[3];
-    for (int a = 0; a < 3; ++a) {
-      const float ext = hi[a] - lo[a];
-      float f = ext > 0.f ? (c[3 * (size_t)i + a] - lo[a]) / ext : 0.f;
-      f = std::min(std::max(f, 0.f), 1.f);
-      q[a] = (uint32_t)(f * 1023.f);
-    }
-    code[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
-  }
   T.perm.resize(ntri);
   for (uint32_t i = 0; i < ntri; ++i) T.perm[i] = i;
-  std::stable_sort(T.perm.begin(), T.perm.end(),
-                   [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
-  std::vector<uint32_t> scode(ntri);
-  for (uint32_t i = 0; i < ntri; ++i) scode[i] = code[T.perm[i]];
-  T.tri.resize(ntri);
-  for (uint32_t i = 0; i < ntri; ++i) T.tri[i] = load_tri(v9, T.perm[i]);
   float big = 1.f;
   for (size_t i = 0; i < 9 * (size_t)ntri; ++i) big = std::max(big, std::fabs(v9[i]));
   const float pad = std::ldexp(big, -11);
   if (ntri == 0) return T;
+  std::vector<SPrim> P(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) {
+    const float* p = v9 + 9 * (size_t)i;
+    for (int k = 0; k < 3; ++k) {
+      P[i].lo[k] = std::min(std::min(p[k], p[3 + k]), p[6 + k]);
+      P[i].hi[k] = std::max(std::max(p[k], p[3 + k]), p[6 + k]);
+      P[i].c[k] = (P[i].lo[k] + P[i].hi[k]) * 0.5f;
+    }
+  }
   if (ntri <= (uint32_t)kSLeaf) {
     const uint32_t m = ntri == 1 ? 1 : ntri / 2;
     SNodeC nd;
@@ -1318,10 +1441,12 @@ SearchTree search_tree(const float* v9, uint32_t ntri) {
         nd.b[j].hi[k] += pad;
       }
     T.nodes.push_back(nd);
-    return T;
+  } else {
+    SBoxC box;
+    s_build_sah(T, P, 0, ntri, 1, pad, box);
   }
-  SBoxC box;
-  s_build(T, v9, scode, 0, ntri, 1, pad, box);
+  T.tri.resize(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) T.tri[i] = load_tri(v9, T.perm[i]);
   return T;
 }
 

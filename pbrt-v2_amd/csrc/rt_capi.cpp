@@ -132,6 +132,7 @@ struct rt_scene {
   // traversal copy for rayhierarchy: Morton-sorted triangles in clusters of 64
   float* d_v9s = nullptr;
   uint32_t* d_perm = nullptr;
+  uint32_t* d_sperm = nullptr;  // search-BVH position -> triangle id
   float4* d_cbox = nullptr;
   float4* d_tbox = nullptr;  // AABB of every sorted triangle
   float4* d_tmt = nullptr;   // ray-bvh: leaf-test operands v1, e1, e2 (3 float4 per triangle)
@@ -281,8 +282,7 @@ uint32_t morton3(const float* p, const float* lo, const float* hi) {
 // of kClusterSize with their vertex AABBs.  Only the traversal order changes: hits carry the
 // original ids through perm.
 void sort_clusters(const float* v9, uint32_t ntri, std::vector<float>& v9s,
-                   std::vector<uint32_t>& perm, std::vector<float4>& cbox,
-                   std::vector<uint32_t>& scode) {
+                   std::vector<uint32_t>& perm, std::vector<float4>& cbox) {
   std::vector<float> c(3 * (size_t)ntri);
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (uint32_t i = 0; i < ntri; ++i)
@@ -299,8 +299,6 @@ void sort_clusters(const float* v9, uint32_t ntri, std::vector<float>& v9s,
   for (uint32_t i = 0; i < ntri; ++i) perm[i] = i;
   std::stable_sort(perm.begin(), perm.end(),
                    [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
-  scode.resize(ntri);
-  for (uint32_t i = 0; i < ntri; ++i) scode[i] = code[perm[i]];
   v9s.resize(9 * (size_t)ntri);
   for (uint32_t i = 0; i < ntri; ++i)
     std::memcpy(&v9s[9 * (size_t)i], v9 + 9 * (size_t)perm[i], 9 * sizeof(float));
@@ -384,6 +382,7 @@ int fill_item(const RhJob& jb, RhItem& it) {
   it.uv6 = jb.s->d_uv6;
   it.v9s = jb.s->d_v9s;
   it.perm = jb.s->d_perm;
+  it.sperm = jb.s->d_sperm;
   it.cbox = jb.s->d_cbox;
   it.tbox = jb.s->d_tbox;
   it.tmt = jb.s->d_tmt;
@@ -680,22 +679,28 @@ int rt_scene_create(rt_ctx* ctx, const float* v9, const float* uv6, uint32_t ntr
     }
   }
   if (e == hipSuccess && desc->kind != RT_ACCEL_NAIVE && ntri) {
-    // Morton-sorted traversal copy: clusters of 64 with AABBs (rayhierarchy block path) and
-    // the search BVH with sorted leaf-test operands (per-ray path of both accelerators)
+    // Morton-sorted traversal copy: clusters of 64 with AABBs (rayhierarchy block path); the
+    // search BVH with its leaf-test operands in search order (per-ray path of both
+    // accelerators)
     std::vector<float> v9s;
-    std::vector<uint32_t> perm, scode;
+    std::vector<uint32_t> perm;
     std::vector<float4> cbox;
-    sort_clusters(v9, ntri, v9s, perm, cbox, scode);
+    sort_clusters(v9, ntri, v9s, perm, cbox);
     e = upload(s->d_v9s, v9s.data(), v9s.size());
     if (e == hipSuccess) e = upload(s->d_perm, perm.data(), perm.size());
     if (e == hipSuccess) {
       std::vector<float4> snode;
-      build_search_bvh(v9s.data(), scode.data(), ntri, snode);
+      std::vector<uint32_t> sorder;
+      build_search_bvh(v9, ntri, snode, sorder);
       e = upload(s->d_snode, snode.data(), snode.size());
-    }
-    if (e == hipSuccess) {
-      const std::vector<float4> mts = mt_operands(v9s.data(), ntri);
-      e = upload(s->d_tmts, mts.data(), mts.size());
+      if (e == hipSuccess) e = upload(s->d_sperm, sorder.data(), sorder.size());
+      if (e == hipSuccess) {
+        std::vector<float> v9o(9 * (size_t)ntri);
+        for (uint32_t i = 0; i < ntri; ++i)
+          std::memcpy(&v9o[9 * (size_t)i], v9 + 9 * (size_t)sorder[i], 9 * sizeof(float));
+        const std::vector<float4> mts = mt_operands(v9o.data(), ntri);
+        e = upload(s->d_tmts, mts.data(), mts.size());
+      }
     }
     if (e == hipSuccess && desc->kind == RT_ACCEL_RAYHIERARCHY) {
       e = upload(s->d_cbox, cbox.data(), cbox.size());
@@ -874,6 +879,7 @@ int rt_scene_destroy(rt_scene* s) {
   (void)hipFree(s->d_uv6);
   (void)hipFree(s->d_v9s);
   (void)hipFree(s->d_perm);
+  (void)hipFree(s->d_sperm);
   (void)hipFree(s->d_cbox);
   (void)hipFree(s->d_tbox);
   (void)hipFree(s->d_tmt);

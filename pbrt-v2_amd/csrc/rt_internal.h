@@ -72,6 +72,7 @@ struct RhItem {
   const float* uv6;      // or null
   const float* v9s;      // Morton-sorted traversal copy, clusters of 64
   const uint32_t* perm;  // sorted slot -> original triangle id
+  const uint32_t* sperm; // search-BVH position -> original triangle id (per-ray path)
   const float4* cbox;    // cluster AABBs (min, max)
   const float4* tbox;    // AABB (min, max) of every sorted triangle
   const float4* tmt;     // ray-bvh: leaf-test operands (v1, e1, e2) of every triangle, original
@@ -142,10 +143,11 @@ constexpr int kDefaultWalk = 3;
 #endif
 constexpr int kSearchDepth = RT_SEARCH_DEPTH;
 constexpr int kSearchLeaf = 4;
-// Builds the search BVH of `ntri` triangles (v9 in the order the leaves index) whose 30-bit
-// Morton codes `code` are sorted ascending.  Returns the node array (4 float4 per node).
-void build_search_bvh(const float* v9, const uint32_t* code, uint32_t ntri,
-                      std::vector<float4>& nodes);
+// Builds the search BVH of `ntri` triangles (v9 in the caller's order): the node array
+// (4 float4 per node) and the search order (position -> triangle index) whose consecutive
+// positions the leaves reference.
+void build_search_bvh(const float* v9, uint32_t ntri, std::vector<float4>& nodes,
+                      std::vector<uint32_t>& order);
 
 // Allocation-free trace: a static device array of kNoAllocScratch bytes per device (defined in
 // rt_kernels.hip).  noalloc_setup looks up its address on `device`, which must be current

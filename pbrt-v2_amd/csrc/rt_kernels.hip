@@ -1414,19 +1414,13 @@ __device__ __forceinline__ bool slab(V3 o, V3 inv, V3 lo, V3 hi, float tlo, floa
   return tn <= tf && tn <= thi && tf >= tlo;
 }
 
-// Hierarchy node of level l above image position (x, y), from the job's node array.
+// Hierarchy node of level l above image position (x, y), from the job's node array.  The
+// job (and so g) may differ between the lanes of a wavefront: the level's table entries are
+// read with the lane's own index, two loads instead of a select over every level.
 template <class P>
 __device__ __forceinline__ bool chain_node(const RHGeom& g, P nodes, int x, int y, int l, Node& n) {
-  int gw = 0, off = 0, cxl = 0, cyl = 0;
-#pragma unroll
-  for (int q = 0; q < kMaxLevels; ++q) {
-    if (q == l) {
-      gw = g.gw[q];
-      off = g.off[q];
-    }
-  }
-  cxl = g.cx << l;
-  cyl = g.cy << l;
+  const int gw = g.gw[l], off = g.off[l];
+  const int cxl = g.cx << l, cyl = g.cy << l;
   const P p = nodes + 4 * ((size_t)off + (size_t)(y / cyl) * gw + (x / cxl));
   const float4 a = ld4(p, 0), b = ld4(p, 1), c = ld4(p, 2), d = ld4(p, 3);
   n.dmin = {a.x, a.y, a.z};
@@ -1436,11 +1430,11 @@ __device__ __forceinline__ bool chain_node(const RHGeom& g, P nodes, int x, int 
   return a.w != 0.f;
 }
 
-// rayhierarchy: the exact cull chain of the ray's leaf for sorted triangle `slot`
+// rayhierarchy: the exact cull chain of the ray's leaf for triangle `id`
 // (intersection6DB.cl:51-119 and variants: every level from the block root to the leaf).
 template <int MODE, int NODE>
-__device__ __forceinline__ bool ray_chain_rh(const RhItem& it, int x, int y, uint32_t slot) {
-  const Tri T = load_tri(glb(it.v9s), slot);
+__device__ __forceinline__ bool ray_chain_rh(const RhItem& it, int x, int y, uint32_t id) {
+  const Tri T = load_tri(glb(it.v9), id);
   for (int l = it.g.L - 1; l >= 0; --l) {
     Node n;
     if (!chain_node(it.g, glb(it.nodes), x, y, l, n)) return false;
@@ -1640,7 +1634,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     const RhItem& it = items[jk];
     const auto snode = glb(it.snode);
     const auto tmts = glb(it.tmts);
-    const auto perm = glb(it.perm);
+    const auto perm = glb(it.sperm);
     // node phase
     for (;;) {
       if (act && cur != kDone && !(cur & kSearchLeafBit)) {
@@ -1686,10 +1680,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
         if (!__any(f < cnt)) break;
         if (f < cnt) {
           const uint32_t slot = first + f;
-          uint32_t ref = slot;
+          uint32_t ref = perm[slot];
           bool ok = true;
           if constexpr (BVH) {
-            const uint32_t bl = glb(it.tleaf)[perm[slot]];
+            const uint32_t bl = glb(it.tleaf)[ref];
             ok = bl != 0xffffffffu;
             ref = bl & 0x7fffffffu;
           }
@@ -1730,7 +1724,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
           if (!um && !ur) continue;
           bool ok;
           if constexpr (BVH) ok = ray_chain_bvh<NODE>(it, x, y, bl);
-          else ok = ray_chain_rh<MODE, NODE>(it, x, y, slot);
+          else ok = ray_chain_rh<MODE, NODE>(it, x, y, id);
           if (!ok) continue;
           if (um) best = key;
           if (ur) rep = fkey(t);

@@ -9,12 +9,16 @@
 // cull chain only for triangles that pass it.  The BVH only has to be conservative: a box
 // test may pass for a box the ray misses, never the other way round.
 //
-// The tree is built over the triangles in ascending 30-bit Morton order of their centroids
-// (the scene's sorted traversal copy), splitting each range at its highest differing code
-// bit, or at the middle when that split would make the tree deeper than kSearchDepth
-// interior levels or the codes are equal.  Leaves hold at most kSearchLeaf consecutive
-// triangles.  Nodes are stored depth first, each holding the boxes and references of its two
-// children.  Every box is the FP32 min/max of its triangles' vertices grown outward by
+// The tree is a binned SAH build over the triangles' bounding boxes: 16 bins on the axis of
+// the largest centroid extent, split cost nL * area(L) + nR * area(R) (the first minimum),
+// stable partitions.  A range of at most kSearchLeaf triangles becomes a leaf unless its best
+// split is cheaper than testing every triangle (node cost kNodeCost, triangle cost
+// kTriCost, per unit area); a range whose centroids coincide on that axis, or whose SAH
+// split would take the tree past kSearchDepth interior levels, is split at the median by
+// (centroid, id).  The build reorders the triangles: leaves hold consecutive positions of
+// that search order, which the scene keeps with the leaf-test operands laid out in it.
+// Nodes are stored depth first, each holding the boxes and references of its two children.
+// Every box is the FP32 min/max of its triangles' vertices grown outward by
 // pad = 2^-11 * max(1, largest vertex coordinate magnitude), which covers the rounding of
 // the strict leaf test's hit point and of the slab test (the argument and its limit are in
 // the design notes of the per-ray path).
@@ -30,8 +34,33 @@
 namespace rti {
 namespace {
 
+constexpr float kNodeCost = 1.f;  // SAH cost of visiting a node, per unit area
+constexpr float kTriCost = 1.f;   // SAH cost of one leaf test, per unit area
+constexpr int kBins = 16;
+
 struct SBox {
   float lo[3], hi[3];
+  void clear() {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = INFINITY;
+      hi[k] = -INFINITY;
+    }
+  }
+  void grow(const SBox& b) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], b.lo[k]);
+      hi[k] = std::max(hi[k], b.hi[k]);
+    }
+  }
+  float area() const {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.f * ((dx * dy + dy * dz) + dz * dx);
+  }
+};
+
+struct Prim {
+  SBox box;
+  float c[3];  // centre of the box
 };
 
 // Interior levels a balanced (median split) tree needs over n triangles.
@@ -46,27 +75,101 @@ int levels_needed(uint32_t n) {
 
 class SearchBuilder {
  public:
-  SearchBuilder(const float* v9, const uint32_t* code, std::vector<float4>& out, float pad)
-      : v9_(v9), code_(code), out_(out), pad_(pad) {}
+  SearchBuilder(const std::vector<Prim>& prims, std::vector<uint32_t>& order,
+                std::vector<float4>& out, float pad)
+      : P_(prims), idx_(order), out_(out), pad_(pad) {}
 
-  // Reference of the subtree over [a, b) whose parent sits at interior depth `depth`
-  // (the root node is depth 1); its box (unpadded) goes to `box`.
+  // Reference of the subtree over search positions [a, b) whose parent sits at interior depth
+  // `depth` (the root node is depth 1); its box (unpadded) goes to `box`.
   uint32_t build(uint32_t a, uint32_t b, int depth, SBox& box) {
     const uint32_t n = b - a;
-    if (n <= (uint32_t)kSearchLeaf) {
-      tri_box(a, b, box);
-      return 0x80000000u | ((n - 1) << 27) | a;
+    if (n == 1) return leaf(a, b, box);
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = a; i < b; ++i)
+      for (int k = 0; k < 3; ++k) {
+        clo[k] = std::min(clo[k], P_[idx_[i]].c[k]);
+        chi[k] = std::max(chi[k], P_[idx_[i]].c[k]);
+      }
+    int ax = 0;
+    for (int k = 1; k < 3; ++k)
+      if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+    const float ext = chi[ax] - clo[ax];
+    bool median = !(ext > 0.f);
+    uint32_t m = 0;
+    if (!median) {
+      const float scale = (float)kBins / ext;
+      auto bin = [&](uint32_t t) {
+        const int k = (int)((P_[t].c[ax] - clo[ax]) * scale);
+        return std::min(std::max(k, 0), kBins - 1);
+      };
+      SBox bb[kBins];
+      uint32_t cnt[kBins] = {0};
+      for (int k = 0; k < kBins; ++k) bb[k].clear();
+      for (uint32_t i = a; i < b; ++i) {
+        const int k = bin(idx_[i]);
+        bb[k].grow(P_[idx_[i]].box);
+        ++cnt[k];
+      }
+      // areas and counts right of each plane, then the sweep from the left
+      float rarea[kBins];
+      uint32_t rcnt[kBins];
+      SBox acc;
+      acc.clear();
+      uint32_t c = 0;
+      for (int k = kBins - 1; k > 0; --k) {
+        c += cnt[k];
+        if (cnt[k]) acc.grow(bb[k]);
+        rarea[k] = c ? acc.area() : 0.f;
+        rcnt[k] = c;
+      }
+      acc.clear();
+      c = 0;
+      int best = -1;
+      float best_cost = INFINITY;
+      for (int k = 0; k < kBins - 1; ++k) {
+        c += cnt[k];
+        if (cnt[k]) acc.grow(bb[k]);
+        if (c == 0 || rcnt[k + 1] == 0) continue;
+        const float cost = (float)c * acc.area() + (float)rcnt[k + 1] * rarea[k + 1];
+        if (cost < best_cost) {
+          best_cost = cost;
+          best = k;
+        }
+      }
+      if (n <= (uint32_t)kSearchLeaf) {
+        SBox all;
+        all.clear();
+        for (uint32_t i = a; i < b; ++i) all.grow(P_[idx_[i]].box);
+        const float A = all.area();
+        if (best < 0 || !(kNodeCost * A + kTriCost * best_cost < kTriCost * (float)n * A))
+          return leaf(a, b, box);
+      }
+      if (best < 0) {
+        median = true;
+      } else {
+        uint32_t* base = idx_.data();
+        uint32_t* mid = std::stable_partition(base + a, base + b,
+                                              [&](uint32_t t) { return bin(t) <= best; });
+        m = (uint32_t)(mid - base);
+        if (depth + levels_needed(m - a) > kSearchDepth ||
+            depth + levels_needed(b - m) > kSearchDepth)
+          median = true;
+      }
+    }
+    if (median) {
+      if (n <= (uint32_t)kSearchLeaf) return leaf(a, b, box);
+      std::stable_sort(idx_.begin() + a, idx_.begin() + b, [&](uint32_t s0, uint32_t s1) {
+        return P_[s0].c[ax] < P_[s1].c[ax] || (P_[s0].c[ax] == P_[s1].c[ax] && s0 < s1);
+      });
+      m = a + n / 2;
     }
     const uint32_t node = reserve();
-    const uint32_t m = split(a, b, depth);
     SBox b0, b1;
     const uint32_t r0 = build(a, m, depth + 1, b0);
     const uint32_t r1 = build(m, b, depth + 1, b1);
     write(node, b0, r0, b1, r1);
-    for (int k = 0; k < 3; ++k) {
-      box.lo[k] = std::min(b0.lo[k], b1.lo[k]);
-      box.hi[k] = std::max(b0.hi[k], b1.hi[k]);
-    }
+    box = b0;
+    box.grow(b1);
     return node;
   }
 
@@ -77,14 +180,10 @@ class SearchBuilder {
       const uint32_t node = reserve();
       const uint32_t m = ntri == 1 ? 1 : ntri / 2;
       SBox b0, b1;
-      tri_box(0, m, b0);
-      const uint32_t r0 = 0x80000000u | ((m - 1) << 27);
+      const uint32_t r0 = leaf(0, m, b0);
       uint32_t r1 = r0;
       b1 = b0;
-      if (ntri > 1) {
-        tri_box(m, ntri, b1);
-        r1 = 0x80000000u | ((ntri - m - 1) << 27) | m;
-      }
+      if (ntri > 1) r1 = leaf(m, ntri, b1);
       write(node, b0, r0, b1, r1);
       return;
     }
@@ -99,39 +198,10 @@ class SearchBuilder {
     return node;
   }
 
-  void tri_box(uint32_t a, uint32_t b, SBox& box) const {
-    for (int k = 0; k < 3; ++k) {
-      box.lo[k] = INFINITY;
-      box.hi[k] = -INFINITY;
-    }
-    for (uint32_t i = a; i < b; ++i)
-      for (int v = 0; v < 3; ++v)
-        for (int k = 0; k < 3; ++k) {
-          const float x = v9_[9 * (size_t)i + 3 * v + k];
-          box.lo[k] = std::min(box.lo[k], x);
-          box.hi[k] = std::max(box.hi[k], x);
-        }
-  }
-
-  uint32_t split(uint32_t a, uint32_t b, int depth) const {
-    const uint32_t mid = a + (b - a) / 2;
-    const uint32_t ca = code_[a], cb = code_[b - 1];
-    if (ca == cb) return mid;
-    const int bit = 31 - __builtin_clz(ca ^ cb);
-    // codes are sorted and agree above `bit`: the first code with the bit set starts the
-    // upper half
-    uint32_t lo = a, hi = b - 1;
-    while (lo < hi) {
-      const uint32_t md = lo + (hi - lo) / 2;
-      if (code_[md] & (1u << bit)) hi = md;
-      else lo = md + 1;
-    }
-    const uint32_t m = lo;
-    // children sit at interior depth depth + 1 (if interior); keep the whole tree within
-    // kSearchDepth interior levels
-    if (depth + levels_needed(m - a) > kSearchDepth || depth + levels_needed(b - m) > kSearchDepth)
-      return mid;
-    return m;
+  uint32_t leaf(uint32_t a, uint32_t b, SBox& box) const {
+    box.clear();
+    for (uint32_t i = a; i < b; ++i) box.grow(P_[idx_[i]].box);
+    return 0x80000000u | ((b - a - 1) << 27) | a;
   }
 
   void write(uint32_t node, const SBox& b0, uint32_t r0, const SBox& b1, uint32_t r1) {
@@ -152,23 +222,34 @@ class SearchBuilder {
     p[3] = make_float4(f0, f1, 0.f, 0.f);
   }
 
-  const float* v9_;
-  const uint32_t* code_;
+  const std::vector<Prim>& P_;
+  std::vector<uint32_t>& idx_;
   std::vector<float4>& out_;
   float pad_;
 };
 
 }  // namespace
 
-void build_search_bvh(const float* v9, const uint32_t* code, uint32_t ntri,
-                      std::vector<float4>& nodes) {
+void build_search_bvh(const float* v9, uint32_t ntri, std::vector<float4>& nodes,
+                      std::vector<uint32_t>& order) {
   nodes.clear();
+  order.resize(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) order[i] = i;
   if (ntri == 0) return;
   float big = 1.f;
   for (size_t i = 0; i < 9 * (size_t)ntri; ++i) big = std::max(big, std::fabs(v9[i]));
   const float pad = std::ldexp(big, -11);
+  std::vector<Prim> prims(ntri);
+  for (uint32_t i = 0; i < ntri; ++i) {
+    const float* p = v9 + 9 * (size_t)i;
+    for (int k = 0; k < 3; ++k) {
+      prims[i].box.lo[k] = std::min(std::min(p[k], p[3 + k]), p[6 + k]);
+      prims[i].box.hi[k] = std::max(std::max(p[k], p[3 + k]), p[6 + k]);
+      prims[i].c[k] = (prims[i].box.lo[k] + prims[i].box.hi[k]) * 0.5f;
+    }
+  }
   nodes.reserve(4 * (size_t)std::max<uint32_t>(1, ntri / 2));
-  SearchBuilder b(v9, code, nodes, pad);
+  SearchBuilder b(prims, order, nodes, pad);
   b.build_root(ntri);
 }
 
