@@ -273,39 +273,45 @@ def bench_uneven(args, rt, scenes, torch, orc):
             smp["ulp_marginal"] += int((ref["flags"] & orc.ULP_MARGINAL != 0).sum())
         ex.close()
 
-    # roofline of the heaviest per-ray kernel: its work is every ray it traces, so it is timed
-    # with every block of its mode sent to it, and the oracle's restatement counts the work
-    heavy = "occlusion" if split["occlusion_per_ray"] >= split["closest_per_ray"] else "closest"
-    hm = 2 if heavy == "occlusion" else rt.RT_CLOSEST_PRIMARY
-    pr = rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED if hm == 2 else rt.RT_PARAM_RAY_CLUSTERS_CLOSEST
-    old = b.get_param(pr)
-    b.set_param(pr, 1)
+    # rooflines of both per-ray kernels: a kernel's work is every ray it traces, so each is timed
+    # with every block of its mode sent to it (median of 3 eager runs), and the oracle's
+    # restatement counts that work; the line's roofline is the one whose kernel takes longer
+    # in the step, the other is reported beside it
     b.set_param(rt.RT_PARAM_GRAPH, 0)
-    b.run(stream=stream)
-    times = []
-    for _ in range(3):
-        b.set_timing(True)
+    rooflines = {}
+    for mname, hm in (("closest", rt.RT_CLOSEST_PRIMARY), ("occlusion", 2)):
+        pr = rt.RT_PARAM_RAY_CLUSTERS_OCCLUDED if hm == 2 else rt.RT_PARAM_RAY_CLUSTERS_CLOSEST
+        old = b.get_param(pr)
+        b.set_param(pr, 1)
         b.run(stream=stream)
-        stream.synchronize()
-        times.append(b.kernel_time(hm + rt.RT_TIME_PER_RAY)[0])
-        b.set_timing(False)
-    t_kernel = statistics.median(times)
-    b.set_param(pr, old)
-    log("oracle counts of the per-ray kernel's work")
-    tot = None
-    for k in range(S):
-        r = jobs[k]["shadow" if hm == 2 else "primary"]
-        st = orc.search_trace(jobs[k]["v9"], r, orc.params("rayhierarchy"), hm,
-                              nthreads=host_threads())["stats"]
-        if tot is None:
-            tot = st
-        else:
-            for f in ("rays", "visits", "tests", "culls", "chains"):
-                setattr(tot, f, getattr(tot, f) + getattr(st, f))
-    kname = f"ray_path_kernel<{hm},0,false>"
-    roofline = per_ray_roofline(kname, t_kernel, tot, hm, "uneven_batch_64")
-    roofline["measured_as"] = (f"the {heavy} jobs of the batch with every block sent to the "
-                               f"per-ray kernel, median of 3 eager runs")
+        times = []
+        for _ in range(3):
+            b.set_timing(True)
+            b.run(stream=stream)
+            stream.synchronize()
+            times.append(b.kernel_time(hm + rt.RT_TIME_PER_RAY)[0])
+            b.set_timing(False)
+        t_kernel = statistics.median(times)
+        b.set_param(pr, old)
+        log(f"oracle counts of the per-ray kernel's work ({mname})")
+        tot = None
+        for k in range(S):
+            r = jobs[k]["shadow" if hm == 2 else "primary"]
+            st = orc.search_trace(jobs[k]["v9"], r, orc.params("rayhierarchy"), hm,
+                                  nthreads=host_threads())["stats"]
+            if tot is None:
+                tot = st
+            else:
+                for f in ("rays", "visits", "tests", "culls", "chains"):
+                    setattr(tot, f, getattr(tot, f) + getattr(st, f))
+        kname = f"ray_path_kernel<{hm},0,false>"
+        rooflines[mname] = per_ray_roofline(kname, t_kernel, tot, hm, "uneven_batch_64")
+        rooflines[mname]["measured_as"] = (
+            f"the {mname} jobs of the batch with every block sent to the per-ray kernel, "
+            f"median of 3 eager runs")
+    heavy = "occlusion" if split["occlusion_per_ray"] >= split["closest_per_ray"] else "closest"
+    roofline = rooflines[heavy]
+    roofline_other = rooflines["closest" if heavy == "occlusion" else "occlusion"]
 
     # CPU baseline: pbrt's BVHAccel (SAH, maxnodeprims 4, inclusive Triangle test) on the
     # camera and shadow rays of every second scene, all host threads, median of 3; one core on
@@ -363,7 +369,8 @@ def bench_uneven(args, rt, scenes, torch, orc):
         "parity": {"batched_vs_plain": {"id_mismatches": mism_id, "t_mismatches": mism_t,
                                         "occlusion_mismatches": mism_occ},
                    "oracle_sample": smp},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "roofline_other_per_ray_kernel": roofline_other,
+        "cpu_baseline": cpu,
     }
     b.close()
     for sc in scs:

@@ -1486,6 +1486,16 @@ __device__ __forceinline__ bool ray_chain_bvh(const RhItem& it, int x, int y, ui
 constexpr uint32_t kChainFlush = 32;
 constexpr uint32_t kChainQ = kChainFlush + 64;  // below kChainFlush, plus one per lane a round
 constexpr uint32_t kChainAge = 512;
+// Closest hits defer their chains the same way.  A candidate (a strict hit that beats the
+// lane's bound) is queued with its t, and the lane goes on pruning with it as a provisional
+// bound.  A passing chain merges the candidate into the ray's global key (atomic min, so the
+// order does not matter); the ray's lane waits at its end until its candidates are evaluated.
+// When a candidate that bounded the search fails its chain, the provisional bound may have
+// pruned the answer, so the lane traces the ray again from the root, evaluating each chain at
+// once (chains pass for most hits, so this is rare).  The queue (four words an entry) is four
+// entries shorter than occlusion's so that five work-groups per SIMD still fit the LDS.
+constexpr uint32_t kChainFlushC = 28;
+constexpr uint32_t kChainQC = kChainFlushC + 64;
 // Five wavefronts per SIMD (at most 96 VGPRs; the LDS of 20 work-groups per CU): the walk is
 // latency-bound and gains more from resident wavefronts than it loses to a few spilled
 // registers of the chain evaluation.
@@ -1494,9 +1504,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     const RhItem* __restrict__ items, const uint2* __restrict__ rq, uint32_t* __restrict__ ctr,
     uint32_t walk) {
   __shared__ uint32_t sStk[kSearchDepth][64];
-  constexpr uint32_t kQ = MODE == 2 ? kChainQ : 1u;
+  constexpr uint32_t kQ = MODE == 2 ? kChainQ : kChainQC;
   __shared__ uint32_t sQIdx[kQ], sQRef[kQ], sQMeta[kQ];
-  __shared__ unsigned long long sQHit;
+  __shared__ float sQT[MODE == 2 ? 1 : kQ];
+  __shared__ unsigned long long sQHit;  // occlusion: rays found occluded; closest: failed bounds
   __shared__ uint32_t sQN;
   constexpr uint32_t kDone = 0xffffffffu;  // not a node reference (leaves hold <= 4 triangles)
   const int lane = threadIdx.x;
@@ -1516,19 +1527,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
   uint64_t best = kMissKey;
   uint32_t rep = kNoKey32;
   bool occluded = false, track_rep = false;
+  // closest: the provisional bound of queued candidates, whether any is queued, and whether the
+  // ray is traced again with its chains evaluated at once
+  uint64_t spec = kMissKey;
+  bool pend = false, nospec = false;
   int sp = 0;
   uint32_t cur = kDone, pl = kDone;
   auto pop = [&]() { return sp > 0 ? sStk[--sp][lane] : kDone; };
   // occlusion: deferred chain candidates (count sQN), the serial of this lane's ray, and the
   // rounds since the queue was last evaluated (wavefront-uniform)
   uint32_t serial = 0, age = 0;
-  if constexpr (MODE == 2) {
-    if (lane == 0) {
-      sQN = 0;
-      sQHit = 0ull;
-    }
-    __syncthreads();
+  if (lane == 0) {
+    sQN = 0;
+    sQHit = 0ull;
   }
+  __syncthreads();
   // Evaluates the cq queued candidates, 64 per pass (one per lane).  Wavefront-uniform, after
   // a barrier that made the entries visible.
   auto run_chains = [&](uint32_t cq) {
@@ -1542,17 +1555,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
         const RhItem& oit = items[meta >> 16];
         const uint32_t ri = sQIdx[e];
         const int ox = (int)(ri % (uint32_t)oit.g.W), oy = (int)(ri / (uint32_t)oit.g.W);
-        bool ok;
-        if constexpr (BVH) ok = ray_chain_bvh<NODE>(oit, ox, oy, sQRef[e]);
-        else ok = ray_chain_rh<MODE, NODE>(oit, ox, oy, sQRef[e]);
-        if (ok) {
-          glb(oit.occ)[ri] = 1;
-          if ((meta & 1023u) == owner_serial) atomicOr(&sQHit, 1ull << owner);
+        if constexpr (MODE == 2) {
+          bool ok;
+          if constexpr (BVH) ok = ray_chain_bvh<NODE>(oit, ox, oy, sQRef[e]);
+          else ok = ray_chain_rh<MODE, NODE>(oit, ox, oy, sQRef[e]);
+          if (ok) {
+            glb(oit.occ)[ri] = 1;
+            if ((meta & 1023u) == owner_serial) atomicOr(&sQHit, 1ull << owner);
+          }
+        } else {
+          // closest: sQRef holds the triangle id, meta bit 1 a main candidate, bit 0 a
+          // repair-region one
+          const uint32_t id = sQRef[e];
+          const float t = sQT[e];
+          bool ok;
+          if constexpr (BVH) ok = ray_chain_bvh<NODE>(oit, ox, oy, glb(oit.tleaf)[id] & 0x7fffffffu);
+          else ok = ray_chain_rh<MODE, NODE>(oit, ox, oy, id);
+          if (ok) {
+            if (meta & 2u) atomicMin((unsigned long long*)&oit.keys[ri], hit_key(t, (int32_t)id));
+            if (meta & 1u) atomicMin(&oit.repkeys[ri], fkey(t));
+          } else if (meta & 2u) {
+            atomicOr(&sQHit, 1ull << owner);
+          }
         }
       }
     }
     __syncthreads();
-    if (act && ((sQHit >> lane) & 1ull)) occluded = true;
+    if constexpr (MODE == 2) {
+      if (act && ((sQHit >> lane) & 1ull)) occluded = true;
+    } else if (act && pend) {
+      pend = false;
+      if ((sQHit >> lane) & 1ull) {  // a bound failed: trace the ray again
+        nospec = true;
+        spec = kMissKey;
+        sp = 0;
+        cur = 0u;
+        pl = kDone;
+      }
+    }
     __syncthreads();
     if (lane == 0) {
       sQN = 0;
@@ -1614,6 +1654,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
           best = kMissKey;
           rep = kNoKey32;
           occluded = false;
+          spec = kMissKey;
+          pend = false;
+          nospec = false;
           track_rep = MODE == 0 && !BVH && it.repkeys != nullptr;
           sp = 0;
           cur = 0u;  // interior node 0 (the root)
@@ -1625,10 +1668,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
       cslot += min(avail, (uint32_t)__popcll(need));
     }
     if (!__any(act)) {
-      if constexpr (MODE == 2) {  // the candidates still queued
-        const uint32_t cq = __builtin_amdgcn_readfirstlane(sQN);
-        if (cq > 0u) run_chains(cq);
-      }
+      const uint32_t cq = __builtin_amdgcn_readfirstlane(sQN);  // the candidates still queued
+      if (cq > 0u) run_chains(cq);
       break;
     }
     const RhItem& it = items[jk];
@@ -1640,8 +1681,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
       if (act && cur != kDone && !(cur & kSearchLeafBit)) {
         const float4 n0 = ld4(snode, 4 * (size_t)cur), n1 = ld4(snode, 4 * (size_t)cur + 1),
                      n2 = ld4(snode, 4 * (size_t)cur + 2), n3 = ld4(snode, 4 * (size_t)cur + 3);
-        float thi = MODE == 2 ? tmax : (best == kMissKey ? __int_as_float(0x7f800000)
-                                                         : fkey_inv((uint32_t)(best >> 32)));
+        const uint64_t bound = spec < best ? spec : best;
+        float thi = MODE == 2 ? tmax : (bound == kMissKey ? __int_as_float(0x7f800000)
+                                                          : fkey_inv((uint32_t)(bound >> 32)));
         thi = thi + fabsf(thi) * kSlack;
         float tn0, tn1;
         const bool h0 = slab(o, inv, V3{n0.x, n0.y, n0.z}, V3{n0.w, n1.x, n1.y}, tlo, thi, tn0);
@@ -1701,55 +1743,85 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
         if (cq >= kChainFlush) run_chains(cq);
       }
       if (haveleaf) pl = kDone;
-    } else if (act && pl != kDone) {
-      const uint32_t first = pl & ((1u << 27) - 1u), cnt = ((pl >> 27) & 15u) + 1u;
-      for (uint32_t f = 0; f < cnt; ++f) {
-        const uint32_t slot = first + f;
-        const uint32_t id = perm[slot];
-        uint32_t bl = 0;
-        if constexpr (BVH) {
-          bl = glb(it.tleaf)[id];
-          if (bl == 0xffffffffu) continue;
-          if (MODE == 1 && NODE != 2 && (bl & 0x80000000u)) continue;  // first primitive only
-          bl &= 0x7fffffffu;
+    } else {
+      // wavefront-uniform rounds over the leaf's triangles, as for occlusion
+      const bool haveleaf = act && pl != kDone;
+      const uint32_t first = pl & ((1u << 27) - 1u);
+      const uint32_t cnt = haveleaf ? ((pl >> 27) & 15u) + 1u : 0u;
+      for (uint32_t f = 0; f < (uint32_t)kSearchLeaf; ++f) {
+        if (!__any(f < cnt)) break;
+        if (f < cnt) {
+          const uint32_t slot = first + f;
+          const uint32_t id = perm[slot];
+          bool ok = true;
+          uint32_t bl = 0;
+          if constexpr (BVH) {
+            bl = glb(it.tleaf)[id];
+            // first primitive only (6DB / IA secondary rays)
+            ok = bl != 0xffffffffu && !(MODE == 1 && NODE != 2 && (bl & 0x80000000u));
+            bl &= 0x7fffffffu;
+          }
+          float t, b1, b2;
+          if (ok && mt_strict(o, d, load_mt(tmts, slot), t, b1, b2)) {
+            const bool main = t > tmin && t < __int_as_float(0x7f800000);
+            const bool rp = track_rep && repair_region(b1, b2) && t >= tmin;
+            const uint64_t key = hit_key(t, (int32_t)id);
+            const bool um = main && key < best && key < spec, ur = rp && fkey(t) < rep;
+            if (um || ur) {
+              if (nospec) {
+                bool pass;
+                if constexpr (BVH) pass = ray_chain_bvh<NODE>(it, x, y, bl);
+                else pass = ray_chain_rh<MODE, NODE>(it, x, y, id);
+                if (pass) {
+                  if (um) best = key;
+                  if (ur) rep = fkey(t);
+                }
+              } else {
+                const uint32_t e = atomicAdd(&sQN, 1u);  // below kChainQC, see kChainQC
+                sQIdx[e] = (uint32_t)idx;
+                sQRef[e] = id;
+                sQT[e] = t;
+                sQMeta[e] = (jk << 16) | ((uint32_t)lane << 10) | (um ? 2u : 0u) | (ur ? 1u : 0u);
+                if (um) spec = key;
+                pend = true;
+              }
+            }
+          }
         }
-        float t, b1, b2;
-        const bool hit = mt_strict(o, d, load_mt(tmts, slot), t, b1, b2);
-        {
-          if (!hit) continue;
-          const bool main = t > tmin && t < __int_as_float(0x7f800000);
-          const bool rp = track_rep && repair_region(b1, b2) && t >= tmin;
-          const uint64_t key = hit_key(t, (int32_t)id);
-          const bool um = main && key < best, ur = rp && fkey(t) < rep;
-          if (!um && !ur) continue;
-          bool ok;
-          if constexpr (BVH) ok = ray_chain_bvh<NODE>(it, x, y, bl);
-          else ok = ray_chain_rh<MODE, NODE>(it, x, y, id);
-          if (!ok) continue;
-          if (um) best = key;
-          if (ur) rep = fkey(t);
-        }
+        __syncthreads();
+        const uint32_t cq = __builtin_amdgcn_readfirstlane(sQN);
+        if (cq >= kChainFlushC) run_chains(cq);
       }
-      pl = kDone;
+      if (haveleaf) pl = kDone;
     }
     if (act && cur != kDone && (cur & kSearchLeafBit)) {  // a second leaf waits for the next round
       pl = cur;
       cur = pop();
     }
-    if constexpr (MODE == 2) {
-      if (++age >= kChainAge) {
+    {
+      // evaluate the queue every kChainAge rounds; closest hits also once a quarter of the
+      // lanes holding a ray wait at its end for their candidates
+      bool flush = ++age >= kChainAge;
+      if constexpr (MODE != 2) {
+        const uint32_t nact = (uint32_t)__popcll(__ballot(act));
+        const uint32_t nwait = (uint32_t)__popcll(__ballot(act && pend && cur == kDone && pl == kDone));
+        flush = flush || (nwait > 0u && 4u * nwait >= nact);
+      }
+      if (flush) {
         const uint32_t cq = __builtin_amdgcn_readfirstlane(sQN);
         if (cq > 0u) run_chains(cq);
         age = 0;
       }
     }
     if (act && occluded) cur = pl = kDone;
-    if (act && cur == kDone && pl == kDone) {  // the ray is done: its result, then a new ray
+    if (act && cur == kDone && pl == kDone && !pend) {  // the ray is done: its result, then a new ray
       if constexpr (MODE == 2) {
         if (occluded) glb(it.occ)[idx] = 1;
       } else {
-        if (best != kMissKey) glb(it.keys)[idx] = best;
-        if (track_rep && rep != kNoKey32) glb(it.repkeys)[idx] = rep;
+        // passing queued candidates were merged by the evaluation; chains evaluated at once
+        // (a ray traced again) leave their result here
+        if (best != kMissKey) atomicMin((unsigned long long*)&it.keys[idx], best);
+        if (track_rep && rep != kNoKey32) atomicMin(&it.repkeys[idx], rep);
       }
       act = false;
     }
