@@ -1799,9 +1799,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
       cur = pop();
     }
     {
-      // evaluate the queue every kChainAge rounds; closest hits also once a quarter of the
-      // lanes holding a ray wait at its end for their candidates
-      bool flush = ++age >= kChainAge;
+      // evaluate the queue every kChainAge rounds, and every round once no chunk is left to
+      // deal (idle lanes cannot take new rays, and a ray found occluded or bounded early ends
+      // sooner); closest hits also once a quarter of the lanes holding a ray wait at its end
+      // for their candidates
+      bool flush = ++age >= kChainAge || !more;
       if constexpr (MODE != 2) {
         const uint32_t nact = (uint32_t)__popcll(__ballot(act));
         const uint32_t nwait = (uint32_t)__popcll(__ballot(act && pend && cur == kDone && pl == kDone));
