@@ -472,6 +472,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
       const size_t n = (size_t)jb.r.W * jb.r.H;
       it.block_base = (uint32_t)blocks;
       it.ray_base = (uint32_t)rays;
+      it.chunk_base = (uint32_t)L.max_chunks;
       blocks += (uint64_t)it.g.ntop;
       rays += n;
       // the 1024-ray unit kernels, for blocks beyond 256 rays or 3 levels that the unit
@@ -1022,6 +1023,7 @@ int rt_trace_closest_noalloc(rt_scene* s, const rt_rays* rays, int32_t mode, con
   if (off > kNoAllocScratch || n >= (1ull << 31)) return RT_ERR_TOO_LARGE;
   it.block_base = 0;
   it.ray_base = 0;
+  it.chunk_base = 0;
   G.items = table;
   G.nitems = 1;
   G.mode = mode;

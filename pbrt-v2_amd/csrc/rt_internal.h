@@ -102,7 +102,8 @@ struct RhItem {
   HitOut out;            // closest-hit outputs
   uint32_t* flags;       // the context's RT_FLAG_* word
   uint32_t block_base, ray_base;
-  int32_t repair_run, pad;
+  int32_t repair_run;
+  uint32_t chunk_base;   // first per-ray chunk slot of the job (its blocks' slots follow)
 };
 
 // A launch group: `items` is the device copy of the job table; units, counters and the
@@ -116,15 +117,16 @@ struct RhGroup {
   int bvh;   // ray-bvh jobs (product traversal with the scene's BVH)
   uint32_t nblocks, nrays;
   uint4* units;       // {item, block, first cluster, count}
-  uint2* rq;          // per-ray path: {item, block << 4 | 64-ray chunk of the block}
+  uint2* rq;          // per-ray path: one slot per 64-ray chunk of every block, at the job's
+                      // chunk_base + block * chunks per block + chunk: {item, block * chunks
+                      // per block + chunk}, or item 0xffffffff for a block of the block path
   uint32_t* ctr;      // [0] units, [1] next unit to take, [2] rays flagged for repair,
-                      // [3] traversal stack overflow (ray-bvh), [4] per-ray chunks,
-                      // [5] next per-ray chunk to take
+                      // [3], [4] unused, [5] next per-ray chunk slot to take
   uint32_t* replist;  // global ray indices flagged for repair (primary with repair only)
   int per_unit;       // passing clusters per work unit
   int grid;           // work-groups of the persistent unit kernel; 0 = one per resident slot
   uint32_t max_units;  // upper bound of the group's work units
-  uint32_t max_chunks;  // upper bound of the group's per-ray chunks
+  uint32_t max_chunks;  // the group's per-ray chunk slots
   // blocks whose root passes at least this many clusters are traced ray by ray; 0 sends every
   // block with a traced ray that way without the cluster scan (job cluster lists unused)
   uint32_t ray_clusters;
@@ -133,10 +135,11 @@ struct RhGroup {
 };
 constexpr int kDefaultWalk = 3;
 
-// Search BVH of the per-ray path over triangles in sorted order (see rt_search_bvh.cpp).
-// Node = 4 float4: child 0 box (lo.xyz, hi.x), (hi.yz, child 1 lo.xy), (lo.z, hi.xyz),
-// (ref0, ref1 as bits, 0, 0).  A ref with bit 31 clear is an interior node index; with bit
-// 31 set it is a leaf of (bits 27..30) + 1 triangles starting at sorted slot (bits 0..26).
+// Search BVH of the per-ray path over triangles in its own search order (see
+// rt_search_bvh.cpp).  Node = 4 float4: child 0 box (lo.xyz, hi.x), (hi.yz, child 1 lo.xy),
+// (lo.z, hi.xyz), (ref0, ref1 as bits, 0, 0).  A ref with bit 31 clear is an interior node
+// index; with bit 31 set it is a leaf of (bits 27..30) + 1 triangles starting at search
+// position (bits 0..26).
 // Boxes are grown outward by the scene's padding.  The depth is at most kSearchDepth.
 #ifndef RT_SEARCH_DEPTH
 #define RT_SEARCH_DEPTH 26

@@ -282,17 +282,22 @@ __device__ void build_block(const RHGeom& g, const RaysDev& r, float4* __restric
   }
 }
 
-// Job of a global block / ray index in a launch group (uniform when the index is).
+// Job of a global block index in a launch group, for an index that is uniform over the
+// wavefront: the lanes read 64 jobs' block prefixes at once and count those at or below the
+// index (the prefixes increase with the job), one round trip per 64 jobs instead of a binary
+// search's chain of dependent loads.
 __device__ __forceinline__ int find_block_item(const RhItem* __restrict__ items, int n,
                                                uint32_t gb) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].block_base <= gb) lo = mid;
-    else hi = mid - 1;
+  int k = 0;
+  for (int c = 0; c < n; c += 64) {
+    const int j = c + (int)__lane_id();
+    const int m = __popcll(__ballot(j < n && items[j].block_base <= gb));
+    k += m;
+    if (m < 64) break;
   }
-  return lo;
+  return k - 1;
 }
+// Job of a global ray index (per lane).
 __device__ __forceinline__ int find_ray_item(const RhItem* __restrict__ items, int n,
                                              uint32_t gr) {
   int lo = 0, hi = n - 1;
@@ -749,7 +754,9 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
 // as many, or whose leaves' ray directions spread by more than kIncoherent on average, is an
 // incoherent ray group: its candidate set
 // holds most (leaf, triangle) pairs, so it is queued for the per-ray path instead, as 64-ray
-// chunks {job, block * chunks per block + chunk}.  ray_clusters == 0 queues every block with a
+// chunks {job, block * chunks per block + chunk} in the block's fixed slots of the queue (no
+// shared counter: thousands of blocks appending through one atomic serialise on it); the
+// slots of every other block are marked empty.  ray_clusters == 0 queues every block with a
 // traced ray that way without scanning the clusters, and so does a block of more than
 // kMaxBlockRays rays or kUnitLevels levels (the unit kernels stage at most that much; the
 // reference's own scenes use such blocks, e.g. 100x4 leaves at height 3 or 4).
@@ -773,7 +780,18 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   const int K = b % g.gw[top], J = b / g.gw[top];
   const float4* p = it.nodes + 4 * ((size_t)g.off[top] + (size_t)J * g.gw[top] + K);
   const float4 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3];
-  if (a0.w == 0.f) return;  // no traced ray below this root (uniform)
+  // this block's chunk slots: filled for the per-ray path, marked empty otherwise
+  const int nch = (g.rb + 63) / 64;
+  uint2* const slots = rq + it.chunk_base + (size_t)b * (size_t)nch;
+  auto queue_chunks = [&](bool per_ray) {
+    for (int q = tid; q < nch; q += kThreads)
+      slots[q] = make_uint2(per_ray ? (uint32_t)k : 0xffffffffu,
+                            (uint32_t)b * (uint32_t)nch + (uint32_t)q);
+  };
+  if (a0.w == 0.f) {  // no traced ray below this root (uniform)
+    queue_chunks(false);
+    return;
+  }
   Node root;
   root.dmin = {a0.x, a0.y, a0.z};
   root.dmax = {a1.x, a1.y, a1.z};
@@ -781,11 +799,7 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   root.omax = {a3.x, a3.y, a3.z};
   if (ray_clusters == 0 || g.rb > kMaxBlockRays || g.L > kUnitLevels) {
     // to the per-ray path without the cluster scan (no cluster list needed)
-    const int nch = (g.rb + 63) / 64;
-    if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
-    __syncthreads();
-    for (int q = tid; q < nch; q += kThreads)
-      rq[sBase + q] = make_uint2((uint32_t)k, (uint32_t)b * (uint32_t)nch + (uint32_t)q);
+    queue_chunks(true);
     return;
   }
   if (tid == 0) {
@@ -814,6 +828,16 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   const uint32_t nclusters = it.nclusters;
   const float4* __restrict__ cbox = it.cbox;
   uint32_t* out = it.clist + (size_t)b * nclusters;
+  __syncthreads();
+  // incoherent: at least `ray_clusters` clusters pass in a scene of at least 4 * ray_clusters
+  // clusters (in smaller scenes the block path's work is bounded by the scene), or the block's
+  // leaves spread their ray directions by more than kIncoherent on average (ray_clusters above
+  // 2^29: never; 1: every block with a passing cluster).  The scan stops as soon as the
+  // passing count decides that (a per-ray block needs no cluster list).
+  const bool wide = sLeaves > 0 && sSpread > kIncoherent * (float)sLeaves;
+  const bool may_heavy = ray_clusters == 1u || (uint64_t)nclusters >= 4ull * ray_clusters;
+  const uint32_t enough = (ray_clusters <= (1u << 29) && wide) ? 1u
+                          : may_heavy ? ray_clusters : 0xffffffffu;
   for (uint32_t cb = 0; cb < nclusters; cb += kThreads) {
     const uint32_t ci = cb + tid;
     bool ok = false;
@@ -822,24 +846,18 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
       ok = cluster_bound<MODE, NODE, BVH>(root, V3{lo.x, lo.y, lo.z}, V3{hi.x, hi.y, hi.z});
     }
     ballot_append(out, &sCnt, ok, ci);
+    if (enough != 0xffffffffu) {
+      __syncthreads();
+      if ((uint32_t)sCnt >= enough) break;  // uniform: every thread reads the same count
+      __syncthreads();
+    }
   }
   __syncthreads();
   const int cnt = sCnt;
-  // incoherent: at least `ray_clusters` clusters pass in a scene of at least 4 * ray_clusters
-  // clusters (in smaller scenes the block path's work is bounded by the scene), or the block's
-  // leaves spread their ray directions by more than kIncoherent on average (ray_clusters above
-  // 2^29: never; 1: every block with a passing cluster)
-  const bool wide = sLeaves > 0 && sSpread > kIncoherent * (float)sLeaves;
-  const bool heavy = (uint32_t)cnt >= ray_clusters &&
-                     (ray_clusters == 1u || (uint64_t)nclusters >= 4ull * ray_clusters);
-  if (heavy || (ray_clusters <= (1u << 29) && cnt > 0 && wide)) {
-    const int nch = (g.rb + 63) / 64;
-    if (tid == 0) sBase = atomicAdd(&ctr[4], (uint32_t)nch);
-    __syncthreads();
-    for (int q = tid; q < nch; q += kThreads)
-      rq[sBase + q] = make_uint2((uint32_t)k, (uint32_t)b * (uint32_t)nch + (uint32_t)q);
-    return;
-  }
+  const bool heavy = (uint32_t)cnt >= ray_clusters && may_heavy;
+  const bool per_ray = heavy || (ray_clusters <= (1u << 29) && cnt > 0 && wide);
+  queue_chunks(per_ray);
+  if (per_ray) return;
   const int nu = (cnt + per_unit - 1) / per_unit;
   if (tid == 0 && nu > 0) sBase = atomicAdd(&ctr[0], (uint32_t)nu);
   __syncthreads();
@@ -1502,7 +1520,7 @@ constexpr uint32_t kChainQC = kChainFlushC + 64;
 template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void ray_path_kernel(
     const RhItem* __restrict__ items, const uint2* __restrict__ rq, uint32_t* __restrict__ ctr,
-    uint32_t walk) {
+    uint32_t nchunks, uint32_t walk) {
   __shared__ uint32_t sStk[kSearchDepth][64];
   constexpr uint32_t kQ = MODE == 2 ? kChainQ : kChainQC;
   __shared__ uint32_t sQIdx[kQ], sQRef[kQ], sQMeta[kQ];
@@ -1512,8 +1530,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
   constexpr uint32_t kDone = 0xffffffffu;  // not a node reference (leaves hold <= 4 triangles)
   const int lane = threadIdx.x;
   const uint64_t lanes_below = (1ull << lane) - 1ull;
-  const uint32_t nchunks = __builtin_amdgcn_readfirstlane(ctr[4]);
-  if (nchunks == 0) return;  // no incoherent block: leave without touching the queue
   // wavefront-uniform: the chunk being dealt and its next slot
   uint32_t cjob = 0, cblk = 0, cpart = 0, cslot = 64;
   bool more = true;
@@ -1615,6 +1631,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
           break;
         }
         const uint2 w = rq[e];
+        if (__builtin_amdgcn_readfirstlane(w.x) == 0xffffffffu) continue;  // a block-path block
         cjob = __builtin_amdgcn_readfirstlane(w.x);
         const uint32_t c = __builtin_amdgcn_readfirstlane(w.y);
         const uint32_t nch = ((uint32_t)items[cjob].g.rb + 63u) / 64u;
@@ -2329,7 +2346,7 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
   auto launch_rays = [&](auto kernel) {
     const dim3 grid(G.ray_grid > 0 ? std::max(1u, std::min((unsigned)G.ray_grid, G.max_chunks))
                                    : resident_grid(kernel, G.max_chunks, 64));
-    hipLaunchKernelGGL(kernel, grid, dim3(64), 0, s, G.items, G.rq, G.ctr,
+    hipLaunchKernelGGL(kernel, grid, dim3(64), 0, s, G.items, G.rq, G.ctr, G.max_chunks,
                        (uint32_t)std::min(std::max(G.walk, 1), 7));
   };
   if (G.bvh)
