@@ -439,7 +439,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
   const size_t tables = align256(sizeof(RhItem) * std::max<size_t>(jobs.size(), 1));
   total += tables;
   struct GroupLayout {
-    size_t units, rq, ctr, replist, max_units, max_chunks;
+    size_t units, ustage, bunits, rq, ctr, replist, max_units, max_chunks;
     uint32_t ray;
     int rgrid, walk;
     uint32_t nblocks, nrays;
@@ -473,6 +473,8 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
       it.block_base = (uint32_t)blocks;
       it.ray_base = (uint32_t)rays;
       it.chunk_base = (uint32_t)L.max_chunks;
+      it.unit_base = (uint32_t)L.max_units;
+      it.unit_stride = (it.nclusters + L.per - 1) / L.per;
       blocks += (uint64_t)it.g.ntop;
       rays += n;
       // the 1024-ray unit kernels, for blocks beyond 256 rays or 3 levels that the unit
@@ -500,6 +502,10 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
                  : 0;
     L.units = total;
     total += align256(sizeof(uint4) * std::max<size_t>(L.max_units, 1));
+    L.ustage = total;
+    total += align256(sizeof(uint4) * std::max<size_t>(L.max_units, 1));
+    L.bunits = total;
+    total += align256(sizeof(uint4) * std::max<size_t>(L.nblocks, 1));
     L.rq = total;
     total += align256(sizeof(uint2) * std::max<size_t>(L.max_chunks, 1));
     L.ctr = total;
@@ -547,6 +553,8 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     G.nblocks = gl[gi].nblocks;
     G.nrays = gl[gi].nrays;
     G.units = (uint4*)(A + gl[gi].units);
+    G.ustage = (uint4*)(A + gl[gi].ustage);
+    G.bunits = (uint4*)(A + gl[gi].bunits);
     G.rq = (uint2*)(A + gl[gi].rq);
     G.max_chunks = (uint32_t)std::min<size_t>(gl[gi].max_chunks, 0xffffffffu);
     G.ray_clusters = gl[gi].ray;
@@ -1017,6 +1025,8 @@ int rt_trace_closest_noalloc(rt_scene* s, const rt_rays* rays, int32_t mode, con
   it.clist = nullptr;
   RhGroup G{};
   G.units = (uint4*)take(sizeof(uint4));
+  G.ustage = G.units;  // every block takes the per-ray path: no unit is staged
+  G.bunits = (uint4*)take(sizeof(uint4) * (size_t)it.g.ntop);
   G.rq = (uint2*)take(sizeof(uint2) * (size_t)it.g.ntop * nch);
   G.ctr = (uint32_t*)take(256);
   G.replist = repair ? (uint32_t*)take(sizeof(uint32_t) * n) : nullptr;

@@ -104,6 +104,8 @@ struct RhItem {
   uint32_t block_base, ray_base;
   int32_t repair_run;
   uint32_t chunk_base;   // first per-ray chunk slot of the job (its blocks' slots follow)
+  uint32_t unit_base;    // first staging slot of the job's work units (fixed per block)
+  uint32_t unit_stride;  // staging slots per block: the most units a block can make
 };
 
 // A launch group: `items` is the device copy of the job table; units, counters and the
@@ -116,7 +118,9 @@ struct RhGroup {
   int big;   // some job has more than 256 rays per block or 4 levels
   int bvh;   // ray-bvh jobs (product traversal with the scene's BVH)
   uint32_t nblocks, nrays;
-  uint4* units;       // {item, block, first cluster, count}
+  uint4* units;       // {item, block, first cluster, count}, compacted from ustage
+  uint4* ustage;      // the units of every block at fixed slots (job unit_base + block * stride)
+  uint4* bunits;      // per block: {units made, first staging slot, first queue slot, 0}
   uint2* rq;          // per-ray path: one slot per 64-ray chunk of every block, at the job's
                       // chunk_base + block * chunks per block + chunk: {item, block * chunks
                       // per block + chunk}, or item 0xffffffff for a block of the block path

@@ -763,12 +763,11 @@ __device__ __forceinline__ void expand_queue(const RHGeom& g, PF v9s, P4 tbox, P
 constexpr float kIncoherent = 0.25f;
 template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
-                                                         int nitems, uint4* __restrict__ units,
-                                                         uint2* __restrict__ rq,
-                                                         uint32_t* __restrict__ ctr,
-                                                         int per_unit, uint32_t ray_clusters) {
+                                                         int nitems, uint4* __restrict__ ustage,
+                                                         uint4* __restrict__ bunits,
+                                                         uint2* __restrict__ rq, int per_unit,
+                                                         uint32_t ray_clusters) {
   __shared__ int sCnt;
-  __shared__ uint32_t sBase;
   __shared__ float sSpread;
   __shared__ int sLeaves;
   const int tid = threadIdx.x;
@@ -787,6 +786,7 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
     for (int q = tid; q < nch; q += kThreads)
       slots[q] = make_uint2(per_ray ? (uint32_t)k : 0xffffffffu,
                             (uint32_t)b * (uint32_t)nch + (uint32_t)q);
+    if (tid == 0) bunits[blockIdx.x] = make_uint4(0u, 0u, 0u, 0u);  // block-path blocks rewrite it
   };
   if (a0.w == 0.f) {  // no traced ray below this root (uniform)
     queue_chunks(false);
@@ -858,12 +858,64 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   const bool per_ray = heavy || (ray_clusters <= (1u << 29) && cnt > 0 && wide);
   queue_chunks(per_ray);
   if (per_ray) return;
+  // the block's units go to its fixed staging slots (no shared append counter: thousands of
+  // blocks appending through one atomic serialise on it); rh_unit_scan_kernel and
+  // rh_unit_pack_kernel pack them
   const int nu = (cnt + per_unit - 1) / per_unit;
-  if (tid == 0 && nu > 0) sBase = atomicAdd(&ctr[0], (uint32_t)nu);
-  __syncthreads();
+  const uint32_t first = it.unit_base + (uint32_t)b * it.unit_stride;
   for (int u = tid; u < nu; u += kThreads)
-    units[sBase + u] = make_uint4((uint32_t)k, (uint32_t)b, (uint32_t)(u * per_unit),
-                                  (uint32_t)min(per_unit, cnt - u * per_unit));
+    ustage[first + u] = make_uint4((uint32_t)k, (uint32_t)b, (uint32_t)(u * per_unit),
+                                   (uint32_t)min(per_unit, cnt - u * per_unit));
+  if (tid == 0) bunits[blockIdx.x] = make_uint4((uint32_t)nu, first, 0u, 0u);
+}
+
+// Packs the staged units of every block of a group into one queue in block order.  The scan
+// (one work-group) gives each block its first queue slot and sets the queue's length ctr[0]:
+// tiles of 1,024 consecutive blocks, one per thread (coalesced), scanned within each wavefront
+// by shuffles and across the 16 wavefronts through LDS.  The pack copies each block's units
+// there, a wavefront per 64 blocks.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void rh_unit_scan_kernel(uint4* __restrict__ bunits,
+                                                                    uint32_t nblocks,
+                                                                    uint32_t* __restrict__ ctr) {
+  __shared__ uint32_t sWave[kScanThreads / 64];
+  __shared__ uint32_t sCarry;
+  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  if (tid == 0) sCarry = 0u;
+  __syncthreads();
+  for (uint32_t base = 0; base < nblocks; base += kScanThreads) {
+    const uint32_t b = base + tid;
+    const uint32_t n = b < nblocks ? bunits[b].x : 0u;
+    uint32_t incl = n;  // inclusive prefix within the wavefront
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint32_t v = (uint32_t)__shfl_up((int)incl, d);
+      if (lane >= d) incl += v;
+    }
+    if (lane == 63) sWave[wave] = incl;
+    __syncthreads();
+    uint32_t before = sCarry;  // units of the earlier tiles and wavefronts
+    for (uint32_t w = 0; w < wave; ++w) before += sWave[w];
+    if (b < nblocks) bunits[b].z = before + incl - n;
+    __syncthreads();
+    if (tid == kScanThreads - 1) sCarry = before + incl;
+    __syncthreads();
+  }
+  if (tid == 0) ctr[0] = sCarry;
+}
+__global__ __launch_bounds__(64) void rh_unit_pack_kernel(const uint4* __restrict__ bunits,
+                                                          uint32_t nblocks,
+                                                          const uint4* __restrict__ ustage,
+                                                          uint4* __restrict__ units) {
+  // each lane reads one block's entry; the wavefront then copies the blocks one after another,
+  // a block's units spread over the lanes
+  const uint32_t lane = threadIdx.x, b = blockIdx.x * 64u + lane;
+  const uint4 bu = b < nblocks ? bunits[b] : make_uint4(0u, 0u, 0u, 0u);
+  for (int l = 0; l < 64; ++l) {
+    const uint32_t nu = (uint32_t)__shfl((int)bu.x, l);
+    if (nu == 0u) continue;
+    const uint32_t from = (uint32_t)__shfl((int)bu.y, l), to = (uint32_t)__shfl((int)bu.z, l);
+    for (uint32_t u = lane; u < nu; u += 64u) units[to + u] = ustage[from + u];
+  }
 }
 
 // Persistent unit kernel: a fixed grid takes units from the group's queue with an atomic
@@ -2314,11 +2366,18 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
                      dim3(kThreads), 0, s, G.items, G.nitems);
   if (G.bvh)
     hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE, true>), dim3(G.nblocks), dim3(kThreads), 0,
-                       s, G.items, G.nitems, G.units, G.rq, G.ctr, G.per_unit, G.ray_clusters);
+                       s, G.items, G.nitems, G.ustage, G.bunits, G.rq, G.per_unit,
+                       G.ray_clusters);
   else
     hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE, false>), dim3(G.nblocks), dim3(kThreads),
-                       0, s, G.items, G.nitems, G.units, G.rq, G.ctr, G.per_unit,
+                       0, s, G.items, G.nitems, G.ustage, G.bunits, G.rq, G.per_unit,
                        G.ray_clusters);
+  if (G.ray_clusters > 1) {  // 0 and 1 send every block with work per-ray: no unit
+    hipLaunchKernelGGL(rh_unit_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, G.bunits,
+                       G.nblocks, G.ctr);
+    hipLaunchKernelGGL(rh_unit_pack_kernel, dim3((G.nblocks + 63) / 64), dim3(64), 0, s,
+                       G.bunits, G.nblocks, G.ustage, G.units);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (t0) {
