@@ -379,7 +379,7 @@ def bench_uneven(args, rt, scenes, torch, orc):
     return result
 
 
-def bench_frame(args, rt, scenes, torch, wl):
+def bench_frame(args, rt, scenes, torch, orc, wl):
     """A whole frame as the renderer's tasks: the image is cut into tiles (one ray image of
     (tile width * spp) x tile height each).  Every pass (camera, then per depth the shadow and
     bounce rays) runs for all tasks either as one batch of jobs (batched form, HIP graph) or
@@ -500,6 +500,38 @@ def bench_frame(args, rt, scenes, torch, wl):
         neq(B[4][d][k].id, P[4][d][k].id) for d in range(depth) for k in range(T))
     mism_t = sum(neq(B[0][k].t.view(torch.int32), P[0][k].t.view(torch.int32)) for k in range(T))
     mism_occ = sum(neq(B[2][d][k], P[2][d][k]) for d in range(depth + 1) for k in range(T))
+
+    # CPU baseline: pbrt's BVHAccel (SAH, maxnodeprims 4, inclusive Triangle test) on the first
+    # task's rays of every pass (camera, bounce and shadow rays as the batched form made them),
+    # all host threads, median of 3; BVH build excluded
+    cpu = None
+    if not args.no_cpu:
+        def host(d):
+            out = {k: getattr(d, k).cpu().numpy() for k in
+                   ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")}
+            out["valid"] = d.valid.cpu().numpy() if d.valid is not None else None
+            out["width"], out["height"] = d.width, d.height
+            return out
+        closest_s = [tasks[0]] + [host(B[3][d][0]) for d in range(depth)]
+        shadow_s = [host(B[1][d][0]) for d in range(depth + 1)]
+        nr = sum(int(r["width"]) * int(r["height"]) if r.get("valid") is None
+                 else int(np.asarray(r["valid"]).sum()) for r in closest_s) + \
+            sum(int(r["valid"].sum()) for r in shadow_s)
+        bv = orc.CpuBvh(v9, max_prims=4)
+        thr = host_threads()
+        ts = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            for r in closest_s:
+                bv.closest(r, nthreads=thr)
+            for r in shadow_s:
+                bv.occluded(r, nthreads=thr)
+            ts.append(time.perf_counter() - c0)
+        bv.close()
+        cpu = {"value": round(nr / statistics.median(ts) / 1e6, 3), "unit": "Mrays/s",
+               "cores": thr, "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"task 0 of {T}: its camera, bounce and shadow rays ({nr} rays), median "
+                         f"of 3 runs; BVH build excluded"}
     result = {
         "metric": "Mrays/s batched vs plain per-scene launch, same hits; p50/p99 request latency",
         "value": round(rays_per_step / t_batch / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1,
@@ -516,7 +548,7 @@ def bench_frame(args, rt, scenes, torch, wl):
         "batched_over_plain": round(t_plain / t_batch, 3),
         "parity": {"id_mismatches": mism_id, "t_mismatches": mism_t,
                    "occlusion_mismatches": mism_occ},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": None, "cpu_baseline": cpu,
     }
     for b in [b_cam] + b_occ + b_sec:
         b.close()
@@ -807,7 +839,7 @@ def main():
     else:
         wl = scenes.WORKLOADS[args.workload]
         if "tiles" in wl:
-            bench_frame(args, rt, scenes, torch, wl)
+            bench_frame(args, rt, scenes, torch, orc, wl)
             return
         result = bench_plain(args, rt, scenes, torch, orc, args.workload)
     if args.copy_mb > 0:
