@@ -8,7 +8,8 @@ traced both ways and must agree bit for bit (id, t, u, v, occlusion), and agree 
 oracle where the oracle finishes in seconds.  Covered: rayhierarchy box_dir and ia, ray-bvh
 box_dir and ia; primary (with the repair pass), secondary (validity; ray-bvh's first-primitive
 rule) and occlusion; coherent camera rays and incoherent random rays; a 1024-ray block
-hierarchy; the repair-at-tmin case; the default threshold (a mix of both paths in one call).
+hierarchy; the repair-at-tmin case; the default threshold (a mix of both paths in one call); a
+closest hit whose provisional bound fails its chain (the lane traces the ray again).
 """
 import numpy as np
 import pytest
@@ -139,4 +140,32 @@ def test_per_ray_raybvh_first_primitive_rule(rt, scenes, orc, gpu_ctx, node):
         _check_paths(out, mode, f"maxBVHPrim 4 {node} mode {mode}")
         ref = _oracle(orc, sc, "ray-bvh", node, rnd, mode)
         assert np.array_equal(out["ray"]["id"], ref["id"]), f"mode {mode} vs oracle"
+    sc.close()
+
+
+@pytest.mark.parametrize("repair_run", [0, 1])
+def test_per_ray_closest_retrace_after_failed_bound(rt, orc, gpu_ctx, repair_run):
+    """The per-ray closest-hit kernel prunes with a queued candidate before its chain is
+    evaluated.  Here the nearest strict hit of ray 0 (triangle A at t = 5) fails its chain:
+    the root's origin box, grown by the other rays' origins at (100, 100, 0), sees A away from
+    every ray direction (intersectsTri rejects it at the root), while the larger triangle B at
+    t = 8 passes at both levels.  The pruning by A's t cut B away, so the lane must trace the
+    ray again and find B, as the oracle and the block path do."""
+    A = [(-1, -1, 5), (1, -1, 5), (0, 1, 5)]
+    B = [(-10, -10, 8), (110, -10, 8), (-10, 110, 8)]
+    v9 = np.array([np.ravel(A), np.ravel(B)], np.float32)
+    o = np.array([(0, 0, 0), (100, 100, 0), (100, 100, 0), (100, 100, 0)], np.float32)
+    d = np.array([(0, 0, 1)] * 4, np.float32)
+    rays = dict(ox=o[:, 0].copy(), oy=o[:, 1].copy(), oz=o[:, 2].copy(), dx=d[:, 0].copy(),
+                dy=d[:, 1].copy(), dz=d[:, 2].copy(), tmin=np.zeros(4, np.float32),
+                tmax=np.full(4, np.inf, np.float32), valid=None, width=2, height=2)
+    desc = rt.default_desc("rayhierarchy", chunk_x=1, chunk_y=1, height=2, repair_run=repair_run)
+    sc = rt.Scene(gpu_ctx, v9, desc=desc)
+    p = orc.params("rayhierarchy", cx=1, cy=1, height=2, repair_run=repair_run)
+    ref = orc.trace(v9, rays, p, orc.CLOSEST_PRIMARY)
+    assert list(ref["id"]) == [1, -1, -1, -1]
+    out = _paths(rt, sc, rt.DeviceRays.from_numpy(rays), 0)
+    _check_paths(out, 0, f"retrace repair {repair_run}")
+    assert np.array_equal(out["ray"]["id"], ref["id"])
+    assert np.array_equal(out["ray"]["t"].view(np.uint32), ref["t"].view(np.uint32))
     sc.close()
