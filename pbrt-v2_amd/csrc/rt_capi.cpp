@@ -509,7 +509,7 @@ int rh_prepare(RhWorkspace& ws, const std::vector<RhJob>& jobs, const RhParams& 
     L.rq = total;
     total += align256(sizeof(uint2) * std::max<size_t>(L.max_chunks, 1));
     L.ctr = total;
-    total += 256;
+    total += align256(kCounterBytes);
     L.replist = total;
     if (L.repair) total += align256(sizeof(uint32_t) * L.nrays);
   }
@@ -1028,7 +1028,7 @@ int rt_trace_closest_noalloc(rt_scene* s, const rt_rays* rays, int32_t mode, con
   G.ustage = G.units;  // every block takes the per-ray path: no unit is staged
   G.bunits = (uint4*)take(sizeof(uint4) * (size_t)it.g.ntop);
   G.rq = (uint2*)take(sizeof(uint2) * (size_t)it.g.ntop * nch);
-  G.ctr = (uint32_t*)take(256);
+  G.ctr = (uint32_t*)take(kCounterBytes);
   G.replist = repair ? (uint32_t*)take(sizeof(uint32_t) * n) : nullptr;
   if (off > kNoAllocScratch || n >= (1ull << 31)) return RT_ERR_TOO_LARGE;
   it.block_base = 0;

@@ -31,6 +31,14 @@ constexpr int kMaxBlockRays = 1024;      // blocks the unit kernels stage in LDS
                                          // blocks always take the per-ray path
 constexpr int64_t kMaxRaysPerBlock = 1 << 24;  // top-level blocks of at most 2^24 rays
 constexpr int kClusterSize = 64;  // triangles per Morton cluster
+// The unit queue and the per-ray chunk slots are claimed through kClaimQueues pairs of
+// counters, 128 bytes apart after the first 64 words of a group's counter block (kCounterBytes
+// in all): range q of the queue is taken first by the work-groups with blockIdx % kClaimQueues
+// == q (those the dispatcher places on one XCD), so each counter is hit from one XCD until its
+// range runs out.
+constexpr int kClaimQueues = 8;
+constexpr int kClaimBase = 64, kClaimStride = 32;
+constexpr int kCounterBytes = 4 * (kClaimBase + kClaimStride * kClaimQueues);
 
 // Ray batch in device memory (structure of arrays, row-major ray image).
 struct RaysDev {
@@ -124,8 +132,9 @@ struct RhGroup {
   uint2* rq;          // per-ray path: one slot per 64-ray chunk of every block, at the job's
                       // chunk_base + block * chunks per block + chunk: {item, block * chunks
                       // per block + chunk}, or item 0xffffffff for a block of the block path
-  uint32_t* ctr;      // [0] units, [1] next unit to take, [2] rays flagged for repair,
-                      // [3], [4] unused, [5] next per-ray chunk slot to take
+  uint32_t* ctr;      // [0] units, [2] rays flagged for repair, [4] set when any block goes
+                      // per-ray; [64 + 32 q] and [64 + 32 q + 1]: next unit and next per-ray
+                      // chunk slot of range q (kClaimQueues ranges)
   uint32_t* replist;  // global ray indices flagged for repair (primary with repair only)
   int per_unit;       // passing clusters per work unit
   int grid;           // work-groups of the persistent unit kernel; 0 = one per resident slot

@@ -372,6 +372,10 @@ __global__ __launch_bounds__(256) void rh_init_kernel(const RhItem* __restrict__
                                                       uint32_t nrays, uint32_t* __restrict__ ctr) {
   const uint32_t gi = blockIdx.x * kThreads + threadIdx.x;
   if (gi < 8) ctr[gi] = 0u;
+  if (gi < kClaimQueues) {
+    ctr[kClaimBase + kClaimStride * gi] = 0u;      // units
+    ctr[kClaimBase + kClaimStride * gi + 1] = 0u;  // per-ray chunk slots
+  }
   if (gi >= nrays) return;
   const RhItem& it = items[find_ray_item(items, nitems, gi)];
   const uint32_t i = gi - it.ray_base;
@@ -765,8 +769,9 @@ template <int MODE, int NODE, bool BVH>
 __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restrict__ items,
                                                          int nitems, uint4* __restrict__ ustage,
                                                          uint4* __restrict__ bunits,
-                                                         uint2* __restrict__ rq, int per_unit,
-                                                         uint32_t ray_clusters) {
+                                                         uint2* __restrict__ rq,
+                                                         uint32_t* __restrict__ ctr,
+                                                         int per_unit, uint32_t ray_clusters) {
   __shared__ int sCnt;
   __shared__ float sSpread;
   __shared__ int sLeaves;
@@ -786,6 +791,7 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
     for (int q = tid; q < nch; q += kThreads)
       slots[q] = make_uint2(per_ray ? (uint32_t)k : 0xffffffffu,
                             (uint32_t)b * (uint32_t)nch + (uint32_t)q);
+    if (per_ray && tid == 0) ctr[4] = 1u;  // the per-ray kernel has work
     if (tid == 0) bunits[blockIdx.x] = make_uint4(0u, 0u, 0u, 0u);  // block-path blocks rewrite it
   };
   if (a0.w == 0.f) {  // no traced ray below this root (uniform)
@@ -918,6 +924,29 @@ __global__ __launch_bounds__(64) void rh_unit_pack_kernel(const uint4* __restric
   }
 }
 
+// Claims the next unit of a queue of nunits split into kClaimQueues contiguous ranges with a
+// counter each.  Work-group w starts in range w % kClaimQueues: the hardware deals
+// work-groups to the 8 XCDs in turn, so each XCD works through its own range of blocks in its
+// own L2, and one counter serves an eighth of the claims (claims of one counter serialise).
+// A work-group whose range is used up moves on to the next.  Returns nunits when none is
+// left.  One thread of the work-group calls it and keeps the state.  word 0 selects the unit
+// counters, word 1 those of the per-ray chunk slots.
+struct UnitClaim {
+  uint32_t q, tried;
+};
+__device__ __forceinline__ uint32_t claim_unit(uint32_t* ctr, uint32_t nunits, UnitClaim& c,
+                                               int word = 0) {
+  while (c.tried < (uint32_t)kClaimQueues) {
+    const uint32_t lo = (uint32_t)((uint64_t)nunits * c.q / kClaimQueues);
+    const uint32_t hi = (uint32_t)((uint64_t)nunits * (c.q + 1) / kClaimQueues);
+    const uint32_t u = lo + atomicAdd(&ctr[kClaimBase + kClaimStride * c.q + word], 1u);
+    if (u < hi) return u;
+    c.q = (c.q + 1) % kClaimQueues;
+    ++c.tried;
+  }
+  return nunits;
+}
+
 // Persistent unit kernel: a fixed grid takes units from the group's queue with an atomic
 // counter (work stealing across every job of the group).  A unit stages its block's rays and
 // subtree, takes the triangles of its clusters through the per-triangle bound (phase A),
@@ -932,8 +961,9 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
   const int tid = threadIdx.x;
   const uint32_t nunits = __builtin_amdgcn_readfirstlane(ctr[0]);
   if (nunits == 0) return;  // every block went to the per-ray path (or had no work)
+  UnitClaim claim{blockIdx.x % kClaimQueues, 0u};
   if (tid == 0) {
-    const uint32_t u = atomicAdd(&ctr[1], 1u);
+    const uint32_t u = claim_unit(ctr, nunits, claim);
     sUnit = u < nunits ? units[u] : make_uint4(kNoUnit, 0u, 0u, 0u);
   }
   if (tid < kUnitLevels) sh.sCnt[tid] = 0;
@@ -1013,7 +1043,7 @@ __global__ __launch_bounds__(256) void rh_unit_kernel(const RhItem* __restrict__
     // descriptor load overlap the merge below.
     uint4 ndesc = make_uint4(kNoUnit, 0u, 0u, 0u);
     if (tid == 0) {
-      const uint32_t next = atomicAdd(&ctr[1], 1u);
+      const uint32_t next = claim_unit(ctr, nunits, claim);
       if (next < nunits) ndesc = units[next];
     }
     for (int slot = tid; slot < g.rb; slot += kThreads) {
@@ -1297,8 +1327,9 @@ __global__ __launch_bounds__(256) void bvh_unit_kernel(const RhItem* __restrict_
   const int tid = threadIdx.x;
   const uint32_t nunits = __builtin_amdgcn_readfirstlane(ctr[0]);
   if (nunits == 0) return;
+  UnitClaim claim{blockIdx.x % kClaimQueues, 0u};
   for (;;) {
-    if (tid == 0) sUnit = atomicAdd(&ctr[1], 1u);
+    if (tid == 0) sUnit = claim_unit(ctr, nunits, claim);
     __syncthreads();
     const uint32_t u = __builtin_amdgcn_readfirstlane(sUnit);
     if (u >= nunits) break;
@@ -1605,6 +1636,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
   // occlusion: deferred chain candidates (count sQN), the serial of this lane's ray, and the
   // rounds since the queue was last evaluated (wavefront-uniform)
   uint32_t serial = 0, age = 0;
+  // the cluster pass sets ctr[4] when it routed any block here
+  if (__builtin_amdgcn_readfirstlane(ctr[4]) == 0u) return;
+  UnitClaim claim{blockIdx.x % kClaimQueues, 0u};
   if (lane == 0) {
     sQN = 0;
     sQHit = 0ull;
@@ -1676,7 +1710,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
       if (need == 0 || !more) break;
       if (cslot >= 64) {
         uint32_t e = 0;
-        if (lane == 0) e = atomicAdd(&ctr[5], 1u);
+        if (lane == 0) e = claim_unit(ctr, nchunks, claim, 1);
         e = __builtin_amdgcn_readfirstlane(e);
         if (e >= nchunks) {
           more = false;
@@ -2366,11 +2400,11 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
                      dim3(kThreads), 0, s, G.items, G.nitems);
   if (G.bvh)
     hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE, true>), dim3(G.nblocks), dim3(kThreads), 0,
-                       s, G.items, G.nitems, G.ustage, G.bunits, G.rq, G.per_unit,
+                       s, G.items, G.nitems, G.ustage, G.bunits, G.rq, G.ctr, G.per_unit,
                        G.ray_clusters);
   else
     hipLaunchKernelGGL((rh_cluster_kernel<MODE, NODE, false>), dim3(G.nblocks), dim3(kThreads),
-                       0, s, G.items, G.nitems, G.ustage, G.bunits, G.rq, G.per_unit,
+                       0, s, G.items, G.nitems, G.ustage, G.bunits, G.rq, G.ctr, G.per_unit,
                        G.ray_clusters);
   if (G.ray_clusters > 1) {  // 0 and 1 send every block with work per-ray: no unit
     hipLaunchKernelGGL(rh_unit_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, G.bunits,
