@@ -879,7 +879,8 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
 // (one work-group) gives each block its first queue slot and sets the queue's length ctr[0]:
 // tiles of 1,024 consecutive blocks, one per thread (coalesced), scanned within each wavefront
 // by shuffles and across the 16 wavefronts through LDS.  The pack copies each block's units
-// there, a wavefront per 64 blocks.
+// there, a wavefront per block (four to a work-group): the copies of all blocks are in flight
+// at once, so the pass costs one load-to-store round trip.
 constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void rh_unit_scan_kernel(uint4* __restrict__ bunits,
                                                                     uint32_t nblocks,
@@ -908,20 +909,14 @@ __global__ __launch_bounds__(kScanThreads) void rh_unit_scan_kernel(uint4* __res
   }
   if (tid == 0) ctr[0] = sCarry;
 }
-__global__ __launch_bounds__(64) void rh_unit_pack_kernel(const uint4* __restrict__ bunits,
-                                                          uint32_t nblocks,
-                                                          const uint4* __restrict__ ustage,
-                                                          uint4* __restrict__ units) {
-  // each lane reads one block's entry; the wavefront then copies the blocks one after another,
-  // a block's units spread over the lanes
-  const uint32_t lane = threadIdx.x, b = blockIdx.x * 64u + lane;
-  const uint4 bu = b < nblocks ? bunits[b] : make_uint4(0u, 0u, 0u, 0u);
-  for (int l = 0; l < 64; ++l) {
-    const uint32_t nu = (uint32_t)__shfl((int)bu.x, l);
-    if (nu == 0u) continue;
-    const uint32_t from = (uint32_t)__shfl((int)bu.y, l), to = (uint32_t)__shfl((int)bu.z, l);
-    for (uint32_t u = lane; u < nu; u += 64u) units[to + u] = ustage[from + u];
-  }
+constexpr int kPackThreads = 256;
+__global__ __launch_bounds__(kPackThreads) void rh_unit_pack_kernel(
+    const uint4* __restrict__ bunits, uint32_t nblocks, const uint4* __restrict__ ustage,
+    uint4* __restrict__ units) {
+  const uint32_t b = blockIdx.x * (kPackThreads / 64u) + (threadIdx.x >> 6);
+  if (b >= nblocks) return;
+  const uint4 bu = bunits[b];
+  for (uint32_t u = __lane_id(); u < bu.x; u += 64u) units[bu.z + u] = ustage[bu.y + u];
 }
 
 // Claims the next unit of a queue of nunits split into kClaimQueues contiguous ranges with a
@@ -2409,8 +2404,9 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
   if (G.ray_clusters > 1) {  // 0 and 1 send every block with work per-ray: no unit
     hipLaunchKernelGGL(rh_unit_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, G.bunits,
                        G.nblocks, G.ctr);
-    hipLaunchKernelGGL(rh_unit_pack_kernel, dim3((G.nblocks + 63) / 64), dim3(64), 0, s,
-                       G.bunits, G.nblocks, G.ustage, G.units);
+    hipLaunchKernelGGL(rh_unit_pack_kernel, dim3((G.nblocks + kPackThreads / 64 - 1) /
+                                                 (kPackThreads / 64)),
+                       dim3(kPackThreads), 0, s, G.bunits, G.nblocks, G.ustage, G.units);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
