@@ -57,6 +57,7 @@ def profile_facts(workload, kernel):
       traffic-per-launch  HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by what the
                           same counters read for a plain copy of known size)
       valu-busy           fraction of the launch the VALU was busy (SQ counters)
+      lane-utilisation    fraction of the VALU lanes active while it issued (SQ counters)
     Returns {fact: value} (empty when not profiled)."""
     out = {}
     for f in sorted((ROOT / "profiles").glob("*.md")):
@@ -697,7 +698,9 @@ def bench_plain(args, rt, scenes, torch, orc, name):
                                      "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
                                      "frac": round(tfl / FP32_PEAK_TFLOPS, 4),
                                      "peak_unpacked": FP32_UNPACKED_TFLOPS,
-                                     "busy_measured": facts.get("valu-busy")}}
+                                     "busy_measured": facts.get("valu-busy"),
+                                     "lane_utilisation_measured":
+                                         facts.get("lane-utilisation")}}
         thr = host_threads()
         bvh = orc.CpuBvh(v9, max_prims=4)
         ts = []
