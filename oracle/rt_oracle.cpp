@@ -1258,11 +1258,13 @@ uint32_t s_build(SearchTree& T, const float* v9, const std::vector<uint32_t>& co
   return node;
 }
 
-// Binned SAH build: 16 bins on the axis of largest centroid extent, split cost nL*area(L) +
-// nR*area(R) (the first minimum), stable partitions.  A range of at most kSLeaf triangles is
-// a leaf unless its best split costs less than testing them all (node and triangle cost 1 per
-// unit area); a range whose centroids coincide on that axis, or whose SAH split would take the
-// tree past kSDepth interior levels, splits at the median by (centroid, id).
+// Binned SAH build: 16 bins on each axis with a centroid extent, split cost nL*area(L) +
+// nR*area(R) (the first minimum over the axes x, y, z in turn and the planes in order),
+// stable partitions.  A range of at most kSLeaf triangles is a leaf unless its best split
+// costs less than testing them all (node and triangle cost 1 per unit area); a range whose
+// centroids coincide on every axis, or whose SAH split would take the tree past kSDepth
+// interior levels, splits at the median by (centroid, id) on the axis of largest centroid
+// extent.
 struct SPrim {
   float lo[3], hi[3], c[3];
 };
@@ -1305,77 +1307,79 @@ uint32_t s_build_sah(SearchTree& T, const std::vector<SPrim>& P, uint32_t a, uin
   int ax = 0;
   for (int k = 1; k < 3; ++k)
     if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
-  const float ext = chi[ax] - clo[ax];
-  uint32_t m = 0;
-  bool median = ext <= 0.f;
-  if (!median) {
-    constexpr int kB = 16;
-    const float scale = (float)kB / ext;
-    auto bin = [&](uint32_t t) {
-      const int k = (int)((P[t].c[ax] - clo[ax]) * scale);
-      return std::min(std::max(k, 0), kB - 1);
-    };
+  constexpr int kB = 16;
+  auto bin_of = [&](uint32_t t, int q) {
+    const float scale = (float)kB / (chi[q] - clo[q]);
+    const int k = (int)((P[t].c[q] - clo[q]) * scale);
+    return std::min(std::max(k, 0), kB - 1);
+  };
+  int best_q = -1, best = -1;
+  float best_cost = kInf;
+  for (int q = 0; q < 3; ++q) {
+    if (!(chi[q] - clo[q] > 0.f)) continue;
     SBoxC bb[kB];
     uint32_t cnt[kB] = {0};
     for (int k = 0; k < kB; ++k) s_empty(bb[k]);
     for (uint32_t i = a; i < b; ++i) {
-      const int k = bin(idx[i]);
+      const int k = bin_of(idx[i], q);
       s_grow(bb[k], P[idx[i]]);
       ++cnt[k];
     }
-    float rarea[kB];
-    uint32_t rcnt[kB];
+    // costs of the kB - 1 planes: left side accumulated upward, right side downward
+    float larea[kB], rarea[kB];
+    uint32_t lcnt[kB], rcnt[kB];
     SBoxC acc;
     s_empty(acc);
     uint32_t c = 0;
-    for (int k = kB - 1; k > 0; --k) {
+    for (int k = 0; k < kB; ++k) {
       c += cnt[k];
-      if (cnt[k]) {
-        for (int q = 0; q < 3; ++q) {
-          acc.lo[q] = std::min(acc.lo[q], bb[k].lo[q]);
-          acc.hi[q] = std::max(acc.hi[q], bb[k].hi[q]);
+      if (cnt[k])
+        for (int z = 0; z < 3; ++z) {
+          acc.lo[z] = std::min(acc.lo[z], bb[k].lo[z]);
+          acc.hi[z] = std::max(acc.hi[z], bb[k].hi[z]);
         }
-      }
-      rarea[k] = c ? s_area(acc) : 0.f;
-      rcnt[k] = c;
+      larea[k] = c ? s_area(acc) : 0.f;
+      lcnt[k] = c;
     }
     s_empty(acc);
     c = 0;
-    int best = -1;
-    float best_cost = kInf;
-    for (int k = 0; k < kB - 1; ++k) {
+    for (int k = kB - 1; k >= 0; --k) {
       c += cnt[k];
-      if (cnt[k]) {
-        for (int q = 0; q < 3; ++q) {
-          acc.lo[q] = std::min(acc.lo[q], bb[k].lo[q]);
-          acc.hi[q] = std::max(acc.hi[q], bb[k].hi[q]);
+      if (cnt[k])
+        for (int z = 0; z < 3; ++z) {
+          acc.lo[z] = std::min(acc.lo[z], bb[k].lo[z]);
+          acc.hi[z] = std::max(acc.hi[z], bb[k].hi[z]);
         }
-      }
-      if (c == 0 || rcnt[k + 1] == 0) continue;
-      const float cost = (float)c * s_area(acc) + (float)rcnt[k + 1] * rarea[k + 1];
+      rarea[k] = c ? s_area(acc) : 0.f;
+      rcnt[k] = c;
+    }
+    for (int k = 0; k + 1 < kB; ++k) {
+      if (lcnt[k] == 0 || rcnt[k + 1] == 0) continue;
+      const float cost = (float)lcnt[k] * larea[k] + (float)rcnt[k + 1] * rarea[k + 1];
       if (cost < best_cost) {
         best_cost = cost;
         best = k;
+        best_q = q;
       }
     }
-    if (n <= (uint32_t)kSLeaf) {
-      SBoxC all;
-      s_empty(all);
-      for (uint32_t i = a; i < b; ++i) s_grow(all, P[idx[i]]);
-      const float A = s_area(all);
-      if (best < 0 || !(kSNodeCost * A + kSTriCost * best_cost < kSTriCost * (float)n * A)) {
-        box = all;
-        return 0x80000000u | ((n - 1) << 27) | a;
-      }
+  }
+  uint32_t m = 0;
+  bool median = best < 0;
+  if (n <= (uint32_t)kSLeaf) {
+    SBoxC all;
+    s_empty(all);
+    for (uint32_t i = a; i < b; ++i) s_grow(all, P[idx[i]]);
+    const float A = s_area(all);
+    if (best < 0 || !(kSNodeCost * A + kSTriCost * best_cost < kSTriCost * (float)n * A)) {
+      box = all;
+      return 0x80000000u | ((n - 1) << 27) | a;
     }
-    if (best < 0) {
-      median = true;
-    } else {
-      uint32_t* mid = std::stable_partition(idx + a, idx + b,
-                                            [&](uint32_t t) { return bin(t) <= best; });
-      m = (uint32_t)(mid - idx);
-      if (depth + s_levels(m - a) > kSDepth || depth + s_levels(b - m) > kSDepth) median = true;
-    }
+  }
+  if (!median) {
+    uint32_t* mid = std::stable_partition(idx + a, idx + b,
+                                          [&](uint32_t t) { return bin_of(t, best_q) <= best; });
+    m = (uint32_t)(mid - idx);
+    if (depth + s_levels(m - a) > kSDepth || depth + s_levels(b - m) > kSDepth) median = true;
   }
   if (median && n <= (uint32_t)kSLeaf) {
     s_empty(box);
