@@ -1173,7 +1173,7 @@ void parallel_for(size_t n, int nthreads, F&& fn) {
 
 // ---- per-ray path restatement ------------------------------------------------------------
 // The library traces incoherent ray groups ray by ray: each ray walks a search BVH of the
-// triangles (binned SAH, leaves of at most 4, boxes grown by 2^-11 * max(1, largest
+// triangles (binned SAH, one triangle per leaf, boxes grown by 2^-11 * max(1, largest
 // |coordinate|)), runs the strict leaf test on the triangles of the leaves it reaches and
 // evaluates the reference's cull chain for the ones that pass.  This is a CPU restatement of
 // that traversal (not of the reference): its results must equal the reference-semantics
@@ -1181,7 +1181,7 @@ void parallel_for(size_t n, int nthreads, F&& fn) {
 // the work model of the per-ray kernel, so the tree is built by the same rules as the
 // library's.
 constexpr int kSDepth = 26;
-constexpr int kSLeaf = 4;
+constexpr int kSLeaf = 1;
 
 struct SBoxC {
   float lo[3], hi[3];
@@ -1261,7 +1261,8 @@ uint32_t s_build(SearchTree& T, const float* v9, const std::vector<uint32_t>& co
 // Binned SAH build: 16 bins on each axis with a centroid extent, split cost nL*area(L) +
 // nR*area(R) (the first minimum over the axes x, y, z in turn and the planes in order),
 // stable partitions.  A range of at most kSLeaf triangles is a leaf unless its best split
-// costs less than testing them all (node and triangle cost 1 per unit area); a range whose
+// costs less than testing them all (node and triangle cost 1 per unit area; with kSLeaf = 1
+// every leaf holds one triangle and the comparison never runs); a range whose
 // centroids coincide on every axis, or whose SAH split would take the tree past kSDepth
 // interior levels, splits at the median by (centroid, id) on the axis of largest centroid
 // extent.

@@ -107,7 +107,7 @@ typedef struct orc_search_stats {
   uint64_t nodes;  /* interior nodes of the search BVH */
   uint64_t chains; /* cull chains evaluated (strict-test hits that could change the ray) */
 } orc_search_stats;
-/* Per-ray traversal of the search BVH (binned SAH, leaves of <= 4, boxes grown by
+/* Per-ray traversal of the search BVH (binned SAH, one triangle per leaf, boxes grown by
  * 2^-11 max(1, max |coordinate|)) with the strict leaf test and the reference's exact cull
  * chain for passing triangles.  kind 1 (rayhierarchy) or 2 (ray-bvh, with its handoff).
  * ray_mask (optional) selects the rays to trace; the others keep the miss / 0 outputs.

@@ -158,7 +158,12 @@ constexpr int kDefaultWalk = 3;
 #define RT_SEARCH_DEPTH 26
 #endif
 constexpr int kSearchDepth = RT_SEARCH_DEPTH;
-constexpr int kSearchLeaf = 4;
+// Most triangles a leaf holds (the ref's count field allows 16).  One: a leaf test costs
+// about as much as a node visit (52 against 64 bytes, 45 against 24 FLOP), and a lane's leaf
+// is then a single test, so the lanes of a wavefront leave the leaf phase together.  Measured
+// on a batch of 64 cluster scenes against caps of 2, 3 and 4: 8.00-8.03 ms per step against
+// 8.18, 8.31-8.43 and 8.44.
+constexpr int kSearchLeaf = 1;
 // Builds the search BVH of `ntri` triangles (v9 in the caller's order): the node array
 // (4 float4 per node) and the search order (position -> triangle index) whose consecutive
 // positions the leaves reference.

@@ -1605,7 +1605,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
   __shared__ float sQT[MODE == 2 ? 1 : kQ];
   __shared__ unsigned long long sQHit;  // occlusion: rays found occluded; closest: failed bounds
   __shared__ uint32_t sQN;
-  constexpr uint32_t kDone = 0xffffffffu;  // not a node reference (leaves hold <= 4 triangles)
+  constexpr uint32_t kDone = 0xffffffffu;  // not a node reference (leaves hold < 16 triangles)
   const int lane = threadIdx.x;
   const uint64_t lanes_below = (1ull << lane) - 1ull;
   // wavefront-uniform: the chunk being dealt and its next slot

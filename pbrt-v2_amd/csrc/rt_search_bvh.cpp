@@ -13,12 +13,14 @@
 // a centroid extent, split cost nL * area(L) + nR * area(R) (the first minimum over the axes
 // x, y, z in turn and the planes in order), stable partitions.  A range of at most
 // kSearchLeaf triangles becomes a leaf unless its best split is cheaper than testing every
-// triangle (node cost kNodeCost, triangle cost kTriCost, per unit area); a range whose
-// centroids coincide on every axis, or whose SAH split would take the tree past kSearchDepth
-// interior levels, is split at the median by (centroid, id) on the axis of largest centroid
-// extent.  Binning every axis in place of the widest one alone made the per-ray kernels about
-// 4% faster on a batch of 64 cluster scenes.  The build reorders the triangles: leaves hold consecutive positions
-// of that search order, which the scene keeps with the leaf-test operands laid out in it.
+// triangle (node cost kNodeCost, triangle cost kTriCost, per unit area); with kSearchLeaf = 1
+// every leaf holds one triangle and that comparison never runs.  A range whose centroids
+// coincide on every axis, or whose SAH split would take the tree past kSearchDepth interior
+// levels, is split at the median by (centroid, id) on the axis of largest centroid extent.
+// Binning every axis in place of the widest one alone made the per-ray kernels about 4%
+// faster on a batch of 64 cluster scenes.  The build reorders the triangles: leaves hold
+// consecutive positions of that search order, which the scene keeps with the leaf-test
+// operands laid out in it.
 // Nodes are stored depth first, each holding the boxes and references of its two children.
 // Every box is the FP32 min/max of its triangles' vertices grown outward by
 // pad = 2^-11 * max(1, largest vertex coordinate magnitude), which covers the rounding of
