@@ -128,7 +128,8 @@ struct RhGroup {
   uint32_t nblocks, nrays;
   uint4* units;       // {item, block, first cluster, count}, compacted from ustage
   uint4* ustage;      // the units of every block at fixed slots (job unit_base + block * stride)
-  uint4* bunits;      // per block: {units made, first staging slot, first queue slot, 0}
+  uint4* bunits;      // per block: {units made, first staging slot, queue slot within its
+                      // scan tile, the tile's unit total (first block of a tile only)}
   uint2* rq;          // per-ray path: one slot per 64-ray chunk of every block, at the job's
                       // chunk_base + block * chunks per block + chunk: {item, block * chunks
                       // per block + chunk}, or item 0xffffffff for a block of the block path

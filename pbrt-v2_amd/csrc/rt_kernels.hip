@@ -875,48 +875,55 @@ __global__ __launch_bounds__(256) void rh_cluster_kernel(const RhItem* __restric
   if (tid == 0) bunits[blockIdx.x] = make_uint4((uint32_t)nu, first, 0u, 0u);
 }
 
-// Packs the staged units of every block of a group into one queue in block order.  The scan
-// (one work-group) gives each block its first queue slot and sets the queue's length ctr[0]:
-// tiles of 1,024 consecutive blocks, one per thread (coalesced), scanned within each wavefront
-// by shuffles and across the 16 wavefronts through LDS.  The pack copies each block's units
-// there, a wavefront per block (four to a work-group): the copies of all blocks are in flight
-// at once, so the pass costs one load-to-store round trip.
+// Packs the staged units of every block of a group into one queue in block order.
+// - The scan takes tiles of 1,024 consecutive blocks, one work-group per tile and one block
+//   per thread (coalesced). It scans within each wavefront by shuffles and across the 16
+//   wavefronts through LDS, leaving each block's exclusive prefix within its tile in .z and
+//   the tile's total in .w of the tile's first block.
+// - The pack copies each block's units to their queue slots, one wavefront per block (four to
+//   a work-group), so the copies of all blocks are in flight at once. Each wavefront first
+//   adds up the totals of the tiles before its block's tile, one tile per lane. The wavefront
+//   of the last block sets the queue's length, ctr[0].
+// A single work-group walking the tiles in turn took ~37 us for the 58,000 blocks of a
+// 64-scene batch.
 constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void rh_unit_scan_kernel(uint4* __restrict__ bunits,
-                                                                    uint32_t nblocks,
-                                                                    uint32_t* __restrict__ ctr) {
+                                                                    uint32_t nblocks) {
   __shared__ uint32_t sWave[kScanThreads / 64];
-  __shared__ uint32_t sCarry;
   const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
-  if (tid == 0) sCarry = 0u;
-  __syncthreads();
-  for (uint32_t base = 0; base < nblocks; base += kScanThreads) {
-    const uint32_t b = base + tid;
-    const uint32_t n = b < nblocks ? bunits[b].x : 0u;
-    uint32_t incl = n;  // inclusive prefix within the wavefront
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-      const uint32_t v = (uint32_t)__shfl_up((int)incl, d);
-      if (lane >= d) incl += v;
-    }
-    if (lane == 63) sWave[wave] = incl;
-    __syncthreads();
-    uint32_t before = sCarry;  // units of the earlier tiles and wavefronts
-    for (uint32_t w = 0; w < wave; ++w) before += sWave[w];
-    if (b < nblocks) bunits[b].z = before + incl - n;
-    __syncthreads();
-    if (tid == kScanThreads - 1) sCarry = before + incl;
-    __syncthreads();
+  const uint32_t b = blockIdx.x * kScanThreads + tid;
+  const uint32_t n = b < nblocks ? bunits[b].x : 0u;
+  uint32_t incl = n;  // inclusive prefix within the wavefront
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t v = (uint32_t)__shfl_up((int)incl, d);
+    if (lane >= d) incl += v;
   }
-  if (tid == 0) ctr[0] = sCarry;
+  if (lane == 63) sWave[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0;  // units of the earlier wavefronts of the tile
+  for (uint32_t w = 0; w < wave; ++w) before += sWave[w];
+  if (b < nblocks) bunits[b].z = before + incl - n;
+  if (tid == 0) {  // b is the tile's first block, inside the grid's range
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < kScanThreads / 64; ++w) total += sWave[w];
+    bunits[b].w = total;
+  }
 }
 constexpr int kPackThreads = 256;
 __global__ __launch_bounds__(kPackThreads) void rh_unit_pack_kernel(
     const uint4* __restrict__ bunits, uint32_t nblocks, const uint4* __restrict__ ustage,
-    uint4* __restrict__ units) {
+    uint4* __restrict__ units, uint32_t* __restrict__ ctr) {
   const uint32_t b = blockIdx.x * (kPackThreads / 64u) + (threadIdx.x >> 6);
-  if (b >= nblocks) return;
+  if (b >= nblocks) return;  // uniform within the wavefront
+  const uint32_t lane = __lane_id();
   const uint4 bu = bunits[b];
-  for (uint32_t u = __lane_id(); u < bu.x; u += 64u) units[bu.z + u] = ustage[bu.y + u];
+  uint32_t off = 0;  // units of the earlier tiles
+  for (uint32_t t = lane; t < b / (uint32_t)kScanThreads; t += 64u)
+    off += bunits[(size_t)t * kScanThreads].w;
+  for (int d = 32; d > 0; d >>= 1) off += (uint32_t)__shfl_xor((int)off, d);
+  const uint32_t to = off + bu.z;
+  for (uint32_t u = lane; u < bu.x; u += 64u) units[to + u] = ustage[bu.y + u];
+  if (b == nblocks - 1 && lane == 0) ctr[0] = to + bu.x;
 }
 
 // Claims the next unit of a queue of nunits split into kClaimQueues contiguous ranges with a
@@ -2409,11 +2416,11 @@ hipError_t rh_group_dispatch(const RhGroup& G, hipStream_t s, hipEvent_t t0, hip
                        0, s, G.items, G.nitems, G.ustage, G.bunits, G.rq, G.ctr, G.per_unit,
                        G.ray_clusters);
   if (G.ray_clusters > 1) {  // 0 and 1 send every block with work per-ray: no unit
-    hipLaunchKernelGGL(rh_unit_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, G.bunits,
-                       G.nblocks, G.ctr);
+    hipLaunchKernelGGL(rh_unit_scan_kernel, dim3((G.nblocks + kScanThreads - 1) / kScanThreads),
+                       dim3(kScanThreads), 0, s, G.bunits, G.nblocks);
     hipLaunchKernelGGL(rh_unit_pack_kernel, dim3((G.nblocks + kPackThreads / 64 - 1) /
                                                  (kPackThreads / 64)),
-                       dim3(kPackThreads), 0, s, G.bunits, G.nblocks, G.ustage, G.units);
+                       dim3(kPackThreads), 0, s, G.bunits, G.nblocks, G.ustage, G.units, G.ctr);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
