@@ -178,6 +178,10 @@ struct rt_batch {
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
   bool graph_valid = false;
+  // side streams (default priority) and events that fork a captured sequence's groups into
+  // parallel branches: forks[0] is recorded on the caller's stream, forks[k + 1] on side[k]
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> forks;
   Timer timer;
   std::mutex mu;
   void drop_graph() {
@@ -187,7 +191,11 @@ struct rt_batch {
     graph = nullptr;
     graph_valid = false;
   }
-  ~rt_batch() { drop_graph(); }
+  ~rt_batch() {
+    drop_graph();
+    for (hipStream_t s : side) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : forks) (void)hipEventDestroy(e);
+  }
 };
 
 namespace {
@@ -1178,6 +1186,43 @@ static int batch_launch(rt_batch* b, const std::vector<RhGroup>& groups, hipStre
   return RT_OK;
 }
 
+// Captured form of batch_launch.  The groups of a batch are independent (disjoint jobs, arena
+// regions and counters), so every group after the first runs on a side stream forked from hs
+// and joined back: the graph holds the groups as parallel branches, and one group's small
+// kernels (build, cluster pass, finalize) overlap another's traversal.  The side streams and
+// events exist before the capture begins (see ensure_side).
+constexpr size_t kSideStreams = 3;  // with the caller's stream, the device's 4 hardware queues
+static int batch_launch_forked(rt_batch* b, const std::vector<RhGroup>& groups, hipStream_t hs) {
+  const size_t nside = std::min(groups.size() - 1, b->side.size());
+  if (groups.size() < 2 || nside == 0) return batch_launch(b, groups, hs, false);
+  RT_TRY(hipEventRecord(b->forks[0], hs));
+  for (size_t k = 0; k < nside; ++k) RT_TRY(hipStreamWaitEvent(b->side[k], b->forks[0], 0));
+  for (size_t gi = 0; gi < groups.size(); ++gi)
+    RT_TRY(launch_rh_group(groups[gi], gi == 0 ? hs : b->side[(gi - 1) % nside], nullptr,
+                           nullptr, nullptr));
+  for (size_t k = 0; k < nside; ++k) {
+    RT_TRY(hipEventRecord(b->forks[k + 1], b->side[k]));
+    RT_TRY(hipStreamWaitEvent(hs, b->forks[k + 1], 0));
+  }
+  return RT_OK;
+}
+
+// Creates the side streams and events that batch_launch_forked needs for `ngroups` groups.
+static int ensure_side(rt_batch* b, size_t ngroups) {
+  const size_t want = std::min(ngroups > 0 ? ngroups - 1 : 0, kSideStreams);
+  while (b->side.size() < want) {
+    hipStream_t s = nullptr;
+    RT_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    b->side.push_back(s);
+  }
+  while (b->forks.size() < b->side.size() + 1) {
+    hipEvent_t e = nullptr;
+    RT_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->forks.push_back(e);
+  }
+  return RT_OK;
+}
+
 static int batch_run_locked(rt_batch* b, hipStream_t hs) {
   RT_TRY(hipSetDevice(b->ctx->device));
   if (b->jobs.empty()) return RT_OK;
@@ -1189,8 +1234,10 @@ static int batch_run_locked(rt_batch* b, hipStream_t hs) {
   if (!b->use_graph || !hs || b->timer.on) return batch_launch(b, groups, hs, true);
   if (changed || !b->graph_valid || b->graph_stream != hs) {
     b->drop_graph();
+    st = ensure_side(b, groups.size());
+    if (st) return st;
     RT_TRY(hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal));
-    st = batch_launch(b, groups, hs, false);
+    st = batch_launch_forked(b, groups, hs);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(hs, &g);
     if (st) {
