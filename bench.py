@@ -443,40 +443,35 @@ def bench_frame(args, rt, scenes, torch, orc, wl):
                 sc.trace_occluded(sh[d + 1][k], out=oc[d + 1][k], stream=stream)
                 src, srch = bo[d][k], bh[d][k]
 
-    # batched: one batch per pass, every task a job of it
+    # batched: every task a job of one batch per pass. A depth's shadow rays and the next
+    # depth's bounce rays both start from that depth's hits, so they share a batch: its
+    # occlusion and closest-hit groups run as parallel branches of the batch's graph.
     hp, sh, oc, bo, bh = B
     b_cam = rt.Batch(ctx)
     for k in range(T):
         b_cam.add_closest(sc, prim[k], rt.RT_CLOSEST_PRIMARY, hits=hp[k])
-    b_occ = []
+    b_pass = []
     for d in range(depth + 1):
         b = rt.Batch(ctx)
         for k in range(T):
             b.add_occluded(sc, sh[d][k], out=oc[d][k])
-        b_occ.append(b)
-    b_sec = []
-    for d in range(depth):
-        b = rt.Batch(ctx)
-        for k in range(T):
-            b.add_closest(sc, bo[d][k], rt.RT_CLOSEST_SECONDARY, hits=bh[d][k])
-        b_sec.append(b)
-    for b in [b_cam] + b_occ + b_sec:
+        if d < depth:
+            for k in range(T):
+                b.add_closest(sc, bo[d][k], rt.RT_CLOSEST_SECONDARY, hits=bh[d][k])
+        b_pass.append(b)
+    for b in [b_cam] + b_pass:
         b.set_param(rt.RT_PARAM_GRAPH, 1)
 
     def batch_step():
         b_cam.run(stream=stream)
-        for k in range(T):
-            rt.make_shadow_rays(prim[k], hp[k], light, stream=stream, out=sh[0][k])
-        b_occ[0].run(stream=stream)
-        for d in range(depth):
+        for d in range(depth + 1):
             for k in range(T):
                 src, srch = (prim[k], hp[k]) if d == 0 else (bo[d - 1][k], bh[d - 1][k])
-                rt.make_bounce_rays(sc, src, srch, seed=17 + k, depth=d + 1, stream=stream,
-                                    out=bo[d][k])
-            b_sec[d].run(stream=stream)
-            for k in range(T):
-                rt.make_shadow_rays(bo[d][k], bh[d][k], light, stream=stream, out=sh[d + 1][k])
-            b_occ[d + 1].run(stream=stream)
+                rt.make_shadow_rays(src, srch, light, stream=stream, out=sh[d][k])
+                if d < depth:
+                    rt.make_bounce_rays(sc, src, srch, seed=17 + k, depth=d + 1, stream=stream,
+                                        out=bo[d][k])
+            b_pass[d].run(stream=stream)
 
     def timed(fn, label):
         for _ in range(args.warmup):
@@ -539,7 +534,9 @@ def bench_frame(args, rt, scenes, torch, orc, wl):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "form": f"batched ({T} task jobs per pass, HIP graph)",
+        "config": {"workload": args.workload,
+                   "form": f"batched ({T} task jobs per pass, HIP graph; a depth's shadow "
+                           "and bounce rays in one batch)",
                    "accelerator": wl["accel"], "triangles": int(len(v9)),
                    "image": f"{W}x{H}x{spp}spp", "tasks": T, "task_ray_image": f"{tw}x{th}",
                    "primary_rays": n, "shadow_rays": n_shadow, "bounce_rays": n_bounce,
@@ -551,7 +548,7 @@ def bench_frame(args, rt, scenes, torch, orc, wl):
                    "occlusion_mismatches": mism_occ},
         "roofline": None, "cpu_baseline": cpu,
     }
-    for b in [b_cam] + b_occ + b_sec:
+    for b in [b_cam] + b_pass:
         b.close()
     sc.close()
     ctx.close()
