@@ -5,8 +5,8 @@ hit ids, t, u, v and occlusion flags.  Covered: scenes of different sizes and no
 one batch, closest primary / secondary and occlusion jobs mixed, jobs of the 256-ray and the
 1024-ray block kernels in one group, launch shapes (grid from one work-group to more than
 the resident slots, several unit sizes), HIP-graph replay (several replays with eager work in
-between, and re-capture after the jobs change), and the autotuner (which must not change any
-result).
+between, re-capture after the jobs change, and a batch's groups as parallel graph branches),
+and the autotuner (which must not change any result).
 """
 import numpy as np
 import pytest
@@ -257,8 +257,17 @@ def test_batch_raybvh_task_jobs(rt, scenes, gpu_ctx):
     b.set_param(rt.RT_PARAM_GRAPH, 1)
     print(f"[batch] ray-bvh: {len(outs) + len(outs_b)} closest + {len(occs)} occlusion jobs",
           flush=True)
-    for rep in range(3):  # eager capture run, then graph replays
-        b.run()
+    import torch
+
+    stream = torch.cuda.Stream()  # graph capture needs a stream of its own
+    for rep in range(3):  # capture and launch, then graph replays (three groups as branches)
+        for o in outs + outs_b:
+            o.id.fill_(-7)
+        for o in occs:
+            o.fill_(7)
+        torch.cuda.synchronize()
+        b.run(stream=stream)
+        stream.synchronize()
         for k in range(len(wins)):
             _same(_np(outs[k]), _np(plain[k]), f"rep {rep} task {k} primary")
             _same(_np(outs_b[k]), _np(plain_b[k]), f"rep {rep} task {k} bounce")
@@ -266,3 +275,44 @@ def test_batch_raybvh_task_jobs(rt, scenes, gpu_ctx):
     assert sum(int((h.id >= 0).sum().item()) for h in plain_b) > 0
     b.close()
     sc.close()
+
+
+def test_batch_graph_branches_closest_and_occlusion(rt, scenes, gpu_ctx):
+    """A captured batch whose groups (closest hits by node type and mode, and occlusion) run
+    as parallel graph branches on side streams: four replays, outputs poisoned before each and
+    eager work between them, every job bit-identical to its plain call each time."""
+    import torch
+
+    sc, jobs = _jobs(rt, scenes, gpu_ctx)
+    dev = [rt.DeviceRays.from_numpy(r) for _, r, _ in jobs]
+    plain = [sc[n].trace_closest(dev[i], m) for i, (n, _, m) in enumerate(jobs)]
+    ref = [_np(h) for h in plain]
+    prim = [i for i, (_, _, m) in enumerate(jobs) if m == 0]
+    shadow = [rt.make_shadow_rays(dev[i], plain[i], (2.0, 2.0, -2.0)) for i in prim]
+    ref_occ = [sc[jobs[i][0]].trace_occluded(s).cpu().numpy() for i, s in zip(prim, shadow)]
+    stream = torch.cuda.Stream()
+    b = rt.Batch(gpu_ctx)
+    b.set_param(rt.RT_PARAM_GRAPH, 1)
+    outs = [b.add_closest(sc[n], dev[i], m) for i, (n, _, m) in enumerate(jobs)]
+    occs = [b.add_occluded(sc[jobs[i][0]], s) for i, s in zip(prim, shadow)]
+    print(f"[batch] graph branches: {len(outs)} closest + {len(occs)} occlusion jobs",
+          flush=True)
+    for rep in range(4):
+        for o in outs:
+            o.id.fill_(-7)
+            o.t.fill_(-1.0)
+        for o in occs:
+            o.fill_(7)
+        torch.cuda.synchronize()
+        b.run(stream=stream)
+        stream.synchronize()
+        for i in range(len(jobs)):
+            _same(_np(outs[i]), ref[i], f"replay {rep} job {i}")
+        for k in range(len(occs)):
+            assert np.array_equal(occs[k].cpu().numpy(), ref_occ[k]), \
+                f"replay {rep} occlusion job {k}"
+        sc["rnd"].trace_occluded(shadow[1])  # eager work between replays
+        torch.cuda.synchronize()
+    b.close()
+    for s in sc.values():
+        s.close()
