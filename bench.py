@@ -818,8 +818,8 @@ def main():
                          "profiler byte counters against a kernel of known traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "sharding is described in DESIGN.md and not launched here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding the "
+              "jobs over several GPUs is not launched by this benchmark.", file=sys.stderr)
         sys.exit(2)
 
     import torch
