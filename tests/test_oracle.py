@@ -1145,3 +1145,5 @@ def test_search_trace_mask_and_counts(orc, scenes):
     np.testing.assert_array_equal(half["occ"], full["occ"] * mask)
     assert half["stats"].rays == 128
     assert 0 < half["stats"].visits < full["stats"].visits
+    # one triangle per leaf: a full binary tree over the scene's triangles
+    assert full["stats"].nodes == len(v9) - 1
