@@ -355,7 +355,8 @@ def bench_uneven(args, rt, scenes, torch, orc):
         "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "uneven_batch_64",
-                   "form": f"batched ({2 * S} jobs, HIP graph, autotuned launch shape)",
+                   "form": f"batched ({2 * S} jobs, HIP graph with the closest and occlusion "
+                           "groups as parallel branches, autotuned launch shape)",
                    "scenes": S, "triangles": int(sum(j["v9"].shape[0] for j in jobs)),
                    "primary_rays": n_prim, "shadow_rays": n_shad, "accelerator": "rayhierarchy",
                    "leaf": "4x4", "height": 3, "node": "box_dir", "launch_shape": tuned},
